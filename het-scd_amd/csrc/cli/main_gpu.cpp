@@ -1,0 +1,140 @@
+// main-gpu <graph> <device> [group size]
+//
+// The reference's single-GPU driver (src/main-gpu.cpp:30-113) on this engine:
+// same arguments, same stdout lines and loop rule.  The initial partition and
+// the initial WCC run on the device (the reference computes them on the host,
+// src/main-gpu.cpp:54-55; the labels are identical).  With the environment
+// variable SCD_LABELS_OUT set, the best labels (uint32, n entries) are also
+// written to that file; the reference writes none.
+#include <cstdio>
+#include <cstdlib>
+#include <iostream>
+#include <stack>
+#include <string>
+#include <sys/time.h>
+#include <utility>
+#include <vector>
+
+#include "scd_cuda_shim.hpp"
+#include "scd_graph_layout.hpp"
+#include "scd_hip.h"
+#include "scd_host.h"
+
+static double now_ms() {
+    struct timeval tv;
+    gettimeofday(&tv, NULL);
+    return tv.tv_sec * 1000.0 + tv.tv_usec / 1000.0;
+}
+
+// Nested wall-clock timers printing "name: X ms" on exit (src/common.cpp:27-52).
+static std::stack<std::pair<std::string, double>> g_timers;
+static void timer_start(const char *name) { g_timers.push({name, now_ms()}); }
+static void timer_end() {
+    if (g_timers.empty()) return;
+    auto t = g_timers.top();
+    g_timers.pop();
+    std::cout << t.first << ": " << (now_ms() - t.second) << " ms" << std::endl;
+}
+
+static void must(int rc, const char *what) {
+    if (rc != SCD_OK) {
+        std::cerr << what << ": " << scd_last_error() << std::endl;
+        exit(EXIT_FAILURE);
+    }
+}
+
+int main(int argc, char *argv[]) {
+    if (argc < 2) {
+        std::cerr << "usage: " << argv[0] << " <filename> <device> [group size]" << std::endl;
+        return EXIT_FAILURE;
+    }
+    const double threshold = 0.01, alpha = 1.0;
+    const int work_group_size = argc >= 4 ? atoi(argv[3]) : 256;
+    if (!cuda_init(argc >= 3 ? atoi(argv[2]) : -1, work_group_size, true)) return EXIT_FAILURE;
+
+    timer_start("load graph");
+    scd_graph *hg = nullptr;
+    if (scd_graph_read(argv[1], &hg) != SCD_OK) {
+        std::cerr << scd_host_last_error() << std::endl;
+        return EXIT_FAILURE;
+    }
+    timer_end();
+    if (hg->n == 0) {
+        std::cerr << "graph has no vertices" << std::endl;
+        return EXIT_FAILURE;
+    }
+    const Graph g(hg->n, hg->m, scd_graph_clustering_coef(hg), hg->tri_deg, hg->tri, hg->indices, hg->adj);
+
+    timer_start("transfer graph to device");
+    if (!cuda_transfer_graph(&g, alpha)) return EXIT_FAILURE;
+    timer_end();
+
+    // the engine's context is the shim's; reach it through the C ABI calls
+    // that take the labels directly
+    const unsigned n = g.n;
+    std::vector<unsigned> best_labels(n);
+    scd_ctx *ctx = scd_shim_context();
+    timer_start("create initial partition");
+    uint32_t k = 0;
+    must(scd_initial_partition(ctx, best_labels.data(), &k), "create initial partition");
+    timer_end();
+    timer_start("compute WCC");
+    double best_wcc = 0.0;
+    must(scd_compute_wcc(ctx, &best_wcc, nullptr, nullptr, nullptr), "compute WCC");
+    timer_end();
+
+    std::cout << std::endl;
+    std::cout << "nodes: " << g.n << std::endl;
+    std::cout << "edges: " << g.m << std::endl;
+    std::cout << "avg. cc: " << g.clustering_coef << std::endl;
+    std::cout << "initial WCC: " << best_wcc << std::endl;
+    std::cout << "work group size: " << work_group_size << std::endl;
+    std::cout << std::endl;
+
+    const double before = now_ms();
+    int iters = 0;
+    while (true) {
+        iters++;
+        timer_start("iteration");
+        double wcc = 0;
+        cuda_improve_labels(&wcc);
+        if (wcc > best_wcc) cuda_store_labels();
+        timer_end();
+        std::cout << "WCC: " << wcc << std::endl << std::endl;
+        if (wcc < best_wcc * (1 + threshold)) break;
+        best_wcc = wcc;
+    }
+    const double after = now_ms();
+
+    timer_start("transfer labels back");
+    cuda_labels_from_device(best_labels.data());
+    timer_end();
+
+    // printed count of the reference's compress_labels (src/scd.cpp:32-55):
+    // n minus the number of distinct valid labels
+    std::vector<char> seen(n, 0);
+    unsigned distinct = 0;
+    for (unsigned v = 0; v < n; v++) {
+        const unsigned l = best_labels[v];
+        if (l != SCD_INVALID_LABEL && l < n && !seen[l]) {
+            seen[l] = 1;
+            distinct++;
+        }
+    }
+    if (const char *out = getenv("SCD_LABELS_OUT")) {
+        FILE *f = fopen(out, "wb");
+        if (!f || fwrite(best_labels.data(), 4, n, f) != n) {
+            std::cerr << "failed to write " << out << std::endl;
+            return EXIT_FAILURE;
+        }
+        fclose(f);
+    }
+    std::cout << "----------------------------------------" << std::endl;
+    std::cout << "num. communities: " << (n - distinct) << std::endl;
+    std::cout << "WCC: " << best_wcc << std::endl;
+    std::cout << "iterations: " << iters << std::endl;
+    std::cout << "time per iteration: " << (after - before) / iters << " ms" << std::endl;
+    std::cout << "----------------------------------------" << std::endl;
+    scd_graph_free(hg);
+    return EXIT_SUCCESS;
+}

@@ -1,0 +1,809 @@
+// libscd_hip.so: the MI355X SCD refinement engine behind the C ABI of
+// include/scd_hip.h.
+//
+// Device state per context (all resident in HBM for the life of the graph):
+//   CSR        indices[n+1], adj[E], tri[n], tri_deg[n]          (uint32/int32)
+//   labels     cur[n], next[n], best[n]                           (uint32)
+//   stats      int4[n]: (size, ext, int, -) per community         (int32)
+//   internal   int_adj[E] (same-label lists at the CSR offsets), int_deg[n]
+//   wcc        terms[n] (fp64), partial sums, the mean
+//   bins       vertex ids grouped by degree bin (host-built at upload)
+//   hubs       per-hub descriptors, tiles, chunks and two scratch segments
+// One iteration (scd_improve) is: move kernels over every bin (old labels
+// -> next), swap, clear statistics, filter+statistics kernels, WCC kernels,
+// two-step deterministic sum; then one device->host copy of the mean.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dev.hpp"
+#include "move_kernels.hpp"
+#include "scd_hip.h"
+#include "stats_kernels.hpp"
+
+using namespace scd_dev;
+
+// ---------------------------------------------------------------------------
+// Errors
+// ---------------------------------------------------------------------------
+static thread_local char g_err[512];
+
+static int fail(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+extern "C" const char *scd_last_error(void) { return g_err; }
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(e_ == hipErrorOutOfMemory ? SCD_ENOMEM : SCD_EDEVICE, "%s: %s (%s:%d)", \
+                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);                     \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+enum BinKind : uint32_t { K_SUB8, K_SUB16, K_SUB32, K_W1, K_W2, K_W4, K_W8, K_W16, K_LDS, K_HUB, K_NB };
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct ProfRec {
+    std::string name;
+    uint64_t launches = 0;
+    double ms = 0.0;
+};
+
+struct PendingEv {
+    int rec;
+    hipEvent_t a, b;
+};
+
+struct scd_ctx {
+    int device = 0;
+    int block_size = 256;
+    bool profile = false;
+    hipStream_t stream = nullptr;
+    uint64_t dev_bytes = 0;
+
+    // graph
+    uint32_t n = 0, m = 0;
+    uint64_t E = 0;
+    double cc = 0.0;
+    bool have_graph = false, have_labels = false;
+    uint32_t num_comms = 0;
+    std::vector<uint32_t> h_deg;  // degrees, kept for re-binning
+
+    uint32_t *indices = nullptr, *adj = nullptr;
+    int32_t *tri = nullptr, *tri_deg = nullptr;
+    uint32_t *cur = nullptr, *next = nullptr, *best = nullptr;
+    int4 *stats = nullptr;
+    uint32_t *int_adj = nullptr, *int_deg = nullptr;
+    double *terms = nullptr, *partial = nullptr, *wcc_dev = nullptr;
+    double *wcc_host = nullptr;  // pinned
+    unsigned long long *counters = nullptr;
+    unsigned long long counters_host[4] = {0, 0, 0, 0};
+
+    // bins
+    uint32_t *bin_verts = nullptr;
+    uint32_t bin_begin[K_NB + 1] = {0};
+
+    // hubs
+    uint32_t nh = 0, n_tiles = 0, n_chunks = 0, max_hub_deg = 0;
+    uint32_t *hub_v = nullptr, *hub_d = nullptr, *hub_off = nullptr, *hub_chunk_begin = nullptr;
+    uint32_t *tile_hub = nullptr, *tile_start = nullptr, *chunk_hub = nullptr, *chunk_start = nullptr;
+    uint32_t *hub_a = nullptr, *hub_b = nullptr, *chunk_cnt = nullptr, *chunk_bl = nullptr,
+             *chunk_has_rem = nullptr;
+    double *chunk_bi = nullptr, *chunk_rem = nullptr;
+    std::vector<DevBuf> hub_bufs;
+
+    scd_tuning tune = {4096, 2048, {0, 0, 0, 0, 0, 0}};
+
+    // profiling
+    std::vector<ProfRec> prof;
+    std::vector<PendingEv> pending;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+
+    std::vector<DevBuf> bufs;
+};
+
+template <typename T>
+static int dalloc(scd_ctx *c, T **p, size_t count, std::vector<DevBuf> *list = nullptr) {
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess)
+        return fail(SCD_ENOMEM, "hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+    *p = (T *)q;
+    (list ? list : &c->bufs)->push_back({q, bytes});
+    c->dev_bytes += bytes;
+    return SCD_OK;
+}
+
+static void free_list(scd_ctx *c, std::vector<DevBuf> &list) {
+    for (auto &b : list) {
+        (void)hipFree(b.p);
+        c->dev_bytes -= b.bytes;
+    }
+    list.clear();
+}
+
+static void free_graph(scd_ctx *c) {
+    free_list(c, c->hub_bufs);
+    free_list(c, c->bufs);
+    c->have_graph = c->have_labels = false;
+    c->indices = c->adj = nullptr;
+    c->nh = c->n_tiles = c->n_chunks = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Profiling: a HIP event pair around each launch on the context's stream.
+// ---------------------------------------------------------------------------
+static int prof_slot(scd_ctx *c, const char *name) {
+    for (size_t i = 0; i < c->prof.size(); i++)
+        if (c->prof[i].name == name) return (int)i;
+    c->prof.push_back({name, 0, 0.0});
+    return (int)c->prof.size() - 1;
+}
+
+static hipEvent_t next_event(scd_ctx *c) {
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_used++];
+}
+
+struct ProfScope {
+    scd_ctx *c;
+    int rec = -1;
+    hipEvent_t a = nullptr;
+    ProfScope(scd_ctx *cc, const char *name) : c(cc) {
+        if (!c->profile) return;
+        rec = prof_slot(c, name);
+        a = next_event(c);
+        if (a) (void)hipEventRecord(a, c->stream);
+    }
+    ~ProfScope() {
+        if (!c->profile || !a) return;
+        hipEvent_t b = next_event(c);
+        if (!b) return;
+        (void)hipEventRecord(b, c->stream);
+        c->pending.push_back({rec, a, b});
+    }
+};
+
+static int prof_collect(scd_ctx *c) {
+    if (!c->profile) return SCD_OK;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (auto &pe : c->pending) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, pe.a, pe.b));
+        c->prof[pe.rec].launches++;
+        c->prof[pe.rec].ms += ms;
+    }
+    c->pending.clear();
+    c->ev_used = 0;
+    return SCD_OK;
+}
+
+#define LAUNCH(c, name, grid, block, kern, ...)                                          \
+    do {                                                                                 \
+        if ((grid) > 0) {                                                                \
+            ProfScope ps_(c, name);                                                      \
+            hipLaunchKernelGGL((kern), dim3(grid), dim3(block), 0, (c)->stream, __VA_ARGS__); \
+        }                                                                                \
+    } while (0)
+
+static int check_launch(const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(SCD_EDEVICE, "%s: kernel launch failed: %s", what, hipGetErrorString(e));
+    return SCD_OK;
+}
+
+static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+// Degree bins and hub work lists (host-built, uploaded once per graph and
+// again when the tuning changes).
+// ---------------------------------------------------------------------------
+static uint32_t bin_of(uint32_t d, uint32_t lds_tile) {
+    if (d <= 8) return K_SUB8;
+    if (d <= 16) return K_SUB16;
+    if (d <= 32) return K_SUB32;
+    if (d <= 64) return K_W1;
+    if (d <= 128) return K_W2;
+    if (d <= 256) return K_W4;
+    if (d <= 512) return K_W8;
+    if (d <= 1024) return K_W16;
+    if (d <= lds_tile) return K_LDS;
+    return K_HUB;
+}
+
+static int build_bins(scd_ctx *c) {
+    free_list(c, c->hub_bufs);
+    const uint32_t n = c->n, T = c->tune.lds_tile, C = c->tune.merge_chunk;
+    std::vector<uint32_t> cnt(K_NB + 1, 0);
+    for (uint32_t v = 0; v < n; v++) cnt[bin_of(c->h_deg[v], T) + 1]++;
+    for (int b = 0; b < K_NB; b++) cnt[b + 1] += cnt[b];
+    for (int b = 0; b <= K_NB; b++) c->bin_begin[b] = cnt[b];
+    std::vector<uint32_t> order(n), pos(cnt.begin(), cnt.end() - 1);
+    for (uint32_t v = 0; v < n; v++) order[pos[bin_of(c->h_deg[v], T)]++] = v;
+    int rc;
+    if ((rc = dalloc(c, &c->bin_verts, n, &c->hub_bufs))) return rc;
+    HIP_TRY(hipMemcpy(c->bin_verts, order.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+
+    // hubs
+    const uint32_t hb0 = c->bin_begin[K_HUB], nh = c->bin_begin[K_HUB + 1] - hb0;
+    c->nh = nh;
+    c->n_tiles = c->n_chunks = c->max_hub_deg = 0;
+    if (nh == 0) return SCD_OK;
+    std::vector<uint32_t> hv(nh), hd(nh), hoff(nh), hcb(nh + 1), th, ts, ch, cs;
+    uint64_t off = 0;
+    for (uint32_t h = 0; h < nh; h++) {
+        const uint32_t v = order[hb0 + h], d = c->h_deg[v];
+        hv[h] = v;
+        hd[h] = d;
+        hoff[h] = (uint32_t)off;
+        off += d;
+        c->max_hub_deg = std::max(c->max_hub_deg, d);
+        for (uint32_t s = 0; s < d; s += T) {
+            th.push_back(h);
+            ts.push_back(s);
+        }
+        hcb[h] = (uint32_t)ch.size();
+        for (uint32_t s = 0; s < d; s += C) {
+            ch.push_back(h);
+            cs.push_back(s);
+        }
+    }
+    hcb[nh] = (uint32_t)ch.size();
+    if (off >= (1ull << 32)) return fail(SCD_ELIMIT, "hub segments exceed 2^32 entries");
+    c->n_tiles = (uint32_t)th.size();
+    c->n_chunks = (uint32_t)ch.size();
+    auto up = [&](uint32_t **dst, const std::vector<uint32_t> &src) -> int {
+        int r = dalloc(c, dst, src.size(), &c->hub_bufs);
+        if (r) return r;
+        HIP_TRY(hipMemcpy(*dst, src.data(), sizeof(uint32_t) * src.size(), hipMemcpyHostToDevice));
+        return SCD_OK;
+    };
+    if ((rc = up(&c->hub_v, hv)) || (rc = up(&c->hub_d, hd)) || (rc = up(&c->hub_off, hoff)) ||
+        (rc = up(&c->hub_chunk_begin, hcb)) || (rc = up(&c->tile_hub, th)) ||
+        (rc = up(&c->tile_start, ts)) || (rc = up(&c->chunk_hub, ch)) || (rc = up(&c->chunk_start, cs)))
+        return rc;
+    if ((rc = dalloc(c, &c->hub_a, off, &c->hub_bufs)) || (rc = dalloc(c, &c->hub_b, off, &c->hub_bufs)) ||
+        (rc = dalloc(c, &c->chunk_cnt, c->n_chunks, &c->hub_bufs)) ||
+        (rc = dalloc(c, &c->chunk_bl, c->n_chunks, &c->hub_bufs)) ||
+        (rc = dalloc(c, &c->chunk_has_rem, c->n_chunks, &c->hub_bufs)) ||
+        (rc = dalloc(c, &c->chunk_bi, c->n_chunks, &c->hub_bufs)) ||
+        (rc = dalloc(c, &c->chunk_rem, c->n_chunks, &c->hub_bufs)))
+        return rc;
+    return SCD_OK;
+}
+
+static HubArgs hub_args(scd_ctx *c) {
+    HubArgs h;
+    h.hub_v = c->hub_v;
+    h.hub_d = c->hub_d;
+    h.hub_off = c->hub_off;
+    h.hub_chunk_begin = c->hub_chunk_begin;
+    h.tile_hub = c->tile_hub;
+    h.tile_start = c->tile_start;
+    h.chunk_hub = c->chunk_hub;
+    h.chunk_start = c->chunk_start;
+    h.chunk_bi = c->chunk_bi;
+    h.chunk_bl = c->chunk_bl;
+    h.chunk_rem = c->chunk_rem;
+    h.chunk_has_rem = c->chunk_has_rem;
+    return h;
+}
+
+static inline uint32_t bin_n(const scd_ctx *c, int b) { return c->bin_begin[b + 1] - c->bin_begin[b]; }
+static inline const uint32_t *bin_v(const scd_ctx *c, int b) { return c->bin_verts + c->bin_begin[b]; }
+
+// ---------------------------------------------------------------------------
+// Phases
+// ---------------------------------------------------------------------------
+template <int T>
+static void launch_lds_and_tiles(scd_ctx *c, const MoveArgs &a, const HubArgs &h) {
+    LAUNCH(c, "move_lds", bin_n(c, K_LDS), 256, (k_move_lds<T>), a, bin_v(c, K_LDS), bin_n(c, K_LDS));
+    LAUNCH(c, "hub_tile_sort", c->n_tiles, 256, (k_hub_tile_sort<T>), a, h, c->hub_a);
+}
+
+// Move step for every vertex: old labels (cur) -> labels_out.  Export mode
+// writes sorted keys / run lengths instead when ex_sorted is set.
+static int phase_move(scd_ctx *c, uint32_t *labels_out, uint32_t *ex_sorted, uint32_t *ex_count) {
+    MoveArgs a;
+    a.indices = c->indices;
+    a.adj = c->adj;
+    a.labels = c->cur;
+    a.stats = c->stats;
+    a.p_ext = c->cc;
+    a.labels_out = labels_out;
+    a.counters = c->counters;
+    a.ex_sorted = ex_sorted;
+    a.ex_count = ex_count;
+    LAUNCH(c, "move_sub8", cdiv(bin_n(c, K_SUB8), 32), 256, k_move_sub<8>, a, bin_v(c, K_SUB8), bin_n(c, K_SUB8));
+    LAUNCH(c, "move_sub16", cdiv(bin_n(c, K_SUB16), 16), 256, k_move_sub<16>, a, bin_v(c, K_SUB16), bin_n(c, K_SUB16));
+    LAUNCH(c, "move_sub32", cdiv(bin_n(c, K_SUB32), 8), 256, k_move_sub<32>, a, bin_v(c, K_SUB32), bin_n(c, K_SUB32));
+    LAUNCH(c, "move_wave1", cdiv(bin_n(c, K_W1), 4), 256, k_move_wave<1>, a, bin_v(c, K_W1), bin_n(c, K_W1));
+    LAUNCH(c, "move_wave2", cdiv(bin_n(c, K_W2), 4), 256, k_move_wave<2>, a, bin_v(c, K_W2), bin_n(c, K_W2));
+    LAUNCH(c, "move_wave4", cdiv(bin_n(c, K_W4), 4), 256, k_move_wave<4>, a, bin_v(c, K_W4), bin_n(c, K_W4));
+    LAUNCH(c, "move_wave8", cdiv(bin_n(c, K_W8), 4), 256, k_move_wave<8>, a, bin_v(c, K_W8), bin_n(c, K_W8));
+    LAUNCH(c, "move_wave16", cdiv(bin_n(c, K_W16), 4), 256, k_move_wave<16>, a, bin_v(c, K_W16), bin_n(c, K_W16));
+    HubArgs h = hub_args(c);
+    switch (c->tune.lds_tile) {
+        case 2048: launch_lds_and_tiles<2048>(c, a, h); break;
+        case 4096: launch_lds_and_tiles<4096>(c, a, h); break;
+        case 8192: launch_lds_and_tiles<8192>(c, a, h); break;
+        default: launch_lds_and_tiles<16384>(c, a, h); break;
+    }
+    if (c->nh) {
+        const uint32_t C = c->tune.merge_chunk;
+        uint32_t *src = c->hub_a, *dst = c->hub_b;
+        for (uint64_t w = c->tune.lds_tile; w < c->max_hub_deg; w *= 2) {
+            LAUNCH(c, "hub_merge", c->n_chunks, 256, k_hub_merge, h, src, dst, (uint32_t)w, C);
+            std::swap(src, dst);
+        }
+        LAUNCH(c, "hub_eval", c->n_chunks, 256, k_hub_eval, a, h, src, C);
+        LAUNCH(c, "hub_finalize", cdiv(c->nh, 64), 64, k_hub_finalize, a, h, c->nh);
+    }
+    return check_launch("move");
+}
+
+// Statistics + internal adjacency of the current labels.
+static int phase_stats(scd_ctx *c) {
+    LAUNCH(c, "clear_stats", cdiv(c->num_comms, 256), 256, k_fill_int4, c->stats, c->num_comms);
+    StatArgs s;
+    s.indices = c->indices;
+    s.adj = c->adj;
+    s.labels = c->cur;
+    s.stats = c->stats;
+    s.int_adj = c->int_adj;
+    s.int_deg = c->int_deg;
+    s.counters = c->counters;
+    LAUNCH(c, "filter_sub8", cdiv(bin_n(c, K_SUB8), 32), 256, k_filter_sub<8>, s, bin_v(c, K_SUB8), bin_n(c, K_SUB8));
+    LAUNCH(c, "filter_sub16", cdiv(bin_n(c, K_SUB16), 16), 256, k_filter_sub<16>, s, bin_v(c, K_SUB16), bin_n(c, K_SUB16));
+    LAUNCH(c, "filter_sub32", cdiv(bin_n(c, K_SUB32), 8), 256, k_filter_sub<32>, s, bin_v(c, K_SUB32), bin_n(c, K_SUB32));
+    const uint32_t nw = c->bin_begin[K_W16 + 1] - c->bin_begin[K_W1];
+    LAUNCH(c, "filter_wave", cdiv(nw, 4), 256, k_filter_wave, s, bin_v(c, K_W1), nw);
+    LAUNCH(c, "filter_block", bin_n(c, K_LDS), 256, k_filter_block, s, bin_v(c, K_LDS), bin_n(c, K_LDS));
+    if (c->nh) {
+        const uint32_t C = c->tune.merge_chunk;
+        LAUNCH(c, "hub_filter_count", c->n_chunks, 256, k_hub_filter_count, s, c->hub_v, c->hub_d,
+               c->chunk_hub, c->chunk_start, C, c->chunk_cnt);
+        LAUNCH(c, "hub_filter_write", c->n_chunks, 256, k_hub_filter_write, s, c->hub_v, c->hub_d,
+               c->hub_chunk_begin, c->chunk_hub, c->chunk_start, C, c->chunk_cnt);
+    }
+    return check_launch("stats");
+}
+
+// WCC terms of the current labels and their mean into wcc_dev.
+static int phase_wcc(scd_ctx *c, int32_t *int_tri_out, int32_t *int_tri_deg_out) {
+    WccArgs w;
+    w.indices = c->indices;
+    w.labels = c->cur;
+    w.stats = c->stats;
+    w.int_adj = c->int_adj;
+    w.int_deg = c->int_deg;
+    w.tri = c->tri;
+    w.tri_deg = c->tri_deg;
+    w.terms = c->terms;
+    w.int_tri_out = int_tri_out;
+    w.int_tri_deg_out = int_tri_deg_out;
+    LAUNCH(c, "wcc_sub8", cdiv(bin_n(c, K_SUB8), 32), 256, k_wcc_sub<8>, w, bin_v(c, K_SUB8), bin_n(c, K_SUB8));
+    LAUNCH(c, "wcc_sub16", cdiv(bin_n(c, K_SUB16), 16), 256, k_wcc_sub<16>, w, bin_v(c, K_SUB16), bin_n(c, K_SUB16));
+    LAUNCH(c, "wcc_sub32", cdiv(bin_n(c, K_SUB32), 8), 256, k_wcc_sub<32>, w, bin_v(c, K_SUB32), bin_n(c, K_SUB32));
+    const uint32_t nw = c->bin_begin[K_W16 + 1] - c->bin_begin[K_W1];
+    LAUNCH(c, "wcc_wave", cdiv(nw, 4), 256, k_wcc_wave, w, bin_v(c, K_W1), nw);
+    const uint32_t nb = c->bin_begin[K_HUB + 1] - c->bin_begin[K_LDS];
+    LAUNCH(c, "wcc_block", nb, 256, k_wcc_block, w, bin_v(c, K_LDS), nb);
+    const uint32_t np = cdiv(c->n, 4096);
+    LAUNCH(c, "wcc_sum_partial", np, 256, k_reduce_partial, c->terms, c->n, c->partial);
+    LAUNCH(c, "wcc_sum_final", 1, 256, k_reduce_final, c->partial, np, c->n, c->wcc_dev);
+    return check_launch("wcc");
+}
+
+static int clear_counters(scd_ctx *c) {
+    LAUNCH(c, "clear_counters", 1, 64, k_fill_u64, c->counters, 4u);
+    return check_launch("clear counters");
+}
+
+static int finish(scd_ctx *c) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return prof_collect(c);
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" int scd_device_count(void) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e == hipErrorNoDevice) return 0;
+    if (e != hipSuccess) return fail(SCD_EDEVICE, "hipGetDeviceCount: %s", hipGetErrorString(e));
+    return n;
+}
+
+extern "C" int scd_init(int device_id, int work_group_size, int profile, scd_ctx **out) {
+    if (!out) return fail(SCD_EINVAL, "scd_init: null output");
+    *out = nullptr;
+    if (work_group_size != 64 && work_group_size != 128 && work_group_size != 256 &&
+        work_group_size != 512 && work_group_size != 1024)
+        return fail(SCD_EINVAL, "scd_init: work group size %d is not 64, 128, 256, 512 or 1024",
+                    work_group_size);
+    int count = scd_device_count();
+    if (count < 0) return count;
+    if (device_id < 0 || device_id >= count)
+        return fail(SCD_EDEVICE, "scd_init: device %d is not one of the %d visible HIP devices",
+                    device_id, count);
+    HIP_TRY(hipSetDevice(device_id));
+    scd_ctx *c = new scd_ctx();
+    c->device = device_id;
+    c->block_size = work_group_size;
+    c->profile = profile != 0;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(SCD_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    e = hipHostMalloc((void **)&c->wcc_host, sizeof(double) * 2, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return fail(SCD_ENOMEM, "hipHostMalloc: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return SCD_OK;
+}
+
+extern "C" void scd_destroy(scd_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_graph(c);
+    for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    (void)hipHostFree(c->wcc_host);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" int scd_upload(scd_ctx *c, uint32_t n, uint32_t m, const uint32_t *indices,
+                          const uint32_t *adj, const int32_t *tri, const int32_t *tri_deg,
+                          double clustering_coef) {
+    if (!c) return fail(SCD_EINVAL, "scd_upload: null context");
+    if (n == 0) return fail(SCD_EINVAL, "scd_upload: graph has no vertices");
+    if (!indices || (m && !adj) || !tri || !tri_deg) return fail(SCD_EINVAL, "scd_upload: null array");
+    if (m >= (1u << 31)) return fail(SCD_ELIMIT, "scd_upload: m = %u needs 2m < 2^32", m);
+    const uint64_t E = 2ull * m;
+    if (indices[0] != 0 || indices[n] != E)
+        return fail(SCD_EINVAL, "scd_upload: indices[0] = %u, indices[n] = %u, expected 0 and %llu",
+                    indices[0], indices[n], (unsigned long long)E);
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    free_graph(c);
+    c->h_deg.resize(n);
+    for (uint32_t v = 0; v < n; v++) {
+        if (indices[v + 1] < indices[v]) return fail(SCD_EINVAL, "scd_upload: indices decrease at %u", v);
+        c->h_deg[v] = indices[v + 1] - indices[v];
+    }
+    c->n = n;
+    c->m = m;
+    c->E = E;
+    c->cc = clustering_coef;
+    int rc;
+    if ((rc = dalloc(c, &c->indices, (size_t)n + 1)) || (rc = dalloc(c, &c->adj, E)) ||
+        (rc = dalloc(c, &c->tri, n)) || (rc = dalloc(c, &c->tri_deg, n)) ||
+        (rc = dalloc(c, &c->cur, n)) || (rc = dalloc(c, &c->next, n)) || (rc = dalloc(c, &c->best, n)) ||
+        (rc = dalloc(c, &c->stats, n)) || (rc = dalloc(c, &c->int_adj, E)) ||
+        (rc = dalloc(c, &c->int_deg, n)) || (rc = dalloc(c, &c->terms, n)) ||
+        (rc = dalloc(c, &c->partial, cdiv(n, 4096))) || (rc = dalloc(c, &c->wcc_dev, 1)) ||
+        (rc = dalloc(c, &c->counters, 4))) {
+        free_graph(c);
+        return rc;
+    }
+    HIP_TRY(hipMemcpyAsync(c->indices, indices, sizeof(uint32_t) * ((size_t)n + 1), hipMemcpyHostToDevice, c->stream));
+    if (E) HIP_TRY(hipMemcpyAsync(c->adj, adj, sizeof(uint32_t) * E, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->tri, tri, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->tri_deg, tri_deg, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if ((rc = build_bins(c))) {
+        free_graph(c);
+        return rc;
+    }
+    c->have_graph = true;
+    return SCD_OK;
+}
+
+static int need_graph(scd_ctx *c, const char *fn) {
+    if (!c) return fail(SCD_EINVAL, "%s: null context", fn);
+    if (!c->have_graph) return fail(SCD_ESTATE, "%s: no graph uploaded", fn);
+    (void)hipSetDevice(c->device);
+    return SCD_OK;
+}
+
+static int need_labels(scd_ctx *c, const char *fn) {
+    int rc = need_graph(c, fn);
+    if (rc) return rc;
+    if (!c->have_labels) return fail(SCD_ESTATE, "%s: no labels on the device", fn);
+    return SCD_OK;
+}
+
+extern "C" int scd_labels_to_device(scd_ctx *c, const uint32_t *labels) {
+    int rc = need_graph(c, "scd_labels_to_device");
+    if (rc) return rc;
+    if (!labels) return fail(SCD_EINVAL, "scd_labels_to_device: null labels");
+    uint32_t mx = 0;
+    for (uint32_t v = 0; v < c->n; v++) {
+        if (labels[v] == SCD_INVALID_LABEL)
+            return fail(SCD_EINVAL, "scd_labels_to_device: vertex %u has the INVALID label", v);
+        mx = std::max(mx, labels[v]);
+    }
+    if (mx >= c->n) return fail(SCD_EINVAL, "scd_labels_to_device: label %u >= n = %u", mx, c->n);
+    c->num_comms = mx + 1;
+    HIP_TRY(hipMemcpyAsync(c->cur, labels, sizeof(uint32_t) * c->n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->best, labels, sizeof(uint32_t) * c->n, hipMemcpyHostToDevice, c->stream));
+    if ((rc = clear_counters(c)) || (rc = phase_stats(c)) || (rc = finish(c))) return rc;
+    c->have_labels = true;
+    return SCD_OK;
+}
+
+extern "C" int scd_improve(scd_ctx *c, double *wcc, uint32_t *labels_out) {
+    int rc = need_labels(c, "scd_improve");
+    if (rc) return rc;
+    if ((rc = clear_counters(c)) || (rc = phase_move(c, c->next, nullptr, nullptr))) return rc;
+    std::swap(c->cur, c->next);
+    if ((rc = phase_stats(c)) || (rc = phase_wcc(c, nullptr, nullptr))) return rc;
+    HIP_TRY(hipMemcpyAsync(c->wcc_host, c->wcc_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->counters_host, c->counters, sizeof(c->counters_host), hipMemcpyDeviceToHost,
+                           c->stream));
+    if (labels_out)
+        HIP_TRY(hipMemcpyAsync(labels_out, c->cur, sizeof(uint32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+    if ((rc = finish(c))) return rc;
+    if (wcc) *wcc = c->wcc_host[0];
+    return SCD_OK;
+}
+
+extern "C" int scd_store_best(scd_ctx *c) {
+    int rc = need_labels(c, "scd_store_best");
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(c->best, c->cur, sizeof(uint32_t) * c->n, hipMemcpyDeviceToDevice, c->stream));
+    return finish(c);
+}
+
+extern "C" int scd_labels_from_device(scd_ctx *c, uint32_t *labels) {
+    int rc = need_labels(c, "scd_labels_from_device");
+    if (rc) return rc;
+    if (!labels) return fail(SCD_EINVAL, "scd_labels_from_device: null labels");
+    HIP_TRY(hipMemcpyAsync(labels, c->best, sizeof(uint32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+    return finish(c);
+}
+
+extern "C" int scd_current_labels(scd_ctx *c, uint32_t *labels) {
+    int rc = need_labels(c, "scd_current_labels");
+    if (rc) return rc;
+    if (!labels) return fail(SCD_EINVAL, "scd_current_labels: null labels");
+    HIP_TRY(hipMemcpyAsync(labels, c->cur, sizeof(uint32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+    return finish(c);
+}
+
+extern "C" int scd_refine(scd_ctx *c, double initial_wcc, uint32_t max_iters, double *best_wcc,
+                          uint32_t *iterations, double *wcc_hist) {
+    int rc = need_labels(c, "scd_refine");
+    if (rc) return rc;
+    const double threshold = 0.01;
+    double best = initial_wcc;
+    uint32_t it = 0;
+    while (it < max_iters) {
+        it++;
+        double wcc = 0.0;
+        if ((rc = scd_improve(c, &wcc, nullptr))) return rc;
+        if (wcc_hist) wcc_hist[it - 1] = wcc;
+        if (wcc > best && (rc = scd_store_best(c))) return rc;
+        if (wcc < best * (1 + threshold)) break;
+        best = wcc;
+    }
+    if (best_wcc) *best_wcc = best;
+    if (iterations) *iterations = it;
+    return SCD_OK;
+}
+
+extern "C" int scd_compute_wcc(scd_ctx *c, double *wcc, double *terms_out, int32_t *int_tri_out,
+                               int32_t *int_tri_deg_out) {
+    int rc = need_labels(c, "scd_compute_wcc");
+    if (rc) return rc;
+    int32_t *d_it = nullptr, *d_itd = nullptr;
+    std::vector<DevBuf> tmp;
+    if (int_tri_out || int_tri_deg_out) {
+        if ((rc = dalloc(c, &d_it, c->n, &tmp)) || (rc = dalloc(c, &d_itd, c->n, &tmp))) {
+            free_list(c, tmp);
+            return rc;
+        }
+    }
+    if ((rc = clear_counters(c)) || (rc = phase_stats(c)) || (rc = phase_wcc(c, d_it, d_itd))) {
+        free_list(c, tmp);
+        return rc;
+    }
+    HIP_TRY(hipMemcpyAsync(c->wcc_host, c->wcc_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (terms_out)
+        HIP_TRY(hipMemcpyAsync(terms_out, c->terms, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
+    if (int_tri_out)
+        HIP_TRY(hipMemcpyAsync(int_tri_out, d_it, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+    if (int_tri_deg_out)
+        HIP_TRY(hipMemcpyAsync(int_tri_deg_out, d_itd, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+    rc = finish(c);
+    free_list(c, tmp);
+    if (rc) return rc;
+    if (wcc) *wcc = c->wcc_host[0];
+    return SCD_OK;
+}
+
+extern "C" int scd_get_stats(scd_ctx *c, uint32_t *num_comms, int32_t *sizes, int32_t *ext_edges,
+                             int32_t *int_edges) {
+    int rc = need_labels(c, "scd_get_stats");
+    if (rc) return rc;
+    std::vector<int4> h(c->num_comms);
+    HIP_TRY(hipMemcpyAsync(h.data(), c->stats, sizeof(int4) * c->num_comms, hipMemcpyDeviceToHost, c->stream));
+    if ((rc = finish(c))) return rc;
+    for (uint32_t i = 0; i < c->num_comms; i++) {
+        if (sizes) sizes[i] = h[i].x;
+        if (ext_edges) ext_edges[i] = h[i].y;
+        if (int_edges) int_edges[i] = h[i].z;
+    }
+    if (num_comms) *num_comms = c->num_comms;
+    return SCD_OK;
+}
+
+// Exclusive scan of n uint32 values (in -> out, may alias) on the stream;
+// returns the total through *total (host).
+static int scan_u32(scd_ctx *c, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *total,
+                    std::vector<DevBuf> &tmp) {
+    const uint32_t nt = cdiv(n, 2048);
+    uint32_t *sums = nullptr, *sums_scan = nullptr;
+    int rc;
+    if ((rc = dalloc(c, &sums, nt, &tmp)) || (rc = dalloc(c, &sums_scan, nt, &tmp))) return rc;
+    LAUNCH(c, "scan_tile_sums", nt, 256, k_scan_tile_sums, in, n, sums);
+    if (nt > 1) {
+        uint32_t dummy;
+        if ((rc = scan_u32(c, sums, sums_scan, nt, &dummy, tmp))) return rc;
+    } else {
+        HIP_TRY(hipMemsetAsync(sums_scan, 0, sizeof(uint32_t), c->stream));
+    }
+    if ((rc = check_launch("scan"))) return rc;
+    // total = last exclusive offset + last tile sum (read before out overwrites in)
+    uint32_t last[2];
+    HIP_TRY(hipMemcpyAsync(&last[0], sums_scan + nt - 1, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&last[1], sums + nt - 1, 4, hipMemcpyDeviceToHost, c->stream));
+    LAUNCH(c, "scan_tiles", nt, 256, k_scan_tiles, in, n, sums_scan, out);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    *total = last[0] + last[1];
+    return check_launch("scan");
+}
+
+extern "C" int scd_neighbor_label_pairs(scd_ctx *c, uint32_t *sorted_out, uint32_t *run_offsets,
+                                        uint32_t *run_label, uint32_t *run_count, uint64_t *num_pairs) {
+    int rc = need_labels(c, "scd_neighbor_label_pairs");
+    if (rc) return rc;
+    std::vector<DevBuf> tmp;
+    uint32_t *d_sorted = nullptr, *d_cnt = nullptr, *d_pos = nullptr, *d_rl = nullptr, *d_rc = nullptr,
+             *d_off = nullptr;
+    const uint64_t E = c->E;
+    if ((rc = dalloc(c, &d_sorted, E, &tmp)) || (rc = dalloc(c, &d_cnt, E, &tmp)) ||
+        (rc = dalloc(c, &d_pos, E, &tmp)) || (rc = dalloc(c, &d_off, (size_t)c->n + 1, &tmp))) {
+        free_list(c, tmp);
+        return rc;
+    }
+    uint32_t total = 0;
+    rc = clear_counters(c);
+    if (!rc) rc = phase_move(c, nullptr, d_sorted, d_cnt);
+    if (!rc && E) {
+        LAUNCH(c, "run_flags", cdiv(E, 256), 256, k_run_flags, d_cnt, E, d_pos);
+        rc = scan_u32(c, d_pos, d_pos, E, &total, tmp);
+    }
+    if (!rc) {
+        if ((rc = dalloc(c, &d_rl, total, &tmp)) || (rc = dalloc(c, &d_rc, total, &tmp))) {
+            free_list(c, tmp);
+            return rc;
+        }
+        if (E) LAUNCH(c, "run_scatter", cdiv(E, 256), 256, k_run_scatter, d_sorted, d_cnt, d_pos, E, d_rl, d_rc);
+        LAUNCH(c, "run_offsets", cdiv((uint64_t)c->n + 1, 256), 256, k_run_offsets, c->indices, d_pos, c->n, E,
+               total, d_off);
+        rc = check_launch("pairs");
+    }
+    if (!rc) {
+        hipStream_t s = c->stream;
+        hipError_t e = hipSuccess;
+        if (sorted_out && E) e = hipMemcpyAsync(sorted_out, d_sorted, 4 * E, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess && run_offsets)
+            e = hipMemcpyAsync(run_offsets, d_off, 4 * ((size_t)c->n + 1), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess && run_label && total) e = hipMemcpyAsync(run_label, d_rl, 4ull * total, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess && run_count && total) e = hipMemcpyAsync(run_count, d_rc, 4ull * total, hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) rc = fail(SCD_EDEVICE, "scd_neighbor_label_pairs: copy: %s", hipGetErrorString(e));
+    }
+    if (!rc) rc = finish(c);
+    free_list(c, tmp);
+    if (rc) return rc;
+    if (num_pairs) *num_pairs = total;
+    return SCD_OK;
+}
+
+extern "C" int scd_get_tuning(const scd_ctx *c, scd_tuning *out) {
+    if (!c || !out) return fail(SCD_EINVAL, "scd_get_tuning: null argument");
+    *out = c->tune;
+    return SCD_OK;
+}
+
+extern "C" int scd_set_tuning(scd_ctx *c, const scd_tuning *t) {
+    if (!c || !t) return fail(SCD_EINVAL, "scd_set_tuning: null argument");
+    const uint32_t T = t->lds_tile, C = t->merge_chunk;
+    if (T != 2048 && T != 4096 && T != 8192 && T != 16384)
+        return fail(SCD_EINVAL, "scd_set_tuning: lds_tile %u is not 2048, 4096, 8192 or 16384", T);
+    if (C < 256 || C > 2 * T || (C & (C - 1)))
+        return fail(SCD_EINVAL, "scd_set_tuning: merge_chunk %u must be a power of two in [256, 2*lds_tile]", C);
+    c->tune = *t;
+    if (c->have_graph) {
+        (void)hipSetDevice(c->device);
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        int rc = build_bins(c);
+        if (rc) return rc;
+    }
+    return SCD_OK;
+}
+
+extern "C" int scd_last_counters(scd_ctx *c, uint64_t out[4]) {
+    if (!c || !out) return fail(SCD_EINVAL, "scd_last_counters: null argument");
+    for (int i = 0; i < 3; i++) out[i] = c->counters_host[i];
+    out[3] = c->nh;
+    return SCD_OK;
+}
+
+extern "C" int scd_prof_count(scd_ctx *c) { return c ? (int)c->prof.size() : 0; }
+
+extern "C" int scd_prof_get(scd_ctx *c, int i, const char **name, uint64_t *launches, double *total_ms) {
+    if (!c || i < 0 || i >= (int)c->prof.size()) return fail(SCD_EINVAL, "scd_prof_get: bad index");
+    if (name) *name = c->prof[i].name.c_str();
+    if (launches) *launches = c->prof[i].launches;
+    if (total_ms) *total_ms = c->prof[i].ms;
+    return SCD_OK;
+}
+
+extern "C" int scd_prof_reset(scd_ctx *c) {
+    if (!c) return fail(SCD_EINVAL, "scd_prof_reset: null context");
+    for (auto &p : c->prof) {
+        p.launches = 0;
+        p.ms = 0.0;
+    }
+    return SCD_OK;
+}
+
+extern "C" uint64_t scd_device_bytes(const scd_ctx *c) { return c ? c->dev_bytes : 0; }
+
+// Placeholders until the device triangle counting and initial partition land.
+extern "C" int scd_count_triangles(scd_ctx *c, int32_t *tri_out, int32_t *tri_deg_out, int replace) {
+    (void)tri_out; (void)tri_deg_out; (void)replace;
+    int rc = need_graph(c, "scd_count_triangles");
+    if (rc) return rc;
+    return fail(SCD_ESTATE, "scd_count_triangles: not available in this build");
+}
+
+extern "C" int scd_initial_partition(scd_ctx *c, uint32_t *labels_out, uint32_t *num_labels_out) {
+    (void)labels_out; (void)num_labels_out;
+    int rc = need_graph(c, "scd_initial_partition");
+    if (rc) return rc;
+    return fail(SCD_ESTATE, "scd_initial_partition: not available in this build");
+}

@@ -1,0 +1,413 @@
+// Best-move kernels: one synchronous SCD move step (src/scd.cpp:221-341).
+//
+// For every vertex p the kernels gather the labels of its neighbours, sort
+// them (own segmented sort: register bitonic across a wave64 or a lane
+// group, LDS bitonic for a workgroup, tiles + merge passes for hubs), reduce
+// equal labels to (label, count) runs with ballots (reduce-by-key), evaluate
+// the improvement of moving p into each neighbouring community with the CPU
+// path's fp64 arithmetic, take the argmax and apply the move rule.  Every
+// read uses the old labels and statistics; the new label goes to a separate
+// array, which is the reference's synchronous update (src/scd.cpp:229, :338).
+//
+// Vertices are split into degree bins on the host (engine.hip); each kernel
+// below serves one bin shape:
+//   k_move_sub<S>   deg <= S (8, 16, 32): S lanes per vertex, one key a lane
+//   k_move_wave<R>  deg <= 64R (R = 1..16): one wave per vertex, R keys a lane
+//   k_move_lds<T>   deg <= T: one workgroup per vertex, bitonic sort in LDS
+//   hub path        deg > T: tiles of T sorted in LDS by separate workgroups,
+//                   merged in global memory, then evaluated in chunks.
+// In export mode (MoveArgs::ex_sorted != nullptr) each kernel also writes
+// the sorted keys and, at every run head, the run length: the segmented
+// sort + reduce-by-key output that replaces the reference GPU's
+// SegSortKeysFromIndices and ReduceByKey (src/cuda.cu:205-227).
+#ifndef SCD_MOVE_KERNELS_HPP
+#define SCD_MOVE_KERNELS_HPP
+
+#include "dev.hpp"
+
+namespace scd_dev {
+
+struct MoveArgs {
+    const uint32_t *__restrict__ indices;
+    const uint32_t *__restrict__ adj;
+    const uint32_t *__restrict__ labels;   // old labels
+    const int4 *__restrict__ stats;        // (size, ext, int, -) of the old labels
+    double p_ext;                          // Graph::clustering_coef
+    uint32_t *__restrict__ labels_out;     // new labels (nullptr: do not write)
+    unsigned long long *__restrict__ counters;  // [0] += evaluated pairs
+    uint32_t *__restrict__ ex_sorted;      // export: sorted keys at adj positions
+    uint32_t *__restrict__ ex_count;       // export: run length at heads, else 0
+};
+
+template <int S>
+__device__ __forceinline__ uint64_t group_bits(uint64_t ballot, int gbase) {
+    if constexpr (S == 64) {
+        return ballot;
+    } else {
+        return (ballot >> gbase) & ((1ull << S) - 1ull);
+    }
+}
+
+__device__ __forceinline__ void count_pairs(unsigned long long *counters, uint32_t heads_in_lane) {
+    // wave sum, then one atomic per wave
+    uint32_t s = heads_in_lane;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, SCD_WAVE);
+    if ((threadIdx.x & 63) == 0 && counters) atomicAdd(counters, (unsigned long long)s);
+}
+
+// ---------------------------------------------------------------------------
+// S lanes per vertex, deg <= S < 64.
+// ---------------------------------------------------------------------------
+template <int S>
+__global__ __launch_bounds__(256) void k_move_sub(MoveArgs a, const uint32_t *__restrict__ verts,
+                                                  uint32_t nv) {
+    const int lane = threadIdx.x & 63;
+    const int lig = threadIdx.x & (S - 1);
+    const int gbase = lane & ~(S - 1);
+    const uint32_t slot = (blockIdx.x * 256u + threadIdx.x) / S;
+    const bool active = slot < nv;
+    const uint32_t p = active ? verts[slot] : 0u;
+    const uint32_t beg = active ? a.indices[p] : 0u;
+    const uint32_t deg = active ? a.indices[p + 1] - beg : 0u;
+    const uint32_t my = active ? a.labels[p] : SCD_INVALID;
+
+    uint32_t v = SCD_INVALID;
+    if ((uint32_t)lig < deg) v = a.labels[a.adj[beg + lig]];
+    v = group_bitonic_sort<S>(v, lig);
+
+    const uint32_t prev = __shfl(v, (lane + 63) & 63, SCD_WAVE);
+    const bool boundary = (lig == 0) || (v != prev);
+    const uint64_t gm = group_bits<S>(__ballot(boundary), gbase);
+    const uint64_t above = (lig + 1 < S) ? (gm >> (lig + 1)) : 0ull;
+    const uint32_t next = above ? (uint32_t)(lig + 1 + __builtin_ctzll(above)) : (uint32_t)S;
+    const uint32_t freq = next - (uint32_t)lig;
+    const bool head = boundary && v != SCD_INVALID;
+
+    double imp = 0.0;
+    if (head) imp = candidate_improvement(v, freq, deg, my, a.stats, a.p_ext);
+
+    const uint64_t mm = group_bits<S>(__ballot(head && v == my), gbase);
+    const int src = mm ? gbase + __builtin_ctzll(mm) : lane;
+    const double ximp = __shfl(imp, src, SCD_WAVE);
+    const double rem = mm ? -ximp : 0.0;
+
+    double bi = 0.0;
+    uint32_t bl = SCD_INVALID;
+    if (head && imp > 0.0) {
+        bi = imp;
+        bl = v;
+    }
+    group_argmax<S>(bi, bl);
+
+    if (active && lig == 0 && a.labels_out) a.labels_out[p] = move_rule(bi, bl, rem, my);
+    if (a.ex_sorted && (uint32_t)lig < deg) {
+        a.ex_sorted[beg + lig] = v;
+        a.ex_count[beg + lig] = head ? freq : 0u;
+    }
+    count_pairs(a.counters, head ? 1u : 0u);
+}
+
+// ---------------------------------------------------------------------------
+// One wave per vertex, deg <= 64 * R; key i in register i / 64 of lane i % 64.
+// ---------------------------------------------------------------------------
+template <int R>
+__global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const uint32_t *__restrict__ verts,
+                                                   uint32_t nv) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t slot = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (slot >= nv) return;  // uniform per wave
+    const uint32_t p = verts[slot];
+    const uint32_t beg = a.indices[p];
+    const uint32_t deg = a.indices[p + 1] - beg;
+    const uint32_t my = a.labels[p];
+
+    uint32_t x[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint32_t i = (uint32_t)(r * 64 + lane);
+        x[r] = (i < deg) ? a.labels[a.adj[beg + i]] : SCD_INVALID;
+    }
+    wave_bitonic_sort<R>(x, lane);
+
+    uint64_t B[R];
+    bool bnd[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint32_t same_reg = __shfl(x[r], (lane + 63) & 63, SCD_WAVE);
+        uint32_t prev_reg = 0;
+        if (r > 0) prev_reg = __shfl(x[r - 1], 63, SCD_WAVE);
+        const uint32_t prev = lane ? same_reg : prev_reg;
+        bnd[r] = (r == 0 && lane == 0) || (x[r] != prev);
+        B[r] = __ballot(bnd[r]);
+    }
+
+    double bi = 0.0, rem_lane = 0.0;
+    uint32_t bl = SCD_INVALID;
+    bool has_rem = false;
+    uint32_t heads = 0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint32_t i = (uint32_t)(r * 64 + lane);
+        const bool head = bnd[r] && x[r] != SCD_INVALID;
+        uint32_t freq = 0;
+        if (head) {
+            const uint64_t above = lane < 63 ? (B[r] >> (lane + 1)) : 0ull;
+            uint32_t next = 64u * R;
+            if (above) {
+                next = i + 1 + (uint32_t)__builtin_ctzll(above);
+            } else {
+#pragma unroll
+                for (int rr = R - 1; rr > r; rr--)
+                    if (B[rr]) next = (uint32_t)(rr * 64 + __builtin_ctzll(B[rr]));
+            }
+            freq = next - i;
+            const double imp = candidate_improvement(x[r], freq, deg, my, a.stats, a.p_ext);
+            if (x[r] == my) {
+                rem_lane = -imp;
+                has_rem = true;
+            }
+            if (imp > 0.0 && better(imp, x[r], bi, bl)) {
+                bi = imp;
+                bl = x[r];
+            }
+            heads++;
+        }
+        if (a.ex_sorted && i < deg) {
+            a.ex_sorted[beg + i] = x[r];
+            a.ex_count[beg + i] = head ? freq : 0u;
+        }
+    }
+    group_argmax<64>(bi, bl);
+    const uint64_t mm = __ballot(has_rem);
+    const int src = mm ? __builtin_ctzll(mm) : lane;
+    const double xr = __shfl(rem_lane, src, SCD_WAVE);
+    const double rem = mm ? xr : 0.0;
+    if (lane == 0 && a.labels_out) a.labels_out[p] = move_rule(bi, bl, rem, my);
+    count_pairs(a.counters, heads);
+}
+
+// Block-wide argmax of (bi, bl) over 256 threads; thread 0 gets the result.
+__device__ __forceinline__ void block_argmax256(double &bi, uint32_t &bl, double *s_bi,
+                                                uint32_t *s_bl) {
+    group_argmax<64>(bi, bl);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s_bi[w] = bi;
+        s_bl[w] = bl;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < 4; k++)
+            if (better(s_bi[k], s_bl[k], bi, bl)) {
+                bi = s_bi[k];
+                bl = s_bl[k];
+            }
+    }
+}
+
+__device__ __forceinline__ uint32_t next_pow2(uint32_t x) {
+    return x <= 2 ? 2u : (1u << (32 - __builtin_clz(x - 1)));
+}
+
+// ---------------------------------------------------------------------------
+// One workgroup (256 threads) per vertex, deg <= T: bitonic sort in LDS.
+// ---------------------------------------------------------------------------
+template <int T>
+__global__ __launch_bounds__(256) void k_move_lds(MoveArgs a, const uint32_t *__restrict__ verts,
+                                                  uint32_t nv) {
+    __shared__ uint32_t keys[T];
+    __shared__ double s_bi[4];
+    __shared__ uint32_t s_bl[4];
+    __shared__ double s_rem;
+    __shared__ int s_has_rem;
+    const int tid = threadIdx.x;
+    const uint32_t p = verts[blockIdx.x];
+    const uint32_t beg = a.indices[p];
+    const uint32_t deg = a.indices[p + 1] - beg;
+    const uint32_t my = a.labels[p];
+    const uint32_t n2 = next_pow2(deg);
+    if (tid == 0) s_has_rem = 0;
+    for (uint32_t i = tid; i < n2; i += 256) keys[i] = i < deg ? a.labels[a.adj[beg + i]] : SCD_INVALID;
+    __syncthreads();
+    lds_bitonic_sort(keys, n2, tid, 256);
+
+    double bi = 0.0;
+    uint32_t bl = SCD_INVALID;
+    uint32_t heads = 0;
+    for (uint32_t i = tid; i < deg; i += 256) {
+        const uint32_t v = keys[i];
+        const bool head = (i == 0 || keys[i - 1] != v) && v != SCD_INVALID;
+        uint32_t freq = 0;
+        if (head) {
+            freq = upper_bound_u32(keys, n2, v) - i;
+            const double imp = candidate_improvement(v, freq, deg, my, a.stats, a.p_ext);
+            if (v == my) {
+                s_rem = -imp;
+                s_has_rem = 1;
+            }
+            if (imp > 0.0 && better(imp, v, bi, bl)) {
+                bi = imp;
+                bl = v;
+            }
+            heads++;
+        }
+        if (a.ex_sorted) {
+            a.ex_sorted[beg + i] = v;
+            a.ex_count[beg + i] = freq;
+        }
+    }
+    count_pairs(a.counters, heads);
+    block_argmax256(bi, bl, s_bi, s_bl);
+    if (tid == 0 && a.labels_out) {
+        const double rem = s_has_rem ? s_rem : 0.0;
+        a.labels_out[p] = move_rule(bi, bl, rem, my);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Hub path (deg > T).  Hub h has vertex hub_v[h], degree hub_d[h] and a
+// scratch segment at hub_off[h] in two ping-pong buffers.  Work items:
+//   tiles  (tile_hub[t], tile_start[t]): T keys each, sorted in LDS;
+//   chunks (chunk_hub[c], chunk_start[c]): C outputs each for merge / eval.
+// ---------------------------------------------------------------------------
+struct HubArgs {
+    const uint32_t *__restrict__ hub_v;
+    const uint32_t *__restrict__ hub_d;
+    const uint32_t *__restrict__ hub_off;
+    const uint32_t *__restrict__ hub_chunk_begin;  // nh + 1
+    const uint32_t *__restrict__ tile_hub;
+    const uint32_t *__restrict__ tile_start;
+    const uint32_t *__restrict__ chunk_hub;
+    const uint32_t *__restrict__ chunk_start;
+    double *__restrict__ chunk_bi;
+    uint32_t *__restrict__ chunk_bl;
+    double *__restrict__ chunk_rem;
+    uint32_t *__restrict__ chunk_has_rem;
+};
+
+template <int T>
+__global__ __launch_bounds__(256) void k_hub_tile_sort(MoveArgs a, HubArgs h,
+                                                       uint32_t *__restrict__ out) {
+    __shared__ uint32_t keys[T];
+    const int tid = threadIdx.x;
+    const uint32_t hb = h.tile_hub[blockIdx.x];
+    const uint32_t st = h.tile_start[blockIdx.x];
+    const uint32_t p = h.hub_v[hb];
+    const uint32_t d = h.hub_d[hb];
+    const uint32_t beg = a.indices[p] + st;
+    const uint32_t len = min((uint32_t)T, d - st);
+    const uint32_t n2 = next_pow2(len);
+    for (uint32_t i = tid; i < n2; i += 256) keys[i] = i < len ? a.labels[a.adj[beg + i]] : SCD_INVALID;
+    __syncthreads();
+    lds_bitonic_sort(keys, n2, tid, 256);
+    uint32_t *dst = out + h.hub_off[hb] + st;
+    for (uint32_t i = tid; i < len; i += 256) dst[i] = keys[i];
+}
+
+// One merge pass: sorted runs of width w -> 2w inside every hub segment.
+// C (outputs per workgroup) is a power of two with C <= 2w, so a chunk never
+// straddles two run pairs.
+__global__ __launch_bounds__(256) void k_hub_merge(HubArgs h, const uint32_t *__restrict__ src,
+                                                   uint32_t *__restrict__ dst, uint32_t w, uint32_t C) {
+    const uint32_t hb = h.chunk_hub[blockIdx.x];
+    const uint32_t o = h.chunk_start[blockIdx.x];
+    const uint32_t d = h.hub_d[hb];
+    const uint32_t base = h.hub_off[hb];
+    const uint32_t ps = (o / (2 * w)) * (2 * w);
+    const uint32_t la = min(w, d - ps);
+    const uint32_t lb = (d - ps > w) ? min(w, d - ps - w) : 0u;
+    const uint32_t *A = src + base + ps;
+    const uint32_t *B = A + la;
+    const uint32_t out_lo = o - ps;
+    const uint32_t out_hi = min(o + C, ps + la + lb) - ps;
+    const uint32_t per = C / 256;
+    const uint32_t t0 = out_lo + threadIdx.x * per;
+    if (t0 >= out_hi) return;
+    const uint32_t t1 = min(t0 + per, out_hi);
+    uint32_t i = merge_path(A, la, B, lb, t0);
+    uint32_t j = t0 - i;
+    uint32_t *D = dst + base + ps;
+    for (uint32_t k = t0; k < t1; k++) {
+        const bool takeA = (j >= lb) || (i < la && A[i] <= B[j]);
+        D[k] = takeA ? A[i++] : B[j++];
+    }
+}
+
+// Reduce-by-key + evaluation over a chunk of a hub's sorted segment; the
+// chunk's best candidate and (if present) the own-label improvement go to
+// per-chunk slots.
+__global__ __launch_bounds__(256) void k_hub_eval(MoveArgs a, HubArgs h,
+                                                  const uint32_t *__restrict__ sorted, uint32_t C) {
+    __shared__ double s_bi[4];
+    __shared__ uint32_t s_bl[4];
+    __shared__ double s_rem;
+    __shared__ int s_has_rem;
+    const int tid = threadIdx.x;
+    const uint32_t c = blockIdx.x;
+    const uint32_t hb = h.chunk_hub[c];
+    const uint32_t o = h.chunk_start[c];
+    const uint32_t d = h.hub_d[hb];
+    const uint32_t p = h.hub_v[hb];
+    const uint32_t my = a.labels[p];
+    const uint32_t beg = a.indices[p];
+    const uint32_t *S = sorted + h.hub_off[hb];
+    if (tid == 0) s_has_rem = 0;
+    __syncthreads();
+    double bi = 0.0;
+    uint32_t bl = SCD_INVALID;
+    uint32_t heads = 0;
+    const uint32_t end = min(o + C, d);
+    for (uint32_t i = o + tid; i < end; i += 256) {
+        const uint32_t v = S[i];
+        const bool head = (i == 0 || S[i - 1] != v) && v != SCD_INVALID;
+        uint32_t freq = 0;
+        if (head) {
+            freq = upper_bound_u32(S, d, v) - i;
+            const double imp = candidate_improvement(v, freq, d, my, a.stats, a.p_ext);
+            if (v == my) {
+                s_rem = -imp;
+                s_has_rem = 1;
+            }
+            if (imp > 0.0 && better(imp, v, bi, bl)) {
+                bi = imp;
+                bl = v;
+            }
+            heads++;
+        }
+        if (a.ex_sorted) {
+            a.ex_sorted[beg + i] = v;
+            a.ex_count[beg + i] = freq;
+        }
+    }
+    count_pairs(a.counters, heads);
+    block_argmax256(bi, bl, s_bi, s_bl);
+    if (tid == 0) {
+        h.chunk_bi[c] = bi;
+        h.chunk_bl[c] = bl;
+        h.chunk_has_rem[c] = (uint32_t)s_has_rem;
+        h.chunk_rem[c] = s_has_rem ? s_rem : 0.0;
+    }
+}
+
+// One thread per hub: combine the hub's chunks and apply the move rule.
+__global__ __launch_bounds__(64) void k_hub_finalize(MoveArgs a, HubArgs h, uint32_t nh) {
+    const uint32_t hb = blockIdx.x * 64u + threadIdx.x;
+    if (hb >= nh) return;
+    const uint32_t p = h.hub_v[hb];
+    const uint32_t my = a.labels[p];
+    double bi = 0.0, rem = 0.0;
+    uint32_t bl = SCD_INVALID;
+    for (uint32_t c = h.hub_chunk_begin[hb]; c < h.hub_chunk_begin[hb + 1]; c++) {
+        if (better(h.chunk_bi[c], h.chunk_bl[c], bi, bl)) {
+            bi = h.chunk_bi[c];
+            bl = h.chunk_bl[c];
+        }
+        if (h.chunk_has_rem[c]) rem = h.chunk_rem[c];
+    }
+    if (a.labels_out) a.labels_out[p] = move_rule(bi, bl, rem, my);
+}
+
+}  // namespace scd_dev
+
+#endif

@@ -1,0 +1,497 @@
+// Community statistics, internal adjacency and WCC kernels.
+//
+// Statistics (collect_stats, src/scd.cpp:57-111): for every vertex p with a
+// valid label L, sizes[L] += 1, ext[L] += #neighbours with another label
+// (INVALID neighbours included), int[L] += #neighbours with label L.  Integer
+// atomics, so the result is independent of order.  The same pass writes p's
+// same-label neighbours, in adjacency order, to int_adj at p's own CSR offset
+// (int_adj[indices[p] .. + int_deg[p]]): the internal subgraph the WCC needs,
+// built without a global scan.
+//
+// WCC (compute_wcc, src/wcc.cpp:31-141): for p with valid label L,
+// sizes[L] > 2 and tri_deg[p] != 0, int_tri = sum over same-label neighbours
+// q of |Nc(p) ∩ N(q)| and int_tri_deg = #q with a non-empty intersection,
+// where Nc(p) is p's same-label neighbour list.  Since every element of Nc(p)
+// has label L = label(q), |Nc(p) ∩ N(q)| = |Nc(p) ∩ Nc(q)|: both lists are in
+// int_adj.  Each vertex's term is written to terms[p]; the mean is a
+// fixed-shape tree sum over vertex order (k_reduce_*), so its bits do not
+// depend on the degree bins or tile sizes.
+#ifndef SCD_STATS_KERNELS_HPP
+#define SCD_STATS_KERNELS_HPP
+
+#include "dev.hpp"
+
+namespace scd_dev {
+
+struct StatArgs {
+    const uint32_t *__restrict__ indices;
+    const uint32_t *__restrict__ adj;
+    const uint32_t *__restrict__ labels;  // current (new) labels
+    int4 *__restrict__ stats;
+    uint32_t *__restrict__ int_adj;
+    uint32_t *__restrict__ int_deg;
+    unsigned long long *__restrict__ counters;  // [1] += internal entries, [2] += deg_int^2
+};
+
+__device__ __forceinline__ void add_stats(int4 *stats, uint32_t L, uint32_t deg, uint32_t cnt) {
+    atomicAdd(&stats[L].x, 1);
+    const uint32_t ext = deg - cnt;
+    if (ext > 0) atomicAdd(&stats[L].y, (int)ext);
+    if (cnt > 0) atomicAdd(&stats[L].z, (int)cnt);
+}
+
+__device__ __forceinline__ void count_internal(unsigned long long *counters, uint64_t ent,
+                                               uint64_t sq) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        ent += __shfl_xor(ent, off, SCD_WAVE);
+        sq += __shfl_xor(sq, off, SCD_WAVE);
+    }
+    if ((threadIdx.x & 63) == 0 && counters) {
+        atomicAdd(counters + 1, (unsigned long long)ent);
+        atomicAdd(counters + 2, (unsigned long long)sq);
+    }
+}
+
+__global__ void k_fill_int4(int4 *__restrict__ p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = make_int4(0, 0, 0, 0);
+}
+
+__global__ void k_fill_u64(unsigned long long *__restrict__ p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// S lanes per vertex (deg <= S <= 64).
+template <int S>
+__global__ __launch_bounds__(256) void k_filter_sub(StatArgs a, const uint32_t *__restrict__ verts,
+                                                    uint32_t nv) {
+    const int lane = threadIdx.x & 63;
+    const int lig = threadIdx.x & (S - 1);
+    const int gbase = lane & ~(S - 1);
+    const uint32_t slot = (blockIdx.x * 256u + threadIdx.x) / S;
+    const bool active = slot < nv;
+    const uint32_t p = active ? verts[slot] : 0u;
+    const uint32_t beg = active ? a.indices[p] : 0u;
+    const uint32_t deg = active ? a.indices[p + 1] - beg : 0u;
+    const uint32_t L = active ? a.labels[p] : SCD_INVALID;
+    bool same = false;
+    uint32_t q = 0;
+    if ((uint32_t)lig < deg) {
+        q = a.adj[beg + lig];
+        same = (L != SCD_INVALID) && a.labels[q] == L;
+    }
+    uint64_t gm;
+    if constexpr (S == 64) gm = __ballot(same);
+    else gm = (__ballot(same) >> gbase) & ((1ull << S) - 1ull);
+    const uint32_t cnt = (uint32_t)__builtin_popcountll(gm);
+    if (same) a.int_adj[beg + __builtin_popcountll(gm & ((1ull << lig) - 1ull))] = q;
+    const bool lead = active && lig == 0;
+    if (lead) {
+        a.int_deg[p] = cnt;
+        if (L != SCD_INVALID) add_stats(a.stats, L, deg, cnt);
+    }
+    count_internal(a.counters, lead ? cnt : 0, lead ? (uint64_t)cnt * cnt : 0);
+}
+
+// One wave per vertex; loops over the list in slices of 64.
+__global__ __launch_bounds__(256) void k_filter_wave(StatArgs a, const uint32_t *__restrict__ verts,
+                                                     uint32_t nv) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t slot = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (slot >= nv) return;
+    const uint32_t p = verts[slot];
+    const uint32_t beg = a.indices[p];
+    const uint32_t deg = a.indices[p + 1] - beg;
+    const uint32_t L = a.labels[p];
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint32_t run = 0;
+    for (uint32_t b = 0; b < deg; b += 64) {
+        const uint32_t i = b + lane;
+        bool same = false;
+        uint32_t q = 0;
+        if (i < deg) {
+            q = a.adj[beg + i];
+            same = (L != SCD_INVALID) && a.labels[q] == L;
+        }
+        const uint64_t m = __ballot(same);
+        if (same) a.int_adj[beg + run + __builtin_popcountll(m & lt)] = q;
+        run += (uint32_t)__builtin_popcountll(m);
+    }
+    if (lane == 0) {
+        a.int_deg[p] = run;
+        if (L != SCD_INVALID) add_stats(a.stats, L, deg, run);
+    }
+    count_internal(a.counters, lane == 0 ? run : 0, lane == 0 ? (uint64_t)run * run : 0);
+}
+
+// Order-preserving compaction of one slice of up to 256 entries by a block
+// of 256 threads: returns the slice's count; `same` entries are written at
+// out[run + rank].
+__device__ __forceinline__ uint32_t block_compact256(bool same, uint32_t q, uint32_t *out,
+                                                     uint32_t run, uint32_t *s_cnt) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t m = __ballot(same);
+    if (lane == 0) s_cnt[w] = (uint32_t)__builtin_popcountll(m);
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        before += (k < w) ? s_cnt[k] : 0u;
+        total += s_cnt[k];
+    }
+    if (same) out[run + before + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = q;
+    __syncthreads();
+    return total;
+}
+
+// One workgroup per vertex (deg up to the LDS tile).
+__global__ __launch_bounds__(256) void k_filter_block(StatArgs a, const uint32_t *__restrict__ verts,
+                                                      uint32_t nv) {
+    __shared__ uint32_t s_cnt[4];
+    const uint32_t p = verts[blockIdx.x];
+    const uint32_t beg = a.indices[p];
+    const uint32_t deg = a.indices[p + 1] - beg;
+    const uint32_t L = a.labels[p];
+    uint32_t run = 0;
+    for (uint32_t b = 0; b < deg; b += 256) {
+        const uint32_t i = b + threadIdx.x;
+        bool same = false;
+        uint32_t q = 0;
+        if (i < deg) {
+            q = a.adj[beg + i];
+            same = (L != SCD_INVALID) && a.labels[q] == L;
+        }
+        run += block_compact256(same, q, a.int_adj + beg, run, s_cnt);
+    }
+    if (threadIdx.x == 0) {
+        a.int_deg[p] = run;
+        if (L != SCD_INVALID) add_stats(a.stats, L, deg, run);
+    }
+    count_internal(a.counters, threadIdx.x == 0 ? run : 0,
+                   threadIdx.x == 0 ? (uint64_t)run * run : 0);
+}
+
+// Hubs, pass 1: same-label count of each chunk of C entries; the chunk's
+// external and internal counts go straight to the statistics.
+__global__ __launch_bounds__(256) void k_hub_filter_count(StatArgs a, const uint32_t *__restrict__ hub_v,
+                                                          const uint32_t *__restrict__ hub_d,
+                                                          const uint32_t *__restrict__ chunk_hub,
+                                                          const uint32_t *__restrict__ chunk_start,
+                                                          uint32_t C, uint32_t *__restrict__ chunk_cnt) {
+    __shared__ uint32_t s_cnt[4];
+    const uint32_t hb = chunk_hub[blockIdx.x];
+    const uint32_t o = chunk_start[blockIdx.x];
+    const uint32_t p = hub_v[hb];
+    const uint32_t d = hub_d[hb];
+    const uint32_t beg = a.indices[p];
+    const uint32_t L = a.labels[p];
+    const uint32_t end = min(o + C, d);
+    uint32_t mine = 0;
+    for (uint32_t i = o + threadIdx.x; i < end; i += 256)
+        mine += (L != SCD_INVALID && a.labels[a.adj[beg + i]] == L) ? 1u : 0u;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off, SCD_WAVE);
+    if ((threadIdx.x & 63) == 0) s_cnt[threadIdx.x >> 6] = mine;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t cnt = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+        chunk_cnt[blockIdx.x] = cnt;
+        if (L != SCD_INVALID) {
+            if (o == 0) atomicAdd(&a.stats[L].x, 1);
+            const uint32_t ext = (end - o) - cnt;
+            if (ext > 0) atomicAdd(&a.stats[L].y, (int)ext);
+            if (cnt > 0) atomicAdd(&a.stats[L].z, (int)cnt);
+        }
+    }
+}
+
+// Hubs, pass 2: compaction of each chunk at the offset given by the counts
+// of the hub's earlier chunks; the hub's first chunk records int_deg.
+__global__ __launch_bounds__(256) void k_hub_filter_write(StatArgs a, const uint32_t *__restrict__ hub_v,
+                                                          const uint32_t *__restrict__ hub_d,
+                                                          const uint32_t *__restrict__ hub_chunk_begin,
+                                                          const uint32_t *__restrict__ chunk_hub,
+                                                          const uint32_t *__restrict__ chunk_start,
+                                                          uint32_t C, const uint32_t *__restrict__ chunk_cnt) {
+    __shared__ uint32_t s_cnt[4];
+    const uint32_t c = blockIdx.x;
+    const uint32_t hb = chunk_hub[c];
+    const uint32_t o = chunk_start[c];
+    const uint32_t p = hub_v[hb];
+    const uint32_t d = hub_d[hb];
+    const uint32_t beg = a.indices[p];
+    const uint32_t L = a.labels[p];
+    uint32_t run = 0;
+    for (uint32_t k = hub_chunk_begin[hb]; k < c; k++) run += chunk_cnt[k];
+    if (o == 0 && threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (uint32_t k = hub_chunk_begin[hb]; k < hub_chunk_begin[hb + 1]; k++) tot += chunk_cnt[k];
+        a.int_deg[p] = tot;
+        if (a.counters) {
+            atomicAdd(a.counters + 1, (unsigned long long)tot);
+            atomicAdd(a.counters + 2, (unsigned long long)tot * tot);
+        }
+    }
+    const uint32_t end = min(o + C, d);
+    for (uint32_t b = o; b < end; b += 256) {
+        const uint32_t i = b + threadIdx.x;
+        bool same = false;
+        uint32_t q = 0;
+        if (i < end) {
+            q = a.adj[beg + i];
+            same = (L != SCD_INVALID) && a.labels[q] == L;
+        }
+        run += block_compact256(same, q, a.int_adj + beg, run, s_cnt);
+    }
+}
+
+struct WccArgs {
+    const uint32_t *__restrict__ indices;
+    const uint32_t *__restrict__ labels;
+    const int4 *__restrict__ stats;
+    const uint32_t *__restrict__ int_adj;
+    const uint32_t *__restrict__ int_deg;
+    const int32_t *__restrict__ tri;
+    const int32_t *__restrict__ tri_deg;
+    double *__restrict__ terms;
+    int32_t *__restrict__ int_tri_out;      // optional
+    int32_t *__restrict__ int_tri_deg_out;  // optional
+};
+
+// Whether p contributes a term (src/wcc.cpp:89-104).
+__device__ __forceinline__ bool wcc_counts(const WccArgs &a, uint32_t p) {
+    const uint32_t L = a.labels[p];
+    return L != SCD_INVALID && a.stats[L].x > 2 && a.tri_deg[p] != 0;
+}
+
+__device__ __forceinline__ uint32_t pair_common(const WccArgs &a, const uint32_t *A, uint32_t na,
+                                                uint32_t q) {
+    return count_common(A, na, a.int_adj + a.indices[q], a.int_deg[q]);
+}
+
+__device__ __forceinline__ void wcc_store(const WccArgs &a, uint32_t p, bool valid, uint32_t itri,
+                                          uint32_t itdeg) {
+    const uint32_t L = a.labels[p];
+    a.terms[p] = valid ? wcc_term(a.tri[p], a.tri_deg[p], a.stats[L].x, (int)itri, (int)itdeg) : 0.0;
+    if (a.int_tri_out) {
+        a.int_tri_out[p] = valid ? (int)itri : 0;
+        a.int_tri_deg_out[p] = valid ? (int)itdeg : 0;
+    }
+}
+
+// S lanes per vertex (deg <= S): lane j handles the pair (p, Nc(p)[j]).
+template <int S>
+__global__ __launch_bounds__(256) void k_wcc_sub(WccArgs a, const uint32_t *__restrict__ verts,
+                                                 uint32_t nv) {
+    const int lig = threadIdx.x & (S - 1);
+    const uint32_t slot = (blockIdx.x * 256u + threadIdx.x) / S;
+    const bool active = slot < nv;
+    const uint32_t p = active ? verts[slot] : 0u;
+    const bool valid = active && wcc_counts(a, p);
+    uint32_t c = 0;
+    if (valid) {
+        const uint32_t na = a.int_deg[p];
+        const uint32_t *A = a.int_adj + a.indices[p];
+        if ((uint32_t)lig < na) c = pair_common(a, A, na, A[lig]);
+    }
+    uint32_t itri = c, itdeg = c > 0 ? 1u : 0u;
+#pragma unroll
+    for (int off = 1; off < S; off <<= 1) {
+        itri += __shfl_xor(itri, off, SCD_WAVE);
+        itdeg += __shfl_xor(itdeg, off, SCD_WAVE);
+    }
+    if (active && lig == 0) wcc_store(a, p, valid, itri, itdeg);
+}
+
+// One wave per vertex.
+__global__ __launch_bounds__(256) void k_wcc_wave(WccArgs a, const uint32_t *__restrict__ verts,
+                                                  uint32_t nv) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t slot = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (slot >= nv) return;
+    const uint32_t p = verts[slot];
+    const bool valid = wcc_counts(a, p);
+    uint32_t itri = 0, itdeg = 0;
+    if (valid) {
+        const uint32_t na = a.int_deg[p];
+        const uint32_t *A = a.int_adj + a.indices[p];
+        for (uint32_t j = lane; j < na; j += 64) {
+            const uint32_t c = pair_common(a, A, na, A[j]);
+            itri += c;
+            itdeg += c > 0 ? 1u : 0u;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        itri += __shfl_xor(itri, off, SCD_WAVE);
+        itdeg += __shfl_xor(itdeg, off, SCD_WAVE);
+    }
+    if (lane == 0) wcc_store(a, p, valid, itri, itdeg);
+}
+
+// One workgroup per vertex (large degrees and hubs).
+__global__ __launch_bounds__(256) void k_wcc_block(WccArgs a, const uint32_t *__restrict__ verts,
+                                                   uint32_t nv) {
+    __shared__ uint32_t s_t[4], s_d[4];
+    const uint32_t p = verts[blockIdx.x];
+    const bool valid = wcc_counts(a, p);
+    uint32_t itri = 0, itdeg = 0;
+    if (valid) {
+        const uint32_t na = a.int_deg[p];
+        const uint32_t *A = a.int_adj + a.indices[p];
+        for (uint32_t j = threadIdx.x; j < na; j += 256) {
+            const uint32_t c = pair_common(a, A, na, A[j]);
+            itri += c;
+            itdeg += c > 0 ? 1u : 0u;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        itri += __shfl_xor(itri, off, SCD_WAVE);
+        itdeg += __shfl_xor(itdeg, off, SCD_WAVE);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        s_t[threadIdx.x >> 6] = itri;
+        s_d[threadIdx.x >> 6] = itdeg;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        wcc_store(a, p, valid, s_t[0] + s_t[1] + s_t[2] + s_t[3], s_d[0] + s_d[1] + s_d[2] + s_d[3]);
+}
+
+// Fixed-shape sum of x[0..n): block b sums x[b*4096 .. +4096) as 256 strided
+// partial sums (thread t adds x[base + t + 256k], k ascending) followed by a
+// pairwise tree; the shape depends on n only.
+__device__ __forceinline__ double block_tree_sum256(double v, double *s) {
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] = s[threadIdx.x] + s[threadIdx.x + w];
+        __syncthreads();
+    }
+    return s[0];
+}
+
+__global__ __launch_bounds__(256) void k_reduce_partial(const double *__restrict__ x, uint32_t n,
+                                                        double *__restrict__ partial) {
+    __shared__ double s[256];
+    const uint64_t base = (uint64_t)blockIdx.x * 4096u;
+    double acc = 0.0;
+    for (uint32_t k = 0; k < 16; k++) {
+        const uint64_t i = base + threadIdx.x + 256u * k;
+        if (i < n) acc = acc + x[i];
+    }
+    const double r = block_tree_sum256(acc, s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
+// Single block: the same shape over the partial sums, then / count.
+__global__ __launch_bounds__(256) void k_reduce_final(const double *__restrict__ partial, uint32_t np,
+                                                      uint32_t count, double *__restrict__ out) {
+    __shared__ double s[256];
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < np; i += 256) acc = acc + partial[i];
+    const double r = block_tree_sum256(acc, s);
+    if (threadIdx.x == 0) *out = count > 0 ? r / (double)count : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of uint32 (wrapping) in three steps: per-tile sums, scan of
+// the tile sums (recursively, host-driven), tile-local scan plus tile offset.
+// Tiles are 2048 entries: 256 threads x 8 consecutive entries.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t *s_w, uint32_t &total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t inc = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(inc, off, SCD_WAVE);
+        if (lane >= off) inc += t;
+    }
+    if (lane == 63) s_w[w] = inc;
+    __syncthreads();
+    uint32_t before = 0;
+    total = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        before += (k < w) ? s_w[k] : 0u;
+        total += s_w[k];
+    }
+    __syncthreads();
+    return before + inc - v;
+}
+
+__global__ __launch_bounds__(256) void k_scan_tile_sums(const uint32_t *__restrict__ in, uint64_t n,
+                                                        uint32_t *__restrict__ sums) {
+    __shared__ uint32_t s_w[4];
+    const uint64_t base = (uint64_t)blockIdx.x * 2048u;
+    uint32_t acc = 0;
+    for (int k = 0; k < 8; k++) {
+        const uint64_t i = base + threadIdx.x + 256u * k;
+        if (i < n) acc += in[i];
+    }
+    uint32_t total;
+    block_excl_scan256(acc, s_w, total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void k_scan_tiles(const uint32_t *__restrict__ in, uint64_t n,
+                                                    const uint32_t *__restrict__ tile_off,
+                                                    uint32_t *__restrict__ out) {
+    __shared__ uint32_t s_v[2048];
+    __shared__ uint32_t s_w[4];
+    const uint64_t base = (uint64_t)blockIdx.x * 2048u;
+    for (int k = 0; k < 8; k++) {
+        const uint64_t i = base + threadIdx.x + 256u * k;
+        s_v[threadIdx.x + 256 * k] = i < n ? in[i] : 0u;
+    }
+    __syncthreads();
+    uint32_t loc[8], acc = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        loc[k] = s_v[threadIdx.x * 8 + k];
+        acc += loc[k];
+    }
+    uint32_t total;
+    uint32_t run = block_excl_scan256(acc, s_w, total) + (tile_off ? tile_off[blockIdx.x] : 0u);
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        s_v[threadIdx.x * 8 + k] = run;
+        run += loc[k];
+    }
+    __syncthreads();
+    for (int k = 0; k < 8; k++) {
+        const uint64_t i = base + threadIdx.x + 256u * k;
+        if (i < n) out[i] = s_v[threadIdx.x + 256 * k];
+    }
+}
+
+// Export of the reduce-by-key runs: flag[i] = (count[i] > 0).
+__global__ void k_run_flags(const uint32_t *__restrict__ cnt, uint64_t n, uint32_t *__restrict__ flag) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[i] = cnt[i] > 0 ? 1u : 0u;
+}
+
+__global__ void k_run_scatter(const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ cnt,
+                              const uint32_t *__restrict__ pos, uint64_t n,
+                              uint32_t *__restrict__ run_label, uint32_t *__restrict__ run_count) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && cnt[i] > 0) {
+        run_label[pos[i]] = sorted[i];
+        run_count[pos[i]] = cnt[i];
+    }
+}
+
+__global__ void k_run_offsets(const uint32_t *__restrict__ indices, const uint32_t *__restrict__ pos,
+                              uint32_t n, uint64_t E, uint32_t total, uint32_t *__restrict__ off) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v > n) return;
+    const uint32_t e = indices[v];
+    off[v] = (v == n || e >= E) ? total : pos[e];
+}
+
+}  // namespace scd_dev
+
+#endif

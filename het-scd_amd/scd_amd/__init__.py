@@ -1,0 +1,11 @@
+"""scd_amd: MI355X-native SCD community detection (host side).
+
+The compute path is the gfx950 HIP engine in libscd_hip.so (include/scd_hip.h);
+this package binds it and the host library (binary format, generators,
+convert) with ctypes.  There is no CPU fallback: Engine raises ScdError when
+the HIP library is missing or no device is present.
+"""
+from ._lib import INVALID_LABEL, ScdError  # noqa: F401
+from .engine import Engine, device_count  # noqa: F401
+from .graph import (Edges, Graph, compress_count, from_edge_pairs, planted_partition,  # noqa: F401
+                    planted_random, rmat)

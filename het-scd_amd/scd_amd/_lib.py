@@ -1,0 +1,165 @@
+"""ctypes loading of the in-tree libraries and the error convention.
+
+Every C entry point returns 0 or a negative SCD_E* code; a failure raises
+ScdError carrying the code and the library's last-error message.  The HIP
+engine library is required: there is no CPU fallback, and loading fails
+loudly when the library has not been built.
+"""
+import ctypes
+import os
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_DIR = os.path.join(PKG_DIR, "lib")
+
+SCD_OK = 0
+SCD_EINVAL = -1
+SCD_EIO = -2
+SCD_ENOMEM = -3
+SCD_EDEVICE = -4
+SCD_ELIMIT = -5
+SCD_ESTATE = -6
+SCD_EBUSY = -7
+INVALID_LABEL = 0xFFFFFFFF
+
+_NAMES = {
+    SCD_EINVAL: "EINVAL", SCD_EIO: "EIO", SCD_ENOMEM: "ENOMEM", SCD_EDEVICE: "EDEVICE",
+    SCD_ELIMIT: "ELIMIT", SCD_ESTATE: "ESTATE", SCD_EBUSY: "EBUSY",
+}
+
+
+class ScdError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"{_NAMES.get(code, code)}: {message}")
+        self.code = code
+        self.message = message
+
+
+_cache = {}
+
+u32p = ctypes.POINTER(ctypes.c_uint32)
+i32p = ctypes.POINTER(ctypes.c_int32)
+f64p = ctypes.POINTER(ctypes.c_double)
+u64p = ctypes.POINTER(ctypes.c_uint64)
+vp = ctypes.c_void_p
+
+
+def _load(name):
+    if name in _cache:
+        return _cache[name]
+    path = os.path.join(LIB_DIR, name)
+    if not os.path.exists(path):
+        raise ScdError(SCD_ESTATE, f"{path} is not built; run `python het-scd_amd/build.py`")
+    lib = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+    _cache[name] = lib
+    return lib
+
+
+class ScdGraphC(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint32), ("m", ctypes.c_uint32), ("indices", u32p), ("adj", u32p),
+                ("tri", i32p), ("tri_deg", i32p)]
+
+
+class ScdEdgesC(ctypes.Structure):
+    _fields_ = [("count", ctypes.c_uint64), ("u", u32p), ("v", u32p)]
+
+
+class ScdTuningC(ctypes.Structure):
+    _fields_ = [("lds_tile", ctypes.c_uint32), ("merge_chunk", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 6)]
+
+
+def host():
+    """libscd_host.so with argtypes set."""
+    if "host_ready" in _cache:
+        return _cache["libscd_host.so"]
+    lib = _load("libscd_host.so")
+    G = ctypes.POINTER(ScdGraphC)
+    Ed = ctypes.POINTER(ScdEdgesC)
+    sig = {
+        "scd_host_last_error": (ctypes.c_char_p, []),
+        "scd_graph_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(G)]),
+        "scd_graph_write": (ctypes.c_int, [G, ctypes.c_char_p]),
+        "scd_graph_from_arrays": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, u32p, u32p, i32p, i32p,
+                                                 ctypes.POINTER(G)]),
+        "scd_graph_free": (None, [G]),
+        "scd_graph_clustering_coef": (ctypes.c_double, [G]),
+        "scd_gen_planted_partition": (ctypes.c_int, [ctypes.c_uint32, u32p, ctypes.c_double, ctypes.c_double,
+                                                     ctypes.c_uint64, ctypes.POINTER(Ed)]),
+        "scd_gen_planted_random": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                                  ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double,
+                                                  ctypes.c_double, u32p, u32p, ctypes.POINTER(Ed)]),
+        "scd_gen_rmat": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_uint64, ctypes.POINTER(Ed)]),
+        "scd_edges_from_arrays": (ctypes.c_int, [ctypes.c_uint64, u32p, u32p, ctypes.POINTER(Ed)]),
+        "scd_edges_free": (None, [Ed]),
+        "scd_edges_write_text": (ctypes.c_int, [Ed, ctypes.c_char_p]),
+        "scd_convert": (ctypes.c_int, [Ed, ctypes.POINTER(G)]),
+    }
+    for k, (res, args) in sig.items():
+        f = getattr(lib, k)
+        f.restype = res
+        f.argtypes = args
+    _cache["host_ready"] = True
+    return lib
+
+
+def hip():
+    """libscd_hip.so with argtypes set."""
+    if "hip_ready" in _cache:
+        return _cache["libscd_hip.so"]
+    host()
+    lib = _load("libscd_hip.so")
+    C = vp
+    sig = {
+        "scd_last_error": (ctypes.c_char_p, []),
+        "scd_device_count": (ctypes.c_int, []),
+        "scd_init": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(C)]),
+        "scd_destroy": (None, [C]),
+        "scd_upload": (ctypes.c_int, [C, ctypes.c_uint32, ctypes.c_uint32, u32p, u32p, i32p, i32p,
+                                      ctypes.c_double]),
+        "scd_count_triangles": (ctypes.c_int, [C, i32p, i32p, ctypes.c_int]),
+        "scd_initial_partition": (ctypes.c_int, [C, u32p, u32p]),
+        "scd_labels_to_device": (ctypes.c_int, [C, u32p]),
+        "scd_improve": (ctypes.c_int, [C, f64p, u32p]),
+        "scd_store_best": (ctypes.c_int, [C]),
+        "scd_labels_from_device": (ctypes.c_int, [C, u32p]),
+        "scd_current_labels": (ctypes.c_int, [C, u32p]),
+        "scd_refine": (ctypes.c_int, [C, ctypes.c_double, ctypes.c_uint32, f64p, u32p, f64p]),
+        "scd_compute_wcc": (ctypes.c_int, [C, f64p, f64p, i32p, i32p]),
+        "scd_get_stats": (ctypes.c_int, [C, u32p, i32p, i32p, i32p]),
+        "scd_neighbor_label_pairs": (ctypes.c_int, [C, u32p, u32p, u32p, u32p, u64p]),
+        "scd_get_tuning": (ctypes.c_int, [C, ctypes.POINTER(ScdTuningC)]),
+        "scd_set_tuning": (ctypes.c_int, [C, ctypes.POINTER(ScdTuningC)]),
+        "scd_last_counters": (ctypes.c_int, [C, u64p]),
+        "scd_prof_count": (ctypes.c_int, [C]),
+        "scd_prof_get": (ctypes.c_int, [C, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), u64p, f64p]),
+        "scd_prof_reset": (ctypes.c_int, [C]),
+        "scd_device_bytes": (ctypes.c_uint64, [C]),
+    }
+    for k, (res, args) in sig.items():
+        f = getattr(lib, k, None)
+        if f is None:
+            continue
+        f.restype = res
+        f.argtypes = args
+    _cache["hip_ready"] = True
+    return lib
+
+
+def check_host(rc):
+    if rc != SCD_OK:
+        raise ScdError(rc, host().scd_host_last_error().decode())
+    return rc
+
+
+def check_hip(rc):
+    if rc < 0:
+        raise ScdError(rc, hip().scd_last_error().decode())
+    return rc
+
+
+def ptr(a, t):
+    """ctypes pointer to a contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(t)

@@ -1,0 +1,175 @@
+"""Python mirror of the reference's single-GPU interface (src/cuda.hpp:23-28)
+over the C ABI of libscd_hip.so (include/scd_hip.h).
+
+    eng = Engine(device=0)                   # cuda_init
+    eng.transfer_graph(g)                    # cuda_transfer_graph
+    eng.labels_to_device(labels)             # cuda_labels_to_device
+    wcc = eng.improve_labels()               # cuda_improve_labels
+    eng.store_labels()                       # cuda_store_labels
+    best = eng.labels_from_device()          # cuda_labels_from_device
+
+Errors raise ScdError instead of exiting the process.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check_hip, f64p, i32p, ptr, u32p, u64p
+
+
+def device_count():
+    return check_hip(_lib.hip().scd_device_count())
+
+
+class Engine:
+    def __init__(self, device=0, work_group_size=256, profile=False):
+        self._ctx = ctypes.c_void_p()
+        check_hip(_lib.hip().scd_init(device, work_group_size, 1 if profile else 0, ctypes.byref(self._ctx)))
+        self.n = 0
+        self.graph = None
+
+    def close(self):
+        if self._ctx:
+            _lib.hip().scd_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- cuda.hpp mirror -------------------------------------------------
+    def transfer_graph(self, g, clustering_coef=None):
+        cc = g.clustering_coef if clustering_coef is None else clustering_coef
+        check_hip(_lib.hip().scd_upload(self._ctx, g.n, g.m, ptr(g.indices, u32p), ptr(g.adj, u32p),
+                                        ptr(g.tri, i32p), ptr(g.tri_deg, i32p), cc))
+        self.n = g.n
+        self.graph = g
+
+    def labels_to_device(self, labels):
+        labels = np.ascontiguousarray(labels, dtype=np.uint32)
+        if labels.shape != (self.n,):
+            raise ValueError(f"labels must hold n = {self.n} entries")
+        check_hip(_lib.hip().scd_labels_to_device(self._ctx, ptr(labels, u32p)))
+
+    def improve_labels(self, return_labels=False):
+        wcc = ctypes.c_double()
+        out = np.empty(self.n, np.uint32) if return_labels else None
+        check_hip(_lib.hip().scd_improve(self._ctx, ctypes.byref(wcc), ptr(out, u32p)))
+        return (wcc.value, out) if return_labels else wcc.value
+
+    def store_labels(self):
+        check_hip(_lib.hip().scd_store_best(self._ctx))
+
+    def labels_from_device(self):
+        out = np.empty(self.n, np.uint32)
+        check_hip(_lib.hip().scd_labels_from_device(self._ctx, ptr(out, u32p)))
+        return out
+
+    # -- the rest of the path ------------------------------------------------
+    def current_labels(self):
+        out = np.empty(self.n, np.uint32)
+        check_hip(_lib.hip().scd_current_labels(self._ctx, ptr(out, u32p)))
+        return out
+
+    def count_triangles(self, replace=False):
+        tri = np.empty(self.n, np.int32)
+        td = np.empty(self.n, np.int32)
+        check_hip(_lib.hip().scd_count_triangles(self._ctx, ptr(tri, i32p), ptr(td, i32p), 1 if replace else 0))
+        return tri, td
+
+    def initial_partition(self):
+        out = np.empty(self.n, np.uint32)
+        k = ctypes.c_uint32()
+        check_hip(_lib.hip().scd_initial_partition(self._ctx, ptr(out, u32p), ctypes.byref(k)))
+        return out, k.value
+
+    def compute_wcc(self, details=False):
+        wcc = ctypes.c_double()
+        if not details:
+            check_hip(_lib.hip().scd_compute_wcc(self._ctx, ctypes.byref(wcc), None, None, None))
+            return wcc.value
+        terms = np.empty(self.n, np.float64)
+        it = np.empty(self.n, np.int32)
+        itd = np.empty(self.n, np.int32)
+        check_hip(_lib.hip().scd_compute_wcc(self._ctx, ctypes.byref(wcc), ptr(terms, f64p), ptr(it, i32p),
+                                             ptr(itd, i32p)))
+        return wcc.value, terms, it, itd
+
+    def stats(self):
+        nc = ctypes.c_uint32()
+        s = np.empty(self.n, np.int32)
+        e = np.empty(self.n, np.int32)
+        i = np.empty(self.n, np.int32)
+        check_hip(_lib.hip().scd_get_stats(self._ctx, ctypes.byref(nc), ptr(s, i32p), ptr(e, i32p), ptr(i, i32p)))
+        k = nc.value
+        return s[:k].copy(), e[:k].copy(), i[:k].copy()
+
+    def neighbor_label_pairs(self):
+        """(sorted[2m], run_offsets[n+1], run_label[P], run_count[P])."""
+        E = 2 * self.graph.m
+        srt = np.empty(max(E, 1), np.uint32)
+        off = np.empty(self.n + 1, np.uint32)
+        rl = np.empty(max(E, 1), np.uint32)
+        rc = np.empty(max(E, 1), np.uint32)
+        P = ctypes.c_uint64()
+        check_hip(_lib.hip().scd_neighbor_label_pairs(self._ctx, ptr(srt, u32p), ptr(off, u32p), ptr(rl, u32p),
+                                                      ptr(rc, u32p), ctypes.byref(P)))
+        P = P.value
+        return srt[:E], off, rl[:P].copy(), rc[:P].copy()
+
+    def refine(self, initial_wcc, max_iters=1000):
+        """The loop of src/main-gpu.cpp:71-96.  Returns (best_wcc, iterations,
+        per-iteration WCC)."""
+        best = ctypes.c_double()
+        iters = ctypes.c_uint32()
+        hist = np.zeros(max_iters, np.float64)
+        check_hip(_lib.hip().scd_refine(self._ctx, initial_wcc, max_iters, ctypes.byref(best),
+                                        ctypes.byref(iters), ptr(hist, f64p)))
+        return best.value, iters.value, hist[:iters.value].copy()
+
+    def get_tuning(self):
+        t = _lib.ScdTuningC()
+        check_hip(_lib.hip().scd_get_tuning(self._ctx, ctypes.byref(t)))
+        return {"lds_tile": t.lds_tile, "merge_chunk": t.merge_chunk}
+
+    def set_tuning(self, lds_tile=None, merge_chunk=None):
+        t = _lib.ScdTuningC()
+        check_hip(_lib.hip().scd_get_tuning(self._ctx, ctypes.byref(t)))
+        if lds_tile is not None:
+            t.lds_tile = lds_tile
+        if merge_chunk is not None:
+            t.merge_chunk = merge_chunk
+        check_hip(_lib.hip().scd_set_tuning(self._ctx, ctypes.byref(t)))
+
+    def last_counters(self):
+        out = np.zeros(4, np.uint64)
+        check_hip(_lib.hip().scd_last_counters(self._ctx, ptr(out, u64p)))
+        return {"pairs": int(out[0]), "internal_entries": int(out[1]), "internal_deg_sq": int(out[2]),
+                "hubs": int(out[3])}
+
+    def profile(self):
+        """{kernel name: (launches, total ms)} from the HIP events."""
+        lib = _lib.hip()
+        res = {}
+        for i in range(lib.scd_prof_count(self._ctx)):
+            name = ctypes.c_char_p()
+            la = ctypes.c_uint64()
+            ms = ctypes.c_double()
+            check_hip(lib.scd_prof_get(self._ctx, i, ctypes.byref(name), ctypes.byref(la), ctypes.byref(ms)))
+            res[name.value.decode()] = (la.value, ms.value)
+        return res
+
+    def profile_reset(self):
+        check_hip(_lib.hip().scd_prof_reset(self._ctx))
+
+    def device_bytes(self):
+        return int(_lib.hip().scd_device_bytes(self._ctx))

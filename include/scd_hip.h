@@ -1,0 +1,199 @@
+/*
+ * scd_hip.h — C ABI of the MI355X SCD refinement engine (libscd_hip.so).
+ *
+ * This is the drop-in boundary for the reference's single-GPU path.  The
+ * reference binds its GPU runtime through six C++ functions with file-static
+ * device state (src/cuda.hpp:23-28, implemented in src/cuda.cu:55-369) and
+ * calls them from src/main-gpu.cpp:40-103.  Here the same steps take an
+ * explicit context handle so one process can hold several graphs (batched and
+ * served modes), and every call returns an int status instead of exiting:
+ *   SCD_OK (0) on success, a negative SCD_E* code (scd_host.h) on failure,
+ *   with a message for the calling thread in scd_last_error().
+ * The reference's exact C++ names and exit-on-error behaviour are provided by
+ * a thin shim over this ABI (scd_cuda_shim.hpp / libscd_cuda_shim.so).
+ *
+ * Types are plain C: uint32 vertex ids and labels, int32 triangle and
+ * community counters, fp64 WCC.  INVALID label = 0xFFFFFFFF (src/scd.hpp:23).
+ * A context owns all device memory it allocates and works on one HIP stream;
+ * every call below returns after the device work it issued has completed.
+ * Calls on one context must not overlap (the reference is single-threaded
+ * over global state, src/cuda.cu:29-52); different contexts are independent.
+ */
+#ifndef SCD_HIP_H
+#define SCD_HIP_H
+
+#include <stdint.h>
+
+#include "scd_host.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SCD_INVALID_LABEL 0xFFFFFFFFu
+
+typedef struct scd_ctx scd_ctx;
+
+/* Message of the last failing call on this thread. */
+const char *scd_last_error(void);
+
+/* Number of visible HIP devices (0 when none), or a negative code. */
+int scd_device_count(void);
+
+/*
+ * Replaces cuda_init(device_id, work_group_size, profile_flag)
+ * (src/cuda.hpp:24, src/cuda.cu:55-82).  An ordinal outside [0, count)
+ * fails with SCD_EDEVICE, as the reference returns false after listing the
+ * devices.  work_group_size is the reference's "group size" argument
+ * (src/main-gpu.cpp:38); it sets the workgroup of the per-vertex kernels and
+ * must be 64, 128, 256, 512 or 1024.  profile != 0 records a HIP event pair
+ * around every kernel launch (scd_prof_*).
+ */
+int scd_init(int device_id, int work_group_size, int profile, scd_ctx **out);
+void scd_destroy(scd_ctx *ctx);
+
+/*
+ * Replaces cuda_transfer_graph(g, alpha) (src/cuda.hpp:25, src/cuda.cu:84-148):
+ * copies the CSR (indices n+1, adj 2m, tri n, tri_deg n) to the device and
+ * sizes every buffer of the refinement loop.  clustering_coef is the graph's
+ * mean local clustering coefficient (Graph::clustering_coef,
+ * src/graph.cpp:57-68; scd_graph_clustering_coef computes it).  Adjacency
+ * lists must be sorted ascending without self loops or duplicates, which the
+ * binary format guarantees.  Replaces any graph uploaded before.  n == 0 is
+ * rejected (the reference would loop forever, src/main.cpp:77).
+ */
+int scd_upload(scd_ctx *ctx, uint32_t n, uint32_t m, const uint32_t *indices,
+               const uint32_t *adj, const int32_t *tri, const int32_t *tri_deg,
+               double clustering_coef);
+
+/*
+ * Per-vertex triangle statistics of the uploaded CSR on the device, with the
+ * convert tool's definition (src/convert.cpp:271-340): t(e) = number of
+ * triangles through edge e, tri[v] = sum of t over v's edges (2 x triangles
+ * at v), tri_deg[v] = number of v's edges with t > 0.  Either output may be
+ * NULL.  With replace != 0 the device copies of tri/tri_deg used by the WCC
+ * are overwritten with the computed ones.
+ */
+int scd_count_triangles(scd_ctx *ctx, int32_t *tri_out, int32_t *tri_deg_out, int replace);
+
+/*
+ * Clustering-coefficient-ordered initial partition on the device
+ * (create_initial_partition, src/scd.cpp:113-171): vertices ordered by
+ * (tri/(deg*(deg-1.0)), deg, id) descending; in that order an unlabelled
+ * vertex opens the next label and claims its unlabelled neighbours.  The
+ * labels become the device's current and best labels and the community
+ * statistics are collected, like create_initial_partition followed by
+ * cuda_labels_to_device.  Either output may be NULL.  Requires every degree
+ * >= 2 (otherwise the reference's sort key is NaN and its order undefined).
+ */
+int scd_initial_partition(scd_ctx *ctx, uint32_t *labels_out, uint32_t *num_labels_out);
+
+/*
+ * Replaces cuda_labels_to_device(labels) (src/cuda.hpp:26, src/cuda.cu:150-174):
+ * n labels, none INVALID; num_comms = max + 1; sets current and best labels
+ * and collects the community statistics.
+ */
+int scd_labels_to_device(scd_ctx *ctx, const uint32_t *labels);
+
+/*
+ * Replaces cuda_improve_labels(&wcc) (src/cuda.hpp:27, src/cuda.cu:176-361):
+ * one synchronous refinement iteration — best move for every vertex
+ * (src/scd.cpp:221-341), community statistics (src/scd.cpp:57-111) and the
+ * mean WCC (src/wcc.cpp:31-141) — with the CPU path's arithmetic.  Writes
+ * the WCC to *wcc; labels_out (n entries, may be NULL) receives the labels
+ * after the move.
+ */
+int scd_improve(scd_ctx *ctx, double *wcc, uint32_t *labels_out);
+
+/* Replaces cuda_store_labels() (src/cuda.cu:363-365): best := current. */
+int scd_store_best(scd_ctx *ctx);
+
+/* Replaces cuda_labels_from_device(labels) (src/cuda.cu:367-369): copies the
+ * best labels (n entries) to the host. */
+int scd_labels_from_device(scd_ctx *ctx, uint32_t *labels);
+
+/* The current (not best) labels, n entries. */
+int scd_current_labels(scd_ctx *ctx, uint32_t *labels);
+
+/*
+ * The refinement loop of src/main-gpu.cpp:71-96 (== src/main.cpp:57-84) run
+ * inside the library: improve; store best when wcc > best; stop when
+ * wcc < best * (1 + 0.01).  initial_wcc is the WCC of the current labels
+ * (scd_compute_wcc).  Outputs (any may be NULL): the final best WCC (the
+ * value the reference prints last), the iteration count, and the WCC of each
+ * iteration (wcc_hist holds max_iters entries).
+ */
+int scd_refine(scd_ctx *ctx, double initial_wcc, uint32_t max_iters, double *best_wcc,
+               uint32_t *iterations, double *wcc_hist);
+
+/*
+ * WCC of the current labels and statistics without moving any vertex
+ * (compute_wcc, src/wcc.cpp:31-141).  terms_out (n entries, may be NULL)
+ * receives each vertex's term; int_tri_out / int_tri_deg_out (may be NULL)
+ * its internal triangle count and internal triangle degree.
+ */
+int scd_compute_wcc(scd_ctx *ctx, double *wcc, double *terms_out, int32_t *int_tri_out,
+                    int32_t *int_tri_deg_out);
+
+/* Community statistics of the current labels (Partition::sizes, ext_edges,
+ * int_edges, src/scd.hpp:25-31), num_comms entries each; *num_comms may be
+ * NULL. Arrays must hold the graph's n entries. */
+int scd_get_stats(scd_ctx *ctx, uint32_t *num_comms, int32_t *sizes, int32_t *ext_edges,
+                  int32_t *int_edges);
+
+/*
+ * The segmented sort + reduce-by-key of the move step on its own
+ * (src/scd.cpp:255-278; the reference GPU's set_adj_labels,
+ * SegSortKeysFromIndices and ReduceByKey, src/cuda.cu:199-227): for the
+ * current labels, each vertex's neighbour labels sorted ascending (unsigned;
+ * INVALID last) into sorted_out (2m entries, may be NULL), and the runs of
+ * valid labels as (label, count) pairs, vertex-major, ascending label, with
+ * run_offsets (n + 1 entries).  run_label/run_count must hold 2m entries.
+ * *num_pairs receives the total number of runs.
+ */
+int scd_neighbor_label_pairs(scd_ctx *ctx, uint32_t *sorted_out, uint32_t *run_offsets,
+                             uint32_t *run_label, uint32_t *run_count, uint64_t *num_pairs);
+
+/*
+ * Kernel shape knobs (degree-bin thresholds and tile sizes).  Results never
+ * depend on them; the autotuner picks them per graph-size class.
+ *   lds_tile:   keys sorted in LDS per workgroup (1024..16384, power of two);
+ *               vertices with larger degree take the multi-tile hub path.
+ *   merge_chunk: outputs per workgroup in the hub merge passes (256..65536).
+ */
+typedef struct scd_tuning {
+    uint32_t lds_tile;
+    uint32_t merge_chunk;
+    uint32_t reserved[6];
+} scd_tuning;
+
+int scd_get_tuning(const scd_ctx *ctx, scd_tuning *out);
+int scd_set_tuning(scd_ctx *ctx, const scd_tuning *t);
+
+/*
+ * Side counters of the last scd_improve (for algorithmic-byte accounting):
+ * [0] P, distinct (vertex, valid neighbour label) pairs evaluated;
+ * [1] internal adjacency entries (same-label directed edges);
+ * [2] sum over vertices of internal degree squared;
+ * [3] vertices with degree above lds_tile (hub path).
+ */
+int scd_last_counters(scd_ctx *ctx, uint64_t out[4]);
+
+/*
+ * Profiling (scd_init with profile != 0): per kernel name, the number of
+ * launches and their summed duration from HIP events recorded on the
+ * context's stream around each launch.
+ */
+int scd_prof_count(scd_ctx *ctx);
+int scd_prof_get(scd_ctx *ctx, int index, const char **name, uint64_t *launches,
+                 double *total_ms);
+int scd_prof_reset(scd_ctx *ctx);
+
+/* Device bytes currently allocated by the context. */
+uint64_t scd_device_bytes(const scd_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

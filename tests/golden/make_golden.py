@@ -1,0 +1,165 @@
+"""Writes the golden fixtures of tests/golden from the reference's own CPU
+code, built from its sources by oracle/Makefile into oracle/_ref:
+
+  _ref/convert            the reference convert tool (src/convert.cpp)
+  _ref/ref_dump           src/{graph,scd,wcc,common}.cpp driven like
+                          src/main.cpp, dumping labels, statistics and WCC of
+                          every iteration
+  _ref/ref_improve_table  src/scd.cpp's calculate_improvement on a table of
+                          arguments
+
+Outputs:
+  manifest.json      per graph: generator recipe, SHA-256 of the reference
+                     convert output, n, m, cc / WCC bit patterns, iteration
+                     count, printed community count, SHA-256 of the labels and
+                     statistics of every iteration
+  full_<name>.npz    the complete label and statistics arrays of small graphs
+  improve_table.npz  calculate_improvement arguments and result bits,
+                     including rows whose int32 products wrap
+
+Run in the build container (it needs the reference build):
+  make -C oracle ref && python tests/golden/make_golden.py
+"""
+import hashlib
+import json
+import os
+import struct
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "het-scd_amd"))
+sys.path.insert(0, os.path.dirname(HERE))
+
+import scd_amd  # noqa: E402
+from golden_graphs import GRAPHS, make_edges  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref")
+FULL_MAX_N = 20000
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def hexf(x):
+    return struct.pack("<d", x).hex()
+
+
+def ref_convert(edges, tmp):
+    txt = os.path.join(tmp, "g.txt")
+    out = os.path.join(tmp, "g.bin")
+    edges.write_text(txt)
+    subprocess.run([os.path.join(REF, "convert"), txt, out], check=True, stdout=subprocess.DEVNULL)
+    with open(out, "rb") as f:
+        return f.read(), out
+
+
+def ref_dump(path, tmp):
+    out = os.path.join(tmp, "dump.bin")
+    # one thread: the reference's OpenMP sums (clustering coefficient,
+    # src/graph.cpp:59; WCC, src/wcc.cpp:49) then run in vertex order, the
+    # canonical order whose bits the oracle and the engine reproduce
+    subprocess.run([os.path.join(REF, "ref_dump"), path, out], check=True, stdout=subprocess.DEVNULL,
+                   env=dict(os.environ, OMP_NUM_THREADS="1"))
+    b = open(out, "rb").read()
+    n, nc, iters, printed = struct.unpack_from("<4I", b, 0)
+    cc, init, final = struct.unpack_from("<3d", b, 16)
+    pos = 40
+    labels, stats, wcc = [], [], []
+    for it in range(iters + 1):
+        labels.append(np.frombuffer(b, np.uint32, n, pos))
+        pos += 4 * n
+        stats.append(np.frombuffer(b, np.int32, 3 * nc, pos).reshape(3, nc))
+        pos += 12 * nc
+        if it > 0:
+            wcc.append(struct.unpack_from("<d", b, pos)[0])
+            pos += 8
+    assert pos == len(b)
+    return dict(n=n, num_comms=nc, iterations=iters, printed=printed, cc=cc, initial_wcc=init, final_wcc=final,
+                labels=labels, stats=stats, wcc=wcc)
+
+
+def improve_table():
+    rng = np.random.default_rng(20261203)
+    rows = []
+    for _ in range(4000):
+        r = int(rng.integers(0, 400))
+        d_in = int(rng.integers(0, 64))
+        d_out = int(rng.integers(0, 2000))
+        c_out = int(rng.integers(0, 5000))
+        p_in = float(rng.random()) if rng.random() < 0.9 else 0.0
+        p_ext = float(rng.random() * 0.6)
+        rows.append((r, d_in, d_out, c_out, p_in, p_ext))
+    # hub-sized arguments whose int32 products wrap (d_out*(d_out-1) etc.)
+    for _ in range(1000):
+        r = int(rng.integers(0, 70000))
+        d_in = int(rng.integers(0, 70000))
+        d_out = int(rng.integers(40000, 400000))
+        c_out = int(rng.integers(0, 2**31 - 1))
+        rows.append((r, d_in, d_out, c_out, float(rng.random()), float(rng.random())))
+    # edge cases: empty and singleton communities, zero probabilities
+    for r in (0, 1, 2, 3):
+        for d_in in (0, 1, 2):
+            for d_out in (0, 1, 5):
+                rows.append((r, d_in, d_out, 3, 0.0, 0.25))
+                rows.append((r, d_in, d_out, 0, 1.0, 0.0))
+    # the table reads the doubles as u64 bit patterns in hex
+    inp = "".join("%d %d %d %d %x %x\n" % (r, a, b, c, struct.unpack("<Q", struct.pack("<d", pi))[0],
+                                          struct.unpack("<Q", struct.pack("<d", pe))[0])
+                  for r, a, b, c, pi, pe in rows)
+    out = subprocess.run([os.path.join(REF, "ref_improve_table")], input=inp, check=True, capture_output=True,
+                         text=True).stdout.split()
+    bits = np.array([int(x, 16) for x in out], dtype=np.uint64)
+    np.savez_compressed(os.path.join(HERE, "improve_table.npz"),
+                        r=np.array([x[0] for x in rows], np.int32),
+                        d_in=np.array([x[1] for x in rows], np.int32),
+                        d_out=np.array([x[2] for x in rows], np.int32),
+                        c_out=np.array([x[3] for x in rows], np.int32),
+                        p_in=np.array([x[4] for x in rows], np.float64),
+                        p_ext=np.array([x[5] for x in rows], np.float64),
+                        bits=bits)
+    return len(rows)
+
+
+def main(names=None):
+    man_path = os.path.join(HERE, "manifest.json")
+    manifest = json.load(open(man_path)) if os.path.exists(man_path) else {"graphs": {}}
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, recipe in GRAPHS.items():
+            if names and name not in names:
+                continue
+            print(f"[golden] {name}", flush=True)
+            edges = make_edges(recipe)
+            ref_bytes, path = ref_convert(edges, tmp)
+            ours = edges.convert()
+            assert ours.to_bytes() == ref_bytes, f"{name}: scd_convert differs from the reference convert"
+            d = ref_dump(path, tmp)
+            ent = {
+                "recipe": recipe,
+                "convert_sha256": hashlib.sha256(ref_bytes).hexdigest(),
+                "n": d["n"], "m": ours.m, "num_comms": d["num_comms"], "iterations": d["iterations"],
+                "printed_num_communities": d["printed"],
+                "cc_hex": hexf(d["cc"]), "initial_wcc_hex": hexf(d["initial_wcc"]),
+                "final_wcc_hex": hexf(d["final_wcc"]), "wcc_hex": [hexf(w) for w in d["wcc"]],
+                "labels_sha256": [sha(l) for l in d["labels"]],
+                "stats_sha256": [sha(s) for s in d["stats"]],
+                "tri_sha256": sha(ours.tri), "tri_deg_sha256": sha(ours.tri_deg),
+                "full": d["n"] <= FULL_MAX_N,
+            }
+            if ent["full"]:
+                np.savez_compressed(os.path.join(HERE, f"full_{name}.npz"), labels=np.stack(d["labels"]),
+                                    stats=np.stack(d["stats"]))
+            manifest["graphs"][name] = ent
+        if not names or "improve_table" in names:
+            manifest["improve_table_rows"] = improve_table()
+    with open(man_path, "w") as f:
+        json.dump(manifest, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

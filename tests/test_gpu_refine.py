@@ -1,0 +1,177 @@
+"""HIP engine parity (MI355X): the refinement path through the C ABI against
+the reference fixtures (tests/golden, written by the reference's own CPU
+code) and the CPU oracle, bit-exact for labels, statistics, sort/reduce-by-key
+runs and per-vertex WCC terms; mean WCC within 1e-9 relative."""
+import hashlib
+import struct
+
+import numpy as np
+import pytest
+
+import oracle_bind as orc
+import scd_amd
+from conftest import full_fixture
+from golden_graphs import LARGE, MEDIUM, SMALL, graph
+
+pytestmark = pytest.mark.gpu
+
+INVALID = 0xFFFFFFFF
+WCC_RTOL = 1e-9
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def unhex(h):
+    return struct.unpack("<d", bytes.fromhex(h))[0]
+
+
+def initial_labels(name, ent, g):
+    full = full_fixture(name)
+    if full is not None:
+        return np.ascontiguousarray(full["labels"][0])
+    lab, _ = orc.initial_partition(g)
+    assert sha(lab) == ent["labels_sha256"][0]
+    return lab
+
+
+def stats_sha(eng):
+    s, e, i = eng.stats()
+    return sha(np.stack([s, e, i]))
+
+
+def run_and_compare(eng, name, ent):
+    g = graph(name)
+    print(f"[gpu] refine {name} n={g.n} m={g.m}", flush=True)
+    eng.transfer_graph(g)
+    lab0 = initial_labels(name, ent, g)
+    eng.labels_to_device(lab0)
+    assert stats_sha(eng) == ent["stats_sha256"][0]
+    best = eng.compute_wcc()
+    assert best == pytest.approx(unhex(ent["initial_wcc_hex"]), rel=WCC_RTOL, abs=0)
+    it = 0
+    while True:
+        it += 1
+        wcc, lab = eng.improve_labels(return_labels=True)
+        assert sha(lab) == ent["labels_sha256"][it], f"{name}: labels differ at iteration {it}"
+        assert stats_sha(eng) == ent["stats_sha256"][it], f"{name}: statistics differ at iteration {it}"
+        assert wcc == pytest.approx(unhex(ent["wcc_hex"][it - 1]), rel=WCC_RTOL, abs=0)
+        if wcc > best:
+            eng.store_labels()
+        if wcc < best * 1.01:
+            break
+        best = wcc
+    assert it == ent["iterations"]
+    assert best == pytest.approx(unhex(ent["final_wcc_hex"]), rel=WCC_RTOL, abs=0)
+    assert scd_amd.compress_count(eng.labels_from_device()) == ent["printed_num_communities"]
+
+
+@pytest.mark.parametrize("name", SMALL + MEDIUM)
+def test_refine_matches_reference(engine, manifest, name):
+    run_and_compare(engine, name, manifest["graphs"][name])
+
+
+@pytest.mark.parametrize("name", LARGE)
+def test_refine_matches_reference_large(engine, manifest, name):
+    run_and_compare(engine, name, manifest["graphs"][name])
+
+
+def test_refine_loop_in_library(engine, manifest):
+    """scd_refine (the loop inside the library) == the host-driven loop."""
+    name = "pp4x16_s4"
+    ent = manifest["graphs"][name]
+    g = graph(name)
+    engine.transfer_graph(g)
+    engine.labels_to_device(initial_labels(name, ent, g))
+    w0 = engine.compute_wcc()
+    best, iters, hist = engine.refine(w0)
+    assert iters == ent["iterations"]
+    assert best == pytest.approx(unhex(ent["final_wcc_hex"]), rel=WCC_RTOL, abs=0)
+    for w, h in zip(hist, ent["wcc_hex"]):
+        assert w == pytest.approx(unhex(h), rel=WCC_RTOL, abs=0)
+
+
+def oracle_step(g, lab, k, cc):
+    s, e, i = orc.collect_stats(g, lab, k)
+    new = orc.improve(g, lab, s, e, i, cc)
+    s2, e2, i2 = orc.collect_stats(g, new, k)
+    w, terms, it, itd = orc.compute_wcc(g, new, s2)
+    return new, (s2, e2, i2), w, terms, it, itd
+
+
+@pytest.mark.parametrize("name,k", [("pp8x40_s5", 7), ("rmat12", 40), ("rmat14", 900), ("c1", 3000),
+                                    ("rmat16", 20000)])
+def test_steps_from_random_labels(engine, name, k):
+    """Two steps from a random partition: the first move sends many vertices
+    to INVALID, so the second step exercises INVALID neighbours and
+    INVALID-labelled vertices.  Per step: labels, statistics, the sorted
+    neighbour labels and reduce-by-key runs, per-vertex WCC terms and
+    internal triangle counts bit-exact; mean within 1e-9."""
+    g = graph(name)
+    print(f"[gpu] random-label steps {name} k={k}", flush=True)
+    rng = np.random.default_rng(k)
+    lab = rng.integers(0, k, g.n).astype(np.uint32)
+    lab[0] = k - 1
+    engine.transfer_graph(g)
+    engine.labels_to_device(lab)
+    cc = g.clustering_coef
+    assert cc == orc.clustering_coef(g)
+    for step in range(2):
+        srt, off, rl, rc = engine.neighbor_label_pairs()
+        osrt, ooff, orl, orc_ = orc.neighbor_label_pairs(g, lab)
+        assert np.array_equal(srt, osrt), f"sorted labels differ (step {step})"
+        assert np.array_equal(off, ooff) and np.array_equal(rl, orl) and np.array_equal(rc, orc_)
+        new, st, w, terms, it, itd = oracle_step(g, lab, k, cc)
+        wcc, got = engine.improve_labels(return_labels=True)
+        assert np.array_equal(got, new), f"labels differ at step {step}: {np.flatnonzero(got != new)[:10]}"
+        s, e, i = engine.stats()
+        assert np.array_equal(s, st[0]) and np.array_equal(e, st[1]) and np.array_equal(i, st[2])
+        w2, t2, it2, itd2 = engine.compute_wcc(details=True)
+        assert np.array_equal(it2, it) and np.array_equal(itd2, itd)
+        assert np.array_equal(t2.view(np.uint64), terms.view(np.uint64))
+        assert wcc == w2
+        assert wcc == pytest.approx(w, rel=WCC_RTOL, abs=0)
+        if step == 0:
+            assert (new == INVALID).sum() > 0
+        lab = new
+
+
+def test_tuning_invariance(engine, manifest):
+    """Labels do not depend on the LDS tile or merge chunk (hub path on vs
+    off)."""
+    name = "rmat16"
+    ent = manifest["graphs"][name]
+    g = graph(name)
+    assert g.degree().max() > 2048
+    outs = []
+    for tile, chunk in ((2048, 256), (2048, 4096), (4096, 1024), (16384, 2048)):
+        engine.transfer_graph(g)
+        engine.set_tuning(lds_tile=tile, merge_chunk=chunk)
+        engine.labels_to_device(initial_labels(name, ent, g))
+        wcc, lab = engine.improve_labels(return_labels=True)
+        hubs = engine.last_counters()["hubs"]
+        assert (hubs > 0) == (tile < g.degree().max())
+        outs.append((wcc, lab))
+    engine.set_tuning(lds_tile=4096, merge_chunk=2048)
+    for wcc, lab in outs:
+        assert wcc == outs[0][0]
+        assert sha(lab) == ent["labels_sha256"][1]
+
+
+def test_error_paths(engine):
+    g = graph("bowtie")
+    engine.transfer_graph(g)
+    with pytest.raises(scd_amd.ScdError) as ei:
+        engine.improve_labels()
+    assert ei.value.code == -6  # no labels yet
+    with pytest.raises(scd_amd.ScdError) as ei:
+        engine.labels_to_device(np.array([0, 1, INVALID, 0, 0], np.uint32))
+    assert ei.value.code == -1
+    empty = scd_amd.Graph(0, 0, np.zeros(1, np.uint32), np.zeros(0, np.uint32), np.zeros(0, np.int32),
+                          np.zeros(0, np.int32))
+    with pytest.raises(scd_amd.ScdError) as ei:
+        engine.transfer_graph(empty, clustering_coef=0.0)
+    assert ei.value.code == -1
+    with pytest.raises(scd_amd.ScdError):
+        engine.set_tuning(lds_tile=3000)
