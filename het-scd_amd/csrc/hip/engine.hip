@@ -25,6 +25,8 @@
 #include "move_kernels.hpp"
 #include "scd_hip.h"
 #include "stats_kernels.hpp"
+#include "tc_kernels.hpp"
+#include "ip_kernels.hpp"
 
 using namespace scd_dev;
 
@@ -111,6 +113,7 @@ struct scd_ctx {
     std::vector<DevBuf> hub_bufs;
 
     scd_tuning tune = {4096, 2048, {0, 0, 0, 0, 0, 0}};
+    uint32_t last_ip_rounds = 0;
 
     // profiling
     std::vector<ProfRec> prof;
@@ -793,17 +796,170 @@ extern "C" int scd_prof_reset(scd_ctx *c) {
 
 extern "C" uint64_t scd_device_bytes(const scd_ctx *c) { return c ? c->dev_bytes : 0; }
 
-// Placeholders until the device triangle counting and initial partition land.
-extern "C" int scd_count_triangles(scd_ctx *c, int32_t *tri_out, int32_t *tri_deg_out, int replace) {
-    (void)tri_out; (void)tri_deg_out; (void)replace;
-    int rc = need_graph(c, "scd_count_triangles");
-    if (rc) return rc;
-    return fail(SCD_ESTATE, "scd_count_triangles: not available in this build");
+// ---------------------------------------------------------------------------
+// Triangle statistics (src/convert.cpp:271-340) on the device.
+// ---------------------------------------------------------------------------
+static int triangles_dev(scd_ctx *c, int32_t *d_tri, uint32_t *d_td, std::vector<DevBuf> &tmp) {
+    const uint32_t n = c->n;
+    const uint64_t E = c->E;
+    TcArgs a;
+    a.indices = c->indices;
+    a.adj = c->adj;
+    a.n = n;
+    int rc;
+    uint32_t *counts = nullptr, *dplus = nullptr, *T = nullptr, *light = nullptr, *heavy = nullptr,
+             *heavy_chunks = nullptr, *gpre = nullptr;
+    uint8_t *flag = nullptr;
+    if ((rc = dalloc(c, &dplus, n, &tmp)) || (rc = dalloc(c, &T, n, &tmp)) || (rc = dalloc(c, &flag, E, &tmp)) ||
+        (rc = dalloc(c, &light, n, &tmp)) || (rc = dalloc(c, &heavy, n, &tmp)) ||
+        (rc = dalloc(c, &heavy_chunks, n, &tmp)) || (rc = dalloc(c, &counts, 4, &tmp)))
+        return rc;
+    a.dplus = dplus;
+    a.T = T;
+    a.flag = flag;
+    a.light = light;
+    a.heavy = heavy;
+    a.heavy_chunks = heavy_chunks;
+    a.td = d_td;
+    a.counts = counts;
+    a.gpre = nullptr;
+    LAUNCH(c, "tc_clear", cdiv(E, 256), 256, k_fill_u8, a.flag, E);
+    LAUNCH(c, "tc_clear", 1, 64, k_fill_u32, counts, 4u);
+    LAUNCH(c, "tc_dplus", cdiv(n, 256), 256, k_tc_dplus, a);
+    if ((rc = check_launch("triangles"))) return rc;
+    // out-lists longer than the LDS stage keep their probe prefix in global
+    // memory; find out whether any exists before sizing that buffer
+    uint32_t max_dp = 0;
+    {
+        std::vector<uint32_t> h(n);
+        HIP_TRY(hipMemcpyAsync(h.data(), a.dplus, 4ull * n, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        for (uint32_t v = 0; v < n; v++) max_dp = std::max(max_dp, h[v]);
+    }
+    if (max_dp > TC_CAP) {
+        if ((rc = dalloc(c, &gpre, E, &tmp))) return rc;
+        a.gpre = gpre;
+    }
+    LAUNCH(c, "tc_classify", cdiv(n, 256), 256, k_tc_classify, a);
+    uint32_t cnt[2];
+    HIP_TRY(hipMemcpyAsync(cnt, counts, 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if ((rc = check_launch("triangles"))) return rc;
+    const uint32_t nlight = cnt[0], nheavy = cnt[1];
+    uint32_t nitems = 0, *item_u = nullptr, *item_c = nullptr, *off = nullptr;
+    if (nheavy) {
+        if ((rc = dalloc(c, &off, nheavy, &tmp))) return rc;
+        if ((rc = scan_u32(c, a.heavy_chunks, off, nheavy, &nitems, tmp))) return rc;
+        if ((rc = dalloc(c, &item_u, nitems, &tmp)) || (rc = dalloc(c, &item_c, nitems, &tmp))) return rc;
+        LAUNCH(c, "tc_items", cdiv(nheavy, 256), 256, k_tc_items, a.heavy, a.heavy_chunks, off, nheavy, item_u,
+               item_c);
+    }
+    LAUNCH(c, "tc_light", cdiv(nlight, 4), 256, k_tc_light, a);
+    LAUNCH(c, "tc_heavy", nitems, 256, k_tc_heavy, a, item_u, item_c);
+    LAUNCH(c, "tc_final", cdiv(n, 4), 256, k_tc_final, a, d_tri);
+    return check_launch("triangles");
 }
 
+extern "C" int scd_count_triangles(scd_ctx *c, int32_t *tri_out, int32_t *tri_deg_out, int replace) {
+    int rc = need_graph(c, "scd_count_triangles");
+    if (rc) return rc;
+    std::vector<DevBuf> tmp;
+    int32_t *d_tri = nullptr;
+    uint32_t *d_td = nullptr;
+    if ((rc = dalloc(c, &d_tri, c->n, &tmp)) || (rc = dalloc(c, &d_td, c->n, &tmp)) ||
+        (rc = triangles_dev(c, d_tri, d_td, tmp))) {
+        free_list(c, tmp);
+        return rc;
+    }
+    hipError_t e = hipSuccess;
+    if (tri_out) e = hipMemcpyAsync(tri_out, d_tri, 4ull * c->n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && tri_deg_out)
+        e = hipMemcpyAsync(tri_deg_out, d_td, 4ull * c->n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && replace) {
+        e = hipMemcpyAsync(c->tri, d_tri, 4ull * c->n, hipMemcpyDeviceToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(c->tri_deg, d_td, 4ull * c->n, hipMemcpyDeviceToDevice, c->stream);
+    }
+    if (e != hipSuccess) rc = fail(SCD_EDEVICE, "scd_count_triangles: copy: %s", hipGetErrorString(e));
+    if (!rc) rc = finish(c);
+    free_list(c, tmp);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Initial partition (src/scd.cpp:113-171) on the device.
+// ---------------------------------------------------------------------------
 extern "C" int scd_initial_partition(scd_ctx *c, uint32_t *labels_out, uint32_t *num_labels_out) {
-    (void)labels_out; (void)num_labels_out;
     int rc = need_graph(c, "scd_initial_partition");
     if (rc) return rc;
-    return fail(SCD_ESTATE, "scd_initial_partition: not available in this build");
+    const uint32_t n = c->n;
+    std::vector<DevBuf> tmp;
+    uint64_t *khi[2] = {nullptr, nullptr};
+    uint32_t *klo[2] = {nullptr, nullptr}, *val[2] = {nullptr, nullptr};
+    uint32_t *flags = nullptr, *hist = nullptr, *hoff = nullptr, *rank = nullptr, *sf = nullptr;
+    unsigned long long *bits = nullptr;
+    uint8_t *state = nullptr;
+    const uint32_t ntiles = cdiv(n, RS_TILE);
+    auto cleanup = [&](int r) {
+        free_list(c, tmp);
+        return r;
+    };
+    if ((rc = dalloc(c, &khi[0], n, &tmp)) || (rc = dalloc(c, &khi[1], n, &tmp)) ||
+        (rc = dalloc(c, &klo[0], n, &tmp)) || (rc = dalloc(c, &klo[1], n, &tmp)) ||
+        (rc = dalloc(c, &val[0], n, &tmp)) || (rc = dalloc(c, &val[1], n, &tmp)) ||
+        (rc = dalloc(c, &flags, 4, &tmp)) || (rc = dalloc(c, &bits, 4, &tmp)) ||
+        (rc = dalloc(c, &hist, 256ull * ntiles, &tmp)) || (rc = dalloc(c, &hoff, 256ull * ntiles, &tmp)) ||
+        (rc = dalloc(c, &rank, n, &tmp)) || (rc = dalloc(c, &sf, n, &tmp)) || (rc = dalloc(c, &state, n, &tmp)))
+        return cleanup(rc);
+    LAUNCH(c, "ip_clear", 1, 64, k_fill_u32, flags, 4u);
+    LAUNCH(c, "ip_clear", 1, 64, k_ip_bits_init, bits);
+    LAUNCH(c, "ip_keys", cdiv(n, 256), 256, k_ip_keys, c->indices, c->tri, n, khi[0], klo[0], val[0], flags, bits);
+    uint32_t hflags[4];
+    unsigned long long hbits[4];
+    if ((rc = check_launch("initial partition"))) return cleanup(rc);
+    if (hipMemcpyAsync(hflags, flags, 16, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipMemcpyAsync(hbits, bits, 32, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return cleanup(fail(SCD_EDEVICE, "scd_initial_partition: readback failed"));
+    if (hflags[0])
+        return cleanup(fail(SCD_EINVAL, "scd_initial_partition: a vertex of degree < 2 makes the reference's "
+                                        "order key NaN (src/scd.cpp:130); convert never produces one"));
+    int cur = 0;
+    for (int pass = 0; pass < 12; pass++) {
+        const uint64_t x = pass < 4 ? (hbits[2] ^ hbits[3]) >> (8 * pass) : (hbits[0] ^ hbits[1]) >> (8 * (pass - 4));
+        if ((x & 0xFF) == 0) continue;  // digit equal in every key
+        LAUNCH(c, "ip_sort_hist", ntiles, 256, k_rs_hist, khi[cur], klo[cur], n, pass, ntiles, hist);
+        if ((rc = check_launch("radix sort"))) return cleanup(rc);
+        uint32_t total = 0;
+        if ((rc = scan_u32(c, hist, hoff, 256ull * ntiles, &total, tmp))) return cleanup(rc);
+        LAUNCH(c, "ip_sort_scatter", ntiles, 256, k_rs_scatter, khi[cur], klo[cur], val[cur], n, pass, ntiles, hoff,
+               khi[1 - cur], klo[1 - cur], val[1 - cur]);
+        cur = 1 - cur;
+    }
+    LAUNCH(c, "ip_rank", cdiv(n, 256), 256, k_ip_rank, val[cur], n, rank, state);
+    uint32_t rounds = 0;
+    while (true) {
+        rounds++;
+        LAUNCH(c, "ip_clear", 1, 64, k_fill_u32, flags, 4u);
+        LAUNCH(c, "ip_round", cdiv(n, 4), 256, k_ip_round, c->indices, c->adj, rank, state, n, flags);
+        if ((rc = check_launch("initial partition"))) return cleanup(rc);
+        if (hipMemcpyAsync(hflags, flags, 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            return cleanup(fail(SCD_EDEVICE, "scd_initial_partition: readback failed"));
+        if (hflags[0] == 0) break;
+        if (rounds > n + 1) return cleanup(fail(SCD_EDEVICE, "scd_initial_partition: no progress"));
+    }
+    LAUNCH(c, "ip_seedflags", cdiv(n, 256), 256, k_ip_seedflags, val[cur], state, n, sf);
+    uint32_t k = 0;
+    if ((rc = scan_u32(c, sf, sf, n, &k, tmp))) return cleanup(rc);
+    LAUNCH(c, "ip_labels", cdiv(n, 4), 256, k_ip_labels, c->indices, c->adj, rank, state, sf, n, c->cur);
+    if ((rc = check_launch("initial partition"))) return cleanup(rc);
+    HIP_TRY(hipMemcpyAsync(c->best, c->cur, 4ull * n, hipMemcpyDeviceToDevice, c->stream));
+    c->num_comms = k;
+    c->last_ip_rounds = rounds;
+    if ((rc = clear_counters(c)) || (rc = phase_stats(c))) return cleanup(rc);
+    if (labels_out) HIP_TRY(hipMemcpyAsync(labels_out, c->cur, 4ull * n, hipMemcpyDeviceToHost, c->stream));
+    if ((rc = finish(c))) return cleanup(rc);
+    c->have_labels = true;
+    if (num_labels_out) *num_labels_out = k;
+    return cleanup(SCD_OK);
 }

@@ -1,0 +1,227 @@
+// Clustering-coefficient-ordered initial partition on the device
+// (create_initial_partition, src/scd.cpp:113-171).
+//
+// The reference sorts (cc = tri/(deg*(deg-1.0)), deg, id) descending and, in
+// that order, lets every unlabelled vertex open the next label and claim its
+// unlabelled neighbours.  A vertex is unlabelled when its turn comes exactly
+// when no earlier vertex adjacent to it opened a label, so the openers
+// ("seeds") are the lexicographically-first maximal independent set of the
+// order, a seed's label is its rank among the seeds, and every other vertex
+// takes the label of its earliest seed neighbour.  The device computes:
+//   1. sort keys and a stable LSD radix sort (8-bit digits) of
+//      (cc desc, deg desc) over ids listed in descending order, which breaks
+//      ties by id descending -> rank[v];
+//   2. seed / non-seed decisions in rounds: undecided v becomes a non-seed
+//      when an earlier neighbour is a seed and a seed when every earlier
+//      neighbour is a non-seed (decisions are final, so a round may read
+//      decisions made in the same round);
+//   3. seed ranks by an exclusive scan over the order, then labels.
+#ifndef SCD_IP_KERNELS_HPP
+#define SCD_IP_KERNELS_HPP
+
+#include "dev.hpp"
+
+namespace scd_dev {
+
+constexpr uint32_t RS_TILE = 4096;  // elements per radix-sort block (256 x 16)
+
+// Maps a double to a uint64 whose unsigned order is the double's order
+// (non-NaN input).
+__device__ __forceinline__ uint64_t orderable(double x) {
+    const uint64_t b = (uint64_t)__double_as_longlong(x);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+// Keys for an ascending sort that yields (cc, deg) descending; values = ids
+// in descending order so the stable sort breaks ties by id descending.
+// flags[0] is set when a degree below 2 makes the reference's key NaN.
+// bits[0..3] = AND / OR over all keys of the high and low words: digits
+// that are equal in every key need no pass.
+__global__ __launch_bounds__(256) void k_ip_keys(const uint32_t *__restrict__ indices, const int32_t *__restrict__ tri,
+                                                 uint32_t n, uint64_t *__restrict__ khi, uint32_t *__restrict__ klo,
+                                                 uint32_t *__restrict__ val, uint32_t *__restrict__ flags,
+                                                 unsigned long long *__restrict__ bits) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t h_and = ~0ull, h_or = 0, l_and = ~0ull, l_or = 0;
+    if (i < n) {
+        const uint32_t p = n - 1 - i;
+        const uint32_t deg = indices[p + 1] - indices[p];
+        if (deg < 2) atomicOr(flags, 1u);
+        const double cc = (double)tri[p] / ((double)deg * ((double)deg - 1.0));
+        const uint64_t h = ~orderable(cc);
+        const uint32_t l = ~(uint32_t)(int)deg;  // the reference compares deg as int
+        khi[i] = h;
+        klo[i] = l;
+        val[i] = p;
+        h_and = h_or = h;
+        l_and = l_or = l;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        h_and &= (uint64_t)__shfl_xor((unsigned long long)h_and, off, SCD_WAVE);
+        h_or |= (uint64_t)__shfl_xor((unsigned long long)h_or, off, SCD_WAVE);
+        l_and &= (uint64_t)__shfl_xor((unsigned long long)l_and, off, SCD_WAVE);
+        l_or |= (uint64_t)__shfl_xor((unsigned long long)l_or, off, SCD_WAVE);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAnd(&bits[0], (unsigned long long)h_and);
+        atomicOr(&bits[1], (unsigned long long)h_or);
+        atomicAnd(&bits[2], (unsigned long long)l_and);
+        atomicOr(&bits[3], (unsigned long long)l_or);
+    }
+}
+
+__global__ void k_ip_bits_init(unsigned long long *bits) {
+    if (threadIdx.x < 4) bits[threadIdx.x] = (threadIdx.x & 1) ? 0ull : ~0ull;
+}
+
+__device__ __forceinline__ uint32_t rs_digit(uint64_t hi, uint32_t lo, int pass) {
+    // passes 0..3: the low word, 4..11: the high word, least significant first
+    return pass < 4 ? (lo >> (8 * pass)) & 0xFFu : (uint32_t)(hi >> (8 * (pass - 4))) & 0xFFu;
+}
+
+// Per-tile digit histogram, digit-major: hist[d * ntiles + tile].
+__global__ __launch_bounds__(256) void k_rs_hist(const uint64_t *__restrict__ khi, const uint32_t *__restrict__ klo,
+                                                 uint32_t n, int pass, uint32_t ntiles, uint32_t *__restrict__ hist) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+    for (uint32_t k = 0; k < RS_TILE / 256; k++) {
+        const uint64_t i = base + k * 256 + threadIdx.x;
+        if (i < n) atomicAdd(&h[rs_digit(khi[i], klo[i], pass)], 1u);
+    }
+    __syncthreads();
+    hist[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter: elements of a tile are placed in index order within each
+// digit.  off[d * ntiles + tile] = exclusive scan of the histogram.
+__global__ __launch_bounds__(256) void k_rs_scatter(const uint64_t *__restrict__ khi, const uint32_t *__restrict__ klo,
+                                                    const uint32_t *__restrict__ val, uint32_t n, int pass,
+                                                    uint32_t ntiles, const uint32_t *__restrict__ off,
+                                                    uint64_t *__restrict__ ohi, uint32_t *__restrict__ olo,
+                                                    uint32_t *__restrict__ oval) {
+    __shared__ uint32_t s_base[256];   // next output position per digit
+    __shared__ uint32_t s_wc[4][256];  // per-wave digit counts of the current round
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    s_base[threadIdx.x] = off[(uint64_t)threadIdx.x * ntiles + blockIdx.x];
+    const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (uint32_t k = 0; k < RS_TILE / 256; k++) {
+        for (int q = 0; q < 4; q++) s_wc[q][threadIdx.x] = 0;
+        __syncthreads();
+        const uint64_t i = base + k * 256 + threadIdx.x;
+        const bool act = i < n;
+        uint64_t hi = 0;
+        uint32_t lo = 0, v = 0, d = 0;
+        if (act) {
+            hi = khi[i];
+            lo = klo[i];
+            v = val[i];
+            d = rs_digit(hi, lo, pass);
+        }
+        // lanes of this wave holding the same digit
+        uint64_t same = __ballot(act);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const uint64_t m = __ballot((d >> b) & 1u);
+            same &= ((d >> b) & 1u) ? m : ~m;
+        }
+        const uint32_t rank_in_wave = (uint32_t)__builtin_popcountll(same & lt);
+        if (act && rank_in_wave == 0) s_wc[w][d] = (uint32_t)__builtin_popcountll(same);
+        __syncthreads();
+        if (act) {
+            uint32_t before = 0;
+            for (int q = 0; q < w; q++) before += s_wc[q][d];
+            const uint32_t pos = s_base[d] + before + rank_in_wave;
+            ohi[pos] = hi;
+            olo[pos] = lo;
+            oval[pos] = v;
+        }
+        __syncthreads();
+        s_base[threadIdx.x] += s_wc[0][threadIdx.x] + s_wc[1][threadIdx.x] + s_wc[2][threadIdx.x] +
+                               s_wc[3][threadIdx.x];
+        __syncthreads();
+    }
+}
+
+__global__ void k_ip_rank(const uint32_t *__restrict__ val, uint32_t n, uint32_t *__restrict__ rank,
+                          uint8_t *__restrict__ state) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    rank[val[i]] = i;
+    state[val[i]] = 0;  // undecided
+}
+
+// One decision round; one wave per undecided vertex.  state: 0 undecided,
+// 1 seed, 2 non-seed.  counts[0] += vertices still undecided.
+__global__ __launch_bounds__(256) void k_ip_round(const uint32_t *__restrict__ indices, const uint32_t *__restrict__ adj,
+                                                  const uint32_t *__restrict__ rank, uint8_t *state, uint32_t n,
+                                                  uint32_t *__restrict__ counts) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t p = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (p >= n) return;
+    if (__atomic_load_n(&state[p], __ATOMIC_RELAXED) != 0) return;
+    const uint32_t b = indices[p], e = indices[p + 1], rp = rank[p];
+    bool pending = false, seed_before = false;
+    for (uint32_t s = b; s < e; s += 64) {
+        const uint32_t i = s + lane;
+        bool earlier_seed = false, earlier_open = false;
+        if (i < e) {
+            const uint32_t q = adj[i];
+            if (rank[q] < rp) {
+                const uint8_t sq = __atomic_load_n(&state[q], __ATOMIC_RELAXED);
+                earlier_seed = sq == 1;
+                earlier_open = sq == 0;
+            }
+        }
+        if (__ballot(earlier_seed)) {
+            seed_before = true;
+            break;
+        }
+        if (__ballot(earlier_open)) pending = true;
+    }
+    if (lane == 0) {
+        if (seed_before) __atomic_store_n(&state[p], (uint8_t)2, __ATOMIC_RELAXED);
+        else if (!pending) __atomic_store_n(&state[p], (uint8_t)1, __ATOMIC_RELAXED);
+        else atomicAdd(&counts[0], 1u);
+    }
+}
+
+// sf[rank] = 1 for seeds (by position in the order).
+__global__ void k_ip_seedflags(const uint32_t *__restrict__ val, const uint8_t *__restrict__ state, uint32_t n,
+                               uint32_t *__restrict__ sf) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    sf[i] = state[val[i]] == 1 ? 1u : 0u;
+}
+
+// Labels: a seed's label is its seed rank (seedpos = exclusive scan of sf);
+// any other vertex takes the label of its earliest seed neighbour.  One
+// wave per vertex.
+__global__ __launch_bounds__(256) void k_ip_labels(const uint32_t *__restrict__ indices, const uint32_t *__restrict__ adj,
+                                                   const uint32_t *__restrict__ rank, const uint8_t *__restrict__ state,
+                                                   const uint32_t *__restrict__ seedpos, uint32_t n,
+                                                   uint32_t *__restrict__ labels) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t p = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (p >= n) return;
+    if (state[p] == 1) {
+        if (lane == 0) labels[p] = seedpos[rank[p]];
+        return;
+    }
+    const uint32_t b = indices[p], e = indices[p + 1];
+    uint32_t best = 0xFFFFFFFFu;
+    for (uint32_t i = b + lane; i < e; i += 64) {
+        const uint32_t q = adj[i];
+        if (state[q] == 1) best = min(best, rank[q]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) best = min(best, (uint32_t)__shfl_xor(best, off, SCD_WAVE));
+    if (lane == 0) labels[p] = seedpos[best];
+}
+
+}  // namespace scd_dev
+
+#endif
