@@ -1,0 +1,278 @@
+#!/usr/bin/env python3
+"""Refine edges/s of the SCD refinement path on one MI355X.
+
+Workload: R-MAT scale 20, edge factor 16 (a=0.57, b=0.19, c=0.19), seed 1,
+run through the convert pipeline (symmetrised, loops and duplicates removed,
+triangle-free edges removed, degree relabel): n = 353,053, m = 13,698,747.
+Set-up on the device (outside the timed region): CSR upload, initial
+partition, initial WCC, snapshot of the initial labels and statistics.
+
+One step = restore the initial labels and statistics (device copies) and run
+the refinement loop of src/main-gpu.cpp:71-96 to its stopping rule (one
+iteration on this graph): best move, statistics, WCC.
+value = m x iterations x steps / wall time of the timed steps (refine edges/s,
+src/main.cpp:57-84's loop time, the reset copies included).
+
+Also reported: the dominant kernel's roofline (algorithmic bytes per launch /
+its mean duration from HIP events recorded on the engine's stream during the
+timed steps), the device triangle counting against HBM peak, and the CPU
+baseline — the reference's own ./main (oracle/_ref/main, built from the
+reference sources) or the oracle restatement, on the same graph file and the
+host's cores.
+"""
+import argparse
+import hashlib
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "het-scd_amd"))
+
+METRIC = "refine edges/s + WCC (bit-exact) at 1 GPU; batched graphs/s & p99 vs plain"
+HBM_PEAK_GBPS = 8000.0
+BIN_OF_KERNEL = {"move_sub8": "sub8", "move_sub16": "sub16", "move_sub32": "sub32", "move_wave1": "wave1",
+                 "move_wave2": "wave2", "move_wave4": "wave4", "move_wave8": "wave8", "move_wave16": "wave16",
+                 "move_lds": "lds", "filter_sub8": "sub8", "filter_sub16": "sub16", "filter_sub32": "sub32",
+                 "filter_block": "lds"}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def move_bytes(v, e, p):
+    """Algorithmic bytes of the move step over a set of v vertices with e
+    adjacency entries and p evaluated pairs: indices 4(v+1), adjacency 4e,
+    label gather 4e, community statistics 12p (size, ext, int), own label
+    read + new label write 8v."""
+    return 4 * (v + 1) + 8 * e + 12 * p + 8 * v
+
+
+def stats_bytes(v, e):
+    """Statistics over v vertices with e entries: indices 4(v+1), adjacency
+    and label gather 8e, own label 4v, three atomic updates 3 x 8v."""
+    return 4 * (v + 1) + 8 * e + 4 * v + 24 * v
+
+
+def kernel_algo_bytes(name, bins, nwave_bins):
+    if name in ("move_sub8", "move_sub16", "move_sub32", "move_wave1", "move_wave2", "move_wave4", "move_wave8",
+                "move_wave16", "move_lds"):
+        v, e, p = bins[BIN_OF_KERNEL[name]]
+        return move_bytes(v, e, p)
+    if name in ("filter_sub8", "filter_sub16", "filter_sub32", "filter_block"):
+        v, e, _ = bins[BIN_OF_KERNEL[name]]
+        return stats_bytes(v, e)
+    if name == "filter_wave":
+        v = sum(bins[b][0] for b in nwave_bins)
+        e = sum(bins[b][1] for b in nwave_bins)
+        return stats_bytes(v, e)
+    return None
+
+
+def tc_algo_bytes(g):
+    """B_tc = 8(n+1) + 4E + 4 sum over oriented edges u->v (v < u) of
+    (d+(u) + d+(v)) + 8n, d+(x) = number of neighbours below x."""
+    import numpy as np
+    n = g.n
+    idx = g.indices.astype(np.int64)
+    src = np.repeat(np.arange(n, dtype=np.int64), np.diff(idx))
+    low = g.adj.astype(np.int64) < src
+    dplus = np.bincount(src[low], minlength=n)
+    term = (dplus[src[low]] + dplus[g.adj[low].astype(np.int64)]).sum()
+    return 8 * (n + 1) + 4 * 2 * g.m + 4 * int(term) + 8 * n
+
+
+def cpu_cores():
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except Exception:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(path, m, reps):
+    """The reference's ./main (oracle/_ref/main) or, if it was not built, the
+    oracle restatement (oracle/main) on the same file; median of reps runs."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "main")
+    port = os.path.join(ROOT, "oracle", "main")
+    exe, kind = (ref, "reference") if os.path.exists(ref) else (port, "port")
+    if not os.path.exists(exe):
+        return None
+    cores = cpu_cores()
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="spread", OMP_PLACES="cores")
+    per_iter = []
+    iters = None
+    for _ in range(reps):
+        out = subprocess.run([exe, path], capture_output=True, text=True, env=env, timeout=600).stdout
+        tpi = re.search(r"time per iteration: ([0-9.eE+-]+) ms", out)
+        it = re.search(r"iterations: (\d+)", out)
+        if not tpi or not it:
+            return None
+        per_iter.append(float(tpi.group(1)))
+        iters = int(it.group(1))
+    per_iter.sort()
+    ms = per_iter[len(per_iter) // 2]
+    return {"value": m / (ms / 1000.0), "unit": "edges/s", "cores": cores, "kind": kind,
+            "sample": f"full ./main run on the same R-MAT s20 graph file ({iters} iteration(s), "
+                      f"{ms:.1f} ms per iteration, median of {reps} runs)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--scale", type=int, default=20)
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tc", action="store_true")
+    ap.add_argument("--profile-json", default=None, help="write per-kernel timings here")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the single-graph SCD path "
+              "has no multi-GPU mode here", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import scd_amd
+
+    t = time.perf_counter()
+    edges = scd_amd.rmat(args.scale, 16, 0.57, 0.19, 0.19, 1)
+    g = edges.convert()
+    del edges
+    log(f"[bench] graph R-MAT s{args.scale}: n={g.n} m={g.m} ({time.perf_counter() - t:.1f} s to generate)")
+    graph_sha = g.sha256()
+
+    eng = scd_amd.Engine(device=0, profile=True)
+    t = time.perf_counter()
+    eng.transfer_graph(g)
+    t_up = time.perf_counter() - t
+    t = time.perf_counter()
+    lab0, k0 = eng.initial_partition()
+    t_ip = time.perf_counter() - t
+    w0 = eng.compute_wcc()
+    eng.snapshot()
+    log(f"[bench] set-up: upload {t_up * 1e3:.1f} ms, initial partition {t_ip * 1e3:.1f} ms "
+        f"({k0} labels), initial WCC {w0:.9f}")
+
+    for _ in range(args.warmup):
+        eng.restore()
+        eng.refine(w0)
+    eng.profile_reset()
+    iters_seen = set()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.restore()
+        best, iters, hist = eng.refine(w0)
+        iters_seen.add(iters)
+    t1 = time.perf_counter()
+    prof = eng.profile()
+    elapsed = t1 - t0
+    iters = max(iters_seen)
+    value = g.m * iters * args.steps / elapsed
+    ms_step = elapsed * 1e3 / args.steps
+    gpu_labels = eng.current_labels()
+    log(f"[bench] {args.steps} steps in {elapsed * 1e3:.1f} ms: {ms_step:.3f} ms/step, {iters} iteration(s), "
+        f"final WCC {hist[-1]:.9f}, {value / 1e9:.2f} G refine edges/s")
+
+    # per-bin pair counts for the algorithmic bytes: one more step with the
+    # side counters on, outside the timed region
+    eng.set_counting(True)
+    eng.restore()
+    eng.refine(w0)
+    eng.set_counting(False)
+    # roofline of the dominant kernel over the timed steps
+    bins = eng.bin_info()
+    wave_bins = ["wave1", "wave2", "wave4", "wave8", "wave16"]
+    kern = sorted(((ms, name, la) for name, (la, ms) in prof.items() if la > 0), reverse=True)
+    step_kernel_ms = sum(ms for ms, name, la in kern if not name.startswith("clear"))
+    roof = None
+    for ms, name, la in kern:
+        b = kernel_algo_bytes(name, bins, wave_bins)
+        if b is None:
+            continue
+        avg_ms = ms / la
+        ach = b / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBPS, 4), "traffic": None, "algo_bytes_per_launch": int(b),
+                "avg_launch_us": round(avg_ms * 1e3, 2),
+                "share_of_step_kernel_time": round(ms / step_kernel_ms, 3)}
+        break
+    per_kernel = {name: {"launches": la, "avg_us": round(ms / la * 1e3, 2), "total_ms": round(ms, 3)}
+                  for ms, name, la in kern}
+
+    # device triangle counting (outside the timed refinement steps)
+    tc = None
+    if not args.no_tc:
+        eng.profile_reset()
+        reps = 3
+        for _ in range(reps):
+            tri, td = eng.count_triangles()
+        assert np.array_equal(tri, g.tri) and np.array_equal(td, g.tri_deg), "triangle counts differ from the file"
+        p = eng.profile()
+        tc_ms = sum(ms for name, (la, ms) in p.items() if name.startswith("tc_")) / reps
+        btc = tc_algo_bytes(g)
+        tc = {"ms": round(tc_ms, 3), "algo_bytes": btc, "achieved_GBps": round(btc / (tc_ms * 1e-3) / 1e9, 1),
+              "frac_of_hbm_peak": round(btc / (tc_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+              "kernels": {name: round(ms / reps, 3) for name, (la, ms) in p.items() if la > 0}}
+        log(f"[bench] triangle counting {tc_ms:.3f} ms, {tc['achieved_GBps']} GB/s algorithmic")
+
+    # CPU baseline and label parity on the same graph file
+    cpu = None
+    parity = None
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "rmat.bin")
+        g.write(path)
+        if not args.no_cpu:
+            cpu = cpu_baseline(path, g.m, args.cpu_reps)
+            if cpu:
+                log(f"[bench] CPU baseline ({cpu['kind']}, {cpu['cores']} threads): "
+                    f"{cpu['value'] / 1e6:.1f} M refine edges/s")
+            dump = os.path.join(ROOT, "oracle", "_ref", "ref_dump")
+            if os.path.exists(dump):
+                out = os.path.join(tmp, "dump.bin")
+                # one thread: the canonical (vertex-order) clustering coefficient
+                subprocess.run([dump, path, out], check=True, capture_output=True,
+                               env=dict(os.environ, OMP_NUM_THREADS="1"))
+                b = open(out, "rb").read()
+                n, nc, it_ref = (int(x) for x in np.frombuffer(b, np.uint32, 3))
+                # 40-byte header; per iteration k: labels (4n), statistics
+                # (12 nc) and, for k >= 1, the WCC (8)
+                off = 40 + it_ref * (4 * n + 12 * nc) + 8 * max(it_ref - 1, 0)
+                last = np.frombuffer(b, np.uint32, n, off)
+                parity = {"iterations_equal": int(it_ref) == iters,
+                          "labels_bitexact": bool(np.array_equal(last, gpu_labels))}
+                log(f"[bench] parity vs the reference's own CPU code: {parity}")
+
+    result = {
+        "metric": METRIC, "value": round(value, 1), "unit": "edges/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp64", "data": "synthetic",
+        "config": {"workload": f"R-MAT scale {args.scale}, edge factor 16 (a=0.57,b=0.19,c=0.19), seed 1, "
+                               "convert-preprocessed; full SCD refinement loop from the initial partition",
+                   "n": g.n, "m": g.m, "iterations": iters, "graph_sha256": graph_sha,
+                   "tuning": eng.get_tuning()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "vs_cpu": round(value / cpu["value"], 2) if cpu else None,
+        "parity_vs_reference_cpu": parity,
+        "setup_ms": {"upload": round(t_up * 1e3, 2), "initial_partition": round(t_ip * 1e3, 2)},
+        "triangle_counting": tc,
+    }
+    if args.profile_json:
+        with open(args.profile_json, "w") as f:
+            json.dump({"per_kernel": per_kernel, "bins": bins, "tc": tc}, f, indent=1)
+    eng.close()
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

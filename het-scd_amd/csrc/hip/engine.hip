@@ -96,8 +96,19 @@ struct scd_ctx {
     uint32_t *int_adj = nullptr, *int_deg = nullptr;
     double *terms = nullptr, *partial = nullptr, *wcc_dev = nullptr;
     double *wcc_host = nullptr;  // pinned
+    // device counters of the last pass: [0, K_NB) evaluated pairs per degree
+    // bin, [K_NB] internal adjacency entries, [K_NB + 1] sum of internal
+    // degree squared
     unsigned long long *counters = nullptr;
-    unsigned long long counters_host[4] = {0, 0, 0, 0};
+    unsigned long long counters_host[16] = {0};
+    uint64_t bin_edges[16] = {0};  // adjacency entries per bin
+    bool counting = false;         // fill the counters (scd_set_counting)
+
+    // snapshot for scd_snapshot / scd_restore
+    uint32_t *snap_labels = nullptr;
+    int4 *snap_stats = nullptr;
+    uint32_t snap_num_comms = 0;
+    bool have_snap = false;
 
     // bins
     uint32_t *bin_verts = nullptr;
@@ -247,6 +258,8 @@ static int build_bins(scd_ctx *c) {
     for (int b = 0; b <= K_NB; b++) c->bin_begin[b] = cnt[b];
     std::vector<uint32_t> order(n), pos(cnt.begin(), cnt.end() - 1);
     for (uint32_t v = 0; v < n; v++) order[pos[bin_of(c->h_deg[v], T)]++] = v;
+    for (int b = 0; b < 16; b++) c->bin_edges[b] = 0;
+    for (uint32_t v = 0; v < n; v++) c->bin_edges[bin_of(c->h_deg[v], T)] += c->h_deg[v];
     int rc;
     if ((rc = dalloc(c, &c->bin_verts, n, &c->hub_bufs))) return rc;
     HIP_TRY(hipMemcpy(c->bin_verts, order.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
@@ -338,17 +351,22 @@ static int phase_move(scd_ctx *c, uint32_t *labels_out, uint32_t *ex_sorted, uin
     a.stats = c->stats;
     a.p_ext = c->cc;
     a.labels_out = labels_out;
-    a.counters = c->counters;
     a.ex_sorted = ex_sorted;
     a.ex_count = ex_count;
-    LAUNCH(c, "move_sub8", cdiv(bin_n(c, K_SUB8), 32), 256, k_move_sub<8>, a, bin_v(c, K_SUB8), bin_n(c, K_SUB8));
-    LAUNCH(c, "move_sub16", cdiv(bin_n(c, K_SUB16), 16), 256, k_move_sub<16>, a, bin_v(c, K_SUB16), bin_n(c, K_SUB16));
-    LAUNCH(c, "move_sub32", cdiv(bin_n(c, K_SUB32), 8), 256, k_move_sub<32>, a, bin_v(c, K_SUB32), bin_n(c, K_SUB32));
-    LAUNCH(c, "move_wave1", cdiv(bin_n(c, K_W1), 4), 256, k_move_wave<1>, a, bin_v(c, K_W1), bin_n(c, K_W1));
-    LAUNCH(c, "move_wave2", cdiv(bin_n(c, K_W2), 4), 256, k_move_wave<2>, a, bin_v(c, K_W2), bin_n(c, K_W2));
-    LAUNCH(c, "move_wave4", cdiv(bin_n(c, K_W4), 4), 256, k_move_wave<4>, a, bin_v(c, K_W4), bin_n(c, K_W4));
-    LAUNCH(c, "move_wave8", cdiv(bin_n(c, K_W8), 4), 256, k_move_wave<8>, a, bin_v(c, K_W8), bin_n(c, K_W8));
-    LAUNCH(c, "move_wave16", cdiv(bin_n(c, K_W16), 4), 256, k_move_wave<16>, a, bin_v(c, K_W16), bin_n(c, K_W16));
+    auto at = [&](int bin) {
+        MoveArgs b = a;
+        b.counters = c->counting ? c->counters + bin : nullptr;
+        return b;
+    };
+    LAUNCH(c, "move_sub8", cdiv(bin_n(c, K_SUB8), 32), 256, k_move_sub<8>, at(K_SUB8), bin_v(c, K_SUB8), bin_n(c, K_SUB8));
+    LAUNCH(c, "move_sub16", cdiv(bin_n(c, K_SUB16), 16), 256, k_move_sub<16>, at(K_SUB16), bin_v(c, K_SUB16), bin_n(c, K_SUB16));
+    LAUNCH(c, "move_sub32", cdiv(bin_n(c, K_SUB32), 8), 256, k_move_sub<32>, at(K_SUB32), bin_v(c, K_SUB32), bin_n(c, K_SUB32));
+    LAUNCH(c, "move_wave1", cdiv(bin_n(c, K_W1), 4), 256, k_move_wave<1>, at(K_W1), bin_v(c, K_W1), bin_n(c, K_W1));
+    LAUNCH(c, "move_wave2", cdiv(bin_n(c, K_W2), 4), 256, k_move_wave<2>, at(K_W2), bin_v(c, K_W2), bin_n(c, K_W2));
+    LAUNCH(c, "move_wave4", cdiv(bin_n(c, K_W4), 4), 256, k_move_wave<4>, at(K_W4), bin_v(c, K_W4), bin_n(c, K_W4));
+    LAUNCH(c, "move_wave8", cdiv(bin_n(c, K_W8), 4), 256, k_move_wave<8>, at(K_W8), bin_v(c, K_W8), bin_n(c, K_W8));
+    LAUNCH(c, "move_wave16", cdiv(bin_n(c, K_W16), 4), 256, k_move_wave<16>, at(K_W16), bin_v(c, K_W16), bin_n(c, K_W16));
+    a.counters = c->counting ? c->counters + K_LDS : nullptr;
     HubArgs h = hub_args(c);
     switch (c->tune.lds_tile) {
         case 2048: launch_lds_and_tiles<2048>(c, a, h); break;
@@ -363,7 +381,7 @@ static int phase_move(scd_ctx *c, uint32_t *labels_out, uint32_t *ex_sorted, uin
             LAUNCH(c, "hub_merge", c->n_chunks, 256, k_hub_merge, h, src, dst, (uint32_t)w, C);
             std::swap(src, dst);
         }
-        LAUNCH(c, "hub_eval", c->n_chunks, 256, k_hub_eval, a, h, src, C);
+        LAUNCH(c, "hub_eval", c->n_chunks, 256, k_hub_eval, at(K_HUB), h, src, C);
         LAUNCH(c, "hub_finalize", cdiv(c->nh, 64), 64, k_hub_finalize, a, h, c->nh);
     }
     return check_launch("move");
@@ -379,7 +397,7 @@ static int phase_stats(scd_ctx *c) {
     s.stats = c->stats;
     s.int_adj = c->int_adj;
     s.int_deg = c->int_deg;
-    s.counters = c->counters;
+    s.counters = c->counting ? c->counters + K_NB : nullptr;
     LAUNCH(c, "filter_sub8", cdiv(bin_n(c, K_SUB8), 32), 256, k_filter_sub<8>, s, bin_v(c, K_SUB8), bin_n(c, K_SUB8));
     LAUNCH(c, "filter_sub16", cdiv(bin_n(c, K_SUB16), 16), 256, k_filter_sub<16>, s, bin_v(c, K_SUB16), bin_n(c, K_SUB16));
     LAUNCH(c, "filter_sub32", cdiv(bin_n(c, K_SUB32), 8), 256, k_filter_sub<32>, s, bin_v(c, K_SUB32), bin_n(c, K_SUB32));
@@ -423,7 +441,7 @@ static int phase_wcc(scd_ctx *c, int32_t *int_tri_out, int32_t *int_tri_deg_out)
 }
 
 static int clear_counters(scd_ctx *c) {
-    LAUNCH(c, "clear_counters", 1, 64, k_fill_u64, c->counters, 4u);
+    LAUNCH(c, "clear_counters", 1, 64, k_fill_u64, c->counters, 16u);
     return check_launch("clear counters");
 }
 
@@ -516,7 +534,8 @@ extern "C" int scd_upload(scd_ctx *c, uint32_t n, uint32_t m, const uint32_t *in
         (rc = dalloc(c, &c->stats, n)) || (rc = dalloc(c, &c->int_adj, E)) ||
         (rc = dalloc(c, &c->int_deg, n)) || (rc = dalloc(c, &c->terms, n)) ||
         (rc = dalloc(c, &c->partial, cdiv(n, 4096))) || (rc = dalloc(c, &c->wcc_dev, 1)) ||
-        (rc = dalloc(c, &c->counters, 4))) {
+        (rc = dalloc(c, &c->counters, 16)) || (rc = dalloc(c, &c->snap_labels, n)) ||
+        (rc = dalloc(c, &c->snap_stats, n))) {
         free_graph(c);
         return rc;
     }
@@ -770,9 +789,92 @@ extern "C" int scd_set_tuning(scd_ctx *c, const scd_tuning *t) {
 
 extern "C" int scd_last_counters(scd_ctx *c, uint64_t out[4]) {
     if (!c || !out) return fail(SCD_EINVAL, "scd_last_counters: null argument");
-    for (int i = 0; i < 3; i++) out[i] = c->counters_host[i];
+    out[0] = 0;
+    for (int b = 0; b < K_NB; b++) out[0] += c->counters_host[b];
+    out[1] = c->counters_host[K_NB];
+    out[2] = c->counters_host[K_NB + 1];
     out[3] = c->nh;
     return SCD_OK;
+}
+
+extern "C" int scd_bin_info(scd_ctx *c, int max_bins, char (*names)[16], uint64_t *vertices,
+                            uint64_t *entries, uint64_t *pairs) {
+    static const char *bin_names[K_NB] = {"sub8", "sub16", "sub32", "wave1", "wave2",
+                                          "wave4", "wave8", "wave16", "lds", "hub"};
+    if (!c) return fail(SCD_EINVAL, "scd_bin_info: null context");
+    if (!c->have_graph) return fail(SCD_ESTATE, "scd_bin_info: no graph uploaded");
+    const int k = std::min<int>(max_bins, K_NB);
+    for (int b = 0; b < k; b++) {
+        if (names) snprintf(names[b], 16, "%s", bin_names[b]);
+        if (vertices) vertices[b] = bin_n(c, b);
+        if (entries) entries[b] = c->bin_edges[b];
+        if (pairs) pairs[b] = c->counters_host[b];
+    }
+    return K_NB;
+}
+
+extern "C" int scd_set_counting(scd_ctx *c, int on) {
+    if (!c) return fail(SCD_EINVAL, "scd_set_counting: null context");
+    c->counting = on != 0;
+    return SCD_OK;
+}
+
+extern "C" int scd_snapshot(scd_ctx *c) {
+    int rc = need_labels(c, "scd_snapshot");
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(c->snap_labels, c->cur, 4ull * c->n, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->snap_stats, c->stats, sizeof(int4) * c->num_comms, hipMemcpyDeviceToDevice,
+                           c->stream));
+    c->snap_num_comms = c->num_comms;
+    c->have_snap = true;
+    return finish(c);
+}
+
+extern "C" int scd_restore(scd_ctx *c) {
+    int rc = need_labels(c, "scd_restore");
+    if (rc) return rc;
+    if (!c->have_snap) return fail(SCD_ESTATE, "scd_restore: no snapshot taken");
+    HIP_TRY(hipMemcpyAsync(c->cur, c->snap_labels, 4ull * c->n, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->best, c->snap_labels, 4ull * c->n, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->stats, c->snap_stats, sizeof(int4) * c->snap_num_comms, hipMemcpyDeviceToDevice,
+                           c->stream));
+    c->num_comms = c->snap_num_comms;
+    return finish(c);
+}
+
+// Measurement helper: a plain streaming copy kernel over `bytes` bytes of
+// device memory, for calibrating hardware traffic counters.
+__global__ void k_calib_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+extern "C" int scd_calibration_copy(scd_ctx *c, uint64_t bytes, int reps, double *ms_per_copy) {
+    if (!c) return fail(SCD_EINVAL, "scd_calibration_copy: null context");
+    (void)hipSetDevice(c->device);
+    std::vector<DevBuf> tmp;
+    uint4 *a = nullptr, *b = nullptr;
+    const uint64_t n = bytes / 16;
+    int rc;
+    if ((rc = dalloc(c, &a, n, &tmp)) || (rc = dalloc(c, &b, n, &tmp))) {
+        free_list(c, tmp);
+        return rc;
+    }
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(cdiv(n, 256), 256ull * 64);
+    HIP_TRY(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_calib_copy, dim3(grid), dim3(256), 0, c->stream, a, b, n);
+    HIP_TRY(hipEventRecord(e1, c->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    free_list(c, tmp);
+    if (ms_per_copy) *ms_per_copy = reps > 0 ? ms / reps : 0.0;
+    return check_launch("calibration copy");
 }
 
 extern "C" int scd_prof_count(scd_ctx *c) { return c ? (int)c->prof.size() : 0; }

@@ -49,7 +49,9 @@ __device__ __forceinline__ uint64_t group_bits(uint64_t ballot, int gbase) {
 }
 
 __device__ __forceinline__ void count_pairs(unsigned long long *counters, uint32_t heads_in_lane) {
-    // wave sum, then one atomic per wave
+    // wave sum, then one atomic per wave; only in counting passes (the
+    // atomics all hit one address)
+    if (!counters) return;
     uint32_t s = heads_in_lane;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, SCD_WAVE);

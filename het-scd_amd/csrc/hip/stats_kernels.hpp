@@ -30,7 +30,7 @@ struct StatArgs {
     int4 *__restrict__ stats;
     uint32_t *__restrict__ int_adj;
     uint32_t *__restrict__ int_deg;
-    unsigned long long *__restrict__ counters;  // [1] += internal entries, [2] += deg_int^2
+    unsigned long long *__restrict__ counters;  // [0] += internal entries, [1] += deg_int^2
 };
 
 __device__ __forceinline__ void add_stats(int4 *stats, uint32_t L, uint32_t deg, uint32_t cnt) {
@@ -42,14 +42,15 @@ __device__ __forceinline__ void add_stats(int4 *stats, uint32_t L, uint32_t deg,
 
 __device__ __forceinline__ void count_internal(unsigned long long *counters, uint64_t ent,
                                                uint64_t sq) {
+    if (!counters) return;  // counting passes only
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         ent += __shfl_xor(ent, off, SCD_WAVE);
         sq += __shfl_xor(sq, off, SCD_WAVE);
     }
     if ((threadIdx.x & 63) == 0 && counters) {
-        atomicAdd(counters + 1, (unsigned long long)ent);
-        atomicAdd(counters + 2, (unsigned long long)sq);
+        atomicAdd(counters + 0, (unsigned long long)ent);
+        atomicAdd(counters + 1, (unsigned long long)sq);
     }
 }
 
@@ -230,8 +231,8 @@ __global__ __launch_bounds__(256) void k_hub_filter_write(StatArgs a, const uint
         for (uint32_t k = hub_chunk_begin[hb]; k < hub_chunk_begin[hb + 1]; k++) tot += chunk_cnt[k];
         a.int_deg[p] = tot;
         if (a.counters) {
-            atomicAdd(a.counters + 1, (unsigned long long)tot);
-            atomicAdd(a.counters + 2, (unsigned long long)tot * tot);
+            atomicAdd(a.counters + 0, (unsigned long long)tot);
+            atomicAdd(a.counters + 1, (unsigned long long)tot * tot);
         }
     }
     const uint32_t end = min(o + C, d);

@@ -135,6 +135,11 @@ def hip():
         "scd_prof_get": (ctypes.c_int, [C, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), u64p, f64p]),
         "scd_prof_reset": (ctypes.c_int, [C]),
         "scd_device_bytes": (ctypes.c_uint64, [C]),
+        "scd_bin_info": (ctypes.c_int, [C, ctypes.c_int, ctypes.c_void_p, u64p, u64p, u64p]),
+        "scd_snapshot": (ctypes.c_int, [C]),
+        "scd_set_counting": (ctypes.c_int, [C, ctypes.c_int]),
+        "scd_restore": (ctypes.c_int, [C]),
+        "scd_calibration_copy": (ctypes.c_int, [C, ctypes.c_uint64, ctypes.c_int, f64p]),
     }
     for k, (res, args) in sig.items():
         f = getattr(lib, k, None)

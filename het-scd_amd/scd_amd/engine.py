@@ -171,5 +171,30 @@ class Engine:
     def profile_reset(self):
         check_hip(_lib.hip().scd_prof_reset(self._ctx))
 
+    def bin_info(self):
+        """Per degree bin: name -> (vertices, adjacency entries, pairs
+        evaluated in the last move step)."""
+        names = (ctypes.c_char * 16 * 16)()
+        v = np.zeros(16, np.uint64)
+        e = np.zeros(16, np.uint64)
+        p = np.zeros(16, np.uint64)
+        k = check_hip(_lib.hip().scd_bin_info(self._ctx, 16, ctypes.cast(names, ctypes.c_void_p), ptr(v, u64p),
+                                              ptr(e, u64p), ptr(p, u64p)))
+        return {names[i].value.decode(): (int(v[i]), int(e[i]), int(p[i])) for i in range(k)}
+
+    def set_counting(self, on=True):
+        check_hip(_lib.hip().scd_set_counting(self._ctx, 1 if on else 0))
+
+    def snapshot(self):
+        check_hip(_lib.hip().scd_snapshot(self._ctx))
+
+    def restore(self):
+        check_hip(_lib.hip().scd_restore(self._ctx))
+
+    def calibration_copy(self, nbytes, reps=10):
+        ms = ctypes.c_double()
+        check_hip(_lib.hip().scd_calibration_copy(self._ctx, nbytes, reps, ctypes.byref(ms)))
+        return ms.value
+
     def device_bytes(self):
         return int(_lib.hip().scd_device_bytes(self._ctx))

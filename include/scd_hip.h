@@ -170,14 +170,37 @@ typedef struct scd_tuning {
 int scd_get_tuning(const scd_ctx *ctx, scd_tuning *out);
 int scd_set_tuning(scd_ctx *ctx, const scd_tuning *t);
 
+/* Turns the side counters below on or off (off by default: they cost one
+ * same-address atomic per wave). */
+int scd_set_counting(scd_ctx *ctx, int on);
+
 /*
- * Side counters of the last scd_improve (for algorithmic-byte accounting):
+ * Side counters of the last scd_improve with counting on (for
+ * algorithmic-byte accounting):
  * [0] P, distinct (vertex, valid neighbour label) pairs evaluated;
  * [1] internal adjacency entries (same-label directed edges);
  * [2] sum over vertices of internal degree squared;
  * [3] vertices with degree above lds_tile (hub path).
  */
 int scd_last_counters(scd_ctx *ctx, uint64_t out[4]);
+
+/*
+ * Per degree bin of the last move step: bin names, vertices, adjacency
+ * entries and evaluated (vertex, label) pairs; returns the number of bins
+ * (at most max_bins are written; any array may be NULL).
+ */
+int scd_bin_info(scd_ctx *ctx, int max_bins, char (*names)[16], uint64_t *vertices,
+                 uint64_t *entries, uint64_t *pairs);
+
+/* Saves the current labels and statistics on the device; scd_restore puts
+ * them back as current and best labels (device-to-device copies only). */
+int scd_snapshot(scd_ctx *ctx);
+int scd_restore(scd_ctx *ctx);
+
+/* Measurement helper: times `reps` launches of a plain streaming copy
+ * kernel over `bytes` bytes (read + write), for calibrating hardware
+ * traffic counters against a kernel of known traffic. */
+int scd_calibration_copy(scd_ctx *ctx, uint64_t bytes, int reps, double *ms_per_copy);
 
 /*
  * Profiling (scd_init with profile != 0): per kernel name, the number of

@@ -146,6 +146,7 @@ def test_tuning_invariance(engine, manifest):
     g = graph(name)
     assert g.degree().max() > 2048
     outs = []
+    engine.set_counting(True)
     for tile, chunk in ((2048, 256), (2048, 4096), (4096, 1024), (16384, 2048)):
         engine.transfer_graph(g)
         engine.set_tuning(lds_tile=tile, merge_chunk=chunk)
@@ -155,6 +156,7 @@ def test_tuning_invariance(engine, manifest):
         assert (hubs > 0) == (tile < g.degree().max())
         outs.append((wcc, lab))
     engine.set_tuning(lds_tile=4096, merge_chunk=2048)
+    engine.set_counting(False)
     for wcc, lab in outs:
         assert wcc == outs[0][0]
         assert sha(lab) == ent["labels_sha256"][1]
