@@ -13,9 +13,11 @@ iteration on this graph): best move, statistics, WCC.
 value = m x iterations x steps / wall time of the timed steps (refine edges/s,
 src/main.cpp:57-84's loop time, the reset copies included).
 
-Also reported: the dominant kernel's roofline (algorithmic bytes per launch /
-its mean duration from HIP events recorded on the engine's stream during the
-timed steps), the device triangle counting against HBM peak, and the CPU
+The timed steps replay each iteration as a captured hipGraph.  The same steps
+are then run once more with eager launches and a HIP event pair around every
+kernel on the engine's stream; from those: the dominant kernel's roofline
+(algorithmic bytes per launch / its mean duration).  Also reported: the
+device triangle counting against HBM peak, and the CPU
 baseline — the reference's own ./main (oracle/_ref/main, built from the
 reference sources) or the oracle restatement, on the same graph file and the
 host's cores.
@@ -38,7 +40,8 @@ HBM_PEAK_GBPS = 8000.0
 BIN_OF_KERNEL = {"move_sub8": "sub8", "move_sub16": "sub16", "move_sub32": "sub32", "move_wave1": "wave1",
                  "move_wave2": "wave2", "move_wave4": "wave4", "move_wave8": "wave8", "move_wave16": "wave16",
                  "move_lds": "lds", "filter_sub8": "sub8", "filter_sub16": "sub16", "filter_sub32": "sub32",
-                 "filter_block": "lds"}
+                 "filter_wave1": "wave1", "filter_wave2": "wave2", "filter_wave4": "wave4", "filter_wave8": "wave8",
+                 "filter_wave16": "wave16", "filter_block": "lds"}
 
 
 def log(*a):
@@ -64,7 +67,8 @@ def kernel_algo_bytes(name, bins, nwave_bins):
                 "move_wave16", "move_lds"):
         v, e, p = bins[BIN_OF_KERNEL[name]]
         return move_bytes(v, e, p)
-    if name in ("filter_sub8", "filter_sub16", "filter_sub32", "filter_block"):
+    if name in ("filter_sub8", "filter_sub16", "filter_sub32", "filter_wave1", "filter_wave2", "filter_wave4",
+                "filter_wave8", "filter_wave16", "filter_block"):
         v, e, _ = bins[BIN_OF_KERNEL[name]]
         return stats_bytes(v, e)
     if name == "filter_wave":
@@ -134,6 +138,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tc", action="store_true")
     ap.add_argument("--profile-json", default=None, help="write per-kernel timings here")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run a plain copy kernel over MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the single-graph SCD path "
@@ -162,10 +168,11 @@ def main():
     log(f"[bench] set-up: upload {t_up * 1e3:.1f} ms, initial partition {t_ip * 1e3:.1f} ms "
         f"({k0} labels), initial WCC {w0:.9f}")
 
+    # timed steps: each iteration replays a captured hipGraph
+    eng.set_profiling(False)
     for _ in range(args.warmup):
         eng.restore()
         eng.refine(w0)
-    eng.profile_reset()
     iters_seen = set()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -173,8 +180,20 @@ def main():
         best, iters, hist = eng.refine(w0)
         iters_seen.add(iters)
     t1 = time.perf_counter()
-    prof = eng.profile()
     elapsed = t1 - t0
+    # the same steps launched eagerly with a HIP event pair around every
+    # kernel on the engine's stream: per-kernel durations for the roofline
+    eng.set_profiling(True)
+    eng.restore()
+    eng.refine(w0)
+    eng.profile_reset()
+    te0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.restore()
+        eng.refine(w0)
+    eager_ms_step = (time.perf_counter() - te0) * 1e3 / args.steps
+    prof = eng.profile()
+    eng.set_profiling(False)
     iters = max(iters_seen)
     value = g.m * iters * args.steps / elapsed
     ms_step = elapsed * 1e3 / args.steps
@@ -211,6 +230,7 @@ def main():
     # device triangle counting (outside the timed refinement steps)
     tc = None
     if not args.no_tc:
+        eng.set_profiling(True)
         eng.profile_reset()
         reps = 3
         for _ in range(reps):
@@ -264,8 +284,13 @@ def main():
         "vs_cpu": round(value / cpu["value"], 2) if cpu else None,
         "parity_vs_reference_cpu": parity,
         "setup_ms": {"upload": round(t_up * 1e3, 2), "initial_partition": round(t_ip * 1e3, 2)},
+        "eager_profiled_ms_per_step": round(eager_ms_step, 4),
         "triangle_counting": tc,
     }
+    if args.calibrate_copy:
+        ms = eng.calibration_copy(args.calibrate_copy << 20, reps=3)
+        log(f"[bench] calibration copy of {args.calibrate_copy} MiB: {ms:.3f} ms "
+            f"({2 * (args.calibrate_copy << 20) / (ms * 1e-3) / 1e9:.0f} GB/s read+write)")
     if args.profile_json:
         with open(args.profile_json, "w") as f:
             json.dump({"per_kernel": per_kernel, "bins": bins, "tc": tc}, f, indent=1)

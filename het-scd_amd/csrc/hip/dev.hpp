@@ -63,11 +63,9 @@ __device__ __forceinline__ double improvement(int r, int d_in, int d_out, int c_
 
 // Candidate set-up of src/scd.cpp:280-307 for moving a vertex of degree
 // `deg` and label `my` into community L, seen `freq` times among its
-// neighbours.  stats[L] = (size, ext_edges, int_edges, unused).
-__device__ __forceinline__ double candidate_improvement(uint32_t L, uint32_t freq, uint32_t deg,
-                                                        uint32_t my, const int4 *__restrict__ stats,
-                                                        double p_ext) {
-    const int4 s = stats[L];
+// neighbours; s = stats[L] = (size, ext_edges, int_edges, unused).
+__device__ __forceinline__ double candidate_improvement_s(int4 s, uint32_t L, uint32_t freq, uint32_t deg,
+                                                          uint32_t my, double p_ext) {
     int size = s.x;
     int ext_edge = s.y;
     int int_edge = s.z / 2;
@@ -81,6 +79,12 @@ __device__ __forceinline__ double candidate_improvement(uint32_t L, uint32_t fre
     double p_int = 0.0;
     if (size > 1) p_int = (2.0 * (double)int_edge) / ((double)size * ((double)size - 1.0));
     return improvement(size, d_in, d_out, ext_edge, p_int, p_ext, 1.0);
+}
+
+__device__ __forceinline__ double candidate_improvement(uint32_t L, uint32_t freq, uint32_t deg,
+                                                        uint32_t my, const int4 *__restrict__ stats,
+                                                        double p_ext) {
+    return candidate_improvement_s(stats[L], L, freq, deg, my, p_ext);
 }
 
 // The best-move order of src/scd.cpp:309-320: the sequential scan over
@@ -215,6 +219,48 @@ __device__ __forceinline__ uint32_t merge_path(const uint32_t *A, uint32_t la, c
         else hi = mid;
     }
     return lo;
+}
+
+// Sort of n2 keys (a power of two, 1024 <= n2 <= T) held in LDS by a block
+// of 256 threads: every wave sorts 1024-key chunks in registers
+// (wave_bitonic_sort<16>), then merge-path passes in LDS double the sorted
+// run width; each thread computes its n2/256 outputs of a pass in registers
+// and writes them back after a barrier.  Ends with a barrier.
+template <int T>
+__device__ __forceinline__ void block_sort_lds(uint32_t *s, uint32_t n2) {
+    constexpr int PER_MAX = T / 256;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (uint32_t c = w; c * 1024 < n2; c += 4) {
+        uint32_t x[16];
+#pragma unroll
+        for (int r = 0; r < 16; r++) x[r] = s[c * 1024 + r * 64 + lane];
+        wave_bitonic_sort<16>(x, lane);
+#pragma unroll
+        for (int r = 0; r < 16; r++) s[c * 1024 + r * 64 + lane] = x[r];
+    }
+    __syncthreads();
+    const uint32_t per = n2 / 256;
+    const uint32_t o0 = threadIdx.x * per;
+    for (uint32_t w2 = 1024; w2 < n2; w2 *= 2) {
+        const uint32_t ps = (o0 / (2 * w2)) * (2 * w2);
+        const uint32_t *A = s + ps;
+        const uint32_t *B = A + w2;
+        uint32_t i = merge_path(A, w2, B, w2, o0 - ps);
+        uint32_t j = o0 - ps - i;
+        uint32_t out[PER_MAX];
+#pragma unroll
+        for (int k = 0; k < PER_MAX; k++) {
+            if ((uint32_t)k < per) {
+                const bool takeA = (j >= w2) || (i < w2 && A[i] <= B[j]);
+                out[k] = takeA ? A[i++] : B[j++];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < PER_MAX; k++)
+            if ((uint32_t)k < per) s[o0 + k] = out[k];
+        __syncthreads();
+    }
 }
 
 // |A ∩ B| of two ascending, duplicate-free lists: walks the shorter one and

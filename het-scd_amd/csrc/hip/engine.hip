@@ -77,6 +77,7 @@ struct PendingEv {
 struct scd_ctx {
     int device = 0;
     int block_size = 256;
+    uint32_t num_cus = 256;
     bool profile = false;
     hipStream_t stream = nullptr;
     uint64_t dev_bytes = 0;
@@ -95,12 +96,12 @@ struct scd_ctx {
     int4 *stats = nullptr;
     uint32_t *int_adj = nullptr, *int_deg = nullptr;
     double *terms = nullptr, *partial = nullptr, *wcc_dev = nullptr;
-    double *wcc_host = nullptr;  // pinned
+    double *wcc_host = nullptr;  // pinned: [0] mean WCC of the last pass
     // device counters of the last pass: [0, K_NB) evaluated pairs per degree
     // bin, [K_NB] internal adjacency entries, [K_NB + 1] sum of internal
     // degree squared
     unsigned long long *counters = nullptr;
-    unsigned long long counters_host[16] = {0};
+    unsigned long long *counters_host = nullptr;  // pinned, 16 entries
     uint64_t bin_edges[16] = {0};  // adjacency entries per bin
     bool counting = false;         // fill the counters (scd_set_counting)
 
@@ -125,6 +126,13 @@ struct scd_ctx {
 
     scd_tuning tune = {4096, 2048, {0, 0, 0, 0, 0, 0}};
     uint32_t last_ip_rounds = 0;
+
+    // per-iteration hipGraphs, one per label-buffer parity (cur == lab_a or
+    // lab_b); dropped whenever a launch parameter changes
+    bool use_graph = true;
+    bool capturing = false;
+    uint32_t *lab_a = nullptr, *lab_b = nullptr;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
 
     // profiling
     std::vector<ProfRec> prof;
@@ -156,7 +164,15 @@ static void free_list(scd_ctx *c, std::vector<DevBuf> &list) {
     list.clear();
 }
 
+static void drop_graphs(scd_ctx *c) {
+    for (int i = 0; i < 2; i++) {
+        if (c->gexec[i]) (void)hipGraphExecDestroy(c->gexec[i]);
+        c->gexec[i] = nullptr;
+    }
+}
+
 static void free_graph(scd_ctx *c) {
+    drop_graphs(c);
     free_list(c, c->hub_bufs);
     free_list(c, c->bufs);
     c->have_graph = c->have_labels = false;
@@ -250,6 +266,7 @@ static uint32_t bin_of(uint32_t d, uint32_t lds_tile) {
 }
 
 static int build_bins(scd_ctx *c) {
+    drop_graphs(c);
     free_list(c, c->hub_bufs);
     const uint32_t n = c->n, T = c->tune.lds_tile, C = c->tune.merge_chunk;
     std::vector<uint32_t> cnt(K_NB + 1, 0);
@@ -401,9 +418,17 @@ static int phase_stats(scd_ctx *c) {
     LAUNCH(c, "filter_sub8", cdiv(bin_n(c, K_SUB8), 32), 256, k_filter_sub<8>, s, bin_v(c, K_SUB8), bin_n(c, K_SUB8));
     LAUNCH(c, "filter_sub16", cdiv(bin_n(c, K_SUB16), 16), 256, k_filter_sub<16>, s, bin_v(c, K_SUB16), bin_n(c, K_SUB16));
     LAUNCH(c, "filter_sub32", cdiv(bin_n(c, K_SUB32), 8), 256, k_filter_sub<32>, s, bin_v(c, K_SUB32), bin_n(c, K_SUB32));
-    const uint32_t nw = c->bin_begin[K_W16 + 1] - c->bin_begin[K_W1];
-    LAUNCH(c, "filter_wave", cdiv(nw, 4), 256, k_filter_wave, s, bin_v(c, K_W1), nw);
-    LAUNCH(c, "filter_block", bin_n(c, K_LDS), 256, k_filter_block, s, bin_v(c, K_LDS), bin_n(c, K_LDS));
+    LAUNCH(c, "filter_wave1", cdiv(bin_n(c, K_W1), 4), 256, k_filter_wave<1>, s, bin_v(c, K_W1), bin_n(c, K_W1));
+    LAUNCH(c, "filter_wave2", cdiv(bin_n(c, K_W2), 4), 256, k_filter_wave<2>, s, bin_v(c, K_W2), bin_n(c, K_W2));
+    LAUNCH(c, "filter_wave4", cdiv(bin_n(c, K_W4), 4), 256, k_filter_wave<4>, s, bin_v(c, K_W4), bin_n(c, K_W4));
+    LAUNCH(c, "filter_wave8", cdiv(bin_n(c, K_W8), 4), 256, k_filter_wave<8>, s, bin_v(c, K_W8), bin_n(c, K_W8));
+    LAUNCH(c, "filter_wave16", cdiv(bin_n(c, K_W16), 4), 256, k_filter_wave<16>, s, bin_v(c, K_W16), bin_n(c, K_W16));
+    switch (c->tune.lds_tile) {
+        case 2048: LAUNCH(c, "filter_block", bin_n(c, K_LDS), 256, k_filter_block<2048>, s, bin_v(c, K_LDS), bin_n(c, K_LDS)); break;
+        case 4096: LAUNCH(c, "filter_block", bin_n(c, K_LDS), 256, k_filter_block<4096>, s, bin_v(c, K_LDS), bin_n(c, K_LDS)); break;
+        case 8192: LAUNCH(c, "filter_block", bin_n(c, K_LDS), 256, k_filter_block<8192>, s, bin_v(c, K_LDS), bin_n(c, K_LDS)); break;
+        default: LAUNCH(c, "filter_block", bin_n(c, K_LDS), 256, k_filter_block<16384>, s, bin_v(c, K_LDS), bin_n(c, K_LDS)); break;
+    }
     if (c->nh) {
         const uint32_t C = c->tune.merge_chunk;
         LAUNCH(c, "hub_filter_count", c->n_chunks, 256, k_hub_filter_count, s, c->hub_v, c->hub_d,
@@ -411,6 +436,8 @@ static int phase_stats(scd_ctx *c) {
         LAUNCH(c, "hub_filter_write", c->n_chunks, 256, k_hub_filter_write, s, c->hub_v, c->hub_d,
                c->hub_chunk_begin, c->chunk_hub, c->chunk_start, C, c->chunk_cnt);
     }
+    LAUNCH(c, "stats_agg", std::min<uint32_t>(cdiv(c->n, 1024), 2u * c->num_cus), 1024, k_stats_agg, c->indices,
+           c->cur, c->int_deg, c->n, c->stats);
     return check_launch("stats");
 }
 
@@ -477,18 +504,25 @@ extern "C" int scd_init(int device_id, int work_group_size, int profile, scd_ctx
     scd_ctx *c = new scd_ctx();
     c->device = device_id;
     c->block_size = work_group_size;
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device_id) == hipSuccess && prop.multiProcessorCount > 0)
+            c->num_cus = (uint32_t)prop.multiProcessorCount;
+    }
     c->profile = profile != 0;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
         return fail(SCD_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
     }
-    e = hipHostMalloc((void **)&c->wcc_host, sizeof(double) * 2, hipHostMallocDefault);
+    e = hipHostMalloc((void **)&c->wcc_host, 256, hipHostMallocDefault);
     if (e != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
         delete c;
         return fail(SCD_ENOMEM, "hipHostMalloc: %s", hipGetErrorString(e));
     }
+    c->counters_host = (unsigned long long *)((char *)c->wcc_host + 64);
+    for (int i = 0; i < 16; i++) c->counters_host[i] = 0;
     *out = c;
     return SCD_OK;
 }
@@ -539,6 +573,8 @@ extern "C" int scd_upload(scd_ctx *c, uint32_t n, uint32_t m, const uint32_t *in
         free_graph(c);
         return rc;
     }
+    c->lab_a = c->cur;
+    c->lab_b = c->next;
     HIP_TRY(hipMemcpyAsync(c->indices, indices, sizeof(uint32_t) * ((size_t)n + 1), hipMemcpyHostToDevice, c->stream));
     if (E) HIP_TRY(hipMemcpyAsync(c->adj, adj, sizeof(uint32_t) * E, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->tri, tri, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
@@ -577,6 +613,7 @@ extern "C" int scd_labels_to_device(scd_ctx *c, const uint32_t *labels) {
         mx = std::max(mx, labels[v]);
     }
     if (mx >= c->n) return fail(SCD_EINVAL, "scd_labels_to_device: label %u >= n = %u", mx, c->n);
+    if (c->num_comms != mx + 1) drop_graphs(c);
     c->num_comms = mx + 1;
     HIP_TRY(hipMemcpyAsync(c->cur, labels, sizeof(uint32_t) * c->n, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->best, labels, sizeof(uint32_t) * c->n, hipMemcpyHostToDevice, c->stream));
@@ -585,19 +622,75 @@ extern "C" int scd_labels_to_device(scd_ctx *c, const uint32_t *labels) {
     return SCD_OK;
 }
 
-extern "C" int scd_improve(scd_ctx *c, double *wcc, uint32_t *labels_out) {
-    int rc = need_labels(c, "scd_improve");
-    if (rc) return rc;
-    if ((rc = clear_counters(c)) || (rc = phase_move(c, c->next, nullptr, nullptr))) return rc;
+// One iteration's device work on the stream: move (cur -> next), swap,
+// statistics, WCC, mean to the pinned host word.  Leaves cur/next swapped.
+static int enqueue_iteration(scd_ctx *c) {
+    int rc;
+    if (c->counting && (rc = clear_counters(c))) return rc;
+    if ((rc = phase_move(c, c->next, nullptr, nullptr))) return rc;
     std::swap(c->cur, c->next);
     if ((rc = phase_stats(c)) || (rc = phase_wcc(c, nullptr, nullptr))) return rc;
     HIP_TRY(hipMemcpyAsync(c->wcc_host, c->wcc_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->counters_host, c->counters, sizeof(c->counters_host), hipMemcpyDeviceToHost,
-                           c->stream));
+    if (c->counting)
+        HIP_TRY(hipMemcpyAsync(c->counters_host, c->counters, 16 * sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, c->stream));
+    return SCD_OK;
+}
+
+// Captures enqueue_iteration for the current parity into gexec[par].
+static int capture_iteration(scd_ctx *c, int par) {
+    uint32_t *cur0 = c->cur, *next0 = c->next;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    int rc = enqueue_iteration(c);
+    c->capturing = false;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->cur = cur0;
+    c->next = next0;
+    if (rc == SCD_OK && e != hipSuccess) rc = fail(SCD_EDEVICE, "stream capture: %s", hipGetErrorString(e));
+    if (rc == SCD_OK) {
+        e = hipGraphInstantiate(&c->gexec[par], g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            c->gexec[par] = nullptr;
+            rc = fail(SCD_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+        }
+    }
+    if (g) (void)hipGraphDestroy(g);
+    if (rc != SCD_OK) (void)hipGetLastError();
+    return rc;
+}
+
+extern "C" int scd_improve(scd_ctx *c, double *wcc, uint32_t *labels_out) {
+    int rc = need_labels(c, "scd_improve");
+    if (rc) return rc;
+    // per-kernel event timing needs eager launches (events recorded by graph
+    // nodes carry no usable timestamps)
+    if (c->use_graph && !c->profile) {
+        const int par = c->cur == c->lab_a ? 0 : 1;
+        if (!c->gexec[par] && (rc = capture_iteration(c, par))) return rc;
+        HIP_TRY(hipGraphLaunch(c->gexec[par], c->stream));
+        std::swap(c->cur, c->next);
+    } else if ((rc = enqueue_iteration(c))) {
+        return rc;
+    }
     if (labels_out)
         HIP_TRY(hipMemcpyAsync(labels_out, c->cur, sizeof(uint32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
     if ((rc = finish(c))) return rc;
     if (wcc) *wcc = c->wcc_host[0];
+    return SCD_OK;
+}
+
+extern "C" int scd_set_profiling(scd_ctx *c, int on) {
+    if (!c) return fail(SCD_EINVAL, "scd_set_profiling: null context");
+    c->profile = on != 0;
+    return SCD_OK;
+}
+
+extern "C" int scd_set_graph_mode(scd_ctx *c, int on) {
+    if (!c) return fail(SCD_EINVAL, "scd_set_graph_mode: null context");
+    if (!on) drop_graphs(c);
+    c->use_graph = on != 0;
     return SCD_OK;
 }
 
@@ -815,6 +908,7 @@ extern "C" int scd_bin_info(scd_ctx *c, int max_bins, char (*names)[16], uint64_
 
 extern "C" int scd_set_counting(scd_ctx *c, int on) {
     if (!c) return fail(SCD_EINVAL, "scd_set_counting: null context");
+    if (c->counting != (on != 0)) drop_graphs(c);
     c->counting = on != 0;
     return SCD_OK;
 }
@@ -838,6 +932,7 @@ extern "C" int scd_restore(scd_ctx *c) {
     HIP_TRY(hipMemcpyAsync(c->best, c->snap_labels, 4ull * c->n, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->stats, c->snap_stats, sizeof(int4) * c->snap_num_comms, hipMemcpyDeviceToDevice,
                            c->stream));
+    if (c->num_comms != c->snap_num_comms) drop_graphs(c);
     c->num_comms = c->snap_num_comms;
     return finish(c);
 }
@@ -1056,6 +1151,7 @@ extern "C" int scd_initial_partition(scd_ctx *c, uint32_t *labels_out, uint32_t 
     LAUNCH(c, "ip_labels", cdiv(n, 4), 256, k_ip_labels, c->indices, c->adj, rank, state, sf, n, c->cur);
     if ((rc = check_launch("initial partition"))) return cleanup(rc);
     HIP_TRY(hipMemcpyAsync(c->best, c->cur, 4ull * n, hipMemcpyDeviceToDevice, c->stream));
+    if (c->num_comms != k) drop_graphs(c);
     c->num_comms = k;
     c->last_ip_rounds = rounds;
     if ((rc = clear_counters(c)) || (rc = phase_stats(c))) return cleanup(rc);

@@ -148,11 +148,13 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const uint32_t *_
     uint32_t bl = SCD_INVALID;
     bool has_rem = false;
     uint32_t heads = 0;
+    uint32_t freq[R];
+    int4 st[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const uint32_t i = (uint32_t)(r * 64 + lane);
         const bool head = bnd[r] && x[r] != SCD_INVALID;
-        uint32_t freq = 0;
+        freq[r] = 0;
         if (head) {
             const uint64_t above = lane < 63 ? (B[r] >> (lane + 1)) : 0ull;
             uint32_t next = 64u * R;
@@ -163,8 +165,17 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const uint32_t *_
                 for (int rr = R - 1; rr > r; rr--)
                     if (B[rr]) next = (uint32_t)(rr * 64 + __builtin_ctzll(B[rr]));
             }
-            freq = next - i;
-            const double imp = candidate_improvement(x[r], freq, deg, my, a.stats, a.p_ext);
+            freq[r] = next - i;
+        }
+    }
+    // all statistics loads before the first evaluation
+#pragma unroll
+    for (int r = 0; r < R; r++) st[r] = freq[r] ? a.stats[x[r]] : make_int4(0, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint32_t i = (uint32_t)(r * 64 + lane);
+        if (freq[r]) {
+            const double imp = candidate_improvement_s(st[r], x[r], freq[r], deg, my, a.p_ext);
             if (x[r] == my) {
                 rem_lane = -imp;
                 has_rem = true;
@@ -177,7 +188,7 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const uint32_t *_
         }
         if (a.ex_sorted && i < deg) {
             a.ex_sorted[beg + i] = x[r];
-            a.ex_count[beg + i] = head ? freq : 0u;
+            a.ex_count[beg + i] = freq[r];
         }
     }
     group_argmax<64>(bi, bl);
@@ -208,12 +219,80 @@ __device__ __forceinline__ void block_argmax256(double &bi, uint32_t &bl, double
     }
 }
 
+// Reduce-by-key + evaluation over positions [lo, hi) of a sorted key array
+// S (length n) by 256 threads, four positions per thread at a time: run
+// heads and run lengths first, then every statistics load, then the
+// evaluations.  The own-label improvement goes to *s_rem (one writer).
+__device__ __forceinline__ void eval_sorted_batched(const MoveArgs &a, const uint32_t *S, uint32_t n, uint32_t lo,
+                                                    uint32_t hi, uint32_t deg, uint32_t my, uint32_t beg,
+                                                    double &bi, uint32_t &bl, uint32_t &heads, double *s_rem,
+                                                    int *s_has_rem) {
+    constexpr int BATCH = 4;
+    for (uint32_t base = lo; base < hi; base += 256 * BATCH) {
+        uint32_t v[BATCH], fr[BATCH];
+        int4 st[BATCH];
+#pragma unroll
+        for (int k = 0; k < BATCH; k++) {
+            const uint32_t i = base + k * 256 + threadIdx.x;
+            v[k] = SCD_INVALID;
+            fr[k] = 0;
+            if (i < hi) {
+                v[k] = S[i];
+                if ((i == 0 || S[i - 1] != v[k]) && v[k] != SCD_INVALID) fr[k] = upper_bound_u32(S, n, v[k]) - i;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < BATCH; k++) st[k] = fr[k] ? a.stats[v[k]] : make_int4(0, 0, 0, 0);
+#pragma unroll
+        for (int k = 0; k < BATCH; k++) {
+            const uint32_t i = base + k * 256 + threadIdx.x;
+            if (fr[k]) {
+                const double imp = candidate_improvement_s(st[k], v[k], fr[k], deg, my, a.p_ext);
+                if (v[k] == my) {
+                    *s_rem = -imp;
+                    *s_has_rem = 1;
+                }
+                if (imp > 0.0 && better(imp, v[k], bi, bl)) {
+                    bi = imp;
+                    bl = v[k];
+                }
+                heads++;
+            }
+            if (a.ex_sorted && i < hi) {
+                a.ex_sorted[beg + i] = v[k];
+                a.ex_count[beg + i] = fr[k];
+            }
+        }
+    }
+}
+
 __device__ __forceinline__ uint32_t next_pow2(uint32_t x) {
     return x <= 2 ? 2u : (1u << (32 - __builtin_clz(x - 1)));
 }
 
+// keys[i] = labels[adj[beg + i]] for i < deg, INVALID up to n2; all of a
+// thread's loads are issued before any is used.  Ends with a barrier.
+template <int T>
+__device__ __forceinline__ void gather_labels(const MoveArgs &a, uint32_t beg, uint32_t deg, uint32_t n2,
+                                              uint32_t *keys) {
+    constexpr int U = T / 256;
+    uint32_t q[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const uint32_t i = u * 256 + threadIdx.x;
+        q[u] = i < deg ? a.adj[beg + i] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const uint32_t i = u * 256 + threadIdx.x;
+        if (i < n2) keys[i] = i < deg ? a.labels[q[u]] : SCD_INVALID;
+    }
+    __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
-// One workgroup (256 threads) per vertex, deg <= T: bitonic sort in LDS.
+// One workgroup (256 threads) per vertex, deg <= T: register sorts of
+// 1024-key chunks and merge passes in LDS.
 // ---------------------------------------------------------------------------
 template <int T>
 __global__ __launch_bounds__(256) void k_move_lds(MoveArgs a, const uint32_t *__restrict__ verts,
@@ -228,37 +307,15 @@ __global__ __launch_bounds__(256) void k_move_lds(MoveArgs a, const uint32_t *__
     const uint32_t beg = a.indices[p];
     const uint32_t deg = a.indices[p + 1] - beg;
     const uint32_t my = a.labels[p];
-    const uint32_t n2 = next_pow2(deg);
+    const uint32_t n2 = max(next_pow2(deg), 1024u);
     if (tid == 0) s_has_rem = 0;
-    for (uint32_t i = tid; i < n2; i += 256) keys[i] = i < deg ? a.labels[a.adj[beg + i]] : SCD_INVALID;
-    __syncthreads();
-    lds_bitonic_sort(keys, n2, tid, 256);
+    gather_labels<T>(a, beg, deg, n2, keys);
+    block_sort_lds<T>(keys, n2);
 
     double bi = 0.0;
     uint32_t bl = SCD_INVALID;
     uint32_t heads = 0;
-    for (uint32_t i = tid; i < deg; i += 256) {
-        const uint32_t v = keys[i];
-        const bool head = (i == 0 || keys[i - 1] != v) && v != SCD_INVALID;
-        uint32_t freq = 0;
-        if (head) {
-            freq = upper_bound_u32(keys, n2, v) - i;
-            const double imp = candidate_improvement(v, freq, deg, my, a.stats, a.p_ext);
-            if (v == my) {
-                s_rem = -imp;
-                s_has_rem = 1;
-            }
-            if (imp > 0.0 && better(imp, v, bi, bl)) {
-                bi = imp;
-                bl = v;
-            }
-            heads++;
-        }
-        if (a.ex_sorted) {
-            a.ex_sorted[beg + i] = v;
-            a.ex_count[beg + i] = freq;
-        }
-    }
+    eval_sorted_batched(a, keys, n2, 0, deg, deg, my, beg, bi, bl, heads, &s_rem, &s_has_rem);
     count_pairs(a.counters, heads);
     block_argmax256(bi, bl, s_bi, s_bl);
     if (tid == 0 && a.labels_out) {
@@ -299,12 +356,12 @@ __global__ __launch_bounds__(256) void k_hub_tile_sort(MoveArgs a, HubArgs h,
     const uint32_t d = h.hub_d[hb];
     const uint32_t beg = a.indices[p] + st;
     const uint32_t len = min((uint32_t)T, d - st);
-    const uint32_t n2 = next_pow2(len);
-    for (uint32_t i = tid; i < n2; i += 256) keys[i] = i < len ? a.labels[a.adj[beg + i]] : SCD_INVALID;
-    __syncthreads();
-    lds_bitonic_sort(keys, n2, tid, 256);
+    const uint32_t n2 = max(next_pow2(len), 1024u);
+    gather_labels<T>(a, beg, len, n2, keys);
+    block_sort_lds<T>(keys, n2);
     uint32_t *dst = out + h.hub_off[hb] + st;
     for (uint32_t i = tid; i < len; i += 256) dst[i] = keys[i];
+    (void)tid;
 }
 
 // One merge pass: sorted runs of width w -> 2w inside every hub segment.
@@ -360,28 +417,7 @@ __global__ __launch_bounds__(256) void k_hub_eval(MoveArgs a, HubArgs h,
     uint32_t bl = SCD_INVALID;
     uint32_t heads = 0;
     const uint32_t end = min(o + C, d);
-    for (uint32_t i = o + tid; i < end; i += 256) {
-        const uint32_t v = S[i];
-        const bool head = (i == 0 || S[i - 1] != v) && v != SCD_INVALID;
-        uint32_t freq = 0;
-        if (head) {
-            freq = upper_bound_u32(S, d, v) - i;
-            const double imp = candidate_improvement(v, freq, d, my, a.stats, a.p_ext);
-            if (v == my) {
-                s_rem = -imp;
-                s_has_rem = 1;
-            }
-            if (imp > 0.0 && better(imp, v, bi, bl)) {
-                bi = imp;
-                bl = v;
-            }
-            heads++;
-        }
-        if (a.ex_sorted) {
-            a.ex_sorted[beg + i] = v;
-            a.ex_count[beg + i] = freq;
-        }
-    }
+    eval_sorted_batched(a, S, d, o, end, d, my, beg, bi, bl, heads, &s_rem, &s_has_rem);
     count_pairs(a.counters, heads);
     block_argmax256(bi, bl, s_bi, s_bl);
     if (tid == 0) {

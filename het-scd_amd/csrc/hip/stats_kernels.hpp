@@ -33,11 +33,58 @@ struct StatArgs {
     unsigned long long *__restrict__ counters;  // [0] += internal entries, [1] += deg_int^2
 };
 
-__device__ __forceinline__ void add_stats(int4 *stats, uint32_t L, uint32_t deg, uint32_t cnt) {
-    atomicAdd(&stats[L].x, 1);
-    const uint32_t ext = deg - cnt;
+// Adds one vertex's (1, external, internal) counts to community L.
+__device__ __forceinline__ void add_stats(int4 *stats, uint32_t L, uint32_t sz, uint32_t ext, uint32_t cnt) {
+    if (sz > 0) atomicAdd(&stats[L].x, (int)sz);
     if (ext > 0) atomicAdd(&stats[L].y, (int)ext);
     if (cnt > 0) atomicAdd(&stats[L].z, (int)cnt);
+}
+
+// Community statistics from the per-vertex internal degrees: sizes[L],
+// ext[L] = sum of (deg - int_deg), int[L] = sum of int_deg over the members
+// of L.  Large communities would serialise on one address, so each block
+// (1024 threads, grid-stride over the vertices) first sums its vertices per
+// label in an LDS hash table and flushes one atomic per label and field;
+// a vertex whose label finds no free slot within STATS_PROBES probes adds
+// its counts directly.  Integer sums, so the result is order-independent.
+constexpr uint32_t STATS_SLOTS = 2048;
+constexpr uint32_t STATS_PROBES = 8;
+
+__global__ __launch_bounds__(1024) void k_stats_agg(const uint32_t *__restrict__ indices,
+                                                    const uint32_t *__restrict__ labels,
+                                                    const uint32_t *__restrict__ int_deg, uint32_t n,
+                                                    int4 *__restrict__ stats) {
+    __shared__ uint32_t s_key[STATS_SLOTS];
+    __shared__ uint32_t s_sz[STATS_SLOTS], s_ext[STATS_SLOTS], s_int[STATS_SLOTS];
+    for (uint32_t h = threadIdx.x; h < STATS_SLOTS; h += 1024) {
+        s_key[h] = SCD_INVALID;
+        s_sz[h] = s_ext[h] = s_int[h] = 0;
+    }
+    __syncthreads();
+    for (uint32_t p = blockIdx.x * 1024u + threadIdx.x; p < n; p += gridDim.x * 1024u) {
+        const uint32_t L = labels[p];
+        if (L == SCD_INVALID) continue;
+        const uint32_t ic = int_deg[p];
+        const uint32_t ext = indices[p + 1] - indices[p] - ic;
+        uint32_t h = (L * 0x9E3779B1u) >> (32 - 11);
+        bool done = false;
+        for (uint32_t k = 0; k < STATS_PROBES && !done; k++, h = (h + 1) & (STATS_SLOTS - 1)) {
+            uint32_t key = s_key[h];
+            if (key == SCD_INVALID) key = atomicCAS(&s_key[h], SCD_INVALID, L);
+            if (key == SCD_INVALID || key == L) {
+                atomicAdd(&s_sz[h], 1u);
+                atomicAdd(&s_ext[h], ext);
+                atomicAdd(&s_int[h], ic);
+                done = true;
+            }
+        }
+        if (!done) add_stats(stats, L, 1, ext, ic);
+    }
+    __syncthreads();
+    for (uint32_t h = threadIdx.x; h < STATS_SLOTS; h += 1024) {
+        const uint32_t L = s_key[h];
+        if (L != SCD_INVALID) add_stats(stats, L, s_sz[h], s_ext[h], s_int[h]);
+    }
 }
 
 __device__ __forceinline__ void count_internal(unsigned long long *counters, uint64_t ent,
@@ -91,12 +138,13 @@ __global__ __launch_bounds__(256) void k_filter_sub(StatArgs a, const uint32_t *
     const bool lead = active && lig == 0;
     if (lead) {
         a.int_deg[p] = cnt;
-        if (L != SCD_INVALID) add_stats(a.stats, L, deg, cnt);
     }
     count_internal(a.counters, lead ? cnt : 0, lead ? (uint64_t)cnt * cnt : 0);
 }
 
-// One wave per vertex; loops over the list in slices of 64.
+// One wave per vertex, deg <= 64 R: every adjacency and label load of the
+// vertex is issued before the compaction.
+template <int R>
 __global__ __launch_bounds__(256) void k_filter_wave(StatArgs a, const uint32_t *__restrict__ verts,
                                                      uint32_t nv) {
     const int lane = threadIdx.x & 63;
@@ -107,24 +155,79 @@ __global__ __launch_bounds__(256) void k_filter_wave(StatArgs a, const uint32_t 
     const uint32_t deg = a.indices[p + 1] - beg;
     const uint32_t L = a.labels[p];
     const uint64_t lt = (1ull << lane) - 1ull;
+    uint32_t q[R], lq[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint32_t i = r * 64 + lane;
+        q[r] = i < deg ? a.adj[beg + i] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint32_t i = r * 64 + lane;
+        lq[r] = i < deg ? a.labels[q[r]] : SCD_INVALID;
+    }
     uint32_t run = 0;
-    for (uint32_t b = 0; b < deg; b += 64) {
-        const uint32_t i = b + lane;
-        bool same = false;
-        uint32_t q = 0;
-        if (i < deg) {
-            q = a.adj[beg + i];
-            same = (L != SCD_INVALID) && a.labels[q] == L;
-        }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const bool same = (L != SCD_INVALID) && lq[r] == L && (uint32_t)(r * 64 + lane) < deg;
         const uint64_t m = __ballot(same);
-        if (same) a.int_adj[beg + run + __builtin_popcountll(m & lt)] = q;
+        if (same) a.int_adj[beg + run + __builtin_popcountll(m & lt)] = q[r];
         run += (uint32_t)__builtin_popcountll(m);
     }
     if (lane == 0) {
         a.int_deg[p] = run;
-        if (L != SCD_INVALID) add_stats(a.stats, L, deg, run);
     }
     count_internal(a.counters, lane == 0 ? run : 0, lane == 0 ? (uint64_t)run * run : 0);
+}
+
+// One workgroup per vertex (deg <= T): all loads first, then one
+// block-wide order-preserving compaction over the T / 256 slices.
+template <int T>
+__global__ __launch_bounds__(256) void k_filter_block(StatArgs a, const uint32_t *__restrict__ verts,
+                                                      uint32_t nv) {
+    constexpr int U = T / 256;
+    __shared__ uint32_t s_cnt[U][4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t p = verts[blockIdx.x];
+    const uint32_t beg = a.indices[p];
+    const uint32_t deg = a.indices[p + 1] - beg;
+    const uint32_t L = a.labels[p];
+    uint32_t q[U];
+    bool same[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const uint32_t i = u * 256 + threadIdx.x;
+        q[u] = i < deg ? a.adj[beg + i] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const uint32_t i = u * 256 + threadIdx.x;
+        same[u] = i < deg && L != SCD_INVALID && a.labels[q[u]] == L;
+    }
+    uint64_t m[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        m[u] = __ballot(same[u]);
+        if (lane == 0) s_cnt[u][w] = (uint32_t)__builtin_popcountll(m[u]);
+    }
+    __syncthreads();
+    uint32_t run = 0;
+    const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        uint32_t before = run;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            before += (k < w) ? s_cnt[u][k] : 0u;
+            run += s_cnt[u][k];
+        }
+        if (same[u]) a.int_adj[beg + before + __builtin_popcountll(m[u] & lt)] = q[u];
+    }
+    if (threadIdx.x == 0) {
+        a.int_deg[p] = run;
+    }
+    count_internal(a.counters, threadIdx.x == 0 ? run : 0,
+                   threadIdx.x == 0 ? (uint64_t)run * run : 0);
 }
 
 // Order-preserving compaction of one slice of up to 256 entries by a block
@@ -147,35 +250,7 @@ __device__ __forceinline__ uint32_t block_compact256(bool same, uint32_t q, uint
     return total;
 }
 
-// One workgroup per vertex (deg up to the LDS tile).
-__global__ __launch_bounds__(256) void k_filter_block(StatArgs a, const uint32_t *__restrict__ verts,
-                                                      uint32_t nv) {
-    __shared__ uint32_t s_cnt[4];
-    const uint32_t p = verts[blockIdx.x];
-    const uint32_t beg = a.indices[p];
-    const uint32_t deg = a.indices[p + 1] - beg;
-    const uint32_t L = a.labels[p];
-    uint32_t run = 0;
-    for (uint32_t b = 0; b < deg; b += 256) {
-        const uint32_t i = b + threadIdx.x;
-        bool same = false;
-        uint32_t q = 0;
-        if (i < deg) {
-            q = a.adj[beg + i];
-            same = (L != SCD_INVALID) && a.labels[q] == L;
-        }
-        run += block_compact256(same, q, a.int_adj + beg, run, s_cnt);
-    }
-    if (threadIdx.x == 0) {
-        a.int_deg[p] = run;
-        if (L != SCD_INVALID) add_stats(a.stats, L, deg, run);
-    }
-    count_internal(a.counters, threadIdx.x == 0 ? run : 0,
-                   threadIdx.x == 0 ? (uint64_t)run * run : 0);
-}
-
-// Hubs, pass 1: same-label count of each chunk of C entries; the chunk's
-// external and internal counts go straight to the statistics.
+// Hubs, pass 1: same-label count of each chunk of C entries.
 __global__ __launch_bounds__(256) void k_hub_filter_count(StatArgs a, const uint32_t *__restrict__ hub_v,
                                                           const uint32_t *__restrict__ hub_d,
                                                           const uint32_t *__restrict__ chunk_hub,
@@ -199,12 +274,6 @@ __global__ __launch_bounds__(256) void k_hub_filter_count(StatArgs a, const uint
     if (threadIdx.x == 0) {
         const uint32_t cnt = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
         chunk_cnt[blockIdx.x] = cnt;
-        if (L != SCD_INVALID) {
-            if (o == 0) atomicAdd(&a.stats[L].x, 1);
-            const uint32_t ext = (end - o) - cnt;
-            if (ext > 0) atomicAdd(&a.stats[L].y, (int)ext);
-            if (cnt > 0) atomicAdd(&a.stats[L].z, (int)cnt);
-        }
     }
 }
 

@@ -137,6 +137,8 @@ def hip():
         "scd_device_bytes": (ctypes.c_uint64, [C]),
         "scd_bin_info": (ctypes.c_int, [C, ctypes.c_int, ctypes.c_void_p, u64p, u64p, u64p]),
         "scd_snapshot": (ctypes.c_int, [C]),
+        "scd_set_graph_mode": (ctypes.c_int, [C, ctypes.c_int]),
+        "scd_set_profiling": (ctypes.c_int, [C, ctypes.c_int]),
         "scd_set_counting": (ctypes.c_int, [C, ctypes.c_int]),
         "scd_restore": (ctypes.c_int, [C]),
         "scd_calibration_copy": (ctypes.c_int, [C, ctypes.c_uint64, ctypes.c_int, f64p]),

@@ -182,6 +182,12 @@ class Engine:
                                               ptr(e, u64p), ptr(p, u64p)))
         return {names[i].value.decode(): (int(v[i]), int(e[i]), int(p[i])) for i in range(k)}
 
+    def set_profiling(self, on=True):
+        check_hip(_lib.hip().scd_set_profiling(self._ctx, 1 if on else 0))
+
+    def set_graph_mode(self, on=True):
+        check_hip(_lib.hip().scd_set_graph_mode(self._ctx, 1 if on else 0))
+
     def set_counting(self, on=True):
         check_hip(_lib.hip().scd_set_counting(self._ctx, 1 if on else 0))
 

@@ -105,6 +105,16 @@ int scd_labels_to_device(scd_ctx *ctx, const uint32_t *labels);
  */
 int scd_improve(scd_ctx *ctx, double *wcc, uint32_t *labels_out);
 
+/* Iteration execution mode: on (default) captures each iteration's launch
+ * sequence once into a hipGraph (one per label-buffer parity) and replays
+ * it; off launches every kernel from the host.  Results are identical.
+ * While profiling is on, iterations launch eagerly (per-kernel HIP events
+ * need stream-recorded events). */
+int scd_set_graph_mode(scd_ctx *ctx, int on);
+
+/* Turns per-kernel HIP event timing (scd_prof_*) on or off. */
+int scd_set_profiling(scd_ctx *ctx, int on);
+
 /* Replaces cuda_store_labels() (src/cuda.cu:363-365): best := current. */
 int scd_store_best(scd_ctx *ctx);
 

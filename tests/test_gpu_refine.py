@@ -178,3 +178,28 @@ def test_error_paths(engine):
     assert ei.value.code == -1
     with pytest.raises(scd_amd.ScdError):
         engine.set_tuning(lds_tile=3000)
+
+
+@pytest.mark.parametrize("name", ["pp4x16_s4", "c1", "rmat16"])
+def test_graph_replays_match_eager(engine, manifest, name):
+    """Captured-iteration replays (several, with eager copies and host
+    synchronisation between them) give the eager launches' labels and WCC
+    every time."""
+    ent = manifest["graphs"][name]
+    g = graph(name)
+    engine.transfer_graph(g)
+    engine.labels_to_device(initial_labels(name, ent, g))
+    engine.snapshot()
+    engine.set_graph_mode(False)
+    eager = []
+    for _ in range(3):
+        eager.append(engine.improve_labels(return_labels=True))
+    engine.set_graph_mode(True)
+    for rep in range(3):
+        engine.restore()
+        for it in range(3):
+            w, lab = engine.improve_labels(return_labels=True)
+            assert w == eager[it][0], (rep, it)
+            assert np.array_equal(lab, eager[it][1]), (rep, it)
+            s, e, i = engine.stats()
+            assert s.sum() == (lab != INVALID).sum()
