@@ -224,6 +224,15 @@ def main():
                 "avg_launch_us": round(avg_ms * 1e3, 2),
                 "share_of_step_kernel_time": round(ms / step_kernel_ms, 3)}
         break
+    if roof:
+        # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE /
+        # WRITE_SIZE summary of the same kernel on the same graph, if any
+        import glob
+        for fn in glob.glob(os.path.join(ROOT, "profiles", "*_traffic.json")):
+            t = json.load(open(fn))
+            if t.get("kernel") == roof["kernel"] and t.get("workload_graph_sha256") == graph_sha:
+                roof["traffic"] = t["traffic_bytes"]
+                roof["traffic_source"] = os.path.relpath(fn, ROOT)
     per_kernel = {name: {"launches": la, "avg_us": round(ms / la * 1e3, 2), "total_ms": round(ms, 3)}
                   for ms, name, la in kern}
 
