@@ -18,6 +18,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <ctime>
 #include <string>
 #include <vector>
 
@@ -105,6 +106,18 @@ struct scd_ctx {
     uint64_t bin_edges[16] = {0};  // adjacency entries per bin
     bool counting = false;         // fill the counters (scd_set_counting)
 
+    // batched arena (G > 0): graph of each vertex, per-graph clustering
+    // coefficient / activity / store flags, per-graph WCC reduction items
+    uint32_t G = 0;
+    std::vector<uint64_t> h_vtx_off;  // G + 1
+    std::vector<uint32_t> h_label_off;  // G: first global label of each graph
+    uint32_t *gid = nullptr, *active_g = nullptr, *store_g = nullptr, *k_g = nullptr;
+    double *cc_g = nullptr, *wcc_g = nullptr, *wcc_g_host = nullptr;
+    uint64_t *vtx_off_d = nullptr;
+    uint32_t *witem_start = nullptr, *witem_end = nullptr, *g_item_begin = nullptr;
+    uint32_t n_witems = 0;
+    double *partial_g = nullptr;
+
     // snapshot for scd_snapshot / scd_restore
     uint32_t *snap_labels = nullptr;
     int4 *snap_stats = nullptr;
@@ -175,6 +188,11 @@ static void free_graph(scd_ctx *c) {
     drop_graphs(c);
     free_list(c, c->hub_bufs);
     free_list(c, c->bufs);
+    if (c->wcc_g_host) (void)hipHostFree(c->wcc_g_host);
+    c->wcc_g_host = nullptr;
+    c->G = 0;
+    c->gid = c->active_g = c->store_g = c->k_g = nullptr;
+    c->cc_g = c->wcc_g = nullptr;
     c->have_graph = c->have_labels = false;
     c->indices = c->adj = nullptr;
     c->nh = c->n_tiles = c->n_chunks = 0;
@@ -370,6 +388,9 @@ static int phase_move(scd_ctx *c, uint32_t *labels_out, uint32_t *ex_sorted, uin
     a.labels_out = labels_out;
     a.ex_sorted = ex_sorted;
     a.ex_count = ex_count;
+    a.gid = c->G ? c->gid : nullptr;
+    a.cc_g = c->cc_g;
+    a.active_g = c->active_g;
     auto at = [&](int bin) {
         MoveArgs b = a;
         b.counters = c->counting ? c->counters + bin : nullptr;
@@ -461,9 +482,15 @@ static int phase_wcc(scd_ctx *c, int32_t *int_tri_out, int32_t *int_tri_deg_out)
     LAUNCH(c, "wcc_wave", cdiv(nw, 4), 256, k_wcc_wave, w, bin_v(c, K_W1), nw);
     const uint32_t nb = c->bin_begin[K_HUB + 1] - c->bin_begin[K_LDS];
     LAUNCH(c, "wcc_block", nb, 256, k_wcc_block, w, bin_v(c, K_LDS), nb);
-    const uint32_t np = cdiv(c->n, 4096);
-    LAUNCH(c, "wcc_sum_partial", np, 256, k_reduce_partial, c->terms, c->n, c->partial);
-    LAUNCH(c, "wcc_sum_final", 1, 256, k_reduce_final, c->partial, np, c->n, c->wcc_dev);
+    if (c->G) {
+        LAUNCH(c, "wcc_sum_partial", c->n_witems, 256, k_reduce_partial_g, c->terms, c->witem_start, c->witem_end,
+               c->partial_g);
+        LAUNCH(c, "wcc_sum_final", c->G, 256, k_reduce_final_g, c->partial_g, c->g_item_begin, c->vtx_off_d, c->wcc_g);
+    } else {
+        const uint32_t np = cdiv(c->n, 4096);
+        LAUNCH(c, "wcc_sum_partial", np, 256, k_reduce_partial, c->terms, c->n, c->partial);
+        LAUNCH(c, "wcc_sum_final", 1, 256, k_reduce_final, c->partial, np, c->n, c->wcc_dev);
+    }
     return check_launch("wcc");
 }
 
@@ -630,7 +657,8 @@ static int enqueue_iteration(scd_ctx *c) {
     if ((rc = phase_move(c, c->next, nullptr, nullptr))) return rc;
     std::swap(c->cur, c->next);
     if ((rc = phase_stats(c)) || (rc = phase_wcc(c, nullptr, nullptr))) return rc;
-    HIP_TRY(hipMemcpyAsync(c->wcc_host, c->wcc_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (c->G) HIP_TRY(hipMemcpyAsync(c->wcc_g_host, c->wcc_g, 8ull * c->G, hipMemcpyDeviceToHost, c->stream));
+    else HIP_TRY(hipMemcpyAsync(c->wcc_host, c->wcc_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     if (c->counting)
         HIP_TRY(hipMemcpyAsync(c->counters_host, c->counters, 16 * sizeof(unsigned long long),
                                hipMemcpyDeviceToHost, c->stream));
@@ -1091,7 +1119,7 @@ extern "C" int scd_initial_partition(scd_ctx *c, uint32_t *labels_out, uint32_t 
     const uint32_t n = c->n;
     std::vector<DevBuf> tmp;
     uint64_t *khi[2] = {nullptr, nullptr};
-    uint32_t *klo[2] = {nullptr, nullptr}, *val[2] = {nullptr, nullptr};
+    uint32_t *klo[2] = {nullptr, nullptr}, *val[2] = {nullptr, nullptr}, *kg[2] = {nullptr, nullptr};
     uint32_t *flags = nullptr, *hist = nullptr, *hoff = nullptr, *rank = nullptr, *sf = nullptr;
     unsigned long long *bits = nullptr;
     uint8_t *state = nullptr;
@@ -1103,33 +1131,37 @@ extern "C" int scd_initial_partition(scd_ctx *c, uint32_t *labels_out, uint32_t 
     if ((rc = dalloc(c, &khi[0], n, &tmp)) || (rc = dalloc(c, &khi[1], n, &tmp)) ||
         (rc = dalloc(c, &klo[0], n, &tmp)) || (rc = dalloc(c, &klo[1], n, &tmp)) ||
         (rc = dalloc(c, &val[0], n, &tmp)) || (rc = dalloc(c, &val[1], n, &tmp)) ||
-        (rc = dalloc(c, &flags, 4, &tmp)) || (rc = dalloc(c, &bits, 4, &tmp)) ||
+        (rc = dalloc(c, &kg[0], n, &tmp)) || (rc = dalloc(c, &kg[1], n, &tmp)) ||
+        (rc = dalloc(c, &flags, 4, &tmp)) || (rc = dalloc(c, &bits, 6, &tmp)) ||
         (rc = dalloc(c, &hist, 256ull * ntiles, &tmp)) || (rc = dalloc(c, &hoff, 256ull * ntiles, &tmp)) ||
         (rc = dalloc(c, &rank, n, &tmp)) || (rc = dalloc(c, &sf, n, &tmp)) || (rc = dalloc(c, &state, n, &tmp)))
         return cleanup(rc);
     LAUNCH(c, "ip_clear", 1, 64, k_fill_u32, flags, 4u);
     LAUNCH(c, "ip_clear", 1, 64, k_ip_bits_init, bits);
-    LAUNCH(c, "ip_keys", cdiv(n, 256), 256, k_ip_keys, c->indices, c->tri, n, khi[0], klo[0], val[0], flags, bits);
+    LAUNCH(c, "ip_keys", cdiv(n, 256), 256, k_ip_keys, c->indices, c->tri, c->G ? c->gid : nullptr, n, kg[0],
+           khi[0], klo[0], val[0], flags, bits);
     uint32_t hflags[4];
-    unsigned long long hbits[4];
+    unsigned long long hbits[6];
     if ((rc = check_launch("initial partition"))) return cleanup(rc);
     if (hipMemcpyAsync(hflags, flags, 16, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-        hipMemcpyAsync(hbits, bits, 32, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipMemcpyAsync(hbits, bits, 48, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess)
         return cleanup(fail(SCD_EDEVICE, "scd_initial_partition: readback failed"));
     if (hflags[0])
         return cleanup(fail(SCD_EINVAL, "scd_initial_partition: a vertex of degree < 2 makes the reference's "
                                         "order key NaN (src/scd.cpp:130); convert never produces one"));
     int cur = 0;
-    for (int pass = 0; pass < 12; pass++) {
-        const uint64_t x = pass < 4 ? (hbits[2] ^ hbits[3]) >> (8 * pass) : (hbits[0] ^ hbits[1]) >> (8 * (pass - 4));
+    for (int pass = 0; pass < 16; pass++) {
+        const uint64_t x = pass < 4    ? (hbits[2] ^ hbits[3]) >> (8 * pass)
+                           : pass < 12 ? (hbits[0] ^ hbits[1]) >> (8 * (pass - 4))
+                                       : (hbits[4] ^ hbits[5]) >> (8 * (pass - 12));
         if ((x & 0xFF) == 0) continue;  // digit equal in every key
-        LAUNCH(c, "ip_sort_hist", ntiles, 256, k_rs_hist, khi[cur], klo[cur], n, pass, ntiles, hist);
+        LAUNCH(c, "ip_sort_hist", ntiles, 256, k_rs_hist, kg[cur], khi[cur], klo[cur], n, pass, ntiles, hist);
         if ((rc = check_launch("radix sort"))) return cleanup(rc);
         uint32_t total = 0;
         if ((rc = scan_u32(c, hist, hoff, 256ull * ntiles, &total, tmp))) return cleanup(rc);
-        LAUNCH(c, "ip_sort_scatter", ntiles, 256, k_rs_scatter, khi[cur], klo[cur], val[cur], n, pass, ntiles, hoff,
-               khi[1 - cur], klo[1 - cur], val[1 - cur]);
+        LAUNCH(c, "ip_sort_scatter", ntiles, 256, k_rs_scatter, kg[cur], khi[cur], klo[cur], val[cur], n, pass,
+               ntiles, hoff, kg[1 - cur], khi[1 - cur], klo[1 - cur], val[1 - cur]);
         cur = 1 - cur;
     }
     LAUNCH(c, "ip_rank", cdiv(n, 256), 256, k_ip_rank, val[cur], n, rank, state);
@@ -1145,7 +1177,9 @@ extern "C" int scd_initial_partition(scd_ctx *c, uint32_t *labels_out, uint32_t 
         if (hflags[0] == 0) break;
         if (rounds > n + 1) return cleanup(fail(SCD_EDEVICE, "scd_initial_partition: no progress"));
     }
-    LAUNCH(c, "ip_seedflags", cdiv(n, 256), 256, k_ip_seedflags, val[cur], state, n, sf);
+    if (c->G) LAUNCH(c, "ip_clear", cdiv(c->G, 256), 256, k_fill_u32, c->k_g, c->G);
+    LAUNCH(c, "ip_seedflags", cdiv(n, 256), 256, k_ip_seedflags, val[cur], state, n, c->G ? c->gid : nullptr, c->k_g,
+           sf);
     uint32_t k = 0;
     if ((rc = scan_u32(c, sf, sf, n, &k, tmp))) return cleanup(rc);
     LAUNCH(c, "ip_labels", cdiv(n, 4), 256, k_ip_labels, c->indices, c->adj, rank, state, sf, n, c->cur);
@@ -1160,4 +1194,184 @@ extern "C" int scd_initial_partition(scd_ctx *c, uint32_t *labels_out, uint32_t 
     c->have_labels = true;
     if (num_labels_out) *num_labels_out = k;
     return cleanup(SCD_OK);
+}
+
+// ---------------------------------------------------------------------------
+// Batched arenas: many graphs refined together as one disjoint union.
+// Labels are global on the device (graph g's labels start at h_label_off[g],
+// preserving each graph's label order and hence its tie-breaks) and local
+// (0-based per graph) at the API.
+// ---------------------------------------------------------------------------
+extern "C" int scd_batch_upload(scd_ctx *c, uint32_t G, const uint64_t *vtx_off, const uint32_t *indices,
+                                const uint32_t *adj, const int32_t *tri, const int32_t *tri_deg, const double *cc) {
+    if (!c || !vtx_off || !indices || !cc) return fail(SCD_EINVAL, "scd_batch_upload: null argument");
+    if (G == 0) return fail(SCD_EINVAL, "scd_batch_upload: no graphs");
+    if (vtx_off[0] != 0) return fail(SCD_EINVAL, "scd_batch_upload: vtx_off[0] must be 0");
+    for (uint32_t g = 0; g < G; g++)
+        if (vtx_off[g + 1] <= vtx_off[g])
+            return fail(SCD_EINVAL, "scd_batch_upload: graph %u has no vertices", g);
+    if (vtx_off[G] >= 0xFFFFFFFFull) return fail(SCD_ELIMIT, "scd_batch_upload: too many vertices");
+    const uint32_t n = (uint32_t)vtx_off[G];
+    const uint32_t E = indices[n];
+    if (E & 1u) return fail(SCD_EINVAL, "scd_batch_upload: odd adjacency length %u", E);
+    int rc = scd_upload(c, n, E / 2, indices, adj, tri, tri_deg, 0.0);
+    if (rc) return rc;
+    std::vector<uint32_t> gid(n), ws, we, gib(G + 1);
+    for (uint32_t g = 0; g < G; g++) {
+        for (uint64_t v = vtx_off[g]; v < vtx_off[g + 1]; v++) gid[v] = g;
+        gib[g] = (uint32_t)ws.size();
+        for (uint64_t s = vtx_off[g]; s < vtx_off[g + 1]; s += 4096) {
+            ws.push_back((uint32_t)s);
+            we.push_back((uint32_t)vtx_off[g + 1]);
+        }
+    }
+    gib[G] = (uint32_t)ws.size();
+    c->n_witems = (uint32_t)ws.size();
+    if ((rc = dalloc(c, &c->gid, n)) || (rc = dalloc(c, &c->cc_g, G)) || (rc = dalloc(c, &c->active_g, G)) ||
+        (rc = dalloc(c, &c->store_g, G)) || (rc = dalloc(c, &c->k_g, G)) || (rc = dalloc(c, &c->wcc_g, G)) ||
+        (rc = dalloc(c, &c->vtx_off_d, G + 1)) || (rc = dalloc(c, &c->witem_start, c->n_witems)) ||
+        (rc = dalloc(c, &c->witem_end, c->n_witems)) || (rc = dalloc(c, &c->g_item_begin, G + 1)) ||
+        (rc = dalloc(c, &c->partial_g, c->n_witems))) {
+        free_graph(c);
+        return rc;
+    }
+    hipError_t e = hipHostMalloc((void **)&c->wcc_g_host, 8ull * G, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        c->wcc_g_host = nullptr;
+        free_graph(c);
+        return fail(SCD_ENOMEM, "hipHostMalloc: %s", hipGetErrorString(e));
+    }
+    std::vector<uint32_t> ones(G, 1u), zeros(G, 0u);
+    HIP_TRY(hipMemcpy(c->gid, gid.data(), 4ull * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->cc_g, cc, 8ull * G, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->active_g, ones.data(), 4ull * G, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->store_g, zeros.data(), 4ull * G, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->vtx_off_d, vtx_off, 8ull * (G + 1), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->witem_start, ws.data(), 4ull * ws.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->witem_end, we.data(), 4ull * we.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->g_item_begin, gib.data(), 4ull * (G + 1), hipMemcpyHostToDevice));
+    c->G = G;
+    c->h_vtx_off.assign(vtx_off, vtx_off + G + 1);
+    c->h_label_off.assign(G, 0);
+    return SCD_OK;
+}
+
+static int need_batch(scd_ctx *c, const char *fn, bool labels) {
+    int rc = labels ? need_labels(c, fn) : need_graph(c, fn);
+    if (rc) return rc;
+    if (!c->G) return fail(SCD_ESTATE, "%s: no batch uploaded", fn);
+    return SCD_OK;
+}
+
+// Device labels (global) -> local labels per graph.
+static int labels_to_local(scd_ctx *c, const uint32_t *dev, uint32_t *out) {
+    HIP_TRY(hipMemcpyAsync(out, dev, 4ull * c->n, hipMemcpyDeviceToHost, c->stream));
+    int rc = finish(c);
+    if (rc) return rc;
+    for (uint32_t g = 0; g < c->G; g++) {
+        const uint32_t off = c->h_label_off[g];
+        for (uint64_t v = c->h_vtx_off[g]; v < c->h_vtx_off[g + 1]; v++)
+            if (out[v] != SCD_INVALID_LABEL) out[v] -= off;
+    }
+    return SCD_OK;
+}
+
+extern "C" int scd_batch_initial_partition(scd_ctx *c, uint32_t *labels_out, uint32_t *k_out) {
+    int rc = need_batch(c, "scd_batch_initial_partition", false);
+    if (rc) return rc;
+    uint32_t ktot = 0;
+    if ((rc = scd_initial_partition(c, nullptr, &ktot))) return rc;
+    std::vector<uint32_t> k(c->G);
+    HIP_TRY(hipMemcpy(k.data(), c->k_g, 4ull * c->G, hipMemcpyDeviceToHost));
+    uint32_t off = 0;
+    for (uint32_t g = 0; g < c->G; g++) {
+        c->h_label_off[g] = off;
+        off += k[g];
+    }
+    if (k_out) memcpy(k_out, k.data(), 4ull * c->G);
+    if (labels_out) return labels_to_local(c, c->cur, labels_out);
+    return SCD_OK;
+}
+
+extern "C" int scd_batch_compute_wcc(scd_ctx *c, double *wcc_out) {
+    int rc = need_batch(c, "scd_batch_compute_wcc", true);
+    if (rc) return rc;
+    if ((rc = clear_counters(c)) || (rc = phase_stats(c)) || (rc = phase_wcc(c, nullptr, nullptr))) return rc;
+    HIP_TRY(hipMemcpyAsync(c->wcc_g_host, c->wcc_g, 8ull * c->G, hipMemcpyDeviceToHost, c->stream));
+    if ((rc = finish(c))) return rc;
+    if (wcc_out) memcpy(wcc_out, c->wcc_g_host, 8ull * c->G);
+    return SCD_OK;
+}
+
+static double wall_ms() {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec * 1e3 + ts.tv_nsec / 1e6;
+}
+
+extern "C" int scd_batch_refine(scd_ctx *c, const double *initial_wcc, uint32_t max_iters, double *best_out,
+                                uint32_t *iters_out, double *done_ms, double *wcc_hist) {
+    int rc = need_batch(c, "scd_batch_refine", true);
+    if (rc) return rc;
+    if (!initial_wcc) return fail(SCD_EINVAL, "scd_batch_refine: null initial WCC");
+    const uint32_t G = c->G;
+    const double threshold = 0.01;
+    std::vector<double> best(initial_wcc, initial_wcc + G), done(G, 0.0);
+    std::vector<uint32_t> active(G, 1u), store(G, 0u), iters(G, 0u);
+    uint32_t remaining = G, it = 0;
+    const double t0 = wall_ms();
+    while (remaining > 0 && it < max_iters) {
+        it++;
+        HIP_TRY(hipMemcpyAsync(c->active_g, active.data(), 4ull * G, hipMemcpyHostToDevice, c->stream));
+        if ((rc = scd_improve(c, nullptr, nullptr))) return rc;
+        const double now = wall_ms() - t0;
+        bool any_store = false;
+        for (uint32_t g = 0; g < G; g++) {
+            store[g] = 0;
+            if (!active[g]) continue;
+            const double w = c->wcc_g_host[g];
+            if (wcc_hist) wcc_hist[(size_t)g * max_iters + iters[g]] = w;
+            iters[g]++;
+            if (w > best[g]) {
+                store[g] = 1;
+                any_store = true;
+            }
+            if (w < best[g] * (1 + threshold)) {
+                active[g] = 0;
+                remaining--;
+                done[g] = now;
+            } else {
+                best[g] = w;
+            }
+        }
+        if (any_store) {
+            HIP_TRY(hipMemcpyAsync(c->store_g, store.data(), 4ull * G, hipMemcpyHostToDevice, c->stream));
+            LAUNCH(c, "store_best", cdiv(c->n, 256), 256, k_store_best_g, c->cur, c->gid, c->store_g, c->n, c->best);
+            if ((rc = check_launch("store best")) || (rc = finish(c))) return rc;
+        }
+    }
+    const double end = wall_ms() - t0;
+    for (uint32_t g = 0; g < G; g++)
+        if (active[g]) done[g] = end;
+    if (best_out) memcpy(best_out, best.data(), 8ull * G);
+    if (iters_out) memcpy(iters_out, iters.data(), 4ull * G);
+    if (done_ms) memcpy(done_ms, done.data(), 8ull * G);
+    // leave every graph active for later calls
+    std::vector<uint32_t> ones(G, 1u);
+    HIP_TRY(hipMemcpyAsync(c->active_g, ones.data(), 4ull * G, hipMemcpyHostToDevice, c->stream));
+    return finish(c);
+}
+
+extern "C" int scd_batch_labels(scd_ctx *c, uint32_t *labels_out) {
+    int rc = need_batch(c, "scd_batch_labels", true);
+    if (rc) return rc;
+    if (!labels_out) return fail(SCD_EINVAL, "scd_batch_labels: null output");
+    return labels_to_local(c, c->best, labels_out);
+}
+
+extern "C" int scd_batch_current_labels(scd_ctx *c, uint32_t *labels_out) {
+    int rc = need_batch(c, "scd_batch_current_labels", true);
+    if (rc) return rc;
+    if (!labels_out) return fail(SCD_EINVAL, "scd_batch_current_labels: null output");
+    return labels_to_local(c, c->cur, labels_out);
 }

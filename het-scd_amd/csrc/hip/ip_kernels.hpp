@@ -37,12 +37,16 @@ __device__ __forceinline__ uint64_t orderable(double x) {
 // flags[0] is set when a degree below 2 makes the reference's key NaN.
 // bits[0..3] = AND / OR over all keys of the high and low words: digits
 // that are equal in every key need no pass.
+// In an arena (gid != nullptr) the graph id is the most significant key
+// (kg), so each graph's vertices form one contiguous, correctly ordered run
+// and seed ranks are numbered graph by graph.
 __global__ __launch_bounds__(256) void k_ip_keys(const uint32_t *__restrict__ indices, const int32_t *__restrict__ tri,
-                                                 uint32_t n, uint64_t *__restrict__ khi, uint32_t *__restrict__ klo,
-                                                 uint32_t *__restrict__ val, uint32_t *__restrict__ flags,
-                                                 unsigned long long *__restrict__ bits) {
+                                                 const uint32_t *__restrict__ gid, uint32_t n,
+                                                 uint32_t *__restrict__ kg, uint64_t *__restrict__ khi,
+                                                 uint32_t *__restrict__ klo, uint32_t *__restrict__ val,
+                                                 uint32_t *__restrict__ flags, unsigned long long *__restrict__ bits) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t h_and = ~0ull, h_or = 0, l_and = ~0ull, l_or = 0;
+    uint64_t h_and = ~0ull, h_or = 0, l_and = ~0ull, l_or = 0, g_and = ~0ull, g_or = 0;
     if (i < n) {
         const uint32_t p = n - 1 - i;
         const uint32_t deg = indices[p + 1] - indices[p];
@@ -50,11 +54,14 @@ __global__ __launch_bounds__(256) void k_ip_keys(const uint32_t *__restrict__ in
         const double cc = (double)tri[p] / ((double)deg * ((double)deg - 1.0));
         const uint64_t h = ~orderable(cc);
         const uint32_t l = ~(uint32_t)(int)deg;  // the reference compares deg as int
+        const uint32_t g = gid ? gid[p] : 0u;
         khi[i] = h;
         klo[i] = l;
+        kg[i] = g;
         val[i] = p;
         h_and = h_or = h;
         l_and = l_or = l;
+        g_and = g_or = g;
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -62,34 +69,42 @@ __global__ __launch_bounds__(256) void k_ip_keys(const uint32_t *__restrict__ in
         h_or |= (uint64_t)__shfl_xor((unsigned long long)h_or, off, SCD_WAVE);
         l_and &= (uint64_t)__shfl_xor((unsigned long long)l_and, off, SCD_WAVE);
         l_or |= (uint64_t)__shfl_xor((unsigned long long)l_or, off, SCD_WAVE);
+        g_and &= (uint64_t)__shfl_xor((unsigned long long)g_and, off, SCD_WAVE);
+        g_or |= (uint64_t)__shfl_xor((unsigned long long)g_or, off, SCD_WAVE);
     }
     if ((threadIdx.x & 63) == 0) {
         atomicAnd(&bits[0], (unsigned long long)h_and);
         atomicOr(&bits[1], (unsigned long long)h_or);
         atomicAnd(&bits[2], (unsigned long long)l_and);
         atomicOr(&bits[3], (unsigned long long)l_or);
+        atomicAnd(&bits[4], (unsigned long long)g_and);
+        atomicOr(&bits[5], (unsigned long long)g_or);
     }
 }
 
 __global__ void k_ip_bits_init(unsigned long long *bits) {
-    if (threadIdx.x < 4) bits[threadIdx.x] = (threadIdx.x & 1) ? 0ull : ~0ull;
+    if (threadIdx.x < 6) bits[threadIdx.x] = (threadIdx.x & 1) ? 0ull : ~0ull;
 }
 
-__device__ __forceinline__ uint32_t rs_digit(uint64_t hi, uint32_t lo, int pass) {
-    // passes 0..3: the low word, 4..11: the high word, least significant first
-    return pass < 4 ? (lo >> (8 * pass)) & 0xFFu : (uint32_t)(hi >> (8 * (pass - 4))) & 0xFFu;
+// passes 0..3: the low word (degree), 4..11: the high word (cc), 12..15: the
+// graph id; least significant first
+__device__ __forceinline__ uint32_t rs_digit(uint32_t g, uint64_t hi, uint32_t lo, int pass) {
+    if (pass < 4) return (lo >> (8 * pass)) & 0xFFu;
+    if (pass < 12) return (uint32_t)(hi >> (8 * (pass - 4))) & 0xFFu;
+    return (g >> (8 * (pass - 12))) & 0xFFu;
 }
 
 // Per-tile digit histogram, digit-major: hist[d * ntiles + tile].
-__global__ __launch_bounds__(256) void k_rs_hist(const uint64_t *__restrict__ khi, const uint32_t *__restrict__ klo,
-                                                 uint32_t n, int pass, uint32_t ntiles, uint32_t *__restrict__ hist) {
+__global__ __launch_bounds__(256) void k_rs_hist(const uint32_t *__restrict__ kg, const uint64_t *__restrict__ khi,
+                                                 const uint32_t *__restrict__ klo, uint32_t n, int pass, uint32_t ntiles,
+                                                 uint32_t *__restrict__ hist) {
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
     const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
     for (uint32_t k = 0; k < RS_TILE / 256; k++) {
         const uint64_t i = base + k * 256 + threadIdx.x;
-        if (i < n) atomicAdd(&h[rs_digit(khi[i], klo[i], pass)], 1u);
+        if (i < n) atomicAdd(&h[rs_digit(kg[i], khi[i], klo[i], pass)], 1u);
     }
     __syncthreads();
     hist[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
@@ -97,9 +112,10 @@ __global__ __launch_bounds__(256) void k_rs_hist(const uint64_t *__restrict__ kh
 
 // Stable scatter: elements of a tile are placed in index order within each
 // digit.  off[d * ntiles + tile] = exclusive scan of the histogram.
-__global__ __launch_bounds__(256) void k_rs_scatter(const uint64_t *__restrict__ khi, const uint32_t *__restrict__ klo,
-                                                    const uint32_t *__restrict__ val, uint32_t n, int pass,
-                                                    uint32_t ntiles, const uint32_t *__restrict__ off,
+__global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t *__restrict__ kg, const uint64_t *__restrict__ khi,
+                                                    const uint32_t *__restrict__ klo, const uint32_t *__restrict__ val,
+                                                    uint32_t n, int pass, uint32_t ntiles,
+                                                    const uint32_t *__restrict__ off, uint32_t *__restrict__ og,
                                                     uint64_t *__restrict__ ohi, uint32_t *__restrict__ olo,
                                                     uint32_t *__restrict__ oval) {
     __shared__ uint32_t s_base[256];   // next output position per digit
@@ -114,12 +130,13 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const uint64_t *__restrict__
         const uint64_t i = base + k * 256 + threadIdx.x;
         const bool act = i < n;
         uint64_t hi = 0;
-        uint32_t lo = 0, v = 0, d = 0;
+        uint32_t g = 0, lo = 0, v = 0, d = 0;
         if (act) {
+            g = kg[i];
             hi = khi[i];
             lo = klo[i];
             v = val[i];
-            d = rs_digit(hi, lo, pass);
+            d = rs_digit(g, hi, lo, pass);
         }
         // lanes of this wave holding the same digit
         uint64_t same = __ballot(act);
@@ -135,6 +152,7 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const uint64_t *__restrict__
             uint32_t before = 0;
             for (int q = 0; q < w; q++) before += s_wc[q][d];
             const uint32_t pos = s_base[d] + before + rank_in_wave;
+            og[pos] = g;
             ohi[pos] = hi;
             olo[pos] = lo;
             oval[pos] = v;
@@ -190,11 +208,15 @@ __global__ __launch_bounds__(256) void k_ip_round(const uint32_t *__restrict__ i
 }
 
 // sf[rank] = 1 for seeds (by position in the order).
+// k_g[g] += seeds of graph g (arenas only).
 __global__ void k_ip_seedflags(const uint32_t *__restrict__ val, const uint8_t *__restrict__ state, uint32_t n,
+                               const uint32_t *__restrict__ gid, uint32_t *__restrict__ k_g,
                                uint32_t *__restrict__ sf) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    sf[i] = state[val[i]] == 1 ? 1u : 0u;
+    const bool seed = state[val[i]] == 1;
+    sf[i] = seed ? 1u : 0u;
+    if (seed && gid) atomicAdd(&k_g[gid[val[i]]], 1u);
 }
 
 // Labels: a seed's label is its seed rank (seedpos = exclusive scan of sf);

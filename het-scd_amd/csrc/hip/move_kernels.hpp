@@ -37,7 +37,21 @@ struct MoveArgs {
     unsigned long long *__restrict__ counters;  // [0] += evaluated pairs
     uint32_t *__restrict__ ex_sorted;      // export: sorted keys at adj positions
     uint32_t *__restrict__ ex_count;       // export: run length at heads, else 0
+    // batched arenas (nullptr for one graph): graph of each vertex, each
+    // graph's clustering coefficient and whether it is still refining
+    const uint32_t *__restrict__ gid;
+    const double *__restrict__ cc_g;
+    const uint32_t *__restrict__ active_g;
 };
+
+__device__ __forceinline__ double vtx_pext(const MoveArgs &a, uint32_t p) {
+    return a.gid ? a.cc_g[a.gid[p]] : a.p_ext;
+}
+
+// A vertex of a graph that has stopped keeps its label.
+__device__ __forceinline__ uint32_t vtx_result(const MoveArgs &a, uint32_t p, uint32_t moved, uint32_t my) {
+    return (a.gid && !a.active_g[a.gid[p]]) ? my : moved;
+}
 
 template <int S>
 __device__ __forceinline__ uint64_t group_bits(uint64_t ballot, int gbase) {
@@ -74,6 +88,7 @@ __global__ __launch_bounds__(256) void k_move_sub(MoveArgs a, const uint32_t *__
     const uint32_t deg = active ? a.indices[p + 1] - beg : 0u;
     const uint32_t my = active ? a.labels[p] : SCD_INVALID;
 
+    const double pext = active ? vtx_pext(a, p) : 0.0;
     uint32_t v = SCD_INVALID;
     if ((uint32_t)lig < deg) v = a.labels[a.adj[beg + lig]];
     v = group_bitonic_sort<S>(v, lig);
@@ -87,7 +102,7 @@ __global__ __launch_bounds__(256) void k_move_sub(MoveArgs a, const uint32_t *__
     const bool head = boundary && v != SCD_INVALID;
 
     double imp = 0.0;
-    if (head) imp = candidate_improvement(v, freq, deg, my, a.stats, a.p_ext);
+    if (head) imp = candidate_improvement(v, freq, deg, my, a.stats, pext);
 
     const uint64_t mm = group_bits<S>(__ballot(head && v == my), gbase);
     const int src = mm ? gbase + __builtin_ctzll(mm) : lane;
@@ -102,7 +117,7 @@ __global__ __launch_bounds__(256) void k_move_sub(MoveArgs a, const uint32_t *__
     }
     group_argmax<S>(bi, bl);
 
-    if (active && lig == 0 && a.labels_out) a.labels_out[p] = move_rule(bi, bl, rem, my);
+    if (active && lig == 0 && a.labels_out) a.labels_out[p] = vtx_result(a, p, move_rule(bi, bl, rem, my), my);
     if (a.ex_sorted && (uint32_t)lig < deg) {
         a.ex_sorted[beg + lig] = v;
         a.ex_count[beg + lig] = head ? freq : 0u;
@@ -123,6 +138,7 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const uint32_t *_
     const uint32_t beg = a.indices[p];
     const uint32_t deg = a.indices[p + 1] - beg;
     const uint32_t my = a.labels[p];
+    const double pext = vtx_pext(a, p);
 
     uint32_t x[R];
 #pragma unroll
@@ -175,7 +191,7 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const uint32_t *_
     for (int r = 0; r < R; r++) {
         const uint32_t i = (uint32_t)(r * 64 + lane);
         if (freq[r]) {
-            const double imp = candidate_improvement_s(st[r], x[r], freq[r], deg, my, a.p_ext);
+            const double imp = candidate_improvement_s(st[r], x[r], freq[r], deg, my, pext);
             if (x[r] == my) {
                 rem_lane = -imp;
                 has_rem = true;
@@ -196,7 +212,7 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const uint32_t *_
     const int src = mm ? __builtin_ctzll(mm) : lane;
     const double xr = __shfl(rem_lane, src, SCD_WAVE);
     const double rem = mm ? xr : 0.0;
-    if (lane == 0 && a.labels_out) a.labels_out[p] = move_rule(bi, bl, rem, my);
+    if (lane == 0 && a.labels_out) a.labels_out[p] = vtx_result(a, p, move_rule(bi, bl, rem, my), my);
     count_pairs(a.counters, heads);
 }
 
@@ -224,7 +240,7 @@ __device__ __forceinline__ void block_argmax256(double &bi, uint32_t &bl, double
 // heads and run lengths first, then every statistics load, then the
 // evaluations.  The own-label improvement goes to *s_rem (one writer).
 __device__ __forceinline__ void eval_sorted_batched(const MoveArgs &a, const uint32_t *S, uint32_t n, uint32_t lo,
-                                                    uint32_t hi, uint32_t deg, uint32_t my, uint32_t beg,
+                                                    uint32_t hi, uint32_t deg, uint32_t my, uint32_t beg, double pext,
                                                     double &bi, uint32_t &bl, uint32_t &heads, double *s_rem,
                                                     int *s_has_rem) {
     constexpr int BATCH = 4;
@@ -247,7 +263,7 @@ __device__ __forceinline__ void eval_sorted_batched(const MoveArgs &a, const uin
         for (int k = 0; k < BATCH; k++) {
             const uint32_t i = base + k * 256 + threadIdx.x;
             if (fr[k]) {
-                const double imp = candidate_improvement_s(st[k], v[k], fr[k], deg, my, a.p_ext);
+                const double imp = candidate_improvement_s(st[k], v[k], fr[k], deg, my, pext);
                 if (v[k] == my) {
                     *s_rem = -imp;
                     *s_has_rem = 1;
@@ -315,12 +331,12 @@ __global__ __launch_bounds__(256) void k_move_lds(MoveArgs a, const uint32_t *__
     double bi = 0.0;
     uint32_t bl = SCD_INVALID;
     uint32_t heads = 0;
-    eval_sorted_batched(a, keys, n2, 0, deg, deg, my, beg, bi, bl, heads, &s_rem, &s_has_rem);
+    eval_sorted_batched(a, keys, n2, 0, deg, deg, my, beg, vtx_pext(a, p), bi, bl, heads, &s_rem, &s_has_rem);
     count_pairs(a.counters, heads);
     block_argmax256(bi, bl, s_bi, s_bl);
     if (tid == 0 && a.labels_out) {
         const double rem = s_has_rem ? s_rem : 0.0;
-        a.labels_out[p] = move_rule(bi, bl, rem, my);
+        a.labels_out[p] = vtx_result(a, p, move_rule(bi, bl, rem, my), my);
     }
 }
 
@@ -417,7 +433,7 @@ __global__ __launch_bounds__(256) void k_hub_eval(MoveArgs a, HubArgs h,
     uint32_t bl = SCD_INVALID;
     uint32_t heads = 0;
     const uint32_t end = min(o + C, d);
-    eval_sorted_batched(a, S, d, o, end, d, my, beg, bi, bl, heads, &s_rem, &s_has_rem);
+    eval_sorted_batched(a, S, d, o, end, d, my, beg, vtx_pext(a, p), bi, bl, heads, &s_rem, &s_has_rem);
     count_pairs(a.counters, heads);
     block_argmax256(bi, bl, s_bi, s_bl);
     if (tid == 0) {
@@ -443,7 +459,7 @@ __global__ __launch_bounds__(64) void k_hub_finalize(MoveArgs a, HubArgs h, uint
         }
         if (h.chunk_has_rem[c]) rem = h.chunk_rem[c];
     }
-    if (a.labels_out) a.labels_out[p] = move_rule(bi, bl, rem, my);
+    if (a.labels_out) a.labels_out[p] = vtx_result(a, p, move_rule(bi, bl, rem, my), my);
 }
 
 }  // namespace scd_dev

@@ -467,6 +467,46 @@ __global__ __launch_bounds__(256) void k_reduce_final(const double *__restrict__
     if (threadIdx.x == 0) *out = count > 0 ? r / (double)count : 0.0;
 }
 
+// Per-graph means of an arena with the single-graph tree shape: item t is
+// block b of graph g (starting at vertex gs = g's first vertex + 4096 b);
+// partial[t] sums x over that block exactly like k_reduce_partial, and
+// k_reduce_final_g reduces graph g's partials like k_reduce_final.
+__global__ __launch_bounds__(256) void k_reduce_partial_g(const double *__restrict__ x,
+                                                          const uint32_t *__restrict__ item_start,
+                                                          const uint32_t *__restrict__ item_end,
+                                                          double *__restrict__ partial) {
+    __shared__ double s[256];
+    const uint32_t base = item_start[blockIdx.x], end = item_end[blockIdx.x];
+    double acc = 0.0;
+    for (uint32_t k = 0; k < 16; k++) {
+        const uint32_t i = base + threadIdx.x + 256u * k;
+        if (i < end) acc = acc + x[i];
+    }
+    const double r = block_tree_sum256(acc, s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(256) void k_reduce_final_g(const double *__restrict__ partial,
+                                                        const uint32_t *__restrict__ g_item_begin,
+                                                        const uint64_t *__restrict__ vtx_off,
+                                                        double *__restrict__ out) {
+    __shared__ double s[256];
+    const uint32_t g = blockIdx.x;
+    const uint32_t b = g_item_begin[g], np = g_item_begin[g + 1] - b;
+    const uint32_t count = (uint32_t)(vtx_off[g + 1] - vtx_off[g]);
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < np; i += 256) acc = acc + partial[b + i];
+    const double r = block_tree_sum256(acc, s);
+    if (threadIdx.x == 0) out[g] = count > 0 ? r / (double)count : 0.0;
+}
+
+// best[p] = cur[p] for the vertices of graphs flagged in store_g.
+__global__ void k_store_best_g(const uint32_t *__restrict__ cur, const uint32_t *__restrict__ gid,
+                               const uint32_t *__restrict__ store_g, uint32_t n, uint32_t *__restrict__ best) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n && store_g[gid[p]]) best[p] = cur[p];
+}
+
 // ---------------------------------------------------------------------------
 // Exclusive scan of uint32 (wrapping) in three steps: per-tile sums, scan of
 // the tile sums (recursively, host-driven), tile-local scan plus tile offset.

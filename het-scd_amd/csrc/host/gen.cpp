@@ -276,7 +276,9 @@ static void radix_sort_u64(std::vector<uint64_t> &keys, int key_bits) {
     const size_t n = keys.size();
     if (n < 2 || key_bits <= 0) return;
     std::vector<uint64_t> tmp(n);
-    const int nthreads = std::max(1, omp_get_max_threads());
+    // inside an enclosing parallel region (one graph per thread) the nested
+    // regions below run with a single thread: size the split for that
+    const int nthreads = omp_in_parallel() ? 1 : std::max(1, omp_get_max_threads());
     const int B = 11, R = 1 << B;
     std::vector<size_t> hist((size_t)nthreads * R);
     for (int shift = 0; shift < key_bits; shift += B) {
@@ -453,4 +455,108 @@ extern "C" int scd_convert(const scd_edges *e, scd_graph **out) {
     }
     *out = g;
     return SCD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Arenas: several graphs as one disjoint union (vertex ids offset per graph)
+// ---------------------------------------------------------------------------
+static int concat_graphs(const std::vector<const scd_graph *> &gs, scd_graph **out, uint64_t *vtx_off) {
+    uint64_t n = 0, m = 0;
+    for (size_t i = 0; i < gs.size(); i++) {
+        vtx_off[i] = n;
+        n += gs[i]->n;
+        m += gs[i]->m;
+    }
+    vtx_off[gs.size()] = n;
+    if (n >= 0xFFFFFFFFull || m >= (1ull << 31)) {
+        set_host_error("arena of %llu vertices and %llu edges exceeds the u32 format",
+                       (unsigned long long)n, (unsigned long long)m);
+        return SCD_ELIMIT;
+    }
+    scd_graph *g = (scd_graph *)calloc(1, sizeof(scd_graph));
+    if (!g) return SCD_ENOMEM;
+    g->n = (uint32_t)n;
+    g->m = (uint32_t)m;
+    g->indices = (uint32_t *)malloc(4 * (n + 1));
+    g->adj = (uint32_t *)malloc(4 * (2 * m + 1));
+    g->tri = (int32_t *)malloc(4 * (n + 1));
+    g->tri_deg = (int32_t *)malloc(4 * (n + 1));
+    if (!g->indices || !g->adj || !g->tri || !g->tri_deg) {
+        scd_graph_free(g);
+        set_host_error("arena: out of host memory");
+        return SCD_ENOMEM;
+    }
+    std::vector<uint64_t> eoff(gs.size() + 1, 0);
+    for (size_t i = 0; i < gs.size(); i++) eoff[i + 1] = eoff[i] + 2ull * gs[i]->m;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t ii = 0; ii < (int64_t)gs.size(); ii++) {
+        const scd_graph *s = gs[ii];
+        const uint32_t vo = (uint32_t)vtx_off[ii], eo = (uint32_t)eoff[ii];
+        for (uint32_t v = 0; v < s->n; v++) {
+            g->indices[vo + v] = eo + s->indices[v];
+            g->tri[vo + v] = s->tri[v];
+            g->tri_deg[vo + v] = s->tri_deg[v];
+        }
+        for (uint64_t k = 0; k < 2ull * s->m; k++) g->adj[eo + k] = vo + s->adj[k];
+    }
+    g->indices[n] = (uint32_t)(2 * m);
+    *out = g;
+    return SCD_OK;
+}
+
+extern "C" int scd_graph_concat(uint32_t count, const scd_graph *const *graphs, scd_graph **out,
+                                uint64_t *vtx_off) {
+    if (!out || !vtx_off || (count && !graphs)) {
+        set_host_error("scd_graph_concat: null argument");
+        return SCD_EINVAL;
+    }
+    std::vector<const scd_graph *> gs(graphs, graphs + count);
+    return concat_graphs(gs, out, vtx_off);
+}
+
+extern "C" int scd_gen_planted_batch(uint64_t seed_base, uint32_t count, uint32_t n_min, uint32_t n_max,
+                                     uint32_t k_min, uint32_t k_max, double p_in, double p_out,
+                                     scd_graph **out, uint64_t *vtx_off, double *cc, uint64_t *seeds) {
+    if (!out || !vtx_off || count == 0) {
+        set_host_error("scd_gen_planted_batch: invalid argument");
+        return SCD_EINVAL;
+    }
+    std::vector<scd_graph *> gs(count, nullptr);
+    std::vector<int> rcs(count, SCD_OK);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t ii = 0; ii < (int64_t)count; ii++) {
+        // a graph that the convert pipeline empties (no triangle) is drawn
+        // again with the next seed of its own sequence
+        for (uint64_t attempt = 0; attempt < 64; attempt++) {
+            const uint64_t seed = seed_base + (uint64_t)ii + attempt * (uint64_t)count;
+            scd_edges *e = nullptr;
+            int rc = scd_gen_planted_random(seed, n_min, n_max, k_min, k_max, p_in, p_out, nullptr, nullptr, &e);
+            scd_graph *g = nullptr;
+            if (rc == SCD_OK) rc = scd_convert(e, &g);
+            scd_edges_free(e);
+            if (rc != SCD_OK) {
+                rcs[ii] = rc;
+                break;
+            }
+            if (g->n > 0) {
+                gs[ii] = g;
+                if (seeds) seeds[ii] = seed;
+                break;
+            }
+            scd_graph_free(g);
+        }
+        if (!gs[ii] && rcs[ii] == SCD_OK) rcs[ii] = SCD_EINVAL;
+    }
+    int rc = SCD_OK;
+    for (uint32_t i = 0; i < count; i++)
+        if (rcs[i] != SCD_OK) rc = rcs[i];
+    if (rc == SCD_OK) {
+        if (cc)
+            for (uint32_t i = 0; i < count; i++) cc[i] = scd_graph_clustering_coef(gs[i]);
+        rc = concat_graphs(std::vector<const scd_graph *>(gs.begin(), gs.end()), out, vtx_off);
+    } else {
+        set_host_error("scd_gen_planted_batch: generating a graph failed (%d)", rc);
+    }
+    for (auto g : gs) scd_graph_free(g);
+    return rc;
 }

@@ -94,6 +94,10 @@ def host():
         "scd_edges_free": (None, [Ed]),
         "scd_edges_write_text": (ctypes.c_int, [Ed, ctypes.c_char_p]),
         "scd_convert": (ctypes.c_int, [Ed, ctypes.POINTER(G)]),
+        "scd_graph_concat": (ctypes.c_int, [ctypes.c_uint32, ctypes.POINTER(G), ctypes.POINTER(G), u64p]),
+        "scd_gen_planted_batch": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                                 ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_double,
+                                                 ctypes.POINTER(G), u64p, f64p, u64p]),
     }
     for k, (res, args) in sig.items():
         f = getattr(lib, k)
@@ -142,6 +146,12 @@ def hip():
         "scd_set_counting": (ctypes.c_int, [C, ctypes.c_int]),
         "scd_restore": (ctypes.c_int, [C]),
         "scd_calibration_copy": (ctypes.c_int, [C, ctypes.c_uint64, ctypes.c_int, f64p]),
+        "scd_batch_upload": (ctypes.c_int, [C, ctypes.c_uint32, u64p, u32p, u32p, i32p, i32p, f64p]),
+        "scd_batch_initial_partition": (ctypes.c_int, [C, u32p, u32p]),
+        "scd_batch_compute_wcc": (ctypes.c_int, [C, f64p]),
+        "scd_batch_refine": (ctypes.c_int, [C, f64p, ctypes.c_uint32, f64p, u32p, f64p, f64p]),
+        "scd_batch_labels": (ctypes.c_int, [C, u32p]),
+        "scd_batch_current_labels": (ctypes.c_int, [C, u32p]),
     }
     for k, (res, args) in sig.items():
         f = getattr(lib, k, None)

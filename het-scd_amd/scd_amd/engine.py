@@ -204,3 +204,66 @@ class Engine:
 
     def device_bytes(self):
         return int(_lib.hip().scd_device_bytes(self._ctx))
+
+
+class BatchEngine:
+    """Batched refinement of an Arena (scd_batch_*): every graph follows the
+    reference loop with its own stopping rule; labels are local per graph."""
+
+    def __init__(self, device=0, profile=False):
+        self.eng = Engine(device=device, profile=profile)
+        self.arena = None
+
+    @property
+    def _ctx(self):
+        return self.eng._ctx
+
+    def close(self):
+        self.eng.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, arena):
+        g = arena.graph
+        check_hip(_lib.hip().scd_batch_upload(self._ctx, arena.count, ptr(arena.vtx_off, u64p), ptr(g.indices, u32p),
+                                              ptr(g.adj, u32p), ptr(g.tri, i32p), ptr(g.tri_deg, i32p),
+                                              ptr(arena.cc, f64p)))
+        self.arena = arena
+        self.eng.n = g.n
+        self.eng.graph = g
+
+    def initial_partition(self):
+        lab = np.empty(self.arena.graph.n, np.uint32)
+        k = np.empty(self.arena.count, np.uint32)
+        check_hip(_lib.hip().scd_batch_initial_partition(self._ctx, ptr(lab, u32p), ptr(k, u32p)))
+        return lab, k
+
+    def compute_wcc(self):
+        w = np.empty(self.arena.count, np.float64)
+        check_hip(_lib.hip().scd_batch_compute_wcc(self._ctx, ptr(w, f64p)))
+        return w
+
+    def refine(self, initial_wcc, max_iters=1000, history=False):
+        G = self.arena.count
+        w0 = np.ascontiguousarray(initial_wcc, np.float64)
+        best = np.empty(G, np.float64)
+        iters = np.empty(G, np.uint32)
+        done = np.empty(G, np.float64)
+        hist = np.zeros((G, max_iters), np.float64) if history else None
+        check_hip(_lib.hip().scd_batch_refine(self._ctx, ptr(w0, f64p), max_iters, ptr(best, f64p), ptr(iters, u32p),
+                                              ptr(done, f64p), ptr(hist, f64p)))
+        return (best, iters, done, hist) if history else (best, iters, done)
+
+    def labels(self):
+        lab = np.empty(self.arena.graph.n, np.uint32)
+        check_hip(_lib.hip().scd_batch_labels(self._ctx, ptr(lab, u32p)))
+        return lab
+
+    def current_labels(self):
+        lab = np.empty(self.arena.graph.n, np.uint32)
+        check_hip(_lib.hip().scd_batch_current_labels(self._ctx, ptr(lab, u32p)))
+        return lab

@@ -165,3 +165,52 @@ def compress_count(labels):
     labels = np.asarray(labels, dtype=np.uint32)
     valid = labels[labels != _lib.INVALID_LABEL]
     return int(len(labels) - len(np.unique(valid)))
+
+
+class Arena:
+    """Several graphs as one disjoint union (scd_graph_concat): `graph` is
+    the union, graph i owns vertices [vtx_off[i], vtx_off[i+1]), cc[i] is its
+    clustering coefficient."""
+
+    def __init__(self, graph, vtx_off, cc, seeds=None):
+        self.graph = graph
+        self.vtx_off = np.ascontiguousarray(vtx_off, dtype=np.uint64)
+        self.cc = np.ascontiguousarray(cc, dtype=np.float64)
+        self.seeds = seeds
+
+    @property
+    def count(self):
+        return len(self.cc)
+
+    def member(self, i):
+        """Graph i on its own (ids 0-based)."""
+        a, b = int(self.vtx_off[i]), int(self.vtx_off[i + 1])
+        g = self.graph
+        e0, e1 = int(g.indices[a]), int(g.indices[b])
+        return Graph(b - a, (e1 - e0) // 2, g.indices[a:b + 1] - g.indices[a], g.adj[e0:e1] - np.uint32(a),
+                     g.tri[a:b], g.tri_deg[a:b])
+
+    @staticmethod
+    def concat(graphs):
+        arr = (_lib.ScdGraphC * len(graphs))()
+        for i, g in enumerate(graphs):
+            arr[i] = g._as_c()
+        ptrs = (ctypes.POINTER(_lib.ScdGraphC) * len(graphs))(*[ctypes.pointer(arr[i]) for i in range(len(graphs))])
+        gp = ctypes.POINTER(_lib.ScdGraphC)()
+        off = np.zeros(len(graphs) + 1, np.uint64)
+        check_host(_lib.host().scd_graph_concat(len(graphs), ptrs, ctypes.byref(gp), ptr(off, _lib.u64p)))
+        return Arena(Graph._from_c(gp), off, [g.clustering_coef for g in graphs])
+
+
+def planted_batch(count, seed_base=1, n_min=64, n_max=20000, k_min=4, k_max=64, p_in=0.15, p_out=0.002):
+    """`count` planted-partition graphs (n log-uniform in [n_min, n_max], k
+    uniform in [k_min, k_max], Dirichlet(1) sizes), each converted, drawn in
+    parallel and concatenated into an Arena."""
+    gp = ctypes.POINTER(_lib.ScdGraphC)()
+    off = np.zeros(count + 1, np.uint64)
+    cc = np.zeros(count, np.float64)
+    seeds = np.zeros(count, np.uint64)
+    check_host(_lib.host().scd_gen_planted_batch(seed_base, count, n_min, n_max, k_min, k_max, p_in, p_out,
+                                                 ctypes.byref(gp), ptr(off, _lib.u64p), ptr(cc, _lib.f64p),
+                                                 ptr(seeds, _lib.u64p)))
+    return Arena(Graph._from_c(gp), off, cc, seeds)

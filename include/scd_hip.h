@@ -225,6 +225,39 @@ int scd_prof_reset(scd_ctx *ctx);
 /* Device bytes currently allocated by the context. */
 uint64_t scd_device_bytes(const scd_ctx *ctx);
 
+/*
+ * Batched arenas (no counterpart in the reference, which refines one graph
+ * per process): G independent graphs refined together, every kernel launch
+ * covering all of them.  The arena is the disjoint union of the graphs
+ * (scd_graph_concat / scd_gen_planted_batch, scd_host.h): graph g owns
+ * vertices [vtx_off[g], vtx_off[g+1]), its adjacency stays inside that range,
+ * cc[g] is its clustering coefficient.  Each graph follows the loop of
+ * src/main-gpu.cpp:71-96 with its own stopping rule; its labels, statistics,
+ * per-iteration WCC (same summation order as a single-graph context) and
+ * best labels equal the single-graph path's.  Labels at this API are local
+ * (0-based per graph).
+ */
+int scd_batch_upload(scd_ctx *ctx, uint32_t G, const uint64_t *vtx_off, const uint32_t *indices,
+                     const uint32_t *adj, const int32_t *tri, const int32_t *tri_deg,
+                     const double *cc);
+/* Initial partition of every graph; k_out (G entries) the label counts. */
+int scd_batch_initial_partition(scd_ctx *ctx, uint32_t *labels_out, uint32_t *k_out);
+/* WCC of every graph's current labels (G entries). */
+int scd_batch_compute_wcc(scd_ctx *ctx, double *wcc_out);
+/*
+ * Refinement loops of all graphs; stops when every graph has stopped or
+ * after max_iters iterations.  Outputs (G entries each, may be NULL): best
+ * WCC, iterations, milliseconds from the call's start to the end of the
+ * graph's last iteration; wcc_hist (G x max_iters, may be NULL) the WCC of
+ * each graph's iterations.
+ */
+int scd_batch_refine(scd_ctx *ctx, const double *initial_wcc, uint32_t max_iters, double *best_out,
+                     uint32_t *iters_out, double *done_ms, double *wcc_hist);
+/* Best labels (local) of every vertex of the arena. */
+int scd_batch_labels(scd_ctx *ctx, uint32_t *labels_out);
+/* Current labels (local). */
+int scd_batch_current_labels(scd_ctx *ctx, uint32_t *labels_out);
+
 #ifdef __cplusplus
 }
 #endif

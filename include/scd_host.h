@@ -96,6 +96,26 @@ int scd_edges_write_text(const scd_edges *e, const char *path);
  */
 int scd_convert(const scd_edges *e, scd_graph **out);
 
+/*
+ * Arena of several graphs as one disjoint union: graph i's vertices are
+ * [vtx_off[i], vtx_off[i+1]) and its adjacency ids are offset by vtx_off[i]
+ * (vtx_off holds count + 1 entries).  The batched engine entry points take
+ * this layout.
+ */
+int scd_graph_concat(uint32_t count, const scd_graph *const *graphs, scd_graph **out,
+                     uint64_t *vtx_off);
+
+/*
+ * `count` planted-partition graphs (scd_gen_planted_random with seeds
+ * seed_base + i), each through scd_convert, generated in parallel and
+ * concatenated into an arena.  A graph that convert empties is drawn again
+ * with seed + count.  cc (count entries, may be NULL) receives each graph's
+ * clustering coefficient, seeds (may be NULL) the seed used.
+ */
+int scd_gen_planted_batch(uint64_t seed_base, uint32_t count, uint32_t n_min, uint32_t n_max,
+                          uint32_t k_min, uint32_t k_max, double p_in, double p_out,
+                          scd_graph **out, uint64_t *vtx_off, double *cc, uint64_t *seeds);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,70 @@
+"""Batched arenas (MI355X): every graph of a batch refined together gets
+exactly what the single-graph path gives it alone — initial labels, label
+counts, per-iteration WCC bits, iteration count, best WCC and best labels —
+and the single-graph path itself matches the oracle."""
+import numpy as np
+import pytest
+
+import oracle_bind as orc
+import scd_amd
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small_batch():
+    # 48 graphs from the batched workload's distribution, capped at 3,000
+    # vertices so the per-graph reference runs stay short
+    return scd_amd.planted_batch(48, seed_base=7, n_min=64, n_max=3000, k_min=4, k_max=64, p_in=0.15, p_out=0.002)
+
+
+def test_batch_equals_single_graph_runs(engine, small_batch):
+    a = small_batch
+    print(f"[gpu] batch of {a.count}: n={a.graph.n} m={a.graph.m}", flush=True)
+    with scd_amd.BatchEngine(device=0) as be:
+        be.upload(a)
+        lab0, k = be.initial_partition()
+        w0 = be.compute_wcc()
+        best, iters, done, hist = be.refine(w0, max_iters=100, history=True)
+        lab_best = be.labels()
+    for i in range(a.count):
+        g = a.member(i)
+        lo, hi = int(a.vtx_off[i]), int(a.vtx_off[i + 1])
+        engine.transfer_graph(g)
+        l0, ki = engine.initial_partition()
+        assert ki == k[i] and np.array_equal(l0, lab0[lo:hi]), f"graph {i}: initial partition"
+        wi = engine.compute_wcc()
+        assert wi == w0[i], f"graph {i}: initial WCC"
+        bi, it, h = engine.refine(wi)
+        assert it == iters[i], f"graph {i}: iterations {it} vs {iters[i]}"
+        assert np.array_equal(h, hist[i, :it]), f"graph {i}: WCC per iteration"
+        assert bi == best[i]
+        assert np.array_equal(engine.labels_from_device(), lab_best[lo:hi]), f"graph {i}: best labels"
+    assert np.all(done > 0)
+
+
+def test_batch_members_match_oracle(small_batch):
+    """A few members through the oracle's driver (the CPU restatement of
+    src/main.cpp): same iteration count, best labels and WCC."""
+    a = small_batch
+    with scd_amd.BatchEngine(device=0) as be:
+        be.upload(a)
+        be.initial_partition()
+        best, iters, done = be.refine(be.compute_wcc())
+        lab = be.labels()
+    for i in (0, 5, 17, 31, 47):
+        g = a.member(i)
+        r = orc.run(g)
+        lo, hi = int(a.vtx_off[i]), int(a.vtx_off[i + 1])
+        assert r["iterations"] == iters[i]
+        assert np.array_equal(r["best_labels"], lab[lo:hi])
+        assert best[i] == pytest.approx(r["final_wcc"], rel=1e-9, abs=0)
+
+
+def test_batch_rejects_empty_member(engine):
+    g = scd_amd.from_edge_pairs([(0, 1), (1, 2), (2, 0)]).convert()
+    arena = scd_amd.Arena.concat([g, g])
+    arena.vtx_off[1] = 0  # graph 0 without vertices
+    with scd_amd.BatchEngine(device=0) as be:
+        with pytest.raises(scd_amd.ScdError):
+            be.upload(arena)
