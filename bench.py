@@ -128,6 +128,77 @@ def cpu_baseline(path, m, reps):
                       f"{ms:.1f} ms per iteration, median of {reps} runs)"}
 
 
+def run_batched(args, scd_amd, np):
+    """Batched workload: planted-partition graphs (n log-uniform in [64,
+    20000], 4-64 communities, Dirichlet(1) sizes, p_in 0.15, p_out 0.002),
+    each through the convert pipeline, resident in HBM.  Per graph the work is
+    initial partition + initial WCC + refinement loop.  Batched: all graphs in
+    one arena, every launch covering all of them, each iteration a captured
+    hipGraph.  Plain: one graph at a time on one context with eager
+    launches (the reference's pattern).  Latency of a graph: batched = from the
+    batch start to the end of its last iteration; plain = its own run."""
+    t = time.perf_counter()
+    a = scd_amd.planted_batch(args.batch_graphs, seed_base=1, n_min=64, n_max=20000, k_min=4, k_max=64,
+                              p_in=0.15, p_out=0.002)
+    gen_s = time.perf_counter() - t
+    G = a.count
+    log(f"[bench] batched workload: {G} graphs, n={a.graph.n} m={a.graph.m} ({gen_s:.1f} s to generate)")
+    be = scd_amd.BatchEngine(device=0)
+    t = time.perf_counter()
+    be.upload(a)
+    up_s = time.perf_counter() - t
+    # warm-up on the same arena (kernel loading, graph capture), then timed
+    be.initial_partition(want_labels=False)
+    be.refine(be.compute_wcc(), max_iters=2)
+    log(f"[bench] batched warm-up done (upload {up_s:.2f} s)")
+    t0 = time.perf_counter()
+    be.initial_partition(want_labels=False)
+    w0 = be.compute_wcc()
+    t1 = time.perf_counter()
+    best, iters, done = be.refine(w0)
+    t2 = time.perf_counter()
+    lab_b = be.labels()
+    batched_s = t2 - t0
+    lat_b = (t1 - t0) * 1e3 + done
+    be.close()
+    log(f"[bench] batched: {batched_s * 1e3:.1f} ms for {G} graphs ({G / batched_s:.0f} graphs/s), "
+        f"iterations max {int(iters.max())}")
+    eng = scd_amd.Engine(device=0)
+    eng.set_graph_mode(False)
+    plain_ms = np.zeros(G)
+    mismatch = 0
+    warm = a.member(0)
+    eng.transfer_graph(warm)
+    eng.initial_partition(want_labels=False)
+    eng.refine(eng.compute_wcc())
+    for i in range(G):
+        g = a.member(i)
+        eng.transfer_graph(g, clustering_coef=float(a.cc[i]))
+        ts = time.perf_counter()
+        eng.initial_partition(want_labels=False)
+        w = eng.compute_wcc()
+        eng.refine(w)
+        plain_ms[i] = (time.perf_counter() - ts) * 1e3
+        lo, hi = int(a.vtx_off[i]), int(a.vtx_off[i + 1])
+        if not np.array_equal(eng.labels_from_device(), lab_b[lo:hi]):
+            mismatch += 1
+        if (i + 1) % 512 == 0:
+            log(f"[bench] plain: {i + 1} graphs, {plain_ms[:i + 1].sum():.0f} ms so far")
+    eng.close()
+    plain_s = plain_ms.sum() / 1e3
+    log(f"[bench] plain: {plain_s * 1e3:.1f} ms summed over {G} graphs ({G / plain_s:.0f} graphs/s); "
+        f"label mismatches {mismatch}")
+    pct = lambda x, q: round(float(np.percentile(x, q)), 3)
+    return {"workload": f"{G} planted-partition graphs, n log-uniform [64, 20000], 4-64 communities "
+                        "Dirichlet(1), p_in 0.15, p_out 0.002, seeds 1.., convert-preprocessed",
+            "graphs": G, "n": a.graph.n, "m": a.graph.m, "generate_s": round(gen_s, 1), "upload_s": round(up_s, 2),
+            "batched": {"graphs_per_s": round(G / batched_s, 1), "total_ms": round(batched_s * 1e3, 2),
+                        "p50_ms": pct(lat_b, 50), "p99_ms": pct(lat_b, 99), "max_iterations": int(iters.max())},
+            "plain": {"graphs_per_s": round(G / plain_s, 1), "total_ms": round(plain_s * 1e3, 2),
+                      "p50_ms": pct(plain_ms, 50), "p99_ms": pct(plain_ms, 99)},
+            "speedup": round(plain_s / batched_s, 2), "labels_identical": mismatch == 0}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +209,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tc", action="store_true")
     ap.add_argument("--profile-json", default=None, help="write per-kernel timings here")
+    ap.add_argument("--batched", action="store_true", help="also measure the batched workload (4,096 graphs)")
+    ap.add_argument("--batch-graphs", type=int, default=4096)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain copy kernel over MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -280,6 +353,9 @@ def main():
                           "labels_bitexact": bool(np.array_equal(last, gpu_labels))}
                 log(f"[bench] parity vs the reference's own CPU code: {parity}")
 
+    tuning = eng.get_tuning()
+    eng.close()
+    batched = run_batched(args, scd_amd, np) if args.batched else None
     result = {
         "metric": METRIC, "value": round(value, 1), "unit": "edges/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
@@ -287,7 +363,7 @@ def main():
         "config": {"workload": f"R-MAT scale {args.scale}, edge factor 16 (a=0.57,b=0.19,c=0.19), seed 1, "
                                "convert-preprocessed; full SCD refinement loop from the initial partition",
                    "n": g.n, "m": g.m, "iterations": iters, "graph_sha256": graph_sha,
-                   "tuning": eng.get_tuning()},
+                   "tuning": tuning},
         "roofline": roof,
         "cpu_baseline": cpu,
         "vs_cpu": round(value / cpu["value"], 2) if cpu else None,
@@ -295,15 +371,16 @@ def main():
         "setup_ms": {"upload": round(t_up * 1e3, 2), "initial_partition": round(t_ip * 1e3, 2)},
         "eager_profiled_ms_per_step": round(eager_ms_step, 4),
         "triangle_counting": tc,
+        "batched": batched,
     }
     if args.calibrate_copy:
-        ms = eng.calibration_copy(args.calibrate_copy << 20, reps=3)
+        with scd_amd.Engine(device=0) as ce:
+            ms = ce.calibration_copy(args.calibrate_copy << 20, reps=3)
         log(f"[bench] calibration copy of {args.calibrate_copy} MiB: {ms:.3f} ms "
             f"({2 * (args.calibrate_copy << 20) / (ms * 1e-3) / 1e9:.0f} GB/s read+write)")
     if args.profile_json:
         with open(args.profile_json, "w") as f:
             json.dump({"per_kernel": per_kernel, "bins": bins, "tc": tc}, f, indent=1)
-    eng.close()
     print(json.dumps(result), flush=True)
     return 0
 

@@ -28,6 +28,7 @@
 #include "stats_kernels.hpp"
 #include "tc_kernels.hpp"
 #include "ip_kernels.hpp"
+#include "wcc_kernels.hpp"
 
 using namespace scd_dev;
 
@@ -96,6 +97,14 @@ struct scd_ctx {
     uint32_t *cur = nullptr, *next = nullptr, *best = nullptr;
     int4 *stats = nullptr;
     uint32_t *int_adj = nullptr, *int_deg = nullptr;
+    uint8_t *iflag = nullptr;                              // per internal entry
+    uint32_t *wdplus = nullptr, *wT = nullptr, *wtd = nullptr;  // internal triangle counts
+    // per label: community member set changed by the last move
+    uint8_t *wdirty = nullptr;
+    // wT / wtd hold the counts of the labels in cur (set by every WCC
+    // evaluation, cleared by everything that writes cur otherwise): the
+    // next iteration then recounts only the communities its moves change
+    bool wcc_valid = false;
     double *terms = nullptr, *partial = nullptr, *wcc_dev = nullptr;
     double *wcc_host = nullptr;  // pinned: [0] mean WCC of the last pass
     // device counters of the last pass: [0, K_NB) evaluated pairs per degree
@@ -128,6 +137,13 @@ struct scd_ctx {
     uint32_t *bin_verts = nullptr;
     uint32_t bin_begin[K_NB + 1] = {0};
 
+    // work queue of the internal triangle count (vertices of the W1..HUB
+    // bins, WTC_VC out-list entries per item): chunk counts, their scan
+    // (device-only, scan levels preallocated), item total and counter
+    uint32_t wq_M = 0;
+    uint32_t *wq_nch = nullptr, *wq_off = nullptr, *wq_tot = nullptr, *wq_next = nullptr;
+    uint32_t *wq_sum[4] = {nullptr}, *wq_scan[4] = {nullptr};
+
     // hubs
     uint32_t nh = 0, n_tiles = 0, n_chunks = 0, max_hub_deg = 0;
     uint32_t *hub_v = nullptr, *hub_d = nullptr, *hub_off = nullptr, *hub_chunk_begin = nullptr;
@@ -145,7 +161,7 @@ struct scd_ctx {
     bool use_graph = true;
     bool capturing = false;
     uint32_t *lab_a = nullptr, *lab_b = nullptr;
-    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [parity][incremental WCC]
 
     // profiling
     std::vector<ProfRec> prof;
@@ -178,10 +194,11 @@ static void free_list(scd_ctx *c, std::vector<DevBuf> &list) {
 }
 
 static void drop_graphs(scd_ctx *c) {
-    for (int i = 0; i < 2; i++) {
-        if (c->gexec[i]) (void)hipGraphExecDestroy(c->gexec[i]);
-        c->gexec[i] = nullptr;
-    }
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) {
+            if (c->gexec[i][j]) (void)hipGraphExecDestroy(c->gexec[i][j]);
+            c->gexec[i][j] = nullptr;
+        }
 }
 
 static void free_graph(scd_ctx *c) {
@@ -194,6 +211,7 @@ static void free_graph(scd_ctx *c) {
     c->gid = c->active_g = c->store_g = c->k_g = nullptr;
     c->cc_g = c->wcc_g = nullptr;
     c->have_graph = c->have_labels = false;
+    c->wcc_valid = false;
     c->indices = c->adj = nullptr;
     c->nh = c->n_tiles = c->n_chunks = 0;
 }
@@ -298,6 +316,25 @@ static int build_bins(scd_ctx *c) {
     int rc;
     if ((rc = dalloc(c, &c->bin_verts, n, &c->hub_bufs))) return rc;
     HIP_TRY(hipMemcpy(c->bin_verts, order.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+
+    // internal triangle work queue
+    c->wq_M = c->bin_begin[K_NB] - c->bin_begin[K_W1];
+    if (c->wq_M) {
+        if ((rc = dalloc(c, &c->wq_nch, c->wq_M, &c->hub_bufs)) ||
+            (rc = dalloc(c, &c->wq_off, c->wq_M, &c->hub_bufs)) || (rc = dalloc(c, &c->wq_tot, 1, &c->hub_bufs)) ||
+            (rc = dalloc(c, &c->wq_next, 1, &c->hub_bufs)))
+            return rc;
+        uint64_t len = c->wq_M;
+        for (int l = 0; l < 4; l++) {
+            const uint64_t nt = cdiv(len, 2048);
+            if ((rc = dalloc(c, &c->wq_sum[l], nt, &c->hub_bufs)) ||
+                (rc = dalloc(c, &c->wq_scan[l], nt, &c->hub_bufs)))
+                return rc;
+            if (nt == 1) break;
+            if (l == 3) return fail(SCD_ELIMIT, "work queue scan deeper than 4 levels");
+            len = nt;
+        }
+    }
 
     // hubs
     const uint32_t hb0 = c->bin_begin[K_HUB], nh = c->bin_begin[K_HUB + 1] - hb0;
@@ -435,6 +472,9 @@ static int phase_stats(scd_ctx *c) {
     s.stats = c->stats;
     s.int_adj = c->int_adj;
     s.int_deg = c->int_deg;
+    s.iflag = c->iflag;
+    s.gid = c->G ? c->gid : nullptr;
+    s.active_g = c->active_g;
     s.counters = c->counting ? c->counters + K_NB : nullptr;
     LAUNCH(c, "filter_sub8", cdiv(bin_n(c, K_SUB8), 32), 256, k_filter_sub<8>, s, bin_v(c, K_SUB8), bin_n(c, K_SUB8));
     LAUNCH(c, "filter_sub16", cdiv(bin_n(c, K_SUB16), 16), 256, k_filter_sub<16>, s, bin_v(c, K_SUB16), bin_n(c, K_SUB16));
@@ -458,13 +498,34 @@ static int phase_stats(scd_ctx *c) {
                c->hub_chunk_begin, c->chunk_hub, c->chunk_start, C, c->chunk_cnt);
     }
     LAUNCH(c, "stats_agg", std::min<uint32_t>(cdiv(c->n, 1024), 2u * c->num_cus), 1024, k_stats_agg, c->indices,
-           c->cur, c->int_deg, c->n, c->stats);
+           c->cur, c->int_deg, c->n, c->G ? c->gid : nullptr, c->active_g, c->stats);
     return check_launch("stats");
 }
 
+// Exclusive scan of n uint32 values (in -> out, no aliasing) on the stream
+// without host involvement: tile sums, scan of the tile sums (recursively,
+// into the level buffers preallocated in wq_sum / wq_scan), tile scans; the
+// total goes to total_dev when given.  Capturable: no host copies, no
+// memset nodes.
+static int scan_u32_dev(scd_ctx *c, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *total_dev, int level) {
+    const uint32_t nt = cdiv(n, 2048);
+    uint32_t *sums = c->wq_sum[level], *scan = c->wq_scan[level];
+    LAUNCH(c, "scan_tile_sums", nt, 256, k_scan_tile_sums, in, n, sums);
+    if (nt > 1) {
+        int rc = scan_u32_dev(c, sums, scan, nt, nullptr, level + 1);
+        if (rc) return rc;
+    } else {
+        LAUNCH(c, "scan_zero", 1, 64, k_fill_u32, scan, 1u);
+    }
+    if (total_dev) LAUNCH(c, "scan_total", 1, 64, k_scan_total, scan, sums, nt, total_dev);
+    LAUNCH(c, "scan_tiles", nt, 256, k_scan_tiles, in, n, scan, out);
+    return SCD_OK;
+}
+
 // WCC terms of the current labels and their mean into wcc_dev.
-static int phase_wcc(scd_ctx *c, int32_t *int_tri_out, int32_t *int_tri_deg_out) {
+static int phase_wcc(scd_ctx *c, int32_t *int_tri_out, int32_t *int_tri_deg_out, bool incremental = false) {
     WccArgs w;
+    w.dirty = incremental ? c->wdirty : nullptr;
     w.indices = c->indices;
     w.labels = c->cur;
     w.stats = c->stats;
@@ -472,16 +533,29 @@ static int phase_wcc(scd_ctx *c, int32_t *int_tri_out, int32_t *int_tri_deg_out)
     w.int_deg = c->int_deg;
     w.tri = c->tri;
     w.tri_deg = c->tri_deg;
+    w.dplus = c->wdplus;
+    w.T = c->wT;
+    w.td = c->wtd;
+    w.flag = c->iflag;
     w.terms = c->terms;
     w.int_tri_out = int_tri_out;
     w.int_tri_deg_out = int_tri_deg_out;
-    LAUNCH(c, "wcc_sub8", cdiv(bin_n(c, K_SUB8), 32), 256, k_wcc_sub<8>, w, bin_v(c, K_SUB8), bin_n(c, K_SUB8));
-    LAUNCH(c, "wcc_sub16", cdiv(bin_n(c, K_SUB16), 16), 256, k_wcc_sub<16>, w, bin_v(c, K_SUB16), bin_n(c, K_SUB16));
-    LAUNCH(c, "wcc_sub32", cdiv(bin_n(c, K_SUB32), 8), 256, k_wcc_sub<32>, w, bin_v(c, K_SUB32), bin_n(c, K_SUB32));
-    const uint32_t nw = c->bin_begin[K_W16 + 1] - c->bin_begin[K_W1];
-    LAUNCH(c, "wcc_wave", cdiv(nw, 4), 256, k_wcc_wave, w, bin_v(c, K_W1), nw);
-    const uint32_t nb = c->bin_begin[K_HUB + 1] - c->bin_begin[K_LDS];
-    LAUNCH(c, "wcc_block", nb, 256, k_wcc_block, w, bin_v(c, K_LDS), nb);
+    LAUNCH(c, "wcc_dplus", cdiv(c->n, 256), 256, k_wtc_dplus, w, c->n);
+    LAUNCH(c, "wcc_tri_sub8", cdiv(bin_n(c, K_SUB8), 32), 256, k_wtc_sub<8>, w, bin_v(c, K_SUB8), bin_n(c, K_SUB8));
+    LAUNCH(c, "wcc_tri_sub16", cdiv(bin_n(c, K_SUB16), 16), 256, k_wtc_sub<16>, w, bin_v(c, K_SUB16), bin_n(c, K_SUB16));
+    LAUNCH(c, "wcc_tri_sub32", cdiv(bin_n(c, K_SUB32), 8), 256, k_wtc_sub<32>, w, bin_v(c, K_SUB32), bin_n(c, K_SUB32));
+    if (c->wq_M) {
+        const uint32_t M = c->wq_M;
+        LAUNCH(c, "wcc_tri_chunks", cdiv(M, 256), 256, k_wtc_count_chunks, c->wdplus, bin_v(c, K_W1), M, c->wq_nch,
+               c->wq_next);
+        LAUNCH(c, "wcc_tri_wave", cdiv(M, 4), 256, k_wtc_wave, w, bin_v(c, K_W1), M);
+        int rc = scan_u32_dev(c, c->wq_nch, c->wq_off, M, c->wq_tot, 0);
+        if (rc) return rc;
+        LAUNCH(c, "wcc_tri_queue", 4 * c->num_cus, 256, k_wtc_queue<2048>, w, bin_v(c, K_W1), M, c->wq_off,
+               c->wq_tot, c->wq_next);
+    }
+    LAUNCH(c, "wcc_tri_final", cdiv(c->n, 4), 256, k_wtc_final, w, c->n);
+    LAUNCH(c, "wcc_terms", cdiv(c->n, 256), 256, k_wcc_terms, w, c->n);
     if (c->G) {
         LAUNCH(c, "wcc_sum_partial", c->n_witems, 256, k_reduce_partial_g, c->terms, c->witem_start, c->witem_end,
                c->partial_g);
@@ -594,6 +668,8 @@ extern "C" int scd_upload(scd_ctx *c, uint32_t n, uint32_t m, const uint32_t *in
         (rc = dalloc(c, &c->cur, n)) || (rc = dalloc(c, &c->next, n)) || (rc = dalloc(c, &c->best, n)) ||
         (rc = dalloc(c, &c->stats, n)) || (rc = dalloc(c, &c->int_adj, E)) ||
         (rc = dalloc(c, &c->int_deg, n)) || (rc = dalloc(c, &c->terms, n)) ||
+        (rc = dalloc(c, &c->iflag, E)) || (rc = dalloc(c, &c->wdplus, n)) || (rc = dalloc(c, &c->wT, n)) ||
+        (rc = dalloc(c, &c->wtd, n)) || (rc = dalloc(c, &c->wdirty, n)) ||
         (rc = dalloc(c, &c->partial, cdiv(n, 4096))) || (rc = dalloc(c, &c->wcc_dev, 1)) ||
         (rc = dalloc(c, &c->counters, 16)) || (rc = dalloc(c, &c->snap_labels, n)) ||
         (rc = dalloc(c, &c->snap_stats, n))) {
@@ -646,17 +722,22 @@ extern "C" int scd_labels_to_device(scd_ctx *c, const uint32_t *labels) {
     HIP_TRY(hipMemcpyAsync(c->best, labels, sizeof(uint32_t) * c->n, hipMemcpyHostToDevice, c->stream));
     if ((rc = clear_counters(c)) || (rc = phase_stats(c)) || (rc = finish(c))) return rc;
     c->have_labels = true;
+    c->wcc_valid = false;
     return SCD_OK;
 }
 
 // One iteration's device work on the stream: move (cur -> next), swap,
 // statistics, WCC, mean to the pinned host word.  Leaves cur/next swapped.
-static int enqueue_iteration(scd_ctx *c) {
+static int enqueue_iteration(scd_ctx *c, bool incremental) {
     int rc;
     if (c->counting && (rc = clear_counters(c))) return rc;
     if ((rc = phase_move(c, c->next, nullptr, nullptr))) return rc;
+    if (incremental) {
+        LAUNCH(c, "wcc_dirty_clear", cdiv(c->n, 256), 256, k_fill_u8, c->wdirty, (uint64_t)c->n);
+        LAUNCH(c, "wcc_dirty_mark", cdiv(c->n, 256), 256, k_mark_dirty, c->cur, c->next, c->n, c->wdirty);
+    }
     std::swap(c->cur, c->next);
-    if ((rc = phase_stats(c)) || (rc = phase_wcc(c, nullptr, nullptr))) return rc;
+    if ((rc = phase_stats(c)) || (rc = phase_wcc(c, nullptr, nullptr, incremental))) return rc;
     if (c->G) HIP_TRY(hipMemcpyAsync(c->wcc_g_host, c->wcc_g, 8ull * c->G, hipMemcpyDeviceToHost, c->stream));
     else HIP_TRY(hipMemcpyAsync(c->wcc_host, c->wcc_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     if (c->counting)
@@ -665,12 +746,13 @@ static int enqueue_iteration(scd_ctx *c) {
     return SCD_OK;
 }
 
-// Captures enqueue_iteration for the current parity into gexec[par].
-static int capture_iteration(scd_ctx *c, int par) {
+// Captures enqueue_iteration for the current parity and WCC mode into
+// gexec[par][inc].
+static int capture_iteration(scd_ctx *c, int par, int inc) {
     uint32_t *cur0 = c->cur, *next0 = c->next;
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     c->capturing = true;
-    int rc = enqueue_iteration(c);
+    int rc = enqueue_iteration(c, inc != 0);
     c->capturing = false;
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
@@ -678,9 +760,9 @@ static int capture_iteration(scd_ctx *c, int par) {
     c->next = next0;
     if (rc == SCD_OK && e != hipSuccess) rc = fail(SCD_EDEVICE, "stream capture: %s", hipGetErrorString(e));
     if (rc == SCD_OK) {
-        e = hipGraphInstantiate(&c->gexec[par], g, nullptr, nullptr, 0);
+        e = hipGraphInstantiate(&c->gexec[par][inc], g, nullptr, nullptr, 0);
         if (e != hipSuccess) {
-            c->gexec[par] = nullptr;
+            c->gexec[par][inc] = nullptr;
             rc = fail(SCD_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
         }
     }
@@ -694,14 +776,17 @@ extern "C" int scd_improve(scd_ctx *c, double *wcc, uint32_t *labels_out) {
     if (rc) return rc;
     // per-kernel event timing needs eager launches (events recorded by graph
     // nodes carry no usable timestamps)
+    const int inc = c->wcc_valid ? 1 : 0;
+    c->wcc_valid = false;
     if (c->use_graph && !c->profile) {
         const int par = c->cur == c->lab_a ? 0 : 1;
-        if (!c->gexec[par] && (rc = capture_iteration(c, par))) return rc;
-        HIP_TRY(hipGraphLaunch(c->gexec[par], c->stream));
+        if (!c->gexec[par][inc] && (rc = capture_iteration(c, par, inc))) return rc;
+        HIP_TRY(hipGraphLaunch(c->gexec[par][inc], c->stream));
         std::swap(c->cur, c->next);
-    } else if ((rc = enqueue_iteration(c))) {
+    } else if ((rc = enqueue_iteration(c, inc != 0))) {
         return rc;
     }
+    c->wcc_valid = true;
     if (labels_out)
         HIP_TRY(hipMemcpyAsync(labels_out, c->cur, sizeof(uint32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
     if ((rc = finish(c))) return rc;
@@ -792,6 +877,7 @@ extern "C" int scd_compute_wcc(scd_ctx *c, double *wcc, double *terms_out, int32
     rc = finish(c);
     free_list(c, tmp);
     if (rc) return rc;
+    c->wcc_valid = true;
     if (wcc) *wcc = c->wcc_host[0];
     return SCD_OK;
 }
@@ -962,6 +1048,7 @@ extern "C" int scd_restore(scd_ctx *c) {
                            c->stream));
     if (c->num_comms != c->snap_num_comms) drop_graphs(c);
     c->num_comms = c->snap_num_comms;
+    c->wcc_valid = false;
     return finish(c);
 }
 
@@ -1192,6 +1279,7 @@ extern "C" int scd_initial_partition(scd_ctx *c, uint32_t *labels_out, uint32_t 
     if (labels_out) HIP_TRY(hipMemcpyAsync(labels_out, c->cur, 4ull * n, hipMemcpyDeviceToHost, c->stream));
     if ((rc = finish(c))) return cleanup(rc);
     c->have_labels = true;
+    c->wcc_valid = false;
     if (num_labels_out) *num_labels_out = k;
     return cleanup(SCD_OK);
 }
@@ -1299,6 +1387,7 @@ extern "C" int scd_batch_compute_wcc(scd_ctx *c, double *wcc_out) {
     if ((rc = clear_counters(c)) || (rc = phase_stats(c)) || (rc = phase_wcc(c, nullptr, nullptr))) return rc;
     HIP_TRY(hipMemcpyAsync(c->wcc_g_host, c->wcc_g, 8ull * c->G, hipMemcpyDeviceToHost, c->stream));
     if ((rc = finish(c))) return rc;
+    c->wcc_valid = true;
     if (wcc_out) memcpy(wcc_out, c->wcc_g_host, 8ull * c->G);
     return SCD_OK;
 }

@@ -44,6 +44,12 @@ struct MoveArgs {
     const uint32_t *__restrict__ active_g;
 };
 
+// Vertices of a batched graph that has stopped are skipped (their label
+// stays).
+__device__ __forceinline__ bool vtx_stopped(const MoveArgs &a, uint32_t p) {
+    return a.gid && !a.active_g[a.gid[p]];
+}
+
 __device__ __forceinline__ double vtx_pext(const MoveArgs &a, uint32_t p) {
     return a.gid ? a.cc_g[a.gid[p]] : a.p_ext;
 }
@@ -84,11 +90,11 @@ __global__ __launch_bounds__(256) void k_move_sub(MoveArgs a, const uint32_t *__
     const uint32_t slot = (blockIdx.x * 256u + threadIdx.x) / S;
     const bool active = slot < nv;
     const uint32_t p = active ? verts[slot] : 0u;
-    const uint32_t beg = active ? a.indices[p] : 0u;
-    const uint32_t deg = active ? a.indices[p + 1] - beg : 0u;
+    const bool live = active && !vtx_stopped(a, p);
+    const uint32_t beg = live ? a.indices[p] : 0u;
+    const uint32_t deg = live ? a.indices[p + 1] - beg : 0u;
     const uint32_t my = active ? a.labels[p] : SCD_INVALID;
-
-    const double pext = active ? vtx_pext(a, p) : 0.0;
+    const double pext = live ? vtx_pext(a, p) : 0.0;
     uint32_t v = SCD_INVALID;
     if ((uint32_t)lig < deg) v = a.labels[a.adj[beg + lig]];
     v = group_bitonic_sort<S>(v, lig);
@@ -135,9 +141,13 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const uint32_t *_
     const uint32_t slot = blockIdx.x * 4u + (threadIdx.x >> 6);
     if (slot >= nv) return;  // uniform per wave
     const uint32_t p = verts[slot];
+    const uint32_t my = a.labels[p];
+    if (vtx_stopped(a, p)) {  // uniform per wave
+        if (lane == 0 && a.labels_out) a.labels_out[p] = my;
+        return;
+    }
     const uint32_t beg = a.indices[p];
     const uint32_t deg = a.indices[p + 1] - beg;
-    const uint32_t my = a.labels[p];
     const double pext = vtx_pext(a, p);
 
     uint32_t x[R];
@@ -320,9 +330,13 @@ __global__ __launch_bounds__(256) void k_move_lds(MoveArgs a, const uint32_t *__
     __shared__ int s_has_rem;
     const int tid = threadIdx.x;
     const uint32_t p = verts[blockIdx.x];
+    const uint32_t my = a.labels[p];
+    if (vtx_stopped(a, p)) {  // uniform per block, before any barrier
+        if (tid == 0 && a.labels_out) a.labels_out[p] = my;
+        return;
+    }
     const uint32_t beg = a.indices[p];
     const uint32_t deg = a.indices[p + 1] - beg;
-    const uint32_t my = a.labels[p];
     const uint32_t n2 = max(next_pow2(deg), 1024u);
     if (tid == 0) s_has_rem = 0;
     gather_labels<T>(a, beg, deg, n2, keys);

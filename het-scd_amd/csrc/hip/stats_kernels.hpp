@@ -8,14 +8,10 @@
 // (int_adj[indices[p] .. + int_deg[p]]): the internal subgraph the WCC needs,
 // built without a global scan.
 //
-// WCC (compute_wcc, src/wcc.cpp:31-141): for p with valid label L,
-// sizes[L] > 2 and tri_deg[p] != 0, int_tri = sum over same-label neighbours
-// q of |Nc(p) ∩ N(q)| and int_tri_deg = #q with a non-empty intersection,
-// where Nc(p) is p's same-label neighbour list.  Since every element of Nc(p)
-// has label L = label(q), |Nc(p) ∩ N(q)| = |Nc(p) ∩ Nc(q)|: both lists are in
-// int_adj.  Each vertex's term is written to terms[p]; the mean is a
-// fixed-shape tree sum over vertex order (k_reduce_*), so its bits do not
-// depend on the degree bins or tile sizes.
+// The WCC kernels (wcc_kernels.hpp) count triangles on this internal
+// subgraph; the mean of the per-vertex terms is a fixed-shape tree sum over
+// vertex order (k_reduce_*), so its bits do not depend on the degree bins or
+// tile sizes.
 #ifndef SCD_STATS_KERNELS_HPP
 #define SCD_STATS_KERNELS_HPP
 
@@ -30,8 +26,16 @@ struct StatArgs {
     int4 *__restrict__ stats;
     uint32_t *__restrict__ int_adj;
     uint32_t *__restrict__ int_deg;
+    uint8_t *__restrict__ iflag;  // per internal entry: cleared here, set by the WCC triangle count
     unsigned long long *__restrict__ counters;  // [0] += internal entries, [1] += deg_int^2
+    // batched arenas: vertices of stopped graphs get no internal list
+    const uint32_t *__restrict__ gid;
+    const uint32_t *__restrict__ active_g;
 };
+
+__device__ __forceinline__ bool stat_stopped(const StatArgs &a, uint32_t p) {
+    return a.gid && !a.active_g[a.gid[p]];
+}
 
 // Adds one vertex's (1, external, internal) counts to community L.
 __device__ __forceinline__ void add_stats(int4 *stats, uint32_t L, uint32_t sz, uint32_t ext, uint32_t cnt) {
@@ -53,6 +57,8 @@ constexpr uint32_t STATS_PROBES = 8;
 __global__ __launch_bounds__(1024) void k_stats_agg(const uint32_t *__restrict__ indices,
                                                     const uint32_t *__restrict__ labels,
                                                     const uint32_t *__restrict__ int_deg, uint32_t n,
+                                                    const uint32_t *__restrict__ gid,
+                                                    const uint32_t *__restrict__ active_g,
                                                     int4 *__restrict__ stats) {
     __shared__ uint32_t s_key[STATS_SLOTS];
     __shared__ uint32_t s_sz[STATS_SLOTS], s_ext[STATS_SLOTS], s_int[STATS_SLOTS];
@@ -63,7 +69,7 @@ __global__ __launch_bounds__(1024) void k_stats_agg(const uint32_t *__restrict__
     __syncthreads();
     for (uint32_t p = blockIdx.x * 1024u + threadIdx.x; p < n; p += gridDim.x * 1024u) {
         const uint32_t L = labels[p];
-        if (L == SCD_INVALID) continue;
+        if (L == SCD_INVALID || (gid && !active_g[gid[p]])) continue;
         const uint32_t ic = int_deg[p];
         const uint32_t ext = indices[p + 1] - indices[p] - ic;
         uint32_t h = (L * 0x9E3779B1u) >> (32 - 11);
@@ -123,10 +129,10 @@ __global__ __launch_bounds__(256) void k_filter_sub(StatArgs a, const uint32_t *
     const uint32_t p = active ? verts[slot] : 0u;
     const uint32_t beg = active ? a.indices[p] : 0u;
     const uint32_t deg = active ? a.indices[p + 1] - beg : 0u;
-    const uint32_t L = active ? a.labels[p] : SCD_INVALID;
+    const uint32_t L = (active && !stat_stopped(a, p)) ? a.labels[p] : SCD_INVALID;
     bool same = false;
     uint32_t q = 0;
-    if ((uint32_t)lig < deg) {
+    if ((uint32_t)lig < deg && L != SCD_INVALID) {
         q = a.adj[beg + lig];
         same = (L != SCD_INVALID) && a.labels[q] == L;
     }
@@ -134,7 +140,11 @@ __global__ __launch_bounds__(256) void k_filter_sub(StatArgs a, const uint32_t *
     if constexpr (S == 64) gm = __ballot(same);
     else gm = (__ballot(same) >> gbase) & ((1ull << S) - 1ull);
     const uint32_t cnt = (uint32_t)__builtin_popcountll(gm);
-    if (same) a.int_adj[beg + __builtin_popcountll(gm & ((1ull << lig) - 1ull))] = q;
+    if (same) {
+        const uint32_t at = beg + __builtin_popcountll(gm & ((1ull << lig) - 1ull));
+        a.int_adj[at] = q;
+        a.iflag[at] = 0;
+    }
     const bool lead = active && lig == 0;
     if (lead) {
         a.int_deg[p] = cnt;
@@ -152,8 +162,12 @@ __global__ __launch_bounds__(256) void k_filter_wave(StatArgs a, const uint32_t 
     if (slot >= nv) return;
     const uint32_t p = verts[slot];
     const uint32_t beg = a.indices[p];
-    const uint32_t deg = a.indices[p + 1] - beg;
     const uint32_t L = a.labels[p];
+    if (L == SCD_INVALID || stat_stopped(a, p)) {  // no internal list; uniform per wave
+        if (lane == 0) a.int_deg[p] = 0;
+        return;
+    }
+    const uint32_t deg = a.indices[p + 1] - beg;
     const uint64_t lt = (1ull << lane) - 1ull;
     uint32_t q[R], lq[R];
 #pragma unroll
@@ -171,7 +185,11 @@ __global__ __launch_bounds__(256) void k_filter_wave(StatArgs a, const uint32_t 
     for (int r = 0; r < R; r++) {
         const bool same = (L != SCD_INVALID) && lq[r] == L && (uint32_t)(r * 64 + lane) < deg;
         const uint64_t m = __ballot(same);
-        if (same) a.int_adj[beg + run + __builtin_popcountll(m & lt)] = q[r];
+        if (same) {
+            const uint32_t at = beg + run + __builtin_popcountll(m & lt);
+            a.int_adj[at] = q[r];
+            a.iflag[at] = 0;
+        }
         run += (uint32_t)__builtin_popcountll(m);
     }
     if (lane == 0) {
@@ -190,8 +208,12 @@ __global__ __launch_bounds__(256) void k_filter_block(StatArgs a, const uint32_t
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t p = verts[blockIdx.x];
     const uint32_t beg = a.indices[p];
-    const uint32_t deg = a.indices[p + 1] - beg;
     const uint32_t L = a.labels[p];
+    if (L == SCD_INVALID || stat_stopped(a, p)) {  // uniform per block, before any barrier
+        if (threadIdx.x == 0) a.int_deg[p] = 0;
+        return;
+    }
+    const uint32_t deg = a.indices[p + 1] - beg;
     uint32_t q[U];
     bool same[U];
 #pragma unroll
@@ -221,7 +243,11 @@ __global__ __launch_bounds__(256) void k_filter_block(StatArgs a, const uint32_t
             before += (k < w) ? s_cnt[u][k] : 0u;
             run += s_cnt[u][k];
         }
-        if (same[u]) a.int_adj[beg + before + __builtin_popcountll(m[u] & lt)] = q[u];
+        if (same[u]) {
+            const uint32_t at = beg + before + __builtin_popcountll(m[u] & lt);
+            a.int_adj[at] = q[u];
+            a.iflag[at] = 0;
+        }
     }
     if (threadIdx.x == 0) {
         a.int_deg[p] = run;
@@ -233,7 +259,7 @@ __global__ __launch_bounds__(256) void k_filter_block(StatArgs a, const uint32_t
 // Order-preserving compaction of one slice of up to 256 entries by a block
 // of 256 threads: returns the slice's count; `same` entries are written at
 // out[run + rank].
-__device__ __forceinline__ uint32_t block_compact256(bool same, uint32_t q, uint32_t *out,
+__device__ __forceinline__ uint32_t block_compact256(bool same, uint32_t q, uint32_t *out, uint8_t *oflag,
                                                      uint32_t run, uint32_t *s_cnt) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint64_t m = __ballot(same);
@@ -245,7 +271,11 @@ __device__ __forceinline__ uint32_t block_compact256(bool same, uint32_t q, uint
         before += (k < w) ? s_cnt[k] : 0u;
         total += s_cnt[k];
     }
-    if (same) out[run + before + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = q;
+    if (same) {
+        const uint32_t at = run + before + __builtin_popcountll(m & ((1ull << lane) - 1ull));
+        out[at] = q;
+        oflag[at] = 0;
+    }
     __syncthreads();
     return total;
 }
@@ -313,122 +343,8 @@ __global__ __launch_bounds__(256) void k_hub_filter_write(StatArgs a, const uint
             q = a.adj[beg + i];
             same = (L != SCD_INVALID) && a.labels[q] == L;
         }
-        run += block_compact256(same, q, a.int_adj + beg, run, s_cnt);
+        run += block_compact256(same, q, a.int_adj + beg, a.iflag + beg, run, s_cnt);
     }
-}
-
-struct WccArgs {
-    const uint32_t *__restrict__ indices;
-    const uint32_t *__restrict__ labels;
-    const int4 *__restrict__ stats;
-    const uint32_t *__restrict__ int_adj;
-    const uint32_t *__restrict__ int_deg;
-    const int32_t *__restrict__ tri;
-    const int32_t *__restrict__ tri_deg;
-    double *__restrict__ terms;
-    int32_t *__restrict__ int_tri_out;      // optional
-    int32_t *__restrict__ int_tri_deg_out;  // optional
-};
-
-// Whether p contributes a term (src/wcc.cpp:89-104).
-__device__ __forceinline__ bool wcc_counts(const WccArgs &a, uint32_t p) {
-    const uint32_t L = a.labels[p];
-    return L != SCD_INVALID && a.stats[L].x > 2 && a.tri_deg[p] != 0;
-}
-
-__device__ __forceinline__ uint32_t pair_common(const WccArgs &a, const uint32_t *A, uint32_t na,
-                                                uint32_t q) {
-    return count_common(A, na, a.int_adj + a.indices[q], a.int_deg[q]);
-}
-
-__device__ __forceinline__ void wcc_store(const WccArgs &a, uint32_t p, bool valid, uint32_t itri,
-                                          uint32_t itdeg) {
-    const uint32_t L = a.labels[p];
-    a.terms[p] = valid ? wcc_term(a.tri[p], a.tri_deg[p], a.stats[L].x, (int)itri, (int)itdeg) : 0.0;
-    if (a.int_tri_out) {
-        a.int_tri_out[p] = valid ? (int)itri : 0;
-        a.int_tri_deg_out[p] = valid ? (int)itdeg : 0;
-    }
-}
-
-// S lanes per vertex (deg <= S): lane j handles the pair (p, Nc(p)[j]).
-template <int S>
-__global__ __launch_bounds__(256) void k_wcc_sub(WccArgs a, const uint32_t *__restrict__ verts,
-                                                 uint32_t nv) {
-    const int lig = threadIdx.x & (S - 1);
-    const uint32_t slot = (blockIdx.x * 256u + threadIdx.x) / S;
-    const bool active = slot < nv;
-    const uint32_t p = active ? verts[slot] : 0u;
-    const bool valid = active && wcc_counts(a, p);
-    uint32_t c = 0;
-    if (valid) {
-        const uint32_t na = a.int_deg[p];
-        const uint32_t *A = a.int_adj + a.indices[p];
-        if ((uint32_t)lig < na) c = pair_common(a, A, na, A[lig]);
-    }
-    uint32_t itri = c, itdeg = c > 0 ? 1u : 0u;
-#pragma unroll
-    for (int off = 1; off < S; off <<= 1) {
-        itri += __shfl_xor(itri, off, SCD_WAVE);
-        itdeg += __shfl_xor(itdeg, off, SCD_WAVE);
-    }
-    if (active && lig == 0) wcc_store(a, p, valid, itri, itdeg);
-}
-
-// One wave per vertex.
-__global__ __launch_bounds__(256) void k_wcc_wave(WccArgs a, const uint32_t *__restrict__ verts,
-                                                  uint32_t nv) {
-    const int lane = threadIdx.x & 63;
-    const uint32_t slot = blockIdx.x * 4u + (threadIdx.x >> 6);
-    if (slot >= nv) return;
-    const uint32_t p = verts[slot];
-    const bool valid = wcc_counts(a, p);
-    uint32_t itri = 0, itdeg = 0;
-    if (valid) {
-        const uint32_t na = a.int_deg[p];
-        const uint32_t *A = a.int_adj + a.indices[p];
-        for (uint32_t j = lane; j < na; j += 64) {
-            const uint32_t c = pair_common(a, A, na, A[j]);
-            itri += c;
-            itdeg += c > 0 ? 1u : 0u;
-        }
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        itri += __shfl_xor(itri, off, SCD_WAVE);
-        itdeg += __shfl_xor(itdeg, off, SCD_WAVE);
-    }
-    if (lane == 0) wcc_store(a, p, valid, itri, itdeg);
-}
-
-// One workgroup per vertex (large degrees and hubs).
-__global__ __launch_bounds__(256) void k_wcc_block(WccArgs a, const uint32_t *__restrict__ verts,
-                                                   uint32_t nv) {
-    __shared__ uint32_t s_t[4], s_d[4];
-    const uint32_t p = verts[blockIdx.x];
-    const bool valid = wcc_counts(a, p);
-    uint32_t itri = 0, itdeg = 0;
-    if (valid) {
-        const uint32_t na = a.int_deg[p];
-        const uint32_t *A = a.int_adj + a.indices[p];
-        for (uint32_t j = threadIdx.x; j < na; j += 256) {
-            const uint32_t c = pair_common(a, A, na, A[j]);
-            itri += c;
-            itdeg += c > 0 ? 1u : 0u;
-        }
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        itri += __shfl_xor(itri, off, SCD_WAVE);
-        itdeg += __shfl_xor(itdeg, off, SCD_WAVE);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        s_t[threadIdx.x >> 6] = itri;
-        s_d[threadIdx.x >> 6] = itdeg;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0)
-        wcc_store(a, p, valid, s_t[0] + s_t[1] + s_t[2] + s_t[3], s_d[0] + s_d[1] + s_d[2] + s_d[3]);
 }
 
 // Fixed-shape sum of x[0..n): block b sums x[b*4096 .. +4096) as 256 strided
@@ -576,6 +492,12 @@ __global__ __launch_bounds__(256) void k_scan_tiles(const uint32_t *__restrict__
         const uint64_t i = base + threadIdx.x + 256u * k;
         if (i < n) out[i] = s_v[threadIdx.x + 256 * k];
     }
+}
+
+// total[0] = exclusive-scan offset of the last tile + its sum.
+__global__ void k_scan_total(const uint32_t *__restrict__ tile_off, const uint32_t *__restrict__ tile_sums,
+                             uint32_t ntiles, uint32_t *__restrict__ total) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) total[0] = tile_off[ntiles - 1] + tile_sums[ntiles - 1];
 }
 
 // Export of the reduce-by-key runs: flag[i] = (count[i] > 0).

@@ -26,6 +26,7 @@
 #define SCD_TC_KERNELS_HPP
 
 #include "dev.hpp"
+#include "stats_kernels.hpp"
 
 namespace scd_dev {
 

@@ -1,0 +1,499 @@
+// WCC evaluation (compute_wcc, src/wcc.cpp:31-141) as triangle counting on
+// the internal subgraph.
+//
+// For p with valid label L, sizes[L] > 2 and tri_deg[p] != 0 the reference
+// sums, over same-label neighbours q, |Nc(p) ∩ N(q)|, where Nc(p) is p's
+// same-label neighbour list (int_tri), and counts the q with a non-empty
+// intersection (int_tri_deg).  Every common neighbour r in Nc(p) ∩ N(q) has
+// label L, so each such (q, r) is a triangle p-q-r inside community L, seen
+// once from q and once from r:
+//   int_tri(p)     = 2 x (internal triangles at p)
+//   int_tri_deg(p) = internal edges at p that lie in an internal triangle.
+// The internal lists (int_adj, sorted) are counted like the whole graph in
+// tc_kernels.hpp: edges point from the higher to the lower id, each internal
+// triangle w < v < u is found once at u, per-vertex counts aggregate in LDS
+// over N+_int(u), and the edges of a found triangle are flagged with plain
+// stores of 1 (the filter pass cleared the flags of the internal entries).
+// All grids are fixed by the degree bins, so an iteration stays capturable.
+#ifndef SCD_WCC_KERNELS_HPP
+#define SCD_WCC_KERNELS_HPP
+
+#include "dev.hpp"
+#include "stats_kernels.hpp"
+
+namespace scd_dev {
+
+struct WccArgs {
+    const uint32_t *__restrict__ indices;
+    const uint32_t *__restrict__ labels;
+    const int4 *__restrict__ stats;
+    const uint32_t *__restrict__ int_adj;
+    const uint32_t *__restrict__ int_deg;
+    const int32_t *__restrict__ tri;
+    const int32_t *__restrict__ tri_deg;
+    uint32_t *__restrict__ dplus;  // internal neighbours below each vertex
+    uint32_t *__restrict__ T;      // internal triangles per vertex
+    uint32_t *__restrict__ td;     // internal edges in an internal triangle
+    uint8_t *__restrict__ flag;    // per int_adj entry (out-slots used)
+    double *__restrict__ terms;
+    int32_t *__restrict__ int_tri_out;      // optional
+    int32_t *__restrict__ int_tri_deg_out;  // optional
+    // optional, per label: 1 if the community's member set changed since
+    // the T / td of its members were counted.  Members of the other
+    // communities keep T and td (the internal subgraph of a community is
+    // a function of its member set).  nullptr: count everything.
+    const uint8_t *__restrict__ dirty;
+};
+
+// dirty[old label] = dirty[new label] = 1 for every vertex that moved
+// (the flags were cleared before).
+__global__ void k_mark_dirty(const uint32_t *__restrict__ old_labels, const uint32_t *__restrict__ new_labels,
+                             uint32_t n, uint8_t *__restrict__ dirty) {
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    const uint32_t a = old_labels[u], b = new_labels[u];
+    if (a == b) return;
+    if (a != SCD_INVALID) dirty[a] = 1;
+    if (b != SCD_INVALID) dirty[b] = 1;
+}
+
+
+__global__ void k_wtc_dplus(WccArgs a, uint32_t n) {
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    const uint32_t lab = a.labels[u];
+    if (a.dirty && lab != SCD_INVALID && !a.dirty[lab]) {
+        a.dplus[u] = 0;  // community unchanged: no counting, T and td kept
+        return;
+    }
+    const uint32_t *L = a.int_adj + a.indices[u];
+    uint32_t lo = 0, hi = a.int_deg[u];
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (L[mid] < u) lo = mid + 1;
+        else hi = mid;
+    }
+    a.dplus[u] = lo;
+    a.T[u] = 0;
+    a.td[u] = 0;
+}
+
+// Vertices of degree <= S: S lanes each, lane j owns v = N+_int(u)[j] and
+// walks N+_int(v); membership in N+_int(u) by binary search in LDS.
+template <int S>
+__global__ __launch_bounds__(256) void k_wtc_sub(WccArgs a, const uint32_t *__restrict__ verts, uint32_t nv) {
+    __shared__ uint32_t s_nk[256], s_cnt[256];
+    const int tid = threadIdx.x;
+    const int lig = tid & (S - 1);
+    const int gb = tid & ~(S - 1);
+    const uint32_t slot = (blockIdx.x * 256u + tid) / S;
+    const bool active = slot < nv;
+    const uint32_t u = active ? verts[slot] : 0u;
+    const uint32_t beg = active ? a.indices[u] : 0u;
+    const uint32_t dp = active ? a.dplus[u] : 0u;
+    s_nk[tid] = (uint32_t)lig < dp ? a.int_adj[beg + lig] : 0xFFFFFFFFu;
+    s_cnt[tid] = 0;
+    __syncthreads();
+    uint32_t tot = 0;
+    if ((uint32_t)lig < dp) {
+        const uint32_t v = s_nk[tid];
+        const uint32_t vb = a.indices[v], dv = a.dplus[v];
+        for (uint32_t k = 0; k < dv; k++) {
+            const uint32_t w = a.int_adj[vb + k];
+            uint32_t lo = 0, hi = dp;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_nk[gb + mid] < w) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo < dp && s_nk[gb + lo] == w) {
+                atomicAdd(&s_cnt[tid], 1u);
+                atomicAdd(&s_cnt[gb + lo], 1u);
+                a.flag[vb + k] = 1;
+                tot++;
+            }
+        }
+    }
+    __syncthreads();
+    if ((uint32_t)lig < dp && s_cnt[tid]) {
+        atomicAdd(&a.T[s_nk[tid]], s_cnt[tid]);
+        a.flag[beg + lig] = 1;
+    }
+#pragma unroll
+    for (int off = 1; off < S; off <<= 1) tot += __shfl_xor(tot, off, SCD_WAVE);
+    if (active && lig == 0 && tot) atomicAdd(&a.T[u], tot);
+}
+
+// Vertices of degree > 32 whose out-list has at most 64 entries: one wave
+// per vertex.  The member lists N+_int(v), v in N+_int(u), are walked
+// flattened over the lanes (WTC_UF loads in flight per lane; the owning v
+// of an entry by binary search over the wave's prefix of list lengths);
+// membership in N+_int(u) by binary search in LDS.  Longer out-lists go to
+// the work queue below.
+constexpr uint32_t WTC_WAVE_MAX = 64;
+constexpr int WTC_UF = 4;  // loads in flight per lane, flattened walks
+
+__global__ __launch_bounds__(256) void k_wtc_wave(WccArgs a, const uint32_t *__restrict__ verts, uint32_t nv) {
+    __shared__ uint32_t s_n[4][64], s_vb[4][64], s_cnt[4][64], s_pre[4][65];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t i = blockIdx.x * 4u + wv;
+    uint32_t u = 0, beg = 0, dp = 0;
+    if (i < nv) {
+        u = verts[i];
+        dp = a.dplus[u];
+        if (dp > WTC_WAVE_MAX) dp = 0;  // work queue
+        beg = a.indices[u];
+    }
+    uint32_t x = SCD_INVALID, len = 0, vb = 0;
+    if ((uint32_t)lane < dp) {
+        x = a.int_adj[beg + lane];
+        len = a.dplus[x];
+        vb = a.indices[x];
+    }
+    uint32_t pre = len;  // inclusive prefix over the lanes
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(pre, o, SCD_WAVE);
+        if (lane >= o) pre += t;
+    }
+    s_n[wv][lane] = x;
+    s_vb[wv][lane] = vb;
+    s_cnt[wv][lane] = 0;
+    s_pre[wv][lane + 1] = pre;
+    if (lane == 0) s_pre[wv][0] = 0;
+    __syncthreads();
+    const uint32_t W = __shfl(pre, 63, SCD_WAVE);
+    uint32_t tot = 0;
+    for (uint32_t f0 = lane; f0 < W; f0 += 64 * WTC_UF) {
+        uint32_t w[WTC_UF], at[WTC_UF], own[WTC_UF];
+#pragma unroll
+        for (int q = 0; q < WTC_UF; q++) {
+            const uint32_t f = f0 + 64 * q;
+            w[q] = SCD_INVALID;
+            if (f < W) {
+                uint32_t l = 0, h = dp;  // largest k with pre[k] <= f
+                while (h - l > 1) {
+                    const uint32_t mid = (l + h) >> 1;
+                    if (s_pre[wv][mid] <= f) l = mid;
+                    else h = mid;
+                }
+                own[q] = l;
+                at[q] = s_vb[wv][l] + (f - s_pre[wv][l]);
+                w[q] = a.int_adj[at[q]];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < WTC_UF; q++) {
+            if (w[q] == SCD_INVALID) continue;
+            uint32_t lo = 0, hi = dp;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_n[wv][mid] < w[q]) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo < dp && s_n[wv][lo] == w[q]) {
+                atomicAdd(&s_cnt[wv][own[q]], 1u);
+                atomicAdd(&s_cnt[wv][lo], 1u);
+                a.flag[at[q]] = 1;
+                tot++;
+            }
+        }
+    }
+    __syncthreads();
+    if ((uint32_t)lane < dp && s_cnt[wv][lane]) {
+        atomicAdd(&a.T[x], s_cnt[wv][lane]);
+        a.flag[beg + lane] = 1;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, SCD_WAVE);
+    if (lane == 0 && tot) atomicAdd(&a.T[u], tot);
+}
+
+// LDS hash set of N+_int(u): slot -> (vertex, position in the out-list).
+__device__ __forceinline__ uint32_t wtc_hash(uint32_t x, int bits) { return (x * 0x9E3779B1u) >> (32 - bits); }
+
+__device__ __forceinline__ int wtc_find(const uint32_t *hk, const uint16_t *hp, int bits, uint32_t w) {
+    const uint32_t mask = (1u << bits) - 1u;
+    for (uint32_t h = wtc_hash(w, bits);; h = (h + 1) & mask) {
+        const uint32_t k = hk[h];
+        if (k == w) return hp[h];
+        if (k == SCD_INVALID) return -1;
+    }
+}
+
+// Out-lists longer than 64 entries are processed as work items (u, c): chunk c holds
+// entries [c VC, (c+1) VC) of N+_int(u).  One persistent kernel claims items
+// from an atomic counter until the device-computed item total is reached, so
+// every block exits and the grid is fixed (the iteration stays capturable).
+// Items are numbered from the highest degree bin down, so the longest
+// out-lists start first.  An item rebuilds the LDS hash set of the whole
+// N+_int(u), walks the member lists of its chunk — one list per wave when
+// they average >= 32 entries, else flattened over the threads — counts per
+// out-list position in LDS and flushes one atomic per non-zero position.
+constexpr uint32_t WTC_VC = 32;  // out-list entries per work item
+constexpr int WTC_UW = 8;        // loads in flight per lane, list-per-wave mode
+
+// nch[i] = chunks of verts[M-1-i]; also resets the item counter.
+__global__ void k_wtc_count_chunks(const uint32_t *__restrict__ dplus, const uint32_t *__restrict__ verts,
+                                   uint32_t M, uint32_t *__restrict__ nch, uint32_t *__restrict__ next) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) next[0] = 0;
+    if (i >= M) return;
+    const uint32_t dp = dplus[verts[M - 1 - i]];
+    nch[i] = dp > WTC_WAVE_MAX ? (dp + WTC_VC - 1) / WTC_VC : 0u;
+}
+
+// Membership set of N+_int(u) (sorted, dp entries) for one work item.  When
+// the id range R of the list fits WTC_TAB words, a bitmap over [lo, lo+R)
+// whose words carry the out-list position of their first entry: w is a
+// member iff its bit is set, and its position is that rank plus the bits
+// below it in the word — one LDS read per probe.  Wider ranges use the
+// linear-probing hash (slot -> vertex, position).
+constexpr uint32_t WTC_TAB = 4000;  // (bits, rank) words: id ranges up to 128000 (4 blocks per CU)
+
+struct WtcSet {
+    uint2 *tab;
+    uint32_t *hk;
+    uint16_t *hp;
+    uint32_t lo, R;
+    int bits;  // 0: bitmap mode
+    __device__ __forceinline__ int find(uint32_t w) const {
+        if (bits == 0) {
+            const uint32_t x = w - lo;
+            if (x >= R) return -1;
+            const uint2 e = tab[x >> 5];
+            const uint32_t b = x & 31u;
+            if (!((e.x >> b) & 1u)) return -1;
+            return (int)(e.y + (uint32_t)__builtin_popcount(e.x & ((1u << b) - 1u)));
+        }
+        return wtc_find(hk, hp, bits, w);
+    }
+};
+
+template <int CAP>
+__global__ __launch_bounds__(256) void k_wtc_queue(WccArgs a, const uint32_t *__restrict__ verts, uint32_t M,
+                                                   const uint32_t *__restrict__ off,
+                                                   const uint32_t *__restrict__ total, uint32_t *__restrict__ next) {
+    static_assert(2 * CAP * 4 + 2 * CAP * 2 <= WTC_TAB * 8, "hash must fit in the bitmap storage");
+    __shared__ uint2 s_tab[WTC_TAB];  // bitmap words, or the hash (keys, positions)
+    __shared__ uint32_t s_cnt[CAP];
+    __shared__ uint32_t s_vb[WTC_VC], s_pre[WTC_VC + 1];
+    __shared__ uint32_t s_item;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    WtcSet set;
+    set.tab = s_tab;
+    set.hk = reinterpret_cast<uint32_t *>(s_tab);
+    set.hp = reinterpret_cast<uint16_t *>(set.hk + 2 * CAP);
+    // the bitmap is all-zero between items (each item clears what it set)
+    for (uint32_t k = tid; k < WTC_TAB; k += 256) s_tab[k] = make_uint2(0u, 0u);
+    const uint32_t items = *total;
+    while (true) {
+        if (tid == 0) s_item = atomicAdd(next, 1u);
+        __syncthreads();
+        const uint32_t item = s_item;
+        if (item >= items) break;  // uniform: every block ends here
+        uint32_t lo = 0, hi = M;  // largest i with off[i] <= item
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (off[mid] <= item) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t u = verts[M - 1 - lo];
+        const uint32_t beg = a.indices[u], dp = a.dplus[u];
+        const uint32_t k0 = (item - off[lo]) * WTC_VC, nk = min(WTC_VC, dp - k0);
+        const uint32_t *N = a.int_adj + beg;
+        uint32_t tot = 0;
+        if (tid < (int)nk) {
+            const uint32_t v = N[k0 + tid];
+            s_vb[tid] = a.indices[v];
+            s_pre[tid + 1] = a.dplus[v];
+        }
+        if (tid == 0) s_pre[0] = 0;
+        if (dp <= (uint32_t)CAP) {
+            set.lo = N[0];
+            set.R = N[dp - 1] - set.lo + 1;
+            set.bits = 0;
+            if (set.R > 32u * WTC_TAB) {
+                set.bits = 1;
+                while ((1u << set.bits) < 2 * dp) set.bits++;
+                for (uint32_t h = tid; h < (1u << set.bits); h += 256) set.hk[h] = SCD_INVALID;
+            }
+            for (uint32_t k = tid; k < dp; k += 256) s_cnt[k] = 0;
+            __syncthreads();
+            if (wv == 0) {  // prefix of the chunk's list lengths
+                uint32_t x = lane < (int)nk ? s_pre[lane + 1] : 0u;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const uint32_t t = __shfl_up(x, o, SCD_WAVE);
+                    if (lane >= o) x += t;
+                }
+                if (lane < (int)nk) s_pre[lane + 1] = x;
+            }
+            if (set.bits == 0) {
+                for (uint32_t k = tid; k < dp; k += 256) {
+                    const uint32_t x = N[k] - set.lo;
+                    atomicOr(&s_tab[x >> 5].x, 1u << (x & 31u));
+                    if (k == 0 || ((N[k - 1] - set.lo) >> 5) != (x >> 5)) s_tab[x >> 5].y = k;
+                }
+            } else {
+                const uint32_t mask = (1u << set.bits) - 1u;
+                for (uint32_t k = tid; k < dp; k += 256) {
+                    const uint32_t v = N[k];
+                    uint32_t h = wtc_hash(v, set.bits);
+                    while (atomicCAS(&set.hk[h], SCD_INVALID, v) != SCD_INVALID) h = (h + 1) & mask;
+                    set.hp[h] = (uint16_t)k;
+                }
+            }
+            __syncthreads();
+            const uint32_t Wc = s_pre[nk];
+            if (Wc >= 32u * nk) {
+                // one list per wave; WTC_UW loads of 64 entries in flight
+                for (uint32_t kk = wv; kk < nk; kk += 4) {
+                    const uint32_t vb = s_vb[kk], dv = s_pre[kk + 1] - s_pre[kk];
+                    uint32_t c = 0;
+                    for (uint32_t j0 = 0; j0 < dv; j0 += 64 * WTC_UW) {
+                        uint32_t w[WTC_UW];
+#pragma unroll
+                        for (int q = 0; q < WTC_UW; q++) {
+                            const uint32_t j = j0 + 64 * q + lane;
+                            w[q] = j < dv ? a.int_adj[vb + j] : SCD_INVALID;
+                        }
+#pragma unroll
+                        for (int q = 0; q < WTC_UW; q++) {
+                            if (w[q] == SCD_INVALID) continue;
+                            const int pos = set.find(w[q]);
+                            if (pos >= 0) {
+                                atomicAdd(&s_cnt[pos], 1u);
+                                a.flag[vb + j0 + 64 * q + lane] = 1;
+                                c++;
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, SCD_WAVE);
+                    if (lane == 0 && c) {
+                        atomicAdd(&s_cnt[k0 + kk], c);
+                        tot += c;
+                    }
+                }
+            } else {
+                // flattened over the threads; WTC_UF loads in flight each
+                for (uint32_t f0 = tid; f0 < Wc; f0 += 256 * WTC_UF) {
+                    uint32_t w[WTC_UF], at[WTC_UF], own[WTC_UF];
+#pragma unroll
+                    for (int q = 0; q < WTC_UF; q++) {
+                        const uint32_t f = f0 + 256 * q;
+                        w[q] = SCD_INVALID;
+                        if (f < Wc) {
+                            uint32_t l = 0, h = nk;  // largest kk with pre[kk] <= f
+                            while (h - l > 1) {
+                                const uint32_t mid = (l + h) >> 1;
+                                if (s_pre[mid] <= f) l = mid;
+                                else h = mid;
+                            }
+                            own[q] = l;
+                            at[q] = s_vb[l] + (f - s_pre[l]);
+                            w[q] = a.int_adj[at[q]];
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < WTC_UF; q++) {
+                        if (w[q] == SCD_INVALID) continue;
+                        const int pos = set.find(w[q]);
+                        if (pos >= 0) {
+                            atomicAdd(&s_cnt[k0 + own[q]], 1u);
+                            atomicAdd(&s_cnt[pos], 1u);
+                            a.flag[at[q]] = 1;
+                            tot++;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            for (uint32_t k = tid; k < dp; k += 256) {
+                const uint32_t c = s_cnt[k];
+                if (c) {
+                    atomicAdd(&a.T[N[k]], c);
+                    a.flag[beg + k] = 1;
+                }
+            }
+            // leave the bitmap all-zero for the next item
+            if (set.bits == 0) {
+                for (uint32_t k = tid; k < dp; k += 256) s_tab[(N[k] - set.lo) >> 5].x = 0;
+            } else {
+                const uint32_t words = (2 * CAP * 4 + 2 * CAP * 2) / 8;
+                for (uint32_t k = tid; k < words; k += 256) s_tab[k] = make_uint2(0u, 0u);
+            }
+        } else {
+            // out-list longer than the LDS set: search it in global memory
+            __syncthreads();
+            for (uint32_t kk = 0; kk < nk; kk++) {
+                const uint32_t v = N[k0 + kk];
+                const uint32_t vb = s_vb[kk], dv = a.dplus[v];
+                for (uint32_t j = tid; j < dv; j += 256) {
+                    const uint32_t w = a.int_adj[vb + j];
+                    uint32_t l2 = 0, h2 = dp;
+                    while (l2 < h2) {
+                        const uint32_t mid = (l2 + h2) >> 1;
+                        if (N[mid] < w) l2 = mid + 1;
+                        else h2 = mid;
+                    }
+                    if (l2 < dp && N[l2] == w) {
+                        atomicAdd(&a.T[v], 1u);
+                        atomicAdd(&a.T[w], 1u);
+                        a.flag[vb + j] = 1;
+                        a.flag[beg + k0 + kk] = 1;
+                        a.flag[beg + l2] = 1;
+                        tot++;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, SCD_WAVE);
+        if (lane == 0 && tot) atomicAdd(&a.T[u], tot);
+        __syncthreads();  // LDS reused by the next item
+    }
+}
+
+// Flagged internal out-edges count at both ends.  One wave per vertex.
+__global__ __launch_bounds__(256) void k_wtc_final(WccArgs a, uint32_t n) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t u = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (u >= n) return;
+    const uint32_t beg = a.indices[u], dp = a.dplus[u];
+    uint32_t mine = 0;
+    for (uint32_t k = lane; k < dp; k += 64) {
+        if (a.flag[beg + k]) {
+            atomicAdd(&a.td[a.int_adj[beg + k]], 1u);
+            mine++;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off, SCD_WAVE);
+    if (lane == 0 && mine) atomicAdd(&a.td[u], mine);
+}
+
+// Per-vertex terms (src/wcc.cpp:89-166): vertices without a valid label, in
+// a community of size <= 2 or without triangles contribute 0.
+__global__ void k_wcc_terms(WccArgs a, uint32_t n) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t L = a.labels[p];
+    bool valid = false;
+    int size = 0;
+    if (L != SCD_INVALID) {
+        size = a.stats[L].x;
+        valid = size > 2 && a.tri_deg[p] != 0;
+    }
+    const uint32_t itri = 2u * a.T[p], itdeg = a.td[p];
+    a.terms[p] = valid ? wcc_term(a.tri[p], a.tri_deg[p], size, (int)itri, (int)itdeg) : 0.0;
+    if (a.int_tri_out) {
+        a.int_tri_out[p] = valid ? (int)itri : 0;
+        a.int_tri_deg_out[p] = valid ? (int)itdeg : 0;
+    }
+}
+
+}  // namespace scd_dev
+
+#endif

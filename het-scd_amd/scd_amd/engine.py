@@ -86,8 +86,8 @@ class Engine:
         check_hip(_lib.hip().scd_count_triangles(self._ctx, ptr(tri, i32p), ptr(td, i32p), 1 if replace else 0))
         return tri, td
 
-    def initial_partition(self):
-        out = np.empty(self.n, np.uint32)
+    def initial_partition(self, want_labels=True):
+        out = np.empty(self.n, np.uint32) if want_labels else None
         k = ctypes.c_uint32()
         check_hip(_lib.hip().scd_initial_partition(self._ctx, ptr(out, u32p), ctypes.byref(k)))
         return out, k.value
@@ -236,8 +236,8 @@ class BatchEngine:
         self.eng.n = g.n
         self.eng.graph = g
 
-    def initial_partition(self):
-        lab = np.empty(self.arena.graph.n, np.uint32)
+    def initial_partition(self, want_labels=True):
+        lab = np.empty(self.arena.graph.n, np.uint32) if want_labels else None
         k = np.empty(self.arena.count, np.uint32)
         check_hip(_lib.hip().scd_batch_initial_partition(self._ctx, ptr(lab, u32p), ptr(k, u32p)))
         return lab, k
