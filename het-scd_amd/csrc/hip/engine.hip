@@ -29,6 +29,7 @@
 #include "tc_kernels.hpp"
 #include "ip_kernels.hpp"
 #include "wcc_kernels.hpp"
+#include "arena_kernels.hpp"
 
 using namespace scd_dev;
 
@@ -170,11 +171,28 @@ struct scd_ctx {
     size_t ev_used = 0;
 
     std::vector<DevBuf> bufs;
+    // buffers of the previous graph, reused by the next upload when a size
+    // fits (a server re-assembles arenas often); freed one upload later
+    std::vector<DevBuf> cache;
 };
 
 template <typename T>
 static int dalloc(scd_ctx *c, T **p, size_t count, std::vector<DevBuf> *list = nullptr) {
     const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (!list) {  // graph buffers: best fit among the cached ones, at most 2x + 1 MiB
+        size_t best = c->cache.size();
+        for (size_t i = 0; i < c->cache.size(); i++) {
+            const size_t b = c->cache[i].bytes;
+            if (b >= bytes && b <= 2 * bytes + (1u << 20) && (best == c->cache.size() || b < c->cache[best].bytes))
+                best = i;
+        }
+        if (best < c->cache.size()) {
+            *p = (T *)c->cache[best].p;
+            c->bufs.push_back(c->cache[best]);
+            c->cache.erase(c->cache.begin() + best);
+            return SCD_OK;
+        }
+    }
     void *q = nullptr;
     hipError_t e = hipMalloc(&q, bytes);
     if (e != hipSuccess)
@@ -201,10 +219,12 @@ static void drop_graphs(scd_ctx *c) {
         }
 }
 
-static void free_graph(scd_ctx *c) {
+static void free_graph(scd_ctx *c, bool keep_cache = true) {
     drop_graphs(c);
     free_list(c, c->hub_bufs);
-    free_list(c, c->bufs);
+    free_list(c, c->cache);
+    if (keep_cache) c->cache.swap(c->bufs);
+    else free_list(c, c->bufs);
     if (c->wcc_g_host) (void)hipHostFree(c->wcc_g_host);
     c->wcc_g_host = nullptr;
     c->G = 0;
@@ -632,12 +652,16 @@ extern "C" void scd_destroy(scd_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    free_graph(c);
+    free_graph(c, false);
+    free_list(c, c->cache);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     (void)hipHostFree(c->wcc_host);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
+
+static int alloc_graph(scd_ctx *c, uint32_t n, uint32_t m, double clustering_coef);
+static int finish_upload(scd_ctx *c, const uint32_t *h_indices);
 
 extern "C" int scd_upload(scd_ctx *c, uint32_t n, uint32_t m, const uint32_t *indices,
                           const uint32_t *adj, const int32_t *tri, const int32_t *tri_deg,
@@ -650,14 +674,24 @@ extern "C" int scd_upload(scd_ctx *c, uint32_t n, uint32_t m, const uint32_t *in
     if (indices[0] != 0 || indices[n] != E)
         return fail(SCD_EINVAL, "scd_upload: indices[0] = %u, indices[n] = %u, expected 0 and %llu",
                     indices[0], indices[n], (unsigned long long)E);
+    for (uint32_t v = 0; v < n; v++)
+        if (indices[v + 1] < indices[v]) return fail(SCD_EINVAL, "scd_upload: indices decrease at %u", v);
+    int rc = alloc_graph(c, n, m, clustering_coef);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(c->indices, indices, sizeof(uint32_t) * ((size_t)n + 1), hipMemcpyHostToDevice, c->stream));
+    if (E) HIP_TRY(hipMemcpyAsync(c->adj, adj, sizeof(uint32_t) * E, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->tri, tri, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->tri_deg, tri_deg, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+    return finish_upload(c, indices);
+}
+
+// Frees the previous graph and allocates every per-vertex and per-entry
+// buffer for n vertices and m edges.
+static int alloc_graph(scd_ctx *c, uint32_t n, uint32_t m, double clustering_coef) {
+    const uint64_t E = 2ull * m;
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));
     free_graph(c);
-    c->h_deg.resize(n);
-    for (uint32_t v = 0; v < n; v++) {
-        if (indices[v + 1] < indices[v]) return fail(SCD_EINVAL, "scd_upload: indices decrease at %u", v);
-        c->h_deg[v] = indices[v + 1] - indices[v];
-    }
     c->n = n;
     c->m = m;
     c->E = E;
@@ -678,12 +712,17 @@ extern "C" int scd_upload(scd_ctx *c, uint32_t n, uint32_t m, const uint32_t *in
     }
     c->lab_a = c->cur;
     c->lab_b = c->next;
-    HIP_TRY(hipMemcpyAsync(c->indices, indices, sizeof(uint32_t) * ((size_t)n + 1), hipMemcpyHostToDevice, c->stream));
-    if (E) HIP_TRY(hipMemcpyAsync(c->adj, adj, sizeof(uint32_t) * E, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->tri, tri, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->tri_deg, tri_deg, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+    return SCD_OK;
+}
+
+// After the graph arrays are on the device (enqueued on the stream): host
+// degrees from the host copy of indices, degree bins.
+static int finish_upload(scd_ctx *c, const uint32_t *h_indices) {
     HIP_TRY(hipStreamSynchronize(c->stream));
-    if ((rc = build_bins(c))) {
+    c->h_deg.resize(c->n);
+    for (uint32_t v = 0; v < c->n; v++) c->h_deg[v] = h_indices[v + 1] - h_indices[v];
+    int rc = build_bins(c);
+    if (rc) {
         free_graph(c);
         return rc;
     }
@@ -1290,6 +1329,8 @@ extern "C" int scd_initial_partition(scd_ctx *c, uint32_t *labels_out, uint32_t 
 // preserving each graph's label order and hence its tie-breaks) and local
 // (0-based per graph) at the API.
 // ---------------------------------------------------------------------------
+static int setup_batch(scd_ctx *c, uint32_t G, const uint64_t *vtx_off, const double *cc);
+
 extern "C" int scd_batch_upload(scd_ctx *c, uint32_t G, const uint64_t *vtx_off, const uint32_t *indices,
                                 const uint32_t *adj, const int32_t *tri, const int32_t *tri_deg, const double *cc) {
     if (!c || !vtx_off || !indices || !cc) return fail(SCD_EINVAL, "scd_batch_upload: null argument");
@@ -1304,6 +1345,15 @@ extern "C" int scd_batch_upload(scd_ctx *c, uint32_t G, const uint64_t *vtx_off,
     if (E & 1u) return fail(SCD_EINVAL, "scd_batch_upload: odd adjacency length %u", E);
     int rc = scd_upload(c, n, E / 2, indices, adj, tri, tri_deg, 0.0);
     if (rc) return rc;
+    return setup_batch(c, G, vtx_off, cc);
+}
+
+// Per-graph state of an arena whose union graph is uploaded: graph of each
+// vertex, clustering coefficients, activity and store flags, WCC reduction
+// items.  Labels offsets start compact (set by the initial partition).
+static int setup_batch(scd_ctx *c, uint32_t G, const uint64_t *vtx_off, const double *cc) {
+    const uint32_t n = c->n;
+    int rc;
     std::vector<uint32_t> gid(n), ws, we, gib(G + 1);
     for (uint32_t g = 0; g < G; g++) {
         for (uint64_t v = vtx_off[g]; v < vtx_off[g + 1]; v++) gid[v] = g;
@@ -1464,3 +1514,116 @@ extern "C" int scd_batch_current_labels(scd_ctx *c, uint32_t *labels_out) {
     if (!labels_out) return fail(SCD_EINVAL, "scd_batch_current_labels: null output");
     return labels_to_local(c, c->cur, labels_out);
 }
+
+// ---------------------------------------------------------------------------
+// Arenas assembled from device-resident graphs, step-wise batch refinement
+// and per-graph label export: the pieces a server needs for continuous
+// batching (graphs join and leave between iterations).
+// ---------------------------------------------------------------------------
+
+extern "C" int scd_batch_upload_parts(scd_ctx *c, uint32_t G, const scd_part *parts,
+                                      const uint32_t *const *cur_labels, const uint32_t *const *best_labels) {
+    if (!c || !parts) return fail(SCD_EINVAL, "scd_batch_upload_parts: null argument");
+    if (G == 0) return fail(SCD_EINVAL, "scd_batch_upload_parts: no graphs");
+    std::vector<uint64_t> vtx_off(G + 1, 0), eoff(G + 1, 0);
+    std::vector<double> cc(G);
+    for (uint32_t g = 0; g < G; g++) {
+        const scd_part &p = parts[g];
+        if (p.n == 0) return fail(SCD_EINVAL, "scd_batch_upload_parts: graph %u has no vertices", g);
+        if (!p.indices || (p.m && !p.adj) || !p.tri || !p.tri_deg)
+            return fail(SCD_EINVAL, "scd_batch_upload_parts: graph %u: null array", g);
+        if (cur_labels && !cur_labels[g])
+            return fail(SCD_EINVAL, "scd_batch_upload_parts: graph %u has no labels", g);
+        vtx_off[g + 1] = vtx_off[g] + p.n;
+        eoff[g + 1] = eoff[g] + 2ull * p.m;
+        cc[g] = p.clustering_coef;
+    }
+    if (vtx_off[G] >= 0xFFFFFFFFull) return fail(SCD_ELIMIT, "scd_batch_upload_parts: too many vertices");
+    if (eoff[G] >= (1ull << 32)) return fail(SCD_ELIMIT, "scd_batch_upload_parts: 2m >= 2^32");
+    const uint32_t n = (uint32_t)vtx_off[G];
+    int rc = alloc_graph(c, n, (uint32_t)(eoff[G] / 2), 0.0);
+    if (rc) return rc;
+    for (uint32_t g = 0; g < G; g++) {
+        const scd_part &p = parts[g];
+        const uint32_t vo = (uint32_t)vtx_off[g], eo = (uint32_t)eoff[g];
+        LAUNCH(c, "arena_indices", cdiv(p.n, 256), 256, k_arena_shift, p.indices, (uint64_t)p.n, eo,
+               c->indices + vo);
+        if (p.m)
+            LAUNCH(c, "arena_adj", cdiv(2ull * p.m, 256), 256, k_arena_shift, p.adj, 2ull * p.m, vo, c->adj + eo);
+        HIP_TRY(hipMemcpyAsync(c->tri + vo, p.tri, 4ull * p.n, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->tri_deg + vo, p.tri_deg, 4ull * p.n, hipMemcpyDeviceToDevice, c->stream));
+    }
+    const uint32_t Eall = (uint32_t)eoff[G];
+    HIP_TRY(hipMemcpyAsync(c->indices + n, &Eall, 4, hipMemcpyHostToDevice, c->stream));
+    if ((rc = check_launch("arena assembly"))) return rc;
+    std::vector<uint32_t> h_ind((size_t)n + 1);
+    HIP_TRY(hipMemcpyAsync(h_ind.data(), c->indices, 4ull * (n + 1), hipMemcpyDeviceToHost, c->stream));
+    if ((rc = finish_upload(c, h_ind.data())) || (rc = setup_batch(c, G, vtx_off.data(), cc.data()))) return rc;
+    if (!cur_labels) return SCD_OK;
+    // labels of graph g live at [vtx_off[g], vtx_off[g+1]) of the label space
+    for (uint32_t g = 0; g < G; g++) {
+        const uint32_t vo = (uint32_t)vtx_off[g];
+        LAUNCH(c, "arena_labels", cdiv(parts[g].n, 256), 256, k_arena_labels, cur_labels[g], parts[g].n, vo, 1,
+               c->cur + vo);
+        const uint32_t *b = best_labels && best_labels[g] ? best_labels[g] : cur_labels[g];
+        LAUNCH(c, "arena_labels", cdiv(parts[g].n, 256), 256, k_arena_labels, b, parts[g].n, vo, 1, c->best + vo);
+        c->h_label_off[g] = vo;
+    }
+    c->num_comms = n;
+    if ((rc = check_launch("arena labels")) || (rc = clear_counters(c)) || (rc = phase_stats(c)) ||
+        (rc = finish(c)))
+        return rc;
+    c->have_labels = true;
+    c->wcc_valid = false;
+    return SCD_OK;
+}
+
+extern "C" int scd_batch_export_labels(scd_ctx *c, uint32_t *const *cur_out, uint32_t *const *best_out) {
+    int rc = need_batch(c, "scd_batch_export_labels", true);
+    if (rc) return rc;
+    for (uint32_t g = 0; g < c->G; g++) {
+        const uint32_t vo = (uint32_t)c->h_vtx_off[g], ng = (uint32_t)(c->h_vtx_off[g + 1] - vo);
+        if (cur_out && cur_out[g])
+            LAUNCH(c, "arena_export", cdiv(ng, 256), 256, k_arena_labels, c->cur + vo, ng, c->h_label_off[g], 0,
+                   cur_out[g]);
+        if (best_out && best_out[g])
+            LAUNCH(c, "arena_export", cdiv(ng, 256), 256, k_arena_labels, c->best + vo, ng, c->h_label_off[g], 0,
+                   best_out[g]);
+    }
+    if ((rc = check_launch("label export"))) return rc;
+    return finish(c);
+}
+
+extern "C" int scd_batch_graph_labels(scd_ctx *c, uint32_t g, int best, uint32_t *labels_out) {
+    int rc = need_batch(c, "scd_batch_graph_labels", true);
+    if (rc) return rc;
+    if (g >= c->G || !labels_out) return fail(SCD_EINVAL, "scd_batch_graph_labels: bad graph or null output");
+    const uint64_t vo = c->h_vtx_off[g], ng = c->h_vtx_off[g + 1] - vo;
+    HIP_TRY(hipMemcpyAsync(labels_out, (best ? c->best : c->cur) + vo, 4ull * ng, hipMemcpyDeviceToHost, c->stream));
+    if ((rc = finish(c))) return rc;
+    const uint32_t off = c->h_label_off[g];
+    for (uint64_t v = 0; v < ng; v++)
+        if (labels_out[v] != SCD_INVALID_LABEL) labels_out[v] -= off;
+    return SCD_OK;
+}
+
+extern "C" int scd_batch_step(scd_ctx *c, const uint32_t *active, double *wcc_out) {
+    int rc = need_batch(c, "scd_batch_step", true);
+    if (rc) return rc;
+    if (!active) return fail(SCD_EINVAL, "scd_batch_step: null activity flags");
+    HIP_TRY(hipMemcpyAsync(c->active_g, active, 4ull * c->G, hipMemcpyHostToDevice, c->stream));
+    if ((rc = scd_improve(c, nullptr, nullptr))) return rc;
+    if (wcc_out) memcpy(wcc_out, c->wcc_g_host, 8ull * c->G);
+    return SCD_OK;
+}
+
+extern "C" int scd_batch_store_best(scd_ctx *c, const uint32_t *store) {
+    int rc = need_batch(c, "scd_batch_store_best", true);
+    if (rc) return rc;
+    if (!store) return fail(SCD_EINVAL, "scd_batch_store_best: null flags");
+    HIP_TRY(hipMemcpyAsync(c->store_g, store, 4ull * c->G, hipMemcpyHostToDevice, c->stream));
+    LAUNCH(c, "store_best", cdiv(c->n, 256), 256, k_store_best_g, c->cur, c->gid, c->store_g, c->n, c->best);
+    if ((rc = check_launch("store best"))) return rc;
+    return finish(c);
+}
+

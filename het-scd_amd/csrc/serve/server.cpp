@@ -1,0 +1,383 @@
+// Serving loop over the batched engine (include/scd_serve.h).
+//
+// Every admitted graph is held on the device in its own buffers (CSR with
+// local ids, current and best labels).  Two engine contexts share the
+// device: `prep` computes the initial partition and initial WCC of newly
+// admitted graphs as one arena, `main` refines all joined graphs as one
+// arena.  A join re-assembles `main` from the resident graphs (the finished
+// ones drop out, the new ones come in with their initial labels, the others
+// carry their current and best labels over), so the iteration loop itself
+// never changes shape.  Each graph follows the stopping rule of
+// src/main-gpu.cpp:71-96 on its own WCC values, which are those of the
+// single-graph path (the batched engine guarantees it), so its result does
+// not depend on which graphs it was served with.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "scd_hip.h"
+#include "scd_host.h"
+#include "scd_serve.h"
+
+namespace {
+
+thread_local char g_err[512];
+
+int sfail(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+// an engine call failed: keep its message
+int engine_fail(int rc, const char *what) {
+    snprintf(g_err, sizeof(g_err), "%s: %s", what, scd_last_error());
+    return rc;
+}
+
+enum State { QUEUED = 0, RESIDENT = 1, REFINING = 2, DONE = 3, FAILED = 4 };
+
+struct Req {
+    scd_request in{};
+    int state = QUEUED;
+    uint32_t *d_ind = nullptr, *d_adj = nullptr, *d_cur = nullptr, *d_best = nullptr;
+    int32_t *d_tri = nullptr, *d_td = nullptr;
+    uint64_t bytes = 0;
+    double best = 0.0;  // best WCC so far (the loop's `best`)
+    scd_result res{};
+    std::vector<uint32_t> labels;
+};
+
+const double kThreshold = 0.01;  // src/main-gpu.cpp:88, stop below best * (1 + 0.01)
+
+}  // namespace
+
+struct scd_server {
+    scd_server_opts o{};
+    scd_ctx *main = nullptr, *prep = nullptr;
+    std::vector<Req> reqs;
+    std::vector<size_t> arena;     // request of each graph of the main arena
+    std::vector<uint32_t> active;  // per arena graph
+    uint32_t n_active = 0;
+    std::vector<size_t> waiting;   // resident, not joined yet (arrival order)
+    uint64_t resident = 0;
+    scd_server_stats st{};
+    std::chrono::steady_clock::time_point t0;
+
+    double now() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+extern "C" const char *scd_server_last_error(void) { return g_err; }
+
+extern "C" uint64_t scd_server_footprint(uint32_t n, uint32_t m) {
+    // resident: CSR (n + 1 + 2m) and tri, tri_deg, two label arrays (4n);
+    // arena share: the engine's per-vertex (about 100 B) and per-entry
+    // (adj, internal adjacency, flags: 9 B) buffers
+    const uint64_t N = n, E = 2ull * m;
+    return 4 * (N + 1 + E) + 16 * N + 104 * N + 9 * E;
+}
+
+extern "C" int scd_server_create(const scd_server_opts *opts, scd_server **out) {
+    if (!opts || !out) return sfail(SCD_EINVAL, "scd_server_create: null argument");
+    *out = nullptr;
+    auto *s = new scd_server();
+    s->o = *opts;
+    if (s->o.max_iters == 0) s->o.max_iters = 1000;
+    if (s->o.join_wait_ms < 0) s->o.join_wait_ms = 0;
+    int rc = scd_init(opts->device, 256, 0, &s->main);
+    if (rc == SCD_OK) rc = scd_init(opts->device, 256, 0, &s->prep);
+    if (rc != SCD_OK) {
+        rc = engine_fail(rc, "scd_server_create");
+        scd_server_destroy(s);
+        return rc;
+    }
+    scd_set_graph_mode(s->main, opts->graph_mode ? 1 : 0);
+    scd_set_graph_mode(s->prep, 0);
+    if (s->o.hbm_budget == 0) {
+        size_t fr = 0, tot = 0;
+        if (hipSetDevice(opts->device) != hipSuccess || hipMemGetInfo(&fr, &tot) != hipSuccess) {
+            scd_server_destroy(s);
+            return sfail(SCD_EDEVICE, "scd_server_create: cannot query device memory");
+        }
+        s->o.hbm_budget = (uint64_t)(0.6 * (double)fr);
+    }
+    *out = s;
+    return SCD_OK;
+}
+
+static void release(scd_server *s, Req &r) {
+    (void)hipFree(r.d_ind);
+    (void)hipFree(r.d_adj);
+    (void)hipFree(r.d_tri);
+    (void)hipFree(r.d_td);
+    (void)hipFree(r.d_cur);
+    (void)hipFree(r.d_best);
+    r.d_ind = r.d_adj = r.d_cur = r.d_best = nullptr;
+    r.d_tri = r.d_td = nullptr;
+    if (r.bytes) s->resident -= r.bytes;
+    r.bytes = 0;
+}
+
+extern "C" void scd_server_destroy(scd_server *s) {
+    if (!s) return;
+    for (auto &r : s->reqs) release(s, r);
+    if (s->main) scd_destroy(s->main);
+    if (s->prep) scd_destroy(s->prep);
+    delete s;
+}
+
+extern "C" int scd_server_submit(scd_server *s, const scd_request *req, uint64_t *id_out) {
+    if (!s || !req) return sfail(SCD_EINVAL, "scd_server_submit: null argument");
+    const scd_request &q = *req;
+    if (q.n == 0) return sfail(SCD_EINVAL, "scd_server_submit: graph has no vertices");
+    if (!q.indices || (q.m && !q.adj) || !q.tri || !q.tri_deg)
+        return sfail(SCD_EINVAL, "scd_server_submit: null array");
+    if (q.m >= (1u << 31)) return sfail(SCD_ELIMIT, "scd_server_submit: m = %u needs 2m < 2^32", q.m);
+    if (q.indices[0] != 0 || q.indices[q.n] != 2ull * q.m)
+        return sfail(SCD_EINVAL, "scd_server_submit: indices[0] = %u, indices[n] = %u, expected 0 and %llu",
+                     q.indices[0], q.indices[q.n], 2ull * q.m);
+    // the initial partition's key needs degree >= 2 (src/scd.cpp:130)
+    for (uint32_t v = 0; v < q.n; v++)
+        if (q.indices[v + 1] < q.indices[v] + 2)
+            return sfail(SCD_EINVAL, "scd_server_submit: vertex %u has degree < 2 (not a convert output)", v);
+    Req r;
+    r.in = q;
+    r.res.status = SCD_OK;
+    r.res.arrival_ms = q.arrival_ms;
+    s->reqs.push_back(std::move(r));
+    if (id_out) *id_out = s->reqs.size() - 1;
+    return SCD_OK;
+}
+
+// Copies a request's graph to the device.
+static int admit(scd_server *s, Req &r) {
+    const scd_request &q = r.in;
+    const uint64_t N = q.n, E = 2ull * q.m;
+    auto dmalloc = [](void **p, uint64_t bytes) {
+        return hipMalloc(p, std::max<uint64_t>(bytes, 4)) == hipSuccess;
+    };
+    if (!dmalloc((void **)&r.d_ind, 4 * (N + 1)) || !dmalloc((void **)&r.d_adj, 4 * E) ||
+        !dmalloc((void **)&r.d_tri, 4 * N) || !dmalloc((void **)&r.d_td, 4 * N) ||
+        !dmalloc((void **)&r.d_cur, 4 * N) || !dmalloc((void **)&r.d_best, 4 * N)) {
+        release(s, r);
+        return sfail(SCD_ENOMEM, "admission: device allocation failed");
+    }
+    if (hipMemcpy(r.d_ind, q.indices, 4 * (N + 1), hipMemcpyHostToDevice) != hipSuccess ||
+        (E && hipMemcpy(r.d_adj, q.adj, 4 * E, hipMemcpyHostToDevice) != hipSuccess) ||
+        hipMemcpy(r.d_tri, q.tri, 4 * N, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(r.d_td, q.tri_deg, 4 * N, hipMemcpyHostToDevice) != hipSuccess) {
+        release(s, r);
+        return sfail(SCD_EDEVICE, "admission: copy to the device failed");
+    }
+    r.bytes = scd_server_footprint(q.n, q.m);
+    s->resident += r.bytes;
+    s->st.peak_bytes = std::max(s->st.peak_bytes, s->resident);
+    r.state = RESIDENT;
+    r.res.admit_ms = s->now();
+    return SCD_OK;
+}
+
+static scd_part part_of(const Req &r) {
+    scd_part p;
+    p.n = r.in.n;
+    p.m = r.in.m;
+    p.indices = r.d_ind;
+    p.adj = r.d_adj;
+    p.tri = r.d_tri;
+    p.tri_deg = r.d_td;
+    p.clustering_coef = r.in.clustering_coef;
+    return p;
+}
+
+// Initial labels of the waiting graphs, then a new main arena from the
+// still-refining graphs (labels carried over) and the waiting ones.
+static int join(scd_server *s) {
+    const uint32_t P = (uint32_t)s->waiting.size();
+    std::vector<scd_part> parts(P);
+    std::vector<uint32_t *> cur(P), best(P);
+    for (uint32_t i = 0; i < P; i++) {
+        const Req &r = s->reqs[s->waiting[i]];
+        parts[i] = part_of(r);
+        cur[i] = r.d_cur;
+        best[i] = r.d_best;
+    }
+    std::vector<double> w0(P);
+    int rc;
+    if ((rc = scd_batch_upload_parts(s->prep, P, parts.data(), nullptr, nullptr)) ||
+        (rc = scd_batch_initial_partition(s->prep, nullptr, nullptr)) ||
+        (rc = scd_batch_compute_wcc(s->prep, w0.data())) ||
+        (rc = scd_batch_export_labels(s->prep, cur.data(), best.data())))
+        return engine_fail(rc, "initial partition of joining graphs");
+    const double t = s->now();
+    for (uint32_t i = 0; i < P; i++) {
+        Req &r = s->reqs[s->waiting[i]];
+        r.best = w0[i];
+        r.res.initial_wcc = w0[i];
+        r.res.start_ms = t;
+    }
+    // carry the refining graphs' labels out of the old arena
+    if (!s->arena.empty()) {
+        const size_t G = s->arena.size();
+        std::vector<uint32_t *> c2(G, nullptr), b2(G, nullptr);
+        for (size_t g = 0; g < G; g++)
+            if (s->active[g]) {
+                c2[g] = s->reqs[s->arena[g]].d_cur;
+                b2[g] = s->reqs[s->arena[g]].d_best;
+            }
+        if ((rc = scd_batch_export_labels(s->main, c2.data(), b2.data())))
+            return engine_fail(rc, "label carry-over");
+    }
+    std::vector<size_t> next;
+    for (size_t g = 0; g < s->arena.size(); g++)
+        if (s->active[g]) next.push_back(s->arena[g]);
+    for (size_t i : s->waiting) next.push_back(i);
+    const uint32_t G = (uint32_t)next.size();
+    std::vector<scd_part> all(G);
+    std::vector<const uint32_t *> ac(G), ab(G);
+    for (uint32_t g = 0; g < G; g++) {
+        const Req &r = s->reqs[next[g]];
+        all[g] = part_of(r);
+        ac[g] = r.d_cur;
+        ab[g] = r.d_best;
+    }
+    if ((rc = scd_batch_upload_parts(s->main, G, all.data(), ac.data(), ab.data())))
+        return engine_fail(rc, "arena assembly");
+    for (size_t i : s->waiting) {
+        s->reqs[i].state = REFINING;
+        s->reqs[i].res.joined_arena_graphs = G;
+    }
+    s->arena.swap(next);
+    s->active.assign(G, 1u);
+    s->n_active = G;
+    s->waiting.clear();
+    s->st.joins++;
+    s->st.peak_graphs = std::max(s->st.peak_graphs, G);
+    return SCD_OK;
+}
+
+// One iteration of every refining graph; finished graphs hand back their
+// best labels and leave.
+static int step(scd_server *s) {
+    const size_t G = s->arena.size();
+    std::vector<double> w(G, 0.0);
+    int rc = scd_batch_step(s->main, s->active.data(), w.data());
+    if (rc) return engine_fail(rc, "iteration");
+    s->st.iterations++;
+    std::vector<uint32_t> store(G, 0u), stop(G, 0u);
+    bool any_store = false;
+    for (size_t g = 0; g < G; g++) {
+        if (!s->active[g]) continue;
+        Req &r = s->reqs[s->arena[g]];
+        r.res.iterations++;
+        if (w[g] > r.best) {
+            store[g] = 1;
+            any_store = true;
+        }
+        if (w[g] < r.best * (1 + kThreshold)) {
+            stop[g] = 1;
+        } else {
+            r.best = w[g];
+            if (r.res.iterations >= s->o.max_iters) stop[g] = 1;
+        }
+    }
+    if (any_store && (rc = scd_batch_store_best(s->main, store.data())))
+        return engine_fail(rc, "store best labels");
+    for (size_t g = 0; g < G; g++) {
+        if (!stop[g]) continue;
+        Req &r = s->reqs[s->arena[g]];
+        r.labels.resize(r.in.n);
+        if ((rc = scd_batch_graph_labels(s->main, (uint32_t)g, 1, r.labels.data())))
+            return engine_fail(rc, "result labels");
+        r.res.best_wcc = r.best;
+        r.res.done_ms = s->now();
+        r.state = DONE;
+        s->active[g] = 0;
+        s->n_active--;
+        release(s, r);
+    }
+    return SCD_OK;
+}
+
+extern "C" int scd_server_run(scd_server *s) {
+    if (!s) return sfail(SCD_EINVAL, "scd_server_run: null server");
+    std::vector<size_t> order;
+    for (size_t i = 0; i < s->reqs.size(); i++)
+        if (s->reqs[i].state == QUEUED) order.push_back(i);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return s->reqs[a].in.arrival_ms < s->reqs[b].in.arrival_ms; });
+    s->t0 = std::chrono::steady_clock::now();
+    size_t qi = 0;
+    while (true) {
+        double now = s->now();
+        // admission, first come first served, within the footprint budget
+        while (qi < order.size()) {
+            Req &r = s->reqs[order[qi]];
+            if (r.in.arrival_ms > now) break;
+            const uint64_t fp = scd_server_footprint(r.in.n, r.in.m);
+            if (fp > s->o.hbm_budget) {
+                r.state = FAILED;
+                r.res.status = SCD_ELIMIT;
+                r.res.done_ms = now;
+                qi++;
+                continue;
+            }
+            if (s->resident + fp > s->o.hbm_budget) break;
+            if (s->o.max_graphs && s->n_active + s->waiting.size() >= s->o.max_graphs) break;
+            const int rc = admit(s, r);
+            if (rc) {
+                r.state = FAILED;
+                r.res.status = rc;
+                r.res.done_ms = now;
+            } else {
+                s->waiting.push_back(order[qi]);
+            }
+            qi++;
+        }
+        if (!s->waiting.empty()) {
+            const double oldest = s->reqs[s->waiting.front()].in.arrival_ms;
+            if (s->n_active == 0 || now - oldest >= s->o.join_wait_ms || 2 * s->n_active <= s->arena.size()) {
+                const double t = s->now();
+                const int rc = join(s);
+                s->st.busy_ms += s->now() - t;
+                if (rc) return rc;
+            }
+        }
+        if (s->n_active == 0) {
+            if (!s->waiting.empty()) continue;
+            if (qi >= order.size()) break;
+            const double wait = s->reqs[order[qi]].in.arrival_ms - s->now();
+            if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double, std::milli>(wait));
+            continue;
+        }
+        const double t = s->now();
+        const int rc = step(s);
+        s->st.busy_ms += s->now() - t;
+        if (rc) return rc;
+    }
+    return SCD_OK;
+}
+
+extern "C" int scd_server_result(scd_server *s, uint64_t id, uint32_t *labels_out, scd_result *out) {
+    if (!s || id >= s->reqs.size()) return sfail(SCD_EINVAL, "scd_server_result: unknown request");
+    const Req &r = s->reqs[id];
+    if (r.state != DONE && r.state != FAILED) return sfail(SCD_ESTATE, "scd_server_result: request not finished");
+    if (out) *out = r.res;
+    if (labels_out && r.state == DONE) memcpy(labels_out, r.labels.data(), 4ull * r.in.n);
+    return SCD_OK;
+}
+
+extern "C" int scd_server_get_stats(const scd_server *s, scd_server_stats *out) {
+    if (!s || !out) return sfail(SCD_EINVAL, "scd_server_get_stats: null argument");
+    *out = s->st;
+    return SCD_OK;
+}

@@ -152,6 +152,12 @@ def hip():
         "scd_batch_refine": (ctypes.c_int, [C, f64p, ctypes.c_uint32, f64p, u32p, f64p, f64p]),
         "scd_batch_labels": (ctypes.c_int, [C, u32p]),
         "scd_batch_current_labels": (ctypes.c_int, [C, u32p]),
+        "scd_batch_upload_parts": (ctypes.c_int, [C, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
+                                                  ctypes.c_void_p]),
+        "scd_batch_export_labels": (ctypes.c_int, [C, ctypes.c_void_p, ctypes.c_void_p]),
+        "scd_batch_graph_labels": (ctypes.c_int, [C, ctypes.c_uint32, ctypes.c_int, u32p]),
+        "scd_batch_step": (ctypes.c_int, [C, u32p, f64p]),
+        "scd_batch_store_best": (ctypes.c_int, [C, u32p]),
     }
     for k, (res, args) in sig.items():
         f = getattr(lib, k, None)
@@ -180,3 +186,57 @@ def ptr(a, t):
     if a is None:
         return None
     return a.ctypes.data_as(t)
+
+
+class ScdServerOptsC(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("hbm_budget", ctypes.c_uint64), ("max_graphs", ctypes.c_uint32),
+                ("join_wait_ms", ctypes.c_double), ("max_iters", ctypes.c_uint32), ("graph_mode", ctypes.c_int)]
+
+
+class ScdRequestC(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint32), ("m", ctypes.c_uint32), ("indices", u32p), ("adj", u32p), ("tri", i32p),
+                ("tri_deg", i32p), ("clustering_coef", ctypes.c_double), ("arrival_ms", ctypes.c_double)]
+
+
+class ScdResultC(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("iterations", ctypes.c_uint32), ("initial_wcc", ctypes.c_double),
+                ("best_wcc", ctypes.c_double), ("arrival_ms", ctypes.c_double), ("admit_ms", ctypes.c_double),
+                ("start_ms", ctypes.c_double), ("done_ms", ctypes.c_double),
+                ("joined_arena_graphs", ctypes.c_uint32)]
+
+
+class ScdServerStatsC(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_uint64), ("joins", ctypes.c_uint64), ("busy_ms", ctypes.c_double),
+                ("peak_bytes", ctypes.c_uint64), ("peak_graphs", ctypes.c_uint32)]
+
+
+def serve():
+    """libscd_serve.so with argtypes set."""
+    if "serve_ready" in _cache:
+        return _cache["libscd_serve.so"]
+    hip()
+    lib = _load("libscd_serve.so")
+    S = vp
+    sig = {
+        "scd_server_last_error": (ctypes.c_char_p, []),
+        "scd_server_create": (ctypes.c_int, [ctypes.POINTER(ScdServerOptsC), ctypes.POINTER(S)]),
+        "scd_server_destroy": (None, [S]),
+        "scd_server_submit": (ctypes.c_int, [S, ctypes.POINTER(ScdRequestC), u64p]),
+        "scd_server_run": (ctypes.c_int, [S]),
+        "scd_server_result": (ctypes.c_int, [S, ctypes.c_uint64, u32p, ctypes.POINTER(ScdResultC)]),
+        "scd_server_get_stats": (ctypes.c_int, [S, ctypes.POINTER(ScdServerStatsC)]),
+        "scd_server_footprint": (ctypes.c_uint64, [ctypes.c_uint32, ctypes.c_uint32]),
+    }
+    for k, (res, args) in sig.items():
+        f = getattr(lib, k)
+        f.restype = res
+        f.argtypes = args
+    _cache["serve_ready"] = True
+    return lib
+
+
+def check_serve(rc):
+    if rc < 0:
+        raise ScdError(rc, serve().scd_server_last_error().decode())
+    return rc
+

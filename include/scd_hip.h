@@ -258,6 +258,44 @@ int scd_batch_labels(scd_ctx *ctx, uint32_t *labels_out);
 /* Current labels (local). */
 int scd_batch_current_labels(scd_ctx *ctx, uint32_t *labels_out);
 
+/*
+ * Arenas from graphs already on the device (no counterpart in the
+ * reference; the building blocks of continuous batching, where graphs join
+ * and leave an arena between iterations).  A part is one graph in the
+ * reference's CSR layout (src/graph.hpp:23-33) with local ids, every array
+ * a device pointer of the context's device.
+ */
+typedef struct scd_part {
+    uint32_t n, m;
+    const uint32_t *indices; /* n + 1 */
+    const uint32_t *adj;     /* 2m, may be NULL when m == 0 */
+    const int32_t *tri;      /* n */
+    const int32_t *tri_deg;  /* n */
+    double clustering_coef;
+} scd_part;
+/*
+ * Replaces the context's graph by the disjoint union of parts[0..G).  With
+ * cur_labels (G device pointers to local labels, n entries each) the arena
+ * starts from those labels (best_labels likewise, NULL entries or a NULL
+ * array: best = current), statistics recomputed, ready for scd_batch_step;
+ * without, an initial partition must follow.
+ */
+int scd_batch_upload_parts(scd_ctx *ctx, uint32_t G, const scd_part *parts,
+                           const uint32_t *const *cur_labels, const uint32_t *const *best_labels);
+/* Current / best local labels of each graph into device buffers (NULL
+ * arrays or entries are skipped). */
+int scd_batch_export_labels(scd_ctx *ctx, uint32_t *const *cur_out, uint32_t *const *best_out);
+/* Best (best != 0) or current local labels of graph g into host memory. */
+int scd_batch_graph_labels(scd_ctx *ctx, uint32_t g, int best, uint32_t *labels_out);
+/*
+ * One iteration (cuda_improve_labels, src/cuda.cu:176-351) of the graphs
+ * whose active[g] != 0 (host array, G entries); the others keep their labels.
+ * wcc_out (G entries, may be NULL): the WCC of each active graph.
+ */
+int scd_batch_step(scd_ctx *ctx, const uint32_t *active, double *wcc_out);
+/* cuda_store_labels (src/cuda.cu:353-357) for the graphs with store[g] != 0. */
+int scd_batch_store_best(scd_ctx *ctx, const uint32_t *store);
+
 #ifdef __cplusplus
 }
 #endif

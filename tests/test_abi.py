@@ -19,10 +19,11 @@ def declared(header):
     return sorted(set(names))
 
 
-@pytest.mark.parametrize("header,lib", [("scd_host.h", "libscd_host.so"), ("scd_hip.h", "libscd_hip.so")])
+@pytest.mark.parametrize("header,lib", [("scd_host.h", "libscd_host.so"), ("scd_hip.h", "libscd_hip.so"),
+                                        ("scd_serve.h", "libscd_serve.so")])
 def test_library_exports_header(header, lib):
     names = declared(header)
-    assert len(names) >= 10
+    assert len(names) >= 8
     L = ctypes.CDLL(os.path.join(_lib.LIB_DIR, lib))
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, f"{lib} lacks {missing}"
@@ -49,3 +50,18 @@ def test_init_rejects_bad_group_size():
     with pytest.raises(scd_amd.ScdError) as ei:
         scd_amd.Engine(device=0, work_group_size=100)
     assert ei.value.code == -1
+
+
+def test_server_without_device_fails_cleanly():
+    if scd_amd.device_count() > 0:
+        pytest.skip("a device is present")
+    with pytest.raises(scd_amd.ScdError):
+        scd_amd.Server(device=0)
+
+
+def test_server_footprint_grows_with_the_graph():
+    f = scd_amd.Server.footprint
+    assert f(1000, 5000) < f(2000, 5000) < f(2000, 10000)
+    # at least the resident CSR and the arena's adjacency copies
+    assert f(1000, 5000) >= 4 * (1001 + 10000) + 4 * 10000
+
