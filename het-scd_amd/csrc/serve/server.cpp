@@ -213,12 +213,14 @@ static int join(scd_server *s) {
     }
     std::vector<double> w0(P);
     int rc;
+    const double tp = s->now();
     if ((rc = scd_batch_upload_parts(s->prep, P, parts.data(), nullptr, nullptr)) ||
         (rc = scd_batch_initial_partition(s->prep, nullptr, nullptr)) ||
         (rc = scd_batch_compute_wcc(s->prep, w0.data())) ||
         (rc = scd_batch_export_labels(s->prep, cur.data(), best.data())))
         return engine_fail(rc, "initial partition of joining graphs");
     const double t = s->now();
+    s->st.prep_ms += t - tp;
     for (uint32_t i = 0; i < P; i++) {
         Req &r = s->reqs[s->waiting[i]];
         r.best = w0[i];
@@ -252,6 +254,7 @@ static int join(scd_server *s) {
     }
     if ((rc = scd_batch_upload_parts(s->main, G, all.data(), ac.data(), ab.data())))
         return engine_fail(rc, "arena assembly");
+    s->st.assemble_ms += s->now() - t;
     for (size_t i : s->waiting) {
         s->reqs[i].state = REFINING;
         s->reqs[i].res.joined_arena_graphs = G;
@@ -333,7 +336,10 @@ extern "C" int scd_server_run(scd_server *s) {
             }
             if (s->resident + fp > s->o.hbm_budget) break;
             if (s->o.max_graphs && s->n_active + s->waiting.size() >= s->o.max_graphs) break;
+            const double ta = s->now();
             const int rc = admit(s, r);
+            s->st.admit_ms += s->now() - ta;
+            s->st.busy_ms += s->now() - ta;
             if (rc) {
                 r.state = FAILED;
                 r.res.status = rc;
@@ -362,6 +368,7 @@ extern "C" int scd_server_run(scd_server *s) {
         const double t = s->now();
         const int rc = step(s);
         s->st.busy_ms += s->now() - t;
+        s->st.step_ms += s->now() - t;
         if (rc) return rc;
     }
     return SCD_OK;

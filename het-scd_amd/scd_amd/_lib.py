@@ -207,7 +207,8 @@ class ScdResultC(ctypes.Structure):
 
 class ScdServerStatsC(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_uint64), ("joins", ctypes.c_uint64), ("busy_ms", ctypes.c_double),
-                ("peak_bytes", ctypes.c_uint64), ("peak_graphs", ctypes.c_uint32)]
+                ("peak_bytes", ctypes.c_uint64), ("peak_graphs", ctypes.c_uint32), ("admit_ms", ctypes.c_double),
+                ("prep_ms", ctypes.c_double), ("assemble_ms", ctypes.c_double), ("step_ms", ctypes.c_double)]
 
 
 def serve():

@@ -72,6 +72,9 @@ typedef struct scd_server_stats {
     double busy_ms;        /* time spent in device work and assembly */
     uint64_t peak_bytes;   /* highest admitted footprint */
     uint32_t peak_graphs;  /* most graphs in one arena */
+    /* where busy_ms went: uploads at admission, initial partitions and
+     * WCC of joining graphs, arena assembly, iterations */
+    double admit_ms, prep_ms, assemble_ms, step_ms;
 } scd_server_stats;
 
 const char *scd_server_last_error(void);
