@@ -19,6 +19,8 @@
 #include <cstdio>
 #include <cstring>
 #include <ctime>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -81,6 +83,7 @@ struct scd_ctx {
     int device = 0;
     int block_size = 256;
     uint32_t num_cus = 256;
+    std::string arch = "unknown";  // gcnArchName without feature suffixes
     bool profile = false;
     hipStream_t stream = nullptr;
     uint64_t dev_bytes = 0;
@@ -154,7 +157,7 @@ struct scd_ctx {
     double *chunk_bi = nullptr, *chunk_rem = nullptr;
     std::vector<DevBuf> hub_bufs;
 
-    scd_tuning tune = {4096, 2048, {0, 0, 0, 0, 0, 0}};
+    scd_tuning tune = {4096, 2048, 1024, {0, 0, 0, 0, 0}};
     uint32_t last_ip_rounds = 0;
 
     // per-iteration hipGraphs, one per label-buffer parity (cur == lab_a or
@@ -308,10 +311,11 @@ static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b -
 // Degree bins and hub work lists (host-built, uploaded once per graph and
 // again when the tuning changes).
 // ---------------------------------------------------------------------------
-static uint32_t bin_of(uint32_t d, uint32_t lds_tile) {
+static uint32_t bin_of(uint32_t d, uint32_t lds_tile, uint32_t wave_max) {
     if (d <= 8) return K_SUB8;
     if (d <= 16) return K_SUB16;
     if (d <= 32) return K_SUB32;
+    if (d > wave_max) return d <= lds_tile ? K_LDS : K_HUB;
     if (d <= 64) return K_W1;
     if (d <= 128) return K_W2;
     if (d <= 256) return K_W4;
@@ -324,15 +328,15 @@ static uint32_t bin_of(uint32_t d, uint32_t lds_tile) {
 static int build_bins(scd_ctx *c) {
     drop_graphs(c);
     free_list(c, c->hub_bufs);
-    const uint32_t n = c->n, T = c->tune.lds_tile, C = c->tune.merge_chunk;
+    const uint32_t n = c->n, T = c->tune.lds_tile, C = c->tune.merge_chunk, Wm = c->tune.wave_max;
     std::vector<uint32_t> cnt(K_NB + 1, 0);
-    for (uint32_t v = 0; v < n; v++) cnt[bin_of(c->h_deg[v], T) + 1]++;
+    for (uint32_t v = 0; v < n; v++) cnt[bin_of(c->h_deg[v], T, Wm) + 1]++;
     for (int b = 0; b < K_NB; b++) cnt[b + 1] += cnt[b];
     for (int b = 0; b <= K_NB; b++) c->bin_begin[b] = cnt[b];
     std::vector<uint32_t> order(n), pos(cnt.begin(), cnt.end() - 1);
-    for (uint32_t v = 0; v < n; v++) order[pos[bin_of(c->h_deg[v], T)]++] = v;
+    for (uint32_t v = 0; v < n; v++) order[pos[bin_of(c->h_deg[v], T, Wm)]++] = v;
     for (int b = 0; b < 16; b++) c->bin_edges[b] = 0;
-    for (uint32_t v = 0; v < n; v++) c->bin_edges[bin_of(c->h_deg[v], T)] += c->h_deg[v];
+    for (uint32_t v = 0; v < n; v++) c->bin_edges[bin_of(c->h_deg[v], T, Wm)] += c->h_deg[v];
     int rc;
     if ((rc = dalloc(c, &c->bin_verts, n, &c->hub_bufs))) return rc;
     HIP_TRY(hipMemcpy(c->bin_verts, order.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
@@ -627,8 +631,12 @@ extern "C" int scd_init(int device_id, int work_group_size, int profile, scd_ctx
     c->block_size = work_group_size;
     {
         hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, device_id) == hipSuccess && prop.multiProcessorCount > 0)
+        if (hipGetDeviceProperties(&prop, device_id) == hipSuccess && prop.multiProcessorCount > 0) {
             c->num_cus = (uint32_t)prop.multiProcessorCount;
+            c->arch = prop.gcnArchName;
+            const size_t colon = c->arch.find(':');
+            if (colon != std::string::npos) c->arch.resize(colon);
+        }
     }
     c->profile = profile != 0;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -1023,7 +1031,11 @@ extern "C" int scd_set_tuning(scd_ctx *c, const scd_tuning *t) {
         return fail(SCD_EINVAL, "scd_set_tuning: lds_tile %u is not 2048, 4096, 8192 or 16384", T);
     if (C < 256 || C > 2 * T || (C & (C - 1)))
         return fail(SCD_EINVAL, "scd_set_tuning: merge_chunk %u must be a power of two in [256, 2*lds_tile]", C);
+    const uint32_t W = t->wave_max ? t->wave_max : 1024u;
+    if (W != 256 && W != 512 && W != 1024)
+        return fail(SCD_EINVAL, "scd_set_tuning: wave_max %u is not 256, 512 or 1024", W);
     c->tune = *t;
+    c->tune.wave_max = W;
     if (c->have_graph) {
         (void)hipSetDevice(c->device);
         HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1625,5 +1637,163 @@ extern "C" int scd_batch_store_best(scd_ctx *c, const uint32_t *store) {
     LAUNCH(c, "store_best", cdiv(c->n, 256), 256, k_store_best_g, c->cur, c->gid, c->store_g, c->n, c->best);
     if ((rc = check_launch("store best"))) return rc;
     return finish(c);
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: the tuning (degree-bin thresholds, tile sizes) with the fastest
+// refinement iteration, per graph-size class.
+// ---------------------------------------------------------------------------
+
+namespace {
+struct TuneEntry {
+    scd_tuning t;
+    double ms;
+};
+std::mutex g_tune_mu;
+std::map<std::string, TuneEntry> g_tune_cache;
+
+uint32_t ilog2u(uint64_t x) {
+    uint32_t r = 0;
+    while (x > 1) {
+        x >>= 1;
+        r++;
+    }
+    return r;
+}
+
+bool read_tune_file(const char *path, const std::string &key, TuneEntry &out) {
+    FILE *f = fopen(path, "r");
+    if (!f) return false;
+    char k[256];
+    unsigned T, C, W;
+    double ms;
+    bool found = false;
+    while (fscanf(f, "%255s %u %u %u %lf", k, &T, &C, &W, &ms) == 5) {
+        if (key == k) {  // the last line of a class wins
+            out.t = scd_tuning{T, C, W, {0, 0, 0, 0, 0}};
+            out.ms = ms;
+            found = true;
+        }
+    }
+    fclose(f);
+    return found;
+}
+}  // namespace
+
+extern "C" int scd_autotune(scd_ctx *c, const char *cache_path, int reps, scd_tuning *chosen,
+                            double *iteration_ms) {
+    int rc = need_labels(c, "scd_autotune");
+    if (rc) return rc;
+    if (reps < 1) reps = 3;
+    uint32_t dmax = 0;
+    for (uint32_t v = 0; v < c->n; v++) dmax = std::max(dmax, c->h_deg[v]);
+    const uint64_t dsum = c->E;
+    char key[256];
+    snprintf(key, sizeof(key), "%s:%ucu:n%u:dmax%u:davg%u", c->arch.c_str(), c->num_cus, ilog2u(c->n),
+             ilog2u(dmax), ilog2u(dsum / std::max<uint32_t>(c->n, 1)));
+    TuneEntry hit;
+    bool have = false;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            hit = it->second;
+            have = true;
+        }
+    }
+    if (!have && cache_path && read_tune_file(cache_path, key, hit)) {
+        have = true;
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_cache[key] = hit;
+    }
+    if (have) {
+        if ((rc = scd_set_tuning(c, &hit.t))) return rc;
+        if (chosen) *chosen = c->tune;
+        if (iteration_ms) *iteration_ms = hit.ms;
+        return SCD_OK;
+    }
+
+    // candidates: only knobs that change a bin or tile for these degrees
+    std::vector<uint32_t> Ws = {1024};
+    if (dmax > 256) Ws = {256, 512, 1024};
+    std::vector<uint32_t> Ts = {2048};
+    if (dmax > 2048) Ts = {2048, 4096, 8192, 16384};
+    std::vector<scd_tuning> cand;
+    for (uint32_t T : Ts)
+        for (uint32_t W : Ws) {
+            std::vector<uint32_t> Cs = {std::min<uint32_t>(2048, 2 * T)};
+            if (dmax > T) Cs = {1024, 2048, 4096};
+            for (uint32_t C : Cs) cand.push_back(scd_tuning{T, C, W, {0, 0, 0, 0, 0}});
+        }
+
+    // the labels, statistics and best labels to start every timed iteration from
+    std::vector<DevBuf> tmp;
+    uint32_t *s_cur = nullptr, *s_best = nullptr;
+    int4 *s_stats = nullptr;
+    const uint32_t nc = c->num_comms;
+    if ((rc = dalloc(c, &s_cur, c->n, &tmp)) || (rc = dalloc(c, &s_best, c->n, &tmp)) ||
+        (rc = dalloc(c, &s_stats, std::max<uint32_t>(nc, 1), &tmp))) {
+        free_list(c, tmp);
+        return rc;
+    }
+    auto copy_state = [&](bool save) -> int {
+        if (save) {
+            HIP_TRY(hipMemcpyAsync(s_cur, c->cur, 4ull * c->n, hipMemcpyDeviceToDevice, c->stream));
+            HIP_TRY(hipMemcpyAsync(s_best, c->best, 4ull * c->n, hipMemcpyDeviceToDevice, c->stream));
+            HIP_TRY(hipMemcpyAsync(s_stats, c->stats, sizeof(int4) * nc, hipMemcpyDeviceToDevice, c->stream));
+        } else {
+            HIP_TRY(hipMemcpyAsync(c->cur, s_cur, 4ull * c->n, hipMemcpyDeviceToDevice, c->stream));
+            HIP_TRY(hipMemcpyAsync(c->best, s_best, 4ull * c->n, hipMemcpyDeviceToDevice, c->stream));
+            HIP_TRY(hipMemcpyAsync(c->stats, s_stats, sizeof(int4) * nc, hipMemcpyDeviceToDevice, c->stream));
+            if (c->num_comms != nc) drop_graphs(c);
+            c->num_comms = nc;
+            c->wcc_valid = false;
+        }
+        return finish(c);
+    };
+    const scd_tuning orig = c->tune;
+    scd_tuning best_t = orig;
+    double best_ms = 1e300;
+    if ((rc = copy_state(true))) {
+        free_list(c, tmp);
+        return rc;
+    }
+    for (const scd_tuning &t : cand) {
+        if ((rc = scd_set_tuning(c, &t)) || (rc = copy_state(false)) || (rc = scd_improve(c, nullptr, nullptr)))
+            break;  // the untimed run captures the iteration's graphs
+        std::vector<double> ms;
+        for (int r = 0; r < reps && rc == SCD_OK; r++) {
+            if ((rc = copy_state(false))) break;
+            const double t0 = wall_ms();
+            rc = scd_improve(c, nullptr, nullptr);
+            ms.push_back(wall_ms() - t0);
+        }
+        if (rc) break;
+        std::sort(ms.begin(), ms.end());
+        const double med = ms[ms.size() / 2];
+        if (med < best_ms) {
+            best_ms = med;
+            best_t = c->tune;
+        }
+    }
+    if (rc == SCD_OK) rc = scd_set_tuning(c, &best_t);
+    else (void)scd_set_tuning(c, &orig);
+    const int rc2 = copy_state(false);
+    free_list(c, tmp);
+    if (rc) return rc;
+    if (rc2) return rc2;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_cache[key] = TuneEntry{best_t, best_ms};
+    }
+    if (cache_path) {
+        FILE *f = fopen(cache_path, "a");
+        if (!f) return fail(SCD_EIO, "scd_autotune: cannot append to %s", cache_path);
+        fprintf(f, "%s %u %u %u %.6f\n", key, best_t.lds_tile, best_t.merge_chunk, best_t.wave_max, best_ms);
+        fclose(f);
+    }
+    if (chosen) *chosen = best_t;
+    if (iteration_ms) *iteration_ms = best_ms;
+    return SCD_OK;
 }
 

@@ -64,8 +64,8 @@ class ScdEdgesC(ctypes.Structure):
 
 
 class ScdTuningC(ctypes.Structure):
-    _fields_ = [("lds_tile", ctypes.c_uint32), ("merge_chunk", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 6)]
+    _fields_ = [("lds_tile", ctypes.c_uint32), ("merge_chunk", ctypes.c_uint32), ("wave_max", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 5)]
 
 
 def host():
@@ -157,6 +157,7 @@ def hip():
         "scd_batch_export_labels": (ctypes.c_int, [C, ctypes.c_void_p, ctypes.c_void_p]),
         "scd_batch_graph_labels": (ctypes.c_int, [C, ctypes.c_uint32, ctypes.c_int, u32p]),
         "scd_batch_step": (ctypes.c_int, [C, u32p, f64p]),
+        "scd_autotune": (ctypes.c_int, [C, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ScdTuningC), f64p]),
         "scd_batch_store_best": (ctypes.c_int, [C, u32p]),
     }
     for k, (res, args) in sig.items():

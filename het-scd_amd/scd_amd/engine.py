@@ -22,6 +22,32 @@ def device_count():
     return check_hip(_lib.hip().scd_device_count())
 
 
+def _get_tuning(ctx):
+    t = _lib.ScdTuningC()
+    check_hip(_lib.hip().scd_get_tuning(ctx, ctypes.byref(t)))
+    return {"lds_tile": t.lds_tile, "merge_chunk": t.merge_chunk, "wave_max": t.wave_max}
+
+
+def _set_tuning(ctx, lds_tile=None, merge_chunk=None, wave_max=None):
+    t = _lib.ScdTuningC()
+    check_hip(_lib.hip().scd_get_tuning(ctx, ctypes.byref(t)))
+    if lds_tile is not None:
+        t.lds_tile = lds_tile
+    if merge_chunk is not None:
+        t.merge_chunk = merge_chunk
+    if wave_max is not None:
+        t.wave_max = wave_max
+    check_hip(_lib.hip().scd_set_tuning(ctx, ctypes.byref(t)))
+
+
+def _autotune(ctx, cache_path, reps):
+    t = _lib.ScdTuningC()
+    ms = ctypes.c_double()
+    path = None if cache_path is None else str(cache_path).encode()
+    check_hip(_lib.hip().scd_autotune(ctx, path, reps, ctypes.byref(t), ctypes.byref(ms)))
+    return {"lds_tile": t.lds_tile, "merge_chunk": t.merge_chunk, "wave_max": t.wave_max}, ms.value
+
+
 class Engine:
     def __init__(self, device=0, work_group_size=256, profile=False):
         self._ctx = ctypes.c_void_p()
@@ -137,18 +163,15 @@ class Engine:
         return best.value, iters.value, hist[:iters.value].copy()
 
     def get_tuning(self):
-        t = _lib.ScdTuningC()
-        check_hip(_lib.hip().scd_get_tuning(self._ctx, ctypes.byref(t)))
-        return {"lds_tile": t.lds_tile, "merge_chunk": t.merge_chunk}
+        return _get_tuning(self._ctx)
 
-    def set_tuning(self, lds_tile=None, merge_chunk=None):
-        t = _lib.ScdTuningC()
-        check_hip(_lib.hip().scd_get_tuning(self._ctx, ctypes.byref(t)))
-        if lds_tile is not None:
-            t.lds_tile = lds_tile
-        if merge_chunk is not None:
-            t.merge_chunk = merge_chunk
-        check_hip(_lib.hip().scd_set_tuning(self._ctx, ctypes.byref(t)))
+    def set_tuning(self, lds_tile=None, merge_chunk=None, wave_max=None):
+        _set_tuning(self._ctx, lds_tile, merge_chunk, wave_max)
+
+    def autotune(self, cache_path=None, reps=3):
+        """scd_autotune: fastest tuning for this graph's size class (cached);
+        returns (tuning, iteration ms)."""
+        return _autotune(self._ctx, cache_path, reps)
 
     def last_counters(self):
         out = np.zeros(4, np.uint64)
@@ -262,6 +285,16 @@ class BatchEngine:
         lab = np.empty(self.arena.graph.n, np.uint32)
         check_hip(_lib.hip().scd_batch_labels(self._ctx, ptr(lab, u32p)))
         return lab
+
+    def autotune(self, cache_path=None, reps=3):
+        """scd_autotune on the arena (after the initial partition)."""
+        return _autotune(self._ctx, cache_path, reps)
+
+    def get_tuning(self):
+        return _get_tuning(self._ctx)
+
+    def set_tuning(self, lds_tile=None, merge_chunk=None, wave_max=None):
+        _set_tuning(self._ctx, lds_tile, merge_chunk, wave_max)
 
     def current_labels(self):
         lab = np.empty(self.arena.graph.n, np.uint32)

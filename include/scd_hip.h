@@ -166,19 +166,38 @@ int scd_neighbor_label_pairs(scd_ctx *ctx, uint32_t *sorted_out, uint32_t *run_o
 
 /*
  * Kernel shape knobs (degree-bin thresholds and tile sizes).  Results never
- * depend on them; the autotuner picks them per graph-size class.
- *   lds_tile:   keys sorted in LDS per workgroup (1024..16384, power of two);
- *               vertices with larger degree take the multi-tile hub path.
- *   merge_chunk: outputs per workgroup in the hub merge passes (256..65536).
+ * depend on them; scd_autotune picks them per graph-size class.
+ *   lds_tile:    keys sorted in LDS per workgroup (2048, 4096, 8192 or
+ *                16384); vertices with larger degree take the multi-tile
+ *                hub path.
+ *   merge_chunk: outputs per workgroup in the hub merge passes (power of
+ *                two, 256..2*lds_tile).
+ *   wave_max:    largest degree sorted by one wavefront in registers (256,
+ *                512 or 1024; 0 means 1024); larger degrees up to lds_tile
+ *                take one workgroup each.
  */
 typedef struct scd_tuning {
     uint32_t lds_tile;
     uint32_t merge_chunk;
-    uint32_t reserved[6];
+    uint32_t wave_max;
+    uint32_t reserved[5];
 } scd_tuning;
 
 int scd_get_tuning(const scd_ctx *ctx, scd_tuning *out);
 int scd_set_tuning(scd_ctx *ctx, const scd_tuning *t);
+
+/*
+ * Autotuner (no counterpart in the reference): times one refinement
+ * iteration from the current labels under each candidate tuning (the
+ * candidates that change the shape for this graph's degrees; reps timed
+ * runs each, median), keeps the fastest and leaves the labels, statistics
+ * and snapshot as they were.  The choice is cached per graph-size class
+ * (device architecture and CU count, log2 of n, of the maximum and of the
+ * mean degree) in memory and, when cache_path is given, in that text file
+ * (one line per class), which later calls read instead of measuring.
+ * chosen / iteration_ms (may be NULL): the tuning set and its time.
+ */
+int scd_autotune(scd_ctx *ctx, const char *cache_path, int reps, scd_tuning *chosen, double *iteration_ms);
 
 /* Turns the side counters below on or off (off by default: they cost one
  * same-address atomic per wave). */

@@ -139,23 +139,24 @@ def test_steps_from_random_labels(engine, name, k, makes_invalid):
 
 
 def test_tuning_invariance(engine, manifest):
-    """Labels do not depend on the LDS tile or merge chunk (hub path on vs
-    off)."""
+    """Labels do not depend on the LDS tile, merge chunk or wave/LDS
+    threshold (hub path on vs off)."""
     name = "rmat16"
     ent = manifest["graphs"][name]
     g = graph(name)
     assert g.degree().max() > 2048
     outs = []
     engine.set_counting(True)
-    for tile, chunk in ((2048, 256), (2048, 4096), (4096, 1024), (16384, 2048)):
+    for tile, chunk, wmax in ((2048, 256, 1024), (2048, 4096, 256), (4096, 1024, 512), (16384, 2048, 1024),
+                              (8192, 2048, 256)):
         engine.transfer_graph(g)
-        engine.set_tuning(lds_tile=tile, merge_chunk=chunk)
+        engine.set_tuning(lds_tile=tile, merge_chunk=chunk, wave_max=wmax)
         engine.labels_to_device(initial_labels(name, ent, g))
         wcc, lab = engine.improve_labels(return_labels=True)
         hubs = engine.last_counters()["hubs"]
         assert (hubs > 0) == (tile < g.degree().max())
         outs.append((wcc, lab))
-    engine.set_tuning(lds_tile=4096, merge_chunk=2048)
+    engine.set_tuning(lds_tile=4096, merge_chunk=2048, wave_max=1024)
     engine.set_counting(False)
     for wcc, lab in outs:
         assert wcc == outs[0][0]
