@@ -578,7 +578,13 @@ static int phase_wcc(scd_ctx *c, int32_t *int_tri_out, int32_t *int_tri_deg_out,
         LAUNCH(c, "wcc_tri_queue", 4 * c->num_cus, 256, k_wtc_queue<2048>, w, bin_v(c, K_W1), M, c->wq_off,
                c->wq_tot, c->wq_next);
     }
-    LAUNCH(c, "wcc_tri_final", cdiv(c->n, 4), 256, k_wtc_final, w, c->n);
+    LAUNCH(c, "wcc_tri_final", cdiv(bin_n(c, K_SUB8), 32), 256, k_wtc_final<8>, w, bin_v(c, K_SUB8),
+           bin_n(c, K_SUB8));
+    LAUNCH(c, "wcc_tri_final", cdiv(bin_n(c, K_SUB16), 16), 256, k_wtc_final<16>, w, bin_v(c, K_SUB16),
+           bin_n(c, K_SUB16));
+    LAUNCH(c, "wcc_tri_final", cdiv(bin_n(c, K_SUB32), 8), 256, k_wtc_final<32>, w, bin_v(c, K_SUB32),
+           bin_n(c, K_SUB32));
+    LAUNCH(c, "wcc_tri_final", cdiv(c->wq_M, 4), 256, k_wtc_final<64>, w, bin_v(c, K_W1), c->wq_M);
     LAUNCH(c, "wcc_terms", cdiv(c->n, 256), 256, k_wcc_terms, w, c->n);
     if (c->G) {
         LAUNCH(c, "wcc_sum_partial", c->n_witems, 256, k_reduce_partial_g, c->terms, c->witem_start, c->witem_end,
@@ -1303,11 +1309,19 @@ extern "C" int scd_initial_partition(scd_ctx *c, uint32_t *labels_out, uint32_t 
         cur = 1 - cur;
     }
     LAUNCH(c, "ip_rank", cdiv(n, 256), 256, k_ip_rank, val[cur], n, rank, state);
+    const uint32_t nbig = c->bin_begin[K_NB] - c->bin_begin[K_W1];  // degree > 32: a wave each
     uint32_t rounds = 0;
     while (true) {
         rounds++;
         LAUNCH(c, "ip_clear", 1, 64, k_fill_u32, flags, 4u);
-        LAUNCH(c, "ip_round", cdiv(n, 4), 256, k_ip_round, c->indices, c->adj, rank, state, n, flags);
+        LAUNCH(c, "ip_round", cdiv(bin_n(c, K_SUB8), 32), 256, k_ip_round<8>, c->indices, c->adj, rank, state,
+               bin_v(c, K_SUB8), bin_n(c, K_SUB8), flags);
+        LAUNCH(c, "ip_round", cdiv(bin_n(c, K_SUB16), 16), 256, k_ip_round<16>, c->indices, c->adj, rank, state,
+               bin_v(c, K_SUB16), bin_n(c, K_SUB16), flags);
+        LAUNCH(c, "ip_round", cdiv(bin_n(c, K_SUB32), 8), 256, k_ip_round<32>, c->indices, c->adj, rank, state,
+               bin_v(c, K_SUB32), bin_n(c, K_SUB32), flags);
+        LAUNCH(c, "ip_round", cdiv(nbig, 4), 256, k_ip_round<64>, c->indices, c->adj, rank, state, bin_v(c, K_W1),
+               nbig, flags);
         if ((rc = check_launch("initial partition"))) return cleanup(rc);
         if (hipMemcpyAsync(hflags, flags, 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
             hipStreamSynchronize(c->stream) != hipSuccess)
@@ -1320,7 +1334,14 @@ extern "C" int scd_initial_partition(scd_ctx *c, uint32_t *labels_out, uint32_t 
            sf);
     uint32_t k = 0;
     if ((rc = scan_u32(c, sf, sf, n, &k, tmp))) return cleanup(rc);
-    LAUNCH(c, "ip_labels", cdiv(n, 4), 256, k_ip_labels, c->indices, c->adj, rank, state, sf, n, c->cur);
+    LAUNCH(c, "ip_labels", cdiv(bin_n(c, K_SUB8), 32), 256, k_ip_labels<8>, c->indices, c->adj, rank, state, sf,
+           bin_v(c, K_SUB8), bin_n(c, K_SUB8), c->cur);
+    LAUNCH(c, "ip_labels", cdiv(bin_n(c, K_SUB16), 16), 256, k_ip_labels<16>, c->indices, c->adj, rank, state, sf,
+           bin_v(c, K_SUB16), bin_n(c, K_SUB16), c->cur);
+    LAUNCH(c, "ip_labels", cdiv(bin_n(c, K_SUB32), 8), 256, k_ip_labels<32>, c->indices, c->adj, rank, state, sf,
+           bin_v(c, K_SUB32), bin_n(c, K_SUB32), c->cur);
+    LAUNCH(c, "ip_labels", cdiv(nbig, 4), 256, k_ip_labels<64>, c->indices, c->adj, rank, state, sf, bin_v(c, K_W1),
+           nbig, c->cur);
     if ((rc = check_launch("initial partition"))) return cleanup(rc);
     HIP_TRY(hipMemcpyAsync(c->best, c->cur, 4ull * n, hipMemcpyDeviceToDevice, c->stream));
     if (c->num_comms != k) drop_graphs(c);

@@ -172,39 +172,70 @@ __global__ void k_ip_rank(const uint32_t *__restrict__ val, uint32_t n, uint32_t
     state[val[i]] = 0;  // undecided
 }
 
-// One decision round; one wave per undecided vertex.  state: 0 undecided,
-// 1 seed, 2 non-seed.  counts[0] += vertices still undecided.
+// Group bits of lane group [gbase, gbase + S) in a wave ballot.
+template <int S>
+__device__ __forceinline__ uint64_t ip_group_bits(uint64_t ballot, int gbase) {
+    if constexpr (S == 64) {
+        return ballot;
+    } else {
+        return (ballot >> gbase) & ((1ull << S) - 1ull);
+    }
+}
+
+// One decision round over the vertices verts[0..nv) (a degree bin), S
+// lanes per vertex walking its neighbours S at a time.  state: 0
+// undecided, 1 seed, 2 non-seed.  An undecided vertex becomes a non-seed
+// when an earlier-ranked neighbour is a seed, a seed when no earlier-ranked
+// neighbour is seed or undecided; otherwise it stays undecided and sets
+// *open_flag.  Decisions are final, so reading states that other vertices
+// set in the same round only speeds the rounds up.
+template <int S>
 __global__ __launch_bounds__(256) void k_ip_round(const uint32_t *__restrict__ indices, const uint32_t *__restrict__ adj,
-                                                  const uint32_t *__restrict__ rank, uint8_t *state, uint32_t n,
-                                                  uint32_t *__restrict__ counts) {
+                                                  const uint32_t *__restrict__ rank, uint8_t *state,
+                                                  const uint32_t *__restrict__ verts, uint32_t nv,
+                                                  uint32_t *__restrict__ open_flag) {
     const int lane = threadIdx.x & 63;
-    const uint32_t p = blockIdx.x * 4u + (threadIdx.x >> 6);
-    if (p >= n) return;
-    if (__atomic_load_n(&state[p], __ATOMIC_RELAXED) != 0) return;
-    const uint32_t b = indices[p], e = indices[p + 1], rp = rank[p];
+    const int lig = lane & (S - 1), gbase = lane & ~(S - 1);
+    const uint32_t slot = (blockIdx.x * 256u + threadIdx.x) / S;
+    uint32_t p = 0, b = 0, e = 0, rp = 0;
+    bool act = slot < nv;
+    if (act) {
+        p = verts[slot];
+        act = __atomic_load_n(&state[p], __ATOMIC_RELAXED) == 0;
+    }
+    if (act) {
+        b = indices[p];
+        e = indices[p + 1];
+        rp = rank[p];
+    }
     bool pending = false, seed_before = false;
-    for (uint32_t s = b; s < e; s += 64) {
-        const uint32_t i = s + lane;
-        bool earlier_seed = false, earlier_open = false;
-        if (i < e) {
-            const uint32_t q = adj[i];
-            if (rank[q] < rp) {
-                const uint8_t sq = __atomic_load_n(&state[q], __ATOMIC_RELAXED);
-                earlier_seed = sq == 1;
-                earlier_open = sq == 0;
+    for (uint32_t s = 0;; s += S) {
+        const bool more = act && !seed_before && b + s < e;
+        if (!__ballot(more)) break;  // wave-uniform exit
+        bool es = false, eo = false;
+        if (more) {
+            const uint32_t i = b + s + lig;
+            if (i < e) {
+                const uint32_t q = adj[i];
+                if (rank[q] < rp) {
+                    const uint8_t sq = __atomic_load_n(&state[q], __ATOMIC_RELAXED);
+                    es = sq == 1;
+                    eo = sq == 0;
+                }
             }
         }
-        if (__ballot(earlier_seed)) {
-            seed_before = true;
-            break;
+        const uint64_t bs = ip_group_bits<S>(__ballot(es), gbase), bo = ip_group_bits<S>(__ballot(eo), gbase);
+        if (more) {
+            if (bs) seed_before = true;
+            else if (bo) pending = true;
         }
-        if (__ballot(earlier_open)) pending = true;
     }
-    if (lane == 0) {
+    const bool still_open = act && !seed_before && pending;
+    if (act && lig == 0) {
         if (seed_before) __atomic_store_n(&state[p], (uint8_t)2, __ATOMIC_RELAXED);
         else if (!pending) __atomic_store_n(&state[p], (uint8_t)1, __ATOMIC_RELAXED);
-        else atomicAdd(&counts[0], 1u);
     }
+    if (__ballot(still_open) && lane == 0) open_flag[0] = 1u;
 }
 
 // sf[rank] = 1 for seeds (by position in the order).
@@ -220,28 +251,31 @@ __global__ void k_ip_seedflags(const uint32_t *__restrict__ val, const uint8_t *
 }
 
 // Labels: a seed's label is its seed rank (seedpos = exclusive scan of sf);
-// any other vertex takes the label of its earliest seed neighbour.  One
-// wave per vertex.
+// any other vertex takes the label of its earliest seed neighbour.  S lanes
+// per vertex of one degree bin.
+template <int S>
 __global__ __launch_bounds__(256) void k_ip_labels(const uint32_t *__restrict__ indices, const uint32_t *__restrict__ adj,
                                                    const uint32_t *__restrict__ rank, const uint8_t *__restrict__ state,
-                                                   const uint32_t *__restrict__ seedpos, uint32_t n,
+                                                   const uint32_t *__restrict__ seedpos,
+                                                   const uint32_t *__restrict__ verts, uint32_t nv,
                                                    uint32_t *__restrict__ labels) {
     const int lane = threadIdx.x & 63;
-    const uint32_t p = blockIdx.x * 4u + (threadIdx.x >> 6);
-    if (p >= n) return;
-    if (state[p] == 1) {
-        if (lane == 0) labels[p] = seedpos[rank[p]];
-        return;
-    }
-    const uint32_t b = indices[p], e = indices[p + 1];
+    const int lig = lane & (S - 1);
+    const uint32_t slot = (blockIdx.x * 256u + threadIdx.x) / S;
+    const bool act = slot < nv;
+    const uint32_t p = act ? verts[slot] : 0u;
+    const bool seed = act && state[p] == 1;
     uint32_t best = 0xFFFFFFFFu;
-    for (uint32_t i = b + lane; i < e; i += 64) {
-        const uint32_t q = adj[i];
-        if (state[q] == 1) best = min(best, rank[q]);
+    if (act && !seed) {
+        const uint32_t b = indices[p], e = indices[p + 1];
+        for (uint32_t i = b + lig; i < e; i += S) {
+            const uint32_t q = adj[i];
+            if (state[q] == 1) best = min(best, rank[q]);
+        }
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) best = min(best, (uint32_t)__shfl_xor(best, off, SCD_WAVE));
-    if (lane == 0) labels[p] = seedpos[best];
+    for (int off = S / 2; off > 0; off >>= 1) best = min(best, (uint32_t)__shfl_xor(best, off, SCD_WAVE));
+    if (act && lig == 0) labels[p] = seed ? seedpos[rank[p]] : seedpos[best];
 }
 
 }  // namespace scd_dev

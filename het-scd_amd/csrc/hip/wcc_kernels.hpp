@@ -232,6 +232,7 @@ __device__ __forceinline__ int wtc_find(const uint32_t *hk, const uint16_t *hp, 
 // out-list position in LDS and flushes one atomic per non-zero position.
 constexpr uint32_t WTC_VC = 32;  // out-list entries per work item
 constexpr int WTC_UW = 8;        // loads in flight per lane, list-per-wave mode
+constexpr uint32_t WTC_CLAIM = 8;  // items claimed at once by a workgroup
 
 // nch[i] = chunks of verts[M-1-i]; also resets the item counter.
 __global__ void k_wtc_count_chunks(const uint32_t *__restrict__ dplus, const uint32_t *__restrict__ verts,
@@ -284,194 +285,218 @@ __global__ __launch_bounds__(256) void k_wtc_queue(WccArgs a, const uint32_t *__
     set.tab = s_tab;
     set.hk = reinterpret_cast<uint32_t *>(s_tab);
     set.hp = reinterpret_cast<uint16_t *>(set.hk + 2 * CAP);
-    // the bitmap is all-zero between items (each item clears what it set)
+    // the bitmap is all-zero between vertices (each clears what it set)
     for (uint32_t k = tid; k < WTC_TAB; k += 256) s_tab[k] = make_uint2(0u, 0u);
     const uint32_t items = *total;
     while (true) {
-        if (tid == 0) s_item = atomicAdd(next, 1u);
+        // claim WTC_CLAIM consecutive items: chunks of one vertex in a claim
+        // share one membership set and one flush of the position counts
+        if (tid == 0) s_item = atomicAdd(next, WTC_CLAIM);
         __syncthreads();
-        const uint32_t item = s_item;
+        uint32_t item = s_item;
         if (item >= items) break;  // uniform: every block ends here
-        uint32_t lo = 0, hi = M;  // largest i with off[i] <= item
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (off[mid] <= item) lo = mid;
-            else hi = mid;
-        }
-        const uint32_t u = verts[M - 1 - lo];
-        const uint32_t beg = a.indices[u], dp = a.dplus[u];
-        const uint32_t k0 = (item - off[lo]) * WTC_VC, nk = min(WTC_VC, dp - k0);
-        const uint32_t *N = a.int_adj + beg;
-        uint32_t tot = 0;
-        if (tid < (int)nk) {
-            const uint32_t v = N[k0 + tid];
-            s_vb[tid] = a.indices[v];
-            s_pre[tid + 1] = a.dplus[v];
-        }
-        if (tid == 0) s_pre[0] = 0;
-        if (dp <= (uint32_t)CAP) {
-            set.lo = N[0];
-            set.R = N[dp - 1] - set.lo + 1;
-            set.bits = 0;
-            if (set.R > 32u * WTC_TAB) {
-                set.bits = 1;
-                while ((1u << set.bits) < 2 * dp) set.bits++;
-                for (uint32_t h = tid; h < (1u << set.bits); h += 256) set.hk[h] = SCD_INVALID;
+        const uint32_t item_end = min(item + WTC_CLAIM, items);
+        while (item < item_end) {  // uniform
+            uint32_t lo = 0, hi = M;  // largest i with off[i] <= item
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (off[mid] <= item) lo = mid;
+                else hi = mid;
             }
-            for (uint32_t k = tid; k < dp; k += 256) s_cnt[k] = 0;
-            __syncthreads();
-            if (wv == 0) {  // prefix of the chunk's list lengths
-                uint32_t x = lane < (int)nk ? s_pre[lane + 1] : 0u;
+            const uint32_t u = verts[M - 1 - lo];
+            const uint32_t beg = a.indices[u], dp = a.dplus[u];
+            const uint32_t vend = lo + 1 < M ? off[lo + 1] : items;
+            const uint32_t c_end = min(vend, item_end);
+            const uint32_t c0 = item - off[lo], c1 = c_end - off[lo];
+            const uint32_t *N = a.int_adj + beg;
+            uint32_t tot = 0;
+            if (dp <= (uint32_t)CAP) {
+                set.lo = N[0];
+                set.R = N[dp - 1] - set.lo + 1;
+                set.bits = 0;
+                if (set.R > 32u * WTC_TAB) {
+                    set.bits = 1;
+                    while ((1u << set.bits) < 2 * dp) set.bits++;
+                    for (uint32_t h = tid; h < (1u << set.bits); h += 256) set.hk[h] = SCD_INVALID;
+                }
+                for (uint32_t k = tid; k < dp; k += 256) s_cnt[k] = 0;
+                __syncthreads();
+                if (set.bits == 0) {
+                    for (uint32_t k = tid; k < dp; k += 256) {
+                        const uint32_t x = N[k] - set.lo;
+                        atomicOr(&s_tab[x >> 5].x, 1u << (x & 31u));
+                        if (k == 0 || ((N[k - 1] - set.lo) >> 5) != (x >> 5)) s_tab[x >> 5].y = k;
+                    }
+                } else {
+                    const uint32_t mask = (1u << set.bits) - 1u;
+                    for (uint32_t k = tid; k < dp; k += 256) {
+                        const uint32_t v = N[k];
+                        uint32_t h = wtc_hash(v, set.bits);
+                        while (atomicCAS(&set.hk[h], SCD_INVALID, v) != SCD_INVALID) h = (h + 1) & mask;
+                        set.hp[h] = (uint16_t)k;
+                    }
+                }
+                for (uint32_t c = c0; c < c1; c++) {
+                    const uint32_t k0 = c * WTC_VC, nk = min(WTC_VC, dp - k0);
+                    if (tid < (int)nk) {
+                        const uint32_t v = N[k0 + tid];
+                        s_vb[tid] = a.indices[v];
+                        s_pre[tid + 1] = a.dplus[v];
+                    }
+                    if (tid == 0) s_pre[0] = 0;
+                    __syncthreads();
+                    if (wv == 0) {  // prefix of the chunk's list lengths
+                        uint32_t x = lane < (int)nk ? s_pre[lane + 1] : 0u;
 #pragma unroll
-                for (int o = 1; o < 64; o <<= 1) {
-                    const uint32_t t = __shfl_up(x, o, SCD_WAVE);
-                    if (lane >= o) x += t;
-                }
-                if (lane < (int)nk) s_pre[lane + 1] = x;
-            }
-            if (set.bits == 0) {
-                for (uint32_t k = tid; k < dp; k += 256) {
-                    const uint32_t x = N[k] - set.lo;
-                    atomicOr(&s_tab[x >> 5].x, 1u << (x & 31u));
-                    if (k == 0 || ((N[k - 1] - set.lo) >> 5) != (x >> 5)) s_tab[x >> 5].y = k;
-                }
-            } else {
-                const uint32_t mask = (1u << set.bits) - 1u;
-                for (uint32_t k = tid; k < dp; k += 256) {
-                    const uint32_t v = N[k];
-                    uint32_t h = wtc_hash(v, set.bits);
-                    while (atomicCAS(&set.hk[h], SCD_INVALID, v) != SCD_INVALID) h = (h + 1) & mask;
-                    set.hp[h] = (uint16_t)k;
-                }
-            }
-            __syncthreads();
-            const uint32_t Wc = s_pre[nk];
-            if (Wc >= 32u * nk) {
-                // one list per wave; WTC_UW loads of 64 entries in flight
-                for (uint32_t kk = wv; kk < nk; kk += 4) {
-                    const uint32_t vb = s_vb[kk], dv = s_pre[kk + 1] - s_pre[kk];
-                    uint32_t c = 0;
-                    for (uint32_t j0 = 0; j0 < dv; j0 += 64 * WTC_UW) {
-                        uint32_t w[WTC_UW];
-#pragma unroll
-                        for (int q = 0; q < WTC_UW; q++) {
-                            const uint32_t j = j0 + 64 * q + lane;
-                            w[q] = j < dv ? a.int_adj[vb + j] : SCD_INVALID;
+                        for (int o = 1; o < 64; o <<= 1) {
+                            const uint32_t t = __shfl_up(x, o, SCD_WAVE);
+                            if (lane >= o) x += t;
                         }
+                        if (lane < (int)nk) s_pre[lane + 1] = x;
+                    }
+                    __syncthreads();
+                    const uint32_t Wc = s_pre[nk];
+                    if (Wc >= 32u * nk) {
+                        // one list per wave; WTC_UW loads of 64 entries in flight
+                        for (uint32_t kk = wv; kk < nk; kk += 4) {
+                            const uint32_t vb = s_vb[kk], dv = s_pre[kk + 1] - s_pre[kk];
+                            uint32_t cnt = 0;
+                            for (uint32_t j0 = 0; j0 < dv; j0 += 64 * WTC_UW) {
+                                uint32_t w[WTC_UW];
 #pragma unroll
-                        for (int q = 0; q < WTC_UW; q++) {
-                            if (w[q] == SCD_INVALID) continue;
-                            const int pos = set.find(w[q]);
-                            if (pos >= 0) {
-                                atomicAdd(&s_cnt[pos], 1u);
-                                a.flag[vb + j0 + 64 * q + lane] = 1;
-                                c++;
+                                for (int q = 0; q < WTC_UW; q++) {
+                                    const uint32_t j = j0 + 64 * q + lane;
+                                    w[q] = j < dv ? a.int_adj[vb + j] : SCD_INVALID;
+                                }
+#pragma unroll
+                                for (int q = 0; q < WTC_UW; q++) {
+                                    if (w[q] == SCD_INVALID) continue;
+                                    const int pos = set.find(w[q]);
+                                    if (pos >= 0) {
+                                        atomicAdd(&s_cnt[pos], 1u);
+                                        a.flag[vb + j0 + 64 * q + lane] = 1;
+                                        cnt++;
+                                    }
+                                }
+                            }
+#pragma unroll
+                            for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, SCD_WAVE);
+                            if (lane == 0 && cnt) {
+                                atomicAdd(&s_cnt[k0 + kk], cnt);
+                                tot += cnt;
+                            }
+                        }
+                    } else {
+                        // flattened over the threads; WTC_UF loads in flight each
+                        for (uint32_t f0 = tid; f0 < Wc; f0 += 256 * WTC_UF) {
+                            uint32_t w[WTC_UF], at[WTC_UF], own[WTC_UF];
+#pragma unroll
+                            for (int q = 0; q < WTC_UF; q++) {
+                                const uint32_t f = f0 + 256 * q;
+                                w[q] = SCD_INVALID;
+                                if (f < Wc) {
+                                    uint32_t l = 0, h = nk;  // largest kk with pre[kk] <= f
+                                    while (h - l > 1) {
+                                        const uint32_t mid = (l + h) >> 1;
+                                        if (s_pre[mid] <= f) l = mid;
+                                        else h = mid;
+                                    }
+                                    own[q] = l;
+                                    at[q] = s_vb[l] + (f - s_pre[l]);
+                                    w[q] = a.int_adj[at[q]];
+                                }
+                            }
+#pragma unroll
+                            for (int q = 0; q < WTC_UF; q++) {
+                                if (w[q] == SCD_INVALID) continue;
+                                const int pos = set.find(w[q]);
+                                if (pos >= 0) {
+                                    atomicAdd(&s_cnt[k0 + own[q]], 1u);
+                                    atomicAdd(&s_cnt[pos], 1u);
+                                    a.flag[at[q]] = 1;
+                                    tot++;
+                                }
                             }
                         }
                     }
-#pragma unroll
-                    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, SCD_WAVE);
-                    if (lane == 0 && c) {
-                        atomicAdd(&s_cnt[k0 + kk], c);
-                        tot += c;
+                    __syncthreads();  // s_vb / s_pre reused by the next chunk
+                }
+                for (uint32_t k = tid; k < dp; k += 256) {
+                    const uint32_t c = s_cnt[k];
+                    if (c) {
+                        atomicAdd(&a.T[N[k]], c);
+                        a.flag[beg + k] = 1;
                     }
                 }
+                // leave the bitmap all-zero for the next vertex
+                if (set.bits == 0) {
+                    for (uint32_t k = tid; k < dp; k += 256) s_tab[(N[k] - set.lo) >> 5].x = 0;
+                } else {
+                    const uint32_t words = (2 * CAP * 4 + 2 * CAP * 2) / 8;
+                    for (uint32_t k = tid; k < words; k += 256) s_tab[k] = make_uint2(0u, 0u);
+                }
             } else {
-                // flattened over the threads; WTC_UF loads in flight each
-                for (uint32_t f0 = tid; f0 < Wc; f0 += 256 * WTC_UF) {
-                    uint32_t w[WTC_UF], at[WTC_UF], own[WTC_UF];
-#pragma unroll
-                    for (int q = 0; q < WTC_UF; q++) {
-                        const uint32_t f = f0 + 256 * q;
-                        w[q] = SCD_INVALID;
-                        if (f < Wc) {
-                            uint32_t l = 0, h = nk;  // largest kk with pre[kk] <= f
-                            while (h - l > 1) {
-                                const uint32_t mid = (l + h) >> 1;
-                                if (s_pre[mid] <= f) l = mid;
-                                else h = mid;
+                // out-list longer than the LDS set: search it in global memory
+                for (uint32_t c = c0; c < c1; c++) {
+                    const uint32_t k0 = c * WTC_VC, nk = min(WTC_VC, dp - k0);
+                    if (tid < (int)nk) s_vb[tid] = a.indices[N[k0 + tid]];
+                    __syncthreads();
+                    for (uint32_t kk = 0; kk < nk; kk++) {
+                        const uint32_t v = N[k0 + kk];
+                        const uint32_t vb = s_vb[kk], dv = a.dplus[v];
+                        for (uint32_t j = tid; j < dv; j += 256) {
+                            const uint32_t w = a.int_adj[vb + j];
+                            uint32_t l2 = 0, h2 = dp;
+                            while (l2 < h2) {
+                                const uint32_t mid = (l2 + h2) >> 1;
+                                if (N[mid] < w) l2 = mid + 1;
+                                else h2 = mid;
                             }
-                            own[q] = l;
-                            at[q] = s_vb[l] + (f - s_pre[l]);
-                            w[q] = a.int_adj[at[q]];
+                            if (l2 < dp && N[l2] == w) {
+                                atomicAdd(&a.T[v], 1u);
+                                atomicAdd(&a.T[w], 1u);
+                                a.flag[vb + j] = 1;
+                                a.flag[beg + k0 + kk] = 1;
+                                a.flag[beg + l2] = 1;
+                                tot++;
+                            }
                         }
                     }
+                    __syncthreads();
+                }
+            }
 #pragma unroll
-                    for (int q = 0; q < WTC_UF; q++) {
-                        if (w[q] == SCD_INVALID) continue;
-                        const int pos = set.find(w[q]);
-                        if (pos >= 0) {
-                            atomicAdd(&s_cnt[k0 + own[q]], 1u);
-                            atomicAdd(&s_cnt[pos], 1u);
-                            a.flag[at[q]] = 1;
-                            tot++;
-                        }
-                    }
-                }
-            }
-            __syncthreads();
-            for (uint32_t k = tid; k < dp; k += 256) {
-                const uint32_t c = s_cnt[k];
-                if (c) {
-                    atomicAdd(&a.T[N[k]], c);
-                    a.flag[beg + k] = 1;
-                }
-            }
-            // leave the bitmap all-zero for the next item
-            if (set.bits == 0) {
-                for (uint32_t k = tid; k < dp; k += 256) s_tab[(N[k] - set.lo) >> 5].x = 0;
-            } else {
-                const uint32_t words = (2 * CAP * 4 + 2 * CAP * 2) / 8;
-                for (uint32_t k = tid; k < words; k += 256) s_tab[k] = make_uint2(0u, 0u);
-            }
-        } else {
-            // out-list longer than the LDS set: search it in global memory
-            __syncthreads();
-            for (uint32_t kk = 0; kk < nk; kk++) {
-                const uint32_t v = N[k0 + kk];
-                const uint32_t vb = s_vb[kk], dv = a.dplus[v];
-                for (uint32_t j = tid; j < dv; j += 256) {
-                    const uint32_t w = a.int_adj[vb + j];
-                    uint32_t l2 = 0, h2 = dp;
-                    while (l2 < h2) {
-                        const uint32_t mid = (l2 + h2) >> 1;
-                        if (N[mid] < w) l2 = mid + 1;
-                        else h2 = mid;
-                    }
-                    if (l2 < dp && N[l2] == w) {
-                        atomicAdd(&a.T[v], 1u);
-                        atomicAdd(&a.T[w], 1u);
-                        a.flag[vb + j] = 1;
-                        a.flag[beg + k0 + kk] = 1;
-                        a.flag[beg + l2] = 1;
-                        tot++;
-                    }
-                }
-            }
+            for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, SCD_WAVE);
+            if (lane == 0 && tot) atomicAdd(&a.T[u], tot);
+            __syncthreads();  // LDS reused by the next vertex
+            item = c_end;
         }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, SCD_WAVE);
-        if (lane == 0 && tot) atomicAdd(&a.T[u], tot);
-        __syncthreads();  // LDS reused by the next item
     }
 }
 
-// Flagged internal out-edges count at both ends.  One wave per vertex.
-__global__ __launch_bounds__(256) void k_wtc_final(WccArgs a, uint32_t n) {
+// Flagged internal out-edges count at both ends.  S lanes per vertex of
+// one degree bin.
+template <int S>
+__global__ __launch_bounds__(256) void k_wtc_final(WccArgs a, const uint32_t *__restrict__ verts, uint32_t nv) {
     const int lane = threadIdx.x & 63;
-    const uint32_t u = blockIdx.x * 4u + (threadIdx.x >> 6);
-    if (u >= n) return;
-    const uint32_t beg = a.indices[u], dp = a.dplus[u];
+    const int lig = lane & (S - 1);
+    const uint32_t slot = (blockIdx.x * 256u + threadIdx.x) / S;
+    uint32_t u = 0, beg = 0, dp = 0;
+    if (slot < nv) {
+        u = verts[slot];
+        dp = a.dplus[u];
+        if (dp) beg = a.indices[u];
+    }
     uint32_t mine = 0;
-    for (uint32_t k = lane; k < dp; k += 64) {
+    for (uint32_t k = lig; k < dp; k += S) {
         if (a.flag[beg + k]) {
             atomicAdd(&a.td[a.int_adj[beg + k]], 1u);
             mine++;
         }
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off, SCD_WAVE);
-    if (lane == 0 && mine) atomicAdd(&a.td[u], mine);
+    for (int off = S / 2; off > 0; off >>= 1) mine += __shfl_xor(mine, off, SCD_WAVE);
+    if (slot < nv && lig == 0 && mine) atomicAdd(&a.td[u], mine);
 }
 
 // Per-vertex terms (src/wcc.cpp:89-166): vertices without a valid label, in
