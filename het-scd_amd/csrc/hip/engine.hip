@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <ctime>
@@ -28,7 +29,6 @@
 #include "move_kernels.hpp"
 #include "scd_hip.h"
 #include "stats_kernels.hpp"
-#include "tc_kernels.hpp"
 #include "ip_kernels.hpp"
 #include "wcc_kernels.hpp"
 #include "arena_kernels.hpp"
@@ -546,6 +546,38 @@ static int scan_u32_dev(scd_ctx *c, const uint32_t *in, uint32_t *out, uint64_t 
     return SCD_OK;
 }
 
+// Triangles of the subgraph given by w.int_adj / w.int_deg (the internal
+// subgraph of the current labels, or with int_deg == nullptr the whole
+// graph): T[u] = triangles at u, td[u] = edges at u in a triangle; the
+// kernels per degree bin and the work queue of long out-lists.
+static int count_internal_triangles(scd_ctx *c, const WccArgs &w) {
+    LAUNCH(c, "wcc_dplus", cdiv(c->n, 256), 256, k_wtc_dplus, w, c->n);
+    LAUNCH(c, "wcc_tri_sub8", cdiv(bin_n(c, K_SUB8), 32), 256, k_wtc_group<8>, w, bin_v(c, K_SUB8),
+           bin_n(c, K_SUB8));
+    LAUNCH(c, "wcc_tri_sub16", cdiv(bin_n(c, K_SUB16), 16), 256, k_wtc_group<16>, w, bin_v(c, K_SUB16),
+           bin_n(c, K_SUB16));
+    LAUNCH(c, "wcc_tri_sub32", cdiv(bin_n(c, K_SUB32), 8), 256, k_wtc_group<32>, w, bin_v(c, K_SUB32),
+           bin_n(c, K_SUB32));
+    if (c->wq_M) {
+        const uint32_t M = c->wq_M;
+        LAUNCH(c, "wcc_tri_chunks", cdiv(M, 256), 256, k_wtc_count_chunks, w.dplus, bin_v(c, K_W1), M, c->wq_nch,
+               c->wq_next);
+        LAUNCH(c, "wcc_tri_wave", cdiv(M, 4), 256, k_wtc_group<64>, w, bin_v(c, K_W1), M);
+        const int rc = scan_u32_dev(c, c->wq_nch, c->wq_off, M, c->wq_tot, 0);
+        if (rc) return rc;
+        LAUNCH(c, "wcc_tri_queue", 4 * c->num_cus, 256, k_wtc_queue<2048>, w, bin_v(c, K_W1), M, c->wq_off,
+               c->wq_tot, c->wq_next);
+    }
+    LAUNCH(c, "wcc_tri_final", cdiv(bin_n(c, K_SUB8), 32), 256, k_wtc_final<8>, w, bin_v(c, K_SUB8),
+           bin_n(c, K_SUB8));
+    LAUNCH(c, "wcc_tri_final", cdiv(bin_n(c, K_SUB16), 16), 256, k_wtc_final<16>, w, bin_v(c, K_SUB16),
+           bin_n(c, K_SUB16));
+    LAUNCH(c, "wcc_tri_final", cdiv(bin_n(c, K_SUB32), 8), 256, k_wtc_final<32>, w, bin_v(c, K_SUB32),
+           bin_n(c, K_SUB32));
+    LAUNCH(c, "wcc_tri_final", cdiv(c->wq_M, 4), 256, k_wtc_final<64>, w, bin_v(c, K_W1), c->wq_M);
+    return SCD_OK;
+}
+
 // WCC terms of the current labels and their mean into wcc_dev.
 static int phase_wcc(scd_ctx *c, int32_t *int_tri_out, int32_t *int_tri_deg_out, bool incremental = false) {
     WccArgs w;
@@ -564,27 +596,8 @@ static int phase_wcc(scd_ctx *c, int32_t *int_tri_out, int32_t *int_tri_deg_out,
     w.terms = c->terms;
     w.int_tri_out = int_tri_out;
     w.int_tri_deg_out = int_tri_deg_out;
-    LAUNCH(c, "wcc_dplus", cdiv(c->n, 256), 256, k_wtc_dplus, w, c->n);
-    LAUNCH(c, "wcc_tri_sub8", cdiv(bin_n(c, K_SUB8), 32), 256, k_wtc_sub<8>, w, bin_v(c, K_SUB8), bin_n(c, K_SUB8));
-    LAUNCH(c, "wcc_tri_sub16", cdiv(bin_n(c, K_SUB16), 16), 256, k_wtc_sub<16>, w, bin_v(c, K_SUB16), bin_n(c, K_SUB16));
-    LAUNCH(c, "wcc_tri_sub32", cdiv(bin_n(c, K_SUB32), 8), 256, k_wtc_sub<32>, w, bin_v(c, K_SUB32), bin_n(c, K_SUB32));
-    if (c->wq_M) {
-        const uint32_t M = c->wq_M;
-        LAUNCH(c, "wcc_tri_chunks", cdiv(M, 256), 256, k_wtc_count_chunks, c->wdplus, bin_v(c, K_W1), M, c->wq_nch,
-               c->wq_next);
-        LAUNCH(c, "wcc_tri_wave", cdiv(M, 4), 256, k_wtc_wave, w, bin_v(c, K_W1), M);
-        int rc = scan_u32_dev(c, c->wq_nch, c->wq_off, M, c->wq_tot, 0);
-        if (rc) return rc;
-        LAUNCH(c, "wcc_tri_queue", 4 * c->num_cus, 256, k_wtc_queue<2048>, w, bin_v(c, K_W1), M, c->wq_off,
-               c->wq_tot, c->wq_next);
-    }
-    LAUNCH(c, "wcc_tri_final", cdiv(bin_n(c, K_SUB8), 32), 256, k_wtc_final<8>, w, bin_v(c, K_SUB8),
-           bin_n(c, K_SUB8));
-    LAUNCH(c, "wcc_tri_final", cdiv(bin_n(c, K_SUB16), 16), 256, k_wtc_final<16>, w, bin_v(c, K_SUB16),
-           bin_n(c, K_SUB16));
-    LAUNCH(c, "wcc_tri_final", cdiv(bin_n(c, K_SUB32), 8), 256, k_wtc_final<32>, w, bin_v(c, K_SUB32),
-           bin_n(c, K_SUB32));
-    LAUNCH(c, "wcc_tri_final", cdiv(c->wq_M, 4), 256, k_wtc_final<64>, w, bin_v(c, K_W1), c->wq_M);
+    int rc = count_internal_triangles(c, w);
+    if (rc) return rc;
     LAUNCH(c, "wcc_terms", cdiv(c->n, 256), 256, k_wcc_terms, w, c->n);
     if (c->G) {
         LAUNCH(c, "wcc_sum_partial", c->n_witems, 256, k_reduce_partial_g, c->terms, c->witem_start, c->witem_end,
@@ -1168,64 +1181,28 @@ extern "C" uint64_t scd_device_bytes(const scd_ctx *c) { return c ? c->dev_bytes
 // ---------------------------------------------------------------------------
 // Triangle statistics (src/convert.cpp:271-340) on the device.
 // ---------------------------------------------------------------------------
+// t / vt (remove_edges_without_triangles, src/convert.cpp:271-351): the
+// internal-triangle kernels over the whole adjacency, tri = 2 x triangles.
 static int triangles_dev(scd_ctx *c, int32_t *d_tri, uint32_t *d_td, std::vector<DevBuf> &tmp) {
     const uint32_t n = c->n;
     const uint64_t E = c->E;
-    TcArgs a;
-    a.indices = c->indices;
-    a.adj = c->adj;
-    a.n = n;
-    int rc;
-    uint32_t *counts = nullptr, *dplus = nullptr, *T = nullptr, *light = nullptr, *heavy = nullptr,
-             *heavy_chunks = nullptr, *gpre = nullptr;
+    uint32_t *dplus = nullptr, *T = nullptr;
     uint8_t *flag = nullptr;
-    if ((rc = dalloc(c, &dplus, n, &tmp)) || (rc = dalloc(c, &T, n, &tmp)) || (rc = dalloc(c, &flag, E, &tmp)) ||
-        (rc = dalloc(c, &light, n, &tmp)) || (rc = dalloc(c, &heavy, n, &tmp)) ||
-        (rc = dalloc(c, &heavy_chunks, n, &tmp)) || (rc = dalloc(c, &counts, 4, &tmp)))
+    int rc;
+    if ((rc = dalloc(c, &dplus, n, &tmp)) || (rc = dalloc(c, &T, n, &tmp)) || (rc = dalloc(c, &flag, E, &tmp)))
         return rc;
-    a.dplus = dplus;
-    a.T = T;
-    a.flag = flag;
-    a.light = light;
-    a.heavy = heavy;
-    a.heavy_chunks = heavy_chunks;
-    a.td = d_td;
-    a.counts = counts;
-    a.gpre = nullptr;
-    LAUNCH(c, "tc_clear", cdiv(E, 256), 256, k_fill_u8, a.flag, E);
-    LAUNCH(c, "tc_clear", 1, 64, k_fill_u32, counts, 4u);
-    LAUNCH(c, "tc_dplus", cdiv(n, 256), 256, k_tc_dplus, a);
-    if ((rc = check_launch("triangles"))) return rc;
-    // out-lists longer than the LDS stage keep their probe prefix in global
-    // memory; find out whether any exists before sizing that buffer
-    uint32_t max_dp = 0;
-    {
-        std::vector<uint32_t> h(n);
-        HIP_TRY(hipMemcpyAsync(h.data(), a.dplus, 4ull * n, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        for (uint32_t v = 0; v < n; v++) max_dp = std::max(max_dp, h[v]);
-    }
-    if (max_dp > TC_CAP) {
-        if ((rc = dalloc(c, &gpre, E, &tmp))) return rc;
-        a.gpre = gpre;
-    }
-    LAUNCH(c, "tc_classify", cdiv(n, 256), 256, k_tc_classify, a);
-    uint32_t cnt[2];
-    HIP_TRY(hipMemcpyAsync(cnt, counts, 8, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    if ((rc = check_launch("triangles"))) return rc;
-    const uint32_t nlight = cnt[0], nheavy = cnt[1];
-    uint32_t nitems = 0, *item_u = nullptr, *item_c = nullptr, *off = nullptr;
-    if (nheavy) {
-        if ((rc = dalloc(c, &off, nheavy, &tmp))) return rc;
-        if ((rc = scan_u32(c, a.heavy_chunks, off, nheavy, &nitems, tmp))) return rc;
-        if ((rc = dalloc(c, &item_u, nitems, &tmp)) || (rc = dalloc(c, &item_c, nitems, &tmp))) return rc;
-        LAUNCH(c, "tc_items", cdiv(nheavy, 256), 256, k_tc_items, a.heavy, a.heavy_chunks, off, nheavy, item_u,
-               item_c);
-    }
-    LAUNCH(c, "tc_light", cdiv(nlight, 4), 256, k_tc_light, a);
-    LAUNCH(c, "tc_heavy", nitems, 256, k_tc_heavy, a, item_u, item_c);
-    LAUNCH(c, "tc_final", cdiv(n, 4), 256, k_tc_final, a, d_tri);
+    WccArgs w;
+    memset(&w, 0, sizeof(w));
+    w.indices = c->indices;
+    w.int_adj = c->adj;
+    w.int_deg = nullptr;
+    w.dplus = dplus;
+    w.T = T;
+    w.td = d_td;
+    w.flag = flag;
+    LAUNCH(c, "tc_clear", cdiv(E, 256), 256, k_fill_u8, flag, E);
+    if ((rc = count_internal_triangles(c, w))) return rc;
+    LAUNCH(c, "tc_double", cdiv(n, 256), 256, k_wtc_double, T, n, d_tri);
     return check_launch("triangles");
 }
 

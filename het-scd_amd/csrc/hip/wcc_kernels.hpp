@@ -9,12 +9,14 @@
 // once from q and once from r:
 //   int_tri(p)     = 2 x (internal triangles at p)
 //   int_tri_deg(p) = internal edges at p that lie in an internal triangle.
-// The internal lists (int_adj, sorted) are counted like the whole graph in
-// tc_kernels.hpp: edges point from the higher to the lower id, each internal
-// triangle w < v < u is found once at u, per-vertex counts aggregate in LDS
-// over N+_int(u), and the edges of a found triangle are flagged with plain
-// stores of 1 (the filter pass cleared the flags of the internal entries).
-// All grids are fixed by the degree bins, so an iteration stays capturable.
+// The internal lists (int_adj, sorted) are oriented from the higher to the
+// lower id, each internal triangle w < v < u is found once at u, per-vertex
+// counts aggregate in LDS over N+_int(u), and the edges of a found triangle
+// are flagged with plain stores of 1 (the filter pass cleared the flags of
+// the internal entries).  All grids are fixed by the degree bins, so an
+// iteration stays capturable.  The same kernels over the whole adjacency
+// (int_deg == nullptr) give the graph's t / vt (remove_edges_without_
+// triangles, src/convert.cpp:271-351): tri = 2 T, tri_deg = td.
 #ifndef SCD_WCC_KERNELS_HPP
 #define SCD_WCC_KERNELS_HPP
 
@@ -61,13 +63,16 @@ __global__ void k_mark_dirty(const uint32_t *__restrict__ old_labels, const uint
 __global__ void k_wtc_dplus(WccArgs a, uint32_t n) {
     const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= n) return;
-    const uint32_t lab = a.labels[u];
-    if (a.dirty && lab != SCD_INVALID && !a.dirty[lab]) {
-        a.dplus[u] = 0;  // community unchanged: no counting, T and td kept
-        return;
+    if (a.dirty) {
+        const uint32_t lab = a.labels[u];
+        if (lab != SCD_INVALID && !a.dirty[lab]) {
+            a.dplus[u] = 0;  // community unchanged: no counting, T and td kept
+            return;
+        }
     }
     const uint32_t *L = a.int_adj + a.indices[u];
-    uint32_t lo = 0, hi = a.int_deg[u];
+    // int_deg == nullptr: the whole adjacency (triangle counting of the graph)
+    uint32_t lo = 0, hi = a.int_deg ? a.int_deg[u] : a.indices[u + 1] - a.indices[u];
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
         if (L[mid] < u) lo = mid + 1;
@@ -78,107 +83,64 @@ __global__ void k_wtc_dplus(WccArgs a, uint32_t n) {
     a.td[u] = 0;
 }
 
-// Vertices of degree <= S: S lanes each, lane j owns v = N+_int(u)[j] and
-// walks N+_int(v); membership in N+_int(u) by binary search in LDS.
+// Out-lists of at most S entries (S = 8, 16, 32, 64): S lanes per vertex.
+// The member lists N+_int(v), v in N+_int(u), are walked flattened over the
+// group's lanes (WTC_UF loads in flight per lane; the owning v of an entry
+// by binary search over the group's prefix of list lengths); membership in
+// N+_int(u) by binary search in LDS.  Counts per out-list position collect
+// in LDS and flush once per vertex.  Longer out-lists (S = 64 groups skip
+// them) go to the work queue below.
+constexpr int WTC_UF = 4;                // loads in flight per lane, flattened walks
+constexpr uint32_t WTC_GROUP_MAX = 64;   // longer out-lists go to the work queue
+
 template <int S>
-__global__ __launch_bounds__(256) void k_wtc_sub(WccArgs a, const uint32_t *__restrict__ verts, uint32_t nv) {
-    __shared__ uint32_t s_nk[256], s_cnt[256];
-    const int tid = threadIdx.x;
-    const int lig = tid & (S - 1);
-    const int gb = tid & ~(S - 1);
+__global__ __launch_bounds__(256) void k_wtc_group(WccArgs a, const uint32_t *__restrict__ verts, uint32_t nv) {
+    __shared__ uint32_t s_n[256], s_vb[256], s_cnt[256], s_inc[256];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int lig = lane & (S - 1), gb = tid & ~(S - 1);  // group base in the block's arrays
     const uint32_t slot = (blockIdx.x * 256u + tid) / S;
-    const bool active = slot < nv;
-    const uint32_t u = active ? verts[slot] : 0u;
-    const uint32_t beg = active ? a.indices[u] : 0u;
-    const uint32_t dp = active ? a.dplus[u] : 0u;
-    s_nk[tid] = (uint32_t)lig < dp ? a.int_adj[beg + lig] : 0xFFFFFFFFu;
-    s_cnt[tid] = 0;
-    __syncthreads();
-    uint32_t tot = 0;
-    if ((uint32_t)lig < dp) {
-        const uint32_t v = s_nk[tid];
-        const uint32_t vb = a.indices[v], dv = a.dplus[v];
-        for (uint32_t k = 0; k < dv; k++) {
-            const uint32_t w = a.int_adj[vb + k];
-            uint32_t lo = 0, hi = dp;
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (s_nk[gb + mid] < w) lo = mid + 1;
-                else hi = mid;
-            }
-            if (lo < dp && s_nk[gb + lo] == w) {
-                atomicAdd(&s_cnt[tid], 1u);
-                atomicAdd(&s_cnt[gb + lo], 1u);
-                a.flag[vb + k] = 1;
-                tot++;
-            }
-        }
-    }
-    __syncthreads();
-    if ((uint32_t)lig < dp && s_cnt[tid]) {
-        atomicAdd(&a.T[s_nk[tid]], s_cnt[tid]);
-        a.flag[beg + lig] = 1;
-    }
-#pragma unroll
-    for (int off = 1; off < S; off <<= 1) tot += __shfl_xor(tot, off, SCD_WAVE);
-    if (active && lig == 0 && tot) atomicAdd(&a.T[u], tot);
-}
-
-// Vertices of degree > 32 whose out-list has at most 64 entries: one wave
-// per vertex.  The member lists N+_int(v), v in N+_int(u), are walked
-// flattened over the lanes (WTC_UF loads in flight per lane; the owning v
-// of an entry by binary search over the wave's prefix of list lengths);
-// membership in N+_int(u) by binary search in LDS.  Longer out-lists go to
-// the work queue below.
-constexpr uint32_t WTC_WAVE_MAX = 64;
-constexpr int WTC_UF = 4;  // loads in flight per lane, flattened walks
-
-__global__ __launch_bounds__(256) void k_wtc_wave(WccArgs a, const uint32_t *__restrict__ verts, uint32_t nv) {
-    __shared__ uint32_t s_n[4][64], s_vb[4][64], s_cnt[4][64], s_pre[4][65];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const uint32_t i = blockIdx.x * 4u + wv;
     uint32_t u = 0, beg = 0, dp = 0;
-    if (i < nv) {
-        u = verts[i];
+    if (slot < nv) {
+        u = verts[slot];
         dp = a.dplus[u];
-        if (dp > WTC_WAVE_MAX) dp = 0;  // work queue
-        beg = a.indices[u];
+        if (dp > (uint32_t)S) dp = 0;  // work queue
+        if (dp) beg = a.indices[u];
     }
     uint32_t x = SCD_INVALID, len = 0, vb = 0;
-    if ((uint32_t)lane < dp) {
-        x = a.int_adj[beg + lane];
+    if ((uint32_t)lig < dp) {
+        x = a.int_adj[beg + lig];
         len = a.dplus[x];
         vb = a.indices[x];
     }
-    uint32_t pre = len;  // inclusive prefix over the lanes
+    uint32_t inc = len;  // inclusive prefix over the group
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t t = __shfl_up(pre, o, SCD_WAVE);
-        if (lane >= o) pre += t;
+    for (int o = 1; o < S; o <<= 1) {
+        const uint32_t t = __shfl_up(inc, o, SCD_WAVE);
+        if (lig >= o) inc += t;
     }
-    s_n[wv][lane] = x;
-    s_vb[wv][lane] = vb;
-    s_cnt[wv][lane] = 0;
-    s_pre[wv][lane + 1] = pre;
-    if (lane == 0) s_pre[wv][0] = 0;
+    s_n[tid] = x;
+    s_vb[tid] = vb;
+    s_cnt[tid] = 0;
+    s_inc[tid] = inc;
     __syncthreads();
-    const uint32_t W = __shfl(pre, 63, SCD_WAVE);
+    const uint32_t W = s_inc[gb + S - 1];
     uint32_t tot = 0;
-    for (uint32_t f0 = lane; f0 < W; f0 += 64 * WTC_UF) {
+    for (uint32_t f0 = lig; f0 < W; f0 += S * WTC_UF) {
         uint32_t w[WTC_UF], at[WTC_UF], own[WTC_UF];
 #pragma unroll
         for (int q = 0; q < WTC_UF; q++) {
-            const uint32_t f = f0 + 64 * q;
+            const uint32_t f = f0 + S * q;
             w[q] = SCD_INVALID;
             if (f < W) {
-                uint32_t l = 0, h = dp;  // largest k with pre[k] <= f
-                while (h - l > 1) {
+                uint32_t l = 0, h = dp - 1;  // smallest k with inc[k] > f
+                while (l < h) {
                     const uint32_t mid = (l + h) >> 1;
-                    if (s_pre[wv][mid] <= f) l = mid;
-                    else h = mid;
+                    if (s_inc[gb + mid] > f) h = mid;
+                    else l = mid + 1;
                 }
                 own[q] = l;
-                at[q] = s_vb[wv][l] + (f - s_pre[wv][l]);
+                const uint32_t excl = l ? s_inc[gb + l - 1] : 0u;
+                at[q] = s_vb[gb + l] + (f - excl);
                 w[q] = a.int_adj[at[q]];
             }
         }
@@ -188,25 +150,25 @@ __global__ __launch_bounds__(256) void k_wtc_wave(WccArgs a, const uint32_t *__r
             uint32_t lo = 0, hi = dp;
             while (lo < hi) {
                 const uint32_t mid = (lo + hi) >> 1;
-                if (s_n[wv][mid] < w[q]) lo = mid + 1;
+                if (s_n[gb + mid] < w[q]) lo = mid + 1;
                 else hi = mid;
             }
-            if (lo < dp && s_n[wv][lo] == w[q]) {
-                atomicAdd(&s_cnt[wv][own[q]], 1u);
-                atomicAdd(&s_cnt[wv][lo], 1u);
+            if (lo < dp && s_n[gb + lo] == w[q]) {
+                atomicAdd(&s_cnt[gb + own[q]], 1u);
+                atomicAdd(&s_cnt[gb + lo], 1u);
                 a.flag[at[q]] = 1;
                 tot++;
             }
         }
     }
     __syncthreads();
-    if ((uint32_t)lane < dp && s_cnt[wv][lane]) {
-        atomicAdd(&a.T[x], s_cnt[wv][lane]);
-        a.flag[beg + lane] = 1;
+    if ((uint32_t)lig < dp && s_cnt[tid]) {
+        atomicAdd(&a.T[x], s_cnt[tid]);
+        a.flag[beg + lig] = 1;
     }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, SCD_WAVE);
-    if (lane == 0 && tot) atomicAdd(&a.T[u], tot);
+    for (int o = S / 2; o > 0; o >>= 1) tot += __shfl_xor(tot, o, SCD_WAVE);
+    if (lig == 0 && tot) atomicAdd(&a.T[u], tot);
 }
 
 // LDS hash set of N+_int(u): slot -> (vertex, position in the out-list).
@@ -241,7 +203,7 @@ __global__ void k_wtc_count_chunks(const uint32_t *__restrict__ dplus, const uin
     if (i == 0) next[0] = 0;
     if (i >= M) return;
     const uint32_t dp = dplus[verts[M - 1 - i]];
-    nch[i] = dp > WTC_WAVE_MAX ? (dp + WTC_VC - 1) / WTC_VC : 0u;
+    nch[i] = dp > WTC_GROUP_MAX ? (dp + WTC_VC - 1) / WTC_VC : 0u;
 }
 
 // Membership set of N+_int(u) (sorted, dp entries) for one work item.  When
@@ -497,6 +459,12 @@ __global__ __launch_bounds__(256) void k_wtc_final(WccArgs a, const uint32_t *__
 #pragma unroll
     for (int off = S / 2; off > 0; off >>= 1) mine += __shfl_xor(mine, off, SCD_WAVE);
     if (slot < nv && lig == 0 && mine) atomicAdd(&a.td[u], mine);
+}
+
+// tri[u] = 2 x triangles at u (the layout of Graph::triangles).
+__global__ void k_wtc_double(const uint32_t *__restrict__ T, uint32_t n, int32_t *__restrict__ tri) {
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < n) tri[u] = (int32_t)(2u * T[u]);
 }
 
 // Per-vertex terms (src/wcc.cpp:89-166): vertices without a valid label, in
