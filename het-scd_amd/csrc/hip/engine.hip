@@ -105,6 +105,10 @@ struct scd_ctx {
     uint32_t *wdplus = nullptr, *wT = nullptr, *wtd = nullptr;  // internal triangle counts
     // per label: community member set changed by the last move
     uint8_t *wdirty = nullptr;
+    // private copies of the triangle counts of the lowest ids (zero between
+    // passes)
+    uint32_t *hot_T = nullptr, *hot_td = nullptr;
+    uint32_t hot_n = 0;
     // wT / wtd hold the counts of the labels in cur (set by every WCC
     // evaluation, cleared by everything that writes cur otherwise): the
     // next iteration then recounts only the communities its moves change
@@ -575,6 +579,7 @@ static int count_internal_triangles(scd_ctx *c, const WccArgs &w) {
     LAUNCH(c, "wcc_tri_final", cdiv(bin_n(c, K_SUB32), 8), 256, k_wtc_final<32>, w, bin_v(c, K_SUB32),
            bin_n(c, K_SUB32));
     LAUNCH(c, "wcc_tri_final", cdiv(c->wq_M, 4), 256, k_wtc_final<64>, w, bin_v(c, K_W1), c->wq_M);
+    LAUNCH(c, "wcc_tri_hot", cdiv(w.hot_n, 256), 256, k_wtc_hot_sum, w);
     return SCD_OK;
 }
 
@@ -594,6 +599,9 @@ static int phase_wcc(scd_ctx *c, int32_t *int_tri_out, int32_t *int_tri_deg_out,
     w.td = c->wtd;
     w.flag = c->iflag;
     w.terms = c->terms;
+    w.T_hot = c->hot_T;
+    w.td_hot = c->hot_td;
+    w.hot_n = c->hot_n;
     w.int_tri_out = int_tri_out;
     w.int_tri_deg_out = int_tri_deg_out;
     int rc = count_internal_triangles(c, w);
@@ -731,6 +739,8 @@ static int alloc_graph(scd_ctx *c, uint32_t n, uint32_t m, double clustering_coe
         (rc = dalloc(c, &c->int_deg, n)) || (rc = dalloc(c, &c->terms, n)) ||
         (rc = dalloc(c, &c->iflag, E)) || (rc = dalloc(c, &c->wdplus, n)) || (rc = dalloc(c, &c->wT, n)) ||
         (rc = dalloc(c, &c->wtd, n)) || (rc = dalloc(c, &c->wdirty, n)) ||
+        (rc = dalloc(c, &c->hot_T, (size_t)WTC_HOT_COPIES * std::min(n, WTC_HOT_MAX))) ||
+        (rc = dalloc(c, &c->hot_td, (size_t)WTC_HOT_COPIES * std::min(n, WTC_HOT_MAX))) ||
         (rc = dalloc(c, &c->partial, cdiv(n, 4096))) || (rc = dalloc(c, &c->wcc_dev, 1)) ||
         (rc = dalloc(c, &c->counters, 16)) || (rc = dalloc(c, &c->snap_labels, n)) ||
         (rc = dalloc(c, &c->snap_stats, n))) {
@@ -739,7 +749,12 @@ static int alloc_graph(scd_ctx *c, uint32_t n, uint32_t m, double clustering_coe
     }
     c->lab_a = c->cur;
     c->lab_b = c->next;
-    return SCD_OK;
+    c->hot_n = std::min(n, WTC_HOT_MAX);
+    LAUNCH(c, "hot_clear", cdiv(WTC_HOT_COPIES * c->hot_n, 256), 256, k_fill_u32, c->hot_T,
+           WTC_HOT_COPIES * c->hot_n);
+    LAUNCH(c, "hot_clear", cdiv(WTC_HOT_COPIES * c->hot_n, 256), 256, k_fill_u32, c->hot_td,
+           WTC_HOT_COPIES * c->hot_n);
+    return check_launch("allocation");
 }
 
 // After the graph arrays are on the device (enqueued on the stream): host
@@ -1200,6 +1215,9 @@ static int triangles_dev(scd_ctx *c, int32_t *d_tri, uint32_t *d_td, std::vector
     w.T = T;
     w.td = d_td;
     w.flag = flag;
+    w.T_hot = c->hot_T;
+    w.td_hot = c->hot_td;
+    w.hot_n = c->hot_n;
     LAUNCH(c, "tc_clear", cdiv(E, 256), 256, k_fill_u8, flag, E);
     if ((rc = count_internal_triangles(c, w))) return rc;
     LAUNCH(c, "tc_double", cdiv(n, 256), 256, k_wtc_double, T, n, d_tri);

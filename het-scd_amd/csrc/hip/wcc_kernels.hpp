@@ -45,7 +45,43 @@ struct WccArgs {
     // communities keep T and td (the internal subgraph of a community is
     // a function of its member set).  nullptr: count everything.
     const uint8_t *__restrict__ dirty;
+    // Counts for the hot_n lowest ids (the highest-degree vertices of a
+    // convert-relabelled graph, which collect the most updates) go to
+    // WTC_HOT_COPIES private copies, one chosen per workgroup, and are summed
+    // into T / td afterwards: fewer atomics on one address.
+    uint32_t *__restrict__ T_hot;   // WTC_HOT_COPIES x hot_n
+    uint32_t *__restrict__ td_hot;  // WTC_HOT_COPIES x hot_n
+    uint32_t hot_n;
 };
+
+constexpr uint32_t WTC_HOT_MAX = 8192;
+constexpr uint32_t WTC_HOT_COPIES = 16;
+
+__device__ __forceinline__ void wtc_add_T(const WccArgs &a, uint32_t x, uint32_t c) {
+    if (x < a.hot_n) atomicAdd(&a.T_hot[(blockIdx.x & (WTC_HOT_COPIES - 1)) * a.hot_n + x], c);
+    else atomicAdd(&a.T[x], c);
+}
+
+__device__ __forceinline__ void wtc_add_td(const WccArgs &a, uint32_t x, uint32_t c) {
+    if (x < a.hot_n) atomicAdd(&a.td_hot[(blockIdx.x & (WTC_HOT_COPIES - 1)) * a.hot_n + x], c);
+    else atomicAdd(&a.td[x], c);
+}
+
+// T[x] += sum of the copies, td likewise, copies cleared for the next pass.
+__global__ void k_wtc_hot_sum(WccArgs a) {
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= a.hot_n) return;
+    uint32_t t = 0, d = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < WTC_HOT_COPIES; k++) {
+        t += a.T_hot[k * a.hot_n + x];
+        d += a.td_hot[k * a.hot_n + x];
+        a.T_hot[k * a.hot_n + x] = 0;
+        a.td_hot[k * a.hot_n + x] = 0;
+    }
+    if (t) a.T[x] += t;
+    if (d) a.td[x] += d;
+}
 
 // dirty[old label] = dirty[new label] = 1 for every vertex that moved
 // (the flags were cleared before).
@@ -163,7 +199,7 @@ __global__ __launch_bounds__(256) void k_wtc_group(WccArgs a, const uint32_t *__
     }
     __syncthreads();
     if ((uint32_t)lig < dp && s_cnt[tid]) {
-        atomicAdd(&a.T[x], s_cnt[tid]);
+        wtc_add_T(a, x, s_cnt[tid]);
         a.flag[beg + lig] = 1;
     }
 #pragma unroll
@@ -386,7 +422,7 @@ __global__ __launch_bounds__(256) void k_wtc_queue(WccArgs a, const uint32_t *__
                 for (uint32_t k = tid; k < dp; k += 256) {
                     const uint32_t c = s_cnt[k];
                     if (c) {
-                        atomicAdd(&a.T[N[k]], c);
+                        wtc_add_T(a, N[k], c);
                         a.flag[beg + k] = 1;
                     }
                 }
@@ -415,8 +451,8 @@ __global__ __launch_bounds__(256) void k_wtc_queue(WccArgs a, const uint32_t *__
                                 else h2 = mid;
                             }
                             if (l2 < dp && N[l2] == w) {
-                                atomicAdd(&a.T[v], 1u);
-                                atomicAdd(&a.T[w], 1u);
+                                wtc_add_T(a, v, 1u);
+                                wtc_add_T(a, w, 1u);
                                 a.flag[vb + j] = 1;
                                 a.flag[beg + k0 + kk] = 1;
                                 a.flag[beg + l2] = 1;
@@ -452,7 +488,7 @@ __global__ __launch_bounds__(256) void k_wtc_final(WccArgs a, const uint32_t *__
     uint32_t mine = 0;
     for (uint32_t k = lig; k < dp; k += S) {
         if (a.flag[beg + k]) {
-            atomicAdd(&a.td[a.int_adj[beg + k]], 1u);
+            wtc_add_td(a, a.int_adj[beg + k], 1u);
             mine++;
         }
     }
