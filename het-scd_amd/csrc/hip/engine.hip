@@ -569,8 +569,8 @@ static int count_internal_triangles(scd_ctx *c, const WccArgs &w) {
         LAUNCH(c, "wcc_tri_wave", cdiv(M, 4), 256, k_wtc_group<64>, w, bin_v(c, K_W1), M);
         const int rc = scan_u32_dev(c, c->wq_nch, c->wq_off, M, c->wq_tot, 0);
         if (rc) return rc;
-        LAUNCH(c, "wcc_tri_queue", 4 * c->num_cus, 256, k_wtc_queue<2048>, w, bin_v(c, K_W1), M, c->wq_off,
-               c->wq_tot, c->wq_next);
+        LAUNCH(c, "wcc_tri_queue", 4 * c->num_cus, WTC_QT, (k_wtc_queue<2048, WTC_QT, 4>), w, bin_v(c, K_W1), M,
+               c->wq_off, c->wq_tot, c->wq_next);
     }
     LAUNCH(c, "wcc_tri_final", cdiv(bin_n(c, K_SUB8), 32), 256, k_wtc_final<8>, w, bin_v(c, K_SUB8),
            bin_n(c, K_SUB8));

@@ -161,6 +161,7 @@ __global__ __launch_bounds__(256) void k_wtc_group(WccArgs a, const uint32_t *__
     __syncthreads();
     const uint32_t W = s_inc[gb + S - 1];
     uint32_t tot = 0;
+    uint32_t cur = 0;  // owner of the lane's entries: the lane's f only grows
     for (uint32_t f0 = lig; f0 < W; f0 += S * WTC_UF) {
         uint32_t w[WTC_UF], at[WTC_UF], own[WTC_UF];
 #pragma unroll
@@ -168,15 +169,10 @@ __global__ __launch_bounds__(256) void k_wtc_group(WccArgs a, const uint32_t *__
             const uint32_t f = f0 + S * q;
             w[q] = SCD_INVALID;
             if (f < W) {
-                uint32_t l = 0, h = dp - 1;  // smallest k with inc[k] > f
-                while (l < h) {
-                    const uint32_t mid = (l + h) >> 1;
-                    if (s_inc[gb + mid] > f) h = mid;
-                    else l = mid + 1;
-                }
-                own[q] = l;
-                const uint32_t excl = l ? s_inc[gb + l - 1] : 0u;
-                at[q] = s_vb[gb + l] + (f - excl);
+                while (s_inc[gb + cur] <= f) cur++;  // smallest k with inc[k] > f
+                own[q] = cur;
+                const uint32_t excl = cur ? s_inc[gb + cur - 1] : 0u;
+                at[q] = s_vb[gb + cur] + (f - excl);
                 w[q] = a.int_adj[at[q]];
             }
         }
@@ -229,7 +225,7 @@ __device__ __forceinline__ int wtc_find(const uint32_t *hk, const uint16_t *hp, 
 // they average >= 32 entries, else flattened over the threads — counts per
 // out-list position in LDS and flushes one atomic per non-zero position.
 constexpr uint32_t WTC_VC = 32;  // out-list entries per work item
-constexpr int WTC_UW = 8;        // loads in flight per lane, list-per-wave mode
+constexpr int WTC_QT = 512;  // work-queue workgroup: 8 waves, 64 VGPRs, 4 workgroups per CU
 constexpr uint32_t WTC_CLAIM = 8;  // items claimed at once by a workgroup
 
 // nch[i] = chunks of verts[M-1-i]; also resets the item counter.
@@ -269,8 +265,8 @@ struct WtcSet {
     }
 };
 
-template <int CAP>
-__global__ __launch_bounds__(256) void k_wtc_queue(WccArgs a, const uint32_t *__restrict__ verts, uint32_t M,
+template <int CAP, int NT, int UW>
+__global__ __launch_bounds__(NT, NT >= 512 ? 8 : 1) void k_wtc_queue(WccArgs a, const uint32_t *__restrict__ verts, uint32_t M,
                                                    const uint32_t *__restrict__ off,
                                                    const uint32_t *__restrict__ total, uint32_t *__restrict__ next) {
     static_assert(2 * CAP * 4 + 2 * CAP * 2 <= WTC_TAB * 8, "hash must fit in the bitmap storage");
@@ -279,12 +275,13 @@ __global__ __launch_bounds__(256) void k_wtc_queue(WccArgs a, const uint32_t *__
     __shared__ uint32_t s_vb[WTC_VC], s_pre[WTC_VC + 1];
     __shared__ uint32_t s_item;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    constexpr int NW = NT / 64;
     WtcSet set;
     set.tab = s_tab;
     set.hk = reinterpret_cast<uint32_t *>(s_tab);
     set.hp = reinterpret_cast<uint16_t *>(set.hk + 2 * CAP);
     // the bitmap is all-zero between vertices (each clears what it set)
-    for (uint32_t k = tid; k < WTC_TAB; k += 256) s_tab[k] = make_uint2(0u, 0u);
+    for (uint32_t k = tid; k < WTC_TAB; k += NT) s_tab[k] = make_uint2(0u, 0u);
     const uint32_t items = *total;
     while (true) {
         // claim WTC_CLAIM consecutive items: chunks of one vertex in a claim
@@ -315,19 +312,19 @@ __global__ __launch_bounds__(256) void k_wtc_queue(WccArgs a, const uint32_t *__
                 if (set.R > 32u * WTC_TAB) {
                     set.bits = 1;
                     while ((1u << set.bits) < 2 * dp) set.bits++;
-                    for (uint32_t h = tid; h < (1u << set.bits); h += 256) set.hk[h] = SCD_INVALID;
+                    for (uint32_t h = tid; h < (1u << set.bits); h += NT) set.hk[h] = SCD_INVALID;
                 }
-                for (uint32_t k = tid; k < dp; k += 256) s_cnt[k] = 0;
+                for (uint32_t k = tid; k < dp; k += NT) s_cnt[k] = 0;
                 __syncthreads();
                 if (set.bits == 0) {
-                    for (uint32_t k = tid; k < dp; k += 256) {
+                    for (uint32_t k = tid; k < dp; k += NT) {
                         const uint32_t x = N[k] - set.lo;
                         atomicOr(&s_tab[x >> 5].x, 1u << (x & 31u));
                         if (k == 0 || ((N[k - 1] - set.lo) >> 5) != (x >> 5)) s_tab[x >> 5].y = k;
                     }
                 } else {
                     const uint32_t mask = (1u << set.bits) - 1u;
-                    for (uint32_t k = tid; k < dp; k += 256) {
+                    for (uint32_t k = tid; k < dp; k += NT) {
                         const uint32_t v = N[k];
                         uint32_t h = wtc_hash(v, set.bits);
                         while (atomicCAS(&set.hk[h], SCD_INVALID, v) != SCD_INVALID) h = (h + 1) & mask;
@@ -355,19 +352,19 @@ __global__ __launch_bounds__(256) void k_wtc_queue(WccArgs a, const uint32_t *__
                     __syncthreads();
                     const uint32_t Wc = s_pre[nk];
                     if (Wc >= 32u * nk) {
-                        // one list per wave; WTC_UW loads of 64 entries in flight
-                        for (uint32_t kk = wv; kk < nk; kk += 4) {
+                        // one list per wave; UW loads of 64 entries in flight
+                        for (uint32_t kk = wv; kk < nk; kk += NW) {
                             const uint32_t vb = s_vb[kk], dv = s_pre[kk + 1] - s_pre[kk];
                             uint32_t cnt = 0;
-                            for (uint32_t j0 = 0; j0 < dv; j0 += 64 * WTC_UW) {
-                                uint32_t w[WTC_UW];
+                            for (uint32_t j0 = 0; j0 < dv; j0 += 64 * UW) {
+                                uint32_t w[UW];
 #pragma unroll
-                                for (int q = 0; q < WTC_UW; q++) {
+                                for (int q = 0; q < UW; q++) {
                                     const uint32_t j = j0 + 64 * q + lane;
                                     w[q] = j < dv ? a.int_adj[vb + j] : SCD_INVALID;
                                 }
 #pragma unroll
-                                for (int q = 0; q < WTC_UW; q++) {
+                                for (int q = 0; q < UW; q++) {
                                     if (w[q] == SCD_INVALID) continue;
                                     const int pos = set.find(w[q]);
                                     if (pos >= 0) {
@@ -386,21 +383,17 @@ __global__ __launch_bounds__(256) void k_wtc_queue(WccArgs a, const uint32_t *__
                         }
                     } else {
                         // flattened over the threads; WTC_UF loads in flight each
-                        for (uint32_t f0 = tid; f0 < Wc; f0 += 256 * WTC_UF) {
+                        uint32_t cur = 0;  // owner list: the thread's f only grows
+                        for (uint32_t f0 = tid; f0 < Wc; f0 += NT * WTC_UF) {
                             uint32_t w[WTC_UF], at[WTC_UF], own[WTC_UF];
 #pragma unroll
                             for (int q = 0; q < WTC_UF; q++) {
-                                const uint32_t f = f0 + 256 * q;
+                                const uint32_t f = f0 + NT * q;
                                 w[q] = SCD_INVALID;
                                 if (f < Wc) {
-                                    uint32_t l = 0, h = nk;  // largest kk with pre[kk] <= f
-                                    while (h - l > 1) {
-                                        const uint32_t mid = (l + h) >> 1;
-                                        if (s_pre[mid] <= f) l = mid;
-                                        else h = mid;
-                                    }
-                                    own[q] = l;
-                                    at[q] = s_vb[l] + (f - s_pre[l]);
+                                    while (s_pre[cur + 1] <= f) cur++;  // largest kk with pre[kk] <= f
+                                    own[q] = cur;
+                                    at[q] = s_vb[cur] + (f - s_pre[cur]);
                                     w[q] = a.int_adj[at[q]];
                                 }
                             }
@@ -419,7 +412,7 @@ __global__ __launch_bounds__(256) void k_wtc_queue(WccArgs a, const uint32_t *__
                     }
                     __syncthreads();  // s_vb / s_pre reused by the next chunk
                 }
-                for (uint32_t k = tid; k < dp; k += 256) {
+                for (uint32_t k = tid; k < dp; k += NT) {
                     const uint32_t c = s_cnt[k];
                     if (c) {
                         wtc_add_T(a, N[k], c);
@@ -428,10 +421,10 @@ __global__ __launch_bounds__(256) void k_wtc_queue(WccArgs a, const uint32_t *__
                 }
                 // leave the bitmap all-zero for the next vertex
                 if (set.bits == 0) {
-                    for (uint32_t k = tid; k < dp; k += 256) s_tab[(N[k] - set.lo) >> 5].x = 0;
+                    for (uint32_t k = tid; k < dp; k += NT) s_tab[(N[k] - set.lo) >> 5].x = 0;
                 } else {
                     const uint32_t words = (2 * CAP * 4 + 2 * CAP * 2) / 8;
-                    for (uint32_t k = tid; k < words; k += 256) s_tab[k] = make_uint2(0u, 0u);
+                    for (uint32_t k = tid; k < words; k += NT) s_tab[k] = make_uint2(0u, 0u);
                 }
             } else {
                 // out-list longer than the LDS set: search it in global memory
@@ -442,7 +435,7 @@ __global__ __launch_bounds__(256) void k_wtc_queue(WccArgs a, const uint32_t *__
                     for (uint32_t kk = 0; kk < nk; kk++) {
                         const uint32_t v = N[k0 + kk];
                         const uint32_t vb = s_vb[kk], dv = a.dplus[v];
-                        for (uint32_t j = tid; j < dv; j += 256) {
+                        for (uint32_t j = tid; j < dv; j += NT) {
                             const uint32_t w = a.int_adj[vb + j];
                             uint32_t l2 = 0, h2 = dp;
                             while (l2 < h2) {

@@ -15,10 +15,14 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <deque>
+#include <mutex>
 #include <thread>
+#include <utility>
 #include <vector>
 
 #include "scd_hip.h"
@@ -43,7 +47,7 @@ int engine_fail(int rc, const char *what) {
     return rc;
 }
 
-enum State { QUEUED = 0, RESIDENT = 1, REFINING = 2, DONE = 3, FAILED = 4 };
+enum State { QUEUED = 0, RESIDENT = 1, REFINING = 2, DONE = 3, FAILED = 4, UPLOADING = 5 };
 
 struct Req {
     scd_request in{};
@@ -71,6 +75,18 @@ struct scd_server {
     uint64_t resident = 0;
     scd_server_stats st{};
     std::chrono::steady_clock::time_point t0;
+
+    // uploads run on their own thread and stream, overlapping the
+    // refinement on the device; the main loop drains finished ones
+    std::thread up_thread;
+    std::mutex mu;
+    std::condition_variable cv, cv_done;
+    std::deque<size_t> up_queue;                 // requests to upload (guarded by mu)
+    std::vector<std::pair<size_t, int>> up_done;  // (request, status) (guarded by mu)
+    bool up_stop = false;                        // guarded by mu
+    hipStream_t up_stream = nullptr;
+    size_t uploading = 0;  // main thread only
+    double upload_ms = 0;  // uploader thread only
 
     double now() const {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -103,6 +119,11 @@ extern "C" int scd_server_create(const scd_server_opts *opts, scd_server **out) 
     }
     scd_set_graph_mode(s->main, opts->graph_mode ? 1 : 0);
     scd_set_graph_mode(s->prep, 0);
+    if (hipSetDevice(opts->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->up_stream, hipStreamNonBlocking) != hipSuccess) {
+        scd_server_destroy(s);
+        return sfail(SCD_EDEVICE, "scd_server_create: cannot create the upload stream");
+    }
     if (s->o.hbm_budget == 0) {
         size_t fr = 0, tot = 0;
         if (hipSetDevice(opts->device) != hipSuccess || hipMemGetInfo(&fr, &tot) != hipSuccess) {
@@ -115,7 +136,9 @@ extern "C" int scd_server_create(const scd_server_opts *opts, scd_server **out) 
     return SCD_OK;
 }
 
-static void release(scd_server *s, Req &r) {
+static void stop_uploader(scd_server *s);
+
+static void free_device(Req &r) {
     (void)hipFree(r.d_ind);
     (void)hipFree(r.d_adj);
     (void)hipFree(r.d_tri);
@@ -124,13 +147,20 @@ static void release(scd_server *s, Req &r) {
     (void)hipFree(r.d_best);
     r.d_ind = r.d_adj = r.d_cur = r.d_best = nullptr;
     r.d_tri = r.d_td = nullptr;
+}
+
+// Frees a request's device buffers and its share of the budget.
+static void release(scd_server *s, Req &r) {
+    free_device(r);
     if (r.bytes) s->resident -= r.bytes;
     r.bytes = 0;
 }
 
 extern "C" void scd_server_destroy(scd_server *s) {
     if (!s) return;
+    stop_uploader(s);
     for (auto &r : s->reqs) release(s, r);
+    if (s->up_stream) (void)hipStreamDestroy(s->up_stream);
     if (s->main) scd_destroy(s->main);
     if (s->prep) scd_destroy(s->prep);
     delete s;
@@ -169,22 +199,52 @@ static int admit(scd_server *s, Req &r) {
     if (!dmalloc((void **)&r.d_ind, 4 * (N + 1)) || !dmalloc((void **)&r.d_adj, 4 * E) ||
         !dmalloc((void **)&r.d_tri, 4 * N) || !dmalloc((void **)&r.d_td, 4 * N) ||
         !dmalloc((void **)&r.d_cur, 4 * N) || !dmalloc((void **)&r.d_best, 4 * N)) {
-        release(s, r);
+        free_device(r);
         return sfail(SCD_ENOMEM, "admission: device allocation failed");
     }
-    if (hipMemcpy(r.d_ind, q.indices, 4 * (N + 1), hipMemcpyHostToDevice) != hipSuccess ||
-        (E && hipMemcpy(r.d_adj, q.adj, 4 * E, hipMemcpyHostToDevice) != hipSuccess) ||
-        hipMemcpy(r.d_tri, q.tri, 4 * N, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(r.d_td, q.tri_deg, 4 * N, hipMemcpyHostToDevice) != hipSuccess) {
-        release(s, r);
+    hipStream_t st = s->up_stream;
+    if (hipMemcpyAsync(r.d_ind, q.indices, 4 * (N + 1), hipMemcpyHostToDevice, st) != hipSuccess ||
+        (E && hipMemcpyAsync(r.d_adj, q.adj, 4 * E, hipMemcpyHostToDevice, st) != hipSuccess) ||
+        hipMemcpyAsync(r.d_tri, q.tri, 4 * N, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(r.d_td, q.tri_deg, 4 * N, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        free_device(r);
         return sfail(SCD_EDEVICE, "admission: copy to the device failed");
     }
-    r.bytes = scd_server_footprint(q.n, q.m);
-    s->resident += r.bytes;
-    s->st.peak_bytes = std::max(s->st.peak_bytes, s->resident);
-    r.state = RESIDENT;
-    r.res.admit_ms = s->now();
     return SCD_OK;
+}
+
+// The uploader thread: device buffers and copies of each admitted request.
+static void uploader(scd_server *s) {
+    (void)hipSetDevice(s->o.device);
+    while (true) {
+        size_t i;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv.wait(lk, [&] { return s->up_stop || !s->up_queue.empty(); });
+            if (s->up_queue.empty()) return;  // stop requested, nothing left
+            i = s->up_queue.front();
+            s->up_queue.pop_front();
+        }
+        const auto t = std::chrono::steady_clock::now();
+        const int rc = admit(s, s->reqs[i]);
+        s->upload_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->up_done.emplace_back(i, rc);
+        }
+        s->cv_done.notify_one();
+    }
+}
+
+static void stop_uploader(scd_server *s) {
+    if (!s->up_thread.joinable()) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->up_stop = true;
+    }
+    s->cv.notify_all();
+    s->up_thread.join();
 }
 
 static scd_part part_of(const Req &r) {
@@ -311,18 +371,35 @@ static int step(scd_server *s) {
     return SCD_OK;
 }
 
-extern "C" int scd_server_run(scd_server *s) {
-    if (!s) return sfail(SCD_EINVAL, "scd_server_run: null server");
-    std::vector<size_t> order;
-    for (size_t i = 0; i < s->reqs.size(); i++)
-        if (s->reqs[i].state == QUEUED) order.push_back(i);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](size_t a, size_t b) { return s->reqs[a].in.arrival_ms < s->reqs[b].in.arrival_ms; });
-    s->t0 = std::chrono::steady_clock::now();
+// Collects finished uploads; waits for one when block is set.
+static void drain_uploads(scd_server *s, bool block) {
+    std::vector<std::pair<size_t, int>> done;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        if (block) s->cv_done.wait(lk, [&] { return !s->up_done.empty(); });
+        done.swap(s->up_done);
+    }
+    for (auto &d : done) {
+        Req &r = s->reqs[d.first];
+        s->uploading--;
+        if (d.second) {
+            release(s, r);
+            r.state = FAILED;
+            r.res.status = d.second;
+            r.res.done_ms = s->now();
+        } else {
+            r.state = RESIDENT;
+            s->waiting.push_back(d.first);
+        }
+    }
+}
+
+static int serve_loop(scd_server *s, const std::vector<size_t> &order) {
     size_t qi = 0;
     while (true) {
-        double now = s->now();
-        // admission, first come first served, within the footprint budget
+        const double now = s->now();
+        // admission, first come first served, within the footprint budget;
+        // the budget is reserved here, the copy runs on the uploader thread
         while (qi < order.size()) {
             Req &r = s->reqs[order[qi]];
             if (r.in.arrival_ms > now) break;
@@ -335,20 +412,21 @@ extern "C" int scd_server_run(scd_server *s) {
                 continue;
             }
             if (s->resident + fp > s->o.hbm_budget) break;
-            if (s->o.max_graphs && s->n_active + s->waiting.size() >= s->o.max_graphs) break;
-            const double ta = s->now();
-            const int rc = admit(s, r);
-            s->st.admit_ms += s->now() - ta;
-            s->st.busy_ms += s->now() - ta;
-            if (rc) {
-                r.state = FAILED;
-                r.res.status = rc;
-                r.res.done_ms = now;
-            } else {
-                s->waiting.push_back(order[qi]);
+            if (s->o.max_graphs && s->n_active + s->waiting.size() + s->uploading >= s->o.max_graphs) break;
+            r.bytes = fp;
+            s->resident += fp;
+            s->st.peak_bytes = std::max(s->st.peak_bytes, s->resident);
+            r.state = UPLOADING;
+            r.res.admit_ms = now;
+            s->uploading++;
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->up_queue.push_back(order[qi]);
             }
+            s->cv.notify_one();
             qi++;
         }
+        drain_uploads(s, false);
         if (!s->waiting.empty()) {
             const double oldest = s->reqs[s->waiting.front()].in.arrival_ms;
             if (s->n_active == 0 || now - oldest >= s->o.join_wait_ms || 2 * s->n_active <= s->arena.size()) {
@@ -360,6 +438,10 @@ extern "C" int scd_server_run(scd_server *s) {
         }
         if (s->n_active == 0) {
             if (!s->waiting.empty()) continue;
+            if (s->uploading) {
+                drain_uploads(s, true);
+                continue;
+            }
             if (qi >= order.size()) break;
             const double wait = s->reqs[order[qi]].in.arrival_ms - s->now();
             if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double, std::milli>(wait));
@@ -370,6 +452,27 @@ extern "C" int scd_server_run(scd_server *s) {
         s->st.busy_ms += s->now() - t;
         s->st.step_ms += s->now() - t;
         if (rc) return rc;
+    }
+    return SCD_OK;
+}
+
+extern "C" int scd_server_run(scd_server *s) {
+    if (!s) return sfail(SCD_EINVAL, "scd_server_run: null server");
+    std::vector<size_t> order;
+    for (size_t i = 0; i < s->reqs.size(); i++)
+        if (s->reqs[i].state == QUEUED) order.push_back(i);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return s->reqs[a].in.arrival_ms < s->reqs[b].in.arrival_ms; });
+    s->t0 = std::chrono::steady_clock::now();
+    s->up_stop = false;
+    s->upload_ms = 0;
+    s->up_thread = std::thread(uploader, s);
+    const int rc = serve_loop(s, order);
+    stop_uploader(s);
+    s->st.admit_ms += s->upload_ms;
+    if (rc) {  // requests left mid-way keep their device buffers until destroy
+        drain_uploads(s, false);
+        return rc;
     }
     return SCD_OK;
 }
