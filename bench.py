@@ -101,7 +101,7 @@ def cpu_cores():
         return os.cpu_count() or 1
 
 
-def cpu_baseline(path, m, reps):
+def cpu_baseline(path, m, reps, what):
     """The reference's ./main (oracle/_ref/main) or, if it was not built, the
     oracle restatement (oracle/main) on the same file; median of reps runs."""
     ref = os.path.join(ROOT, "oracle", "_ref", "main")
@@ -124,7 +124,7 @@ def cpu_baseline(path, m, reps):
     per_iter.sort()
     ms = per_iter[len(per_iter) // 2]
     return {"value": m / (ms / 1000.0), "unit": "edges/s", "cores": cores, "kind": kind,
-            "sample": f"full ./main run on the same R-MAT s20 graph file ({iters} iteration(s), "
+            "sample": f"full ./main run on the same {what} graph file ({iters} iteration(s), "
                       f"{ms:.1f} ms per iteration, median of {reps} runs)"}
 
 
@@ -469,7 +469,7 @@ def main():
         path = os.path.join(tmp, "rmat.bin")
         g.write(path)
         if not args.no_cpu:
-            cpu = cpu_baseline(path, g.m, args.cpu_reps)
+            cpu = cpu_baseline(path, g.m, args.cpu_reps, f"R-MAT s{args.scale}")
             if cpu:
                 log(f"[bench] CPU baseline ({cpu['kind']}, {cpu['cores']} threads): "
                     f"{cpu['value'] / 1e6:.1f} M refine edges/s")
