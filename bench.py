@@ -370,6 +370,14 @@ def main():
     t = time.perf_counter()
     lab0, k0 = eng.initial_partition()
     t_ip = time.perf_counter() - t
+    # the autotuner's pick for this graph's size class (set-up, outside the
+    # timed steps; cached per class)
+    eng.set_profiling(False)
+    t = time.perf_counter()
+    tune, tune_ms = eng.autotune(reps=3)
+    t_tune = time.perf_counter() - t
+    eng.set_profiling(True)
+    log(f"[bench] autotune {t_tune:.1f} s -> {tune} ({tune_ms:.3f} ms per iteration)")
     w0 = eng.compute_wcc()
     eng.snapshot()
     log(f"[bench] set-up: upload {t_up * 1e3:.1f} ms, initial partition {t_ip * 1e3:.1f} ms "
@@ -437,7 +445,8 @@ def main():
         import glob
         for fn in glob.glob(os.path.join(ROOT, "profiles", "*_traffic.json")):
             t = json.load(open(fn))
-            if t.get("kernel") == roof["kernel"] and t.get("workload_graph_sha256") == graph_sha:
+            if (t.get("kernel") == roof["kernel"] and t.get("workload_graph_sha256") == graph_sha
+                    and t.get("tuning") == eng.get_tuning()):
                 roof["traffic"] = t["traffic_bytes"]
                 roof["traffic_source"] = os.path.relpath(fn, ROOT)
     per_kernel = {name: {"launches": la, "avg_us": round(ms / la * 1e3, 2), "total_ms": round(ms, 3)}
@@ -505,7 +514,8 @@ def main():
         "cpu_baseline": cpu,
         "vs_cpu": round(value / cpu["value"], 2) if cpu else None,
         "parity_vs_reference_cpu": parity,
-        "setup_ms": {"upload": round(t_up * 1e3, 2), "initial_partition": round(t_ip * 1e3, 2)},
+        "setup_ms": {"upload": round(t_up * 1e3, 2), "initial_partition": round(t_ip * 1e3, 2),
+                     "autotune": round(t_tune * 1e3, 1)},
         "eager_profiled_ms_per_step": round(eager_ms_step, 4),
         "triangle_counting": tc,
         "batched": batched,

@@ -159,9 +159,17 @@ struct scd_ctx {
     uint32_t *hub_a = nullptr, *hub_b = nullptr, *chunk_cnt = nullptr, *chunk_bl = nullptr,
              *chunk_has_rem = nullptr;
     double *chunk_bi = nullptr, *chunk_rem = nullptr;
+    // hash tables of the hub candidates (move without export): slots, per-hub
+    // offset and capacity, chunks of slots and their results
+    unsigned long long *hub_slots = nullptr;
+    uint64_t *tab_off = nullptr;
+    uint32_t *tab_cap = nullptr, *tchunk_hub = nullptr, *tchunk_start = nullptr, *hub_tchunk_begin = nullptr;
+    uint32_t *tchunk_bl = nullptr, *tchunk_has_rem = nullptr;
+    double *tchunk_bi = nullptr, *tchunk_rem = nullptr;
+    uint32_t n_tchunks = 0;
     std::vector<DevBuf> hub_bufs;
 
-    scd_tuning tune = {4096, 2048, 1024, {0, 0, 0, 0, 0}};
+    scd_tuning tune = {4096, 2048, 1024, 0, {0, 0, 0, 0}};
     uint32_t last_ip_rounds = 0;
 
     // per-iteration hipGraphs, one per label-buffer parity (cur == lab_a or
@@ -409,7 +417,35 @@ static int build_bins(scd_ctx *c) {
         (rc = dalloc(c, &c->chunk_bi, c->n_chunks, &c->hub_bufs)) ||
         (rc = dalloc(c, &c->chunk_rem, c->n_chunks, &c->hub_bufs)))
         return rc;
-    return SCD_OK;
+
+    // hash tables: 2 x degree slots rounded up to a power of two, C per chunk
+    std::vector<uint64_t> toff(nh);
+    std::vector<uint32_t> tcap(nh), tch, tcs, htb(nh + 1);
+    uint64_t slots = 0;
+    for (uint32_t hb = 0; hb < nh; hb++) {
+        uint32_t cap = 1;
+        while (cap < 2 * hd[hb]) cap <<= 1;
+        toff[hb] = slots;
+        tcap[hb] = cap;
+        slots += cap;
+        htb[hb] = (uint32_t)tch.size();
+        for (uint32_t st = 0; st < cap; st += C) {
+            tch.push_back(hb);
+            tcs.push_back(st);
+        }
+    }
+    htb[nh] = (uint32_t)tch.size();
+    c->n_tchunks = (uint32_t)tch.size();
+    if ((rc = dalloc(c, &c->hub_slots, slots, &c->hub_bufs)) || (rc = dalloc(c, &c->tab_off, nh, &c->hub_bufs)) ||
+        (rc = up(&c->tab_cap, tcap)) || (rc = up(&c->tchunk_hub, tch)) || (rc = up(&c->tchunk_start, tcs)) ||
+        (rc = up(&c->hub_tchunk_begin, htb)) || (rc = dalloc(c, &c->tchunk_bl, c->n_tchunks, &c->hub_bufs)) ||
+        (rc = dalloc(c, &c->tchunk_has_rem, c->n_tchunks, &c->hub_bufs)) ||
+        (rc = dalloc(c, &c->tchunk_bi, c->n_tchunks, &c->hub_bufs)) ||
+        (rc = dalloc(c, &c->tchunk_rem, c->n_tchunks, &c->hub_bufs)))
+        return rc;
+    HIP_TRY(hipMemcpy(c->tab_off, toff.data(), 8ull * nh, hipMemcpyHostToDevice));
+    LAUNCH(c, "hub_table_clear", cdiv(slots, 256), 256, k_fill_u64v, c->hub_slots, slots, HUB_EMPTY);
+    return check_launch("hub tables");
 }
 
 static HubArgs hub_args(scd_ctx *c) {
@@ -436,9 +472,22 @@ static inline const uint32_t *bin_v(const scd_ctx *c, int b) { return c->bin_ver
 // Phases
 // ---------------------------------------------------------------------------
 template <int T>
-static void launch_lds_and_tiles(scd_ctx *c, const MoveArgs &a, const HubArgs &h) {
+static void launch_lds_and_tiles(scd_ctx *c, const MoveArgs &a, const HubArgs &h, const HubTable &t) {
     LAUNCH(c, "move_lds", bin_n(c, K_LDS), 256, (k_move_lds<T>), a, bin_v(c, K_LDS), bin_n(c, K_LDS));
-    LAUNCH(c, "hub_tile_sort", c->n_tiles, 256, (k_hub_tile_sort<T>), a, h, c->hub_a);
+    if (a.ex_sorted || !c->tune.hub_table)
+        LAUNCH(c, "hub_tile_sort", c->n_tiles, 256, (k_hub_tile_sort<T>), a, h, c->hub_a);
+    else
+        LAUNCH(c, "hub_tile_count", c->n_tiles, 256, (k_hub_tile_count<T>), a, h, t);
+}
+
+static HubTable hub_table(scd_ctx *c) {
+    HubTable t;
+    t.slots = c->hub_slots;
+    t.tab_off = c->tab_off;
+    t.tab_cap = c->tab_cap;
+    t.tchunk_hub = c->tchunk_hub;
+    t.tchunk_start = c->tchunk_start;
+    return t;
 }
 
 // Move step for every vertex: old labels (cur) -> labels_out.  Export mode
@@ -471,13 +520,23 @@ static int phase_move(scd_ctx *c, uint32_t *labels_out, uint32_t *ex_sorted, uin
     LAUNCH(c, "move_wave16", cdiv(bin_n(c, K_W16), 4), 256, k_move_wave<16>, at(K_W16), bin_v(c, K_W16), bin_n(c, K_W16));
     a.counters = c->counting ? c->counters + K_LDS : nullptr;
     HubArgs h = hub_args(c);
+    const HubTable t = hub_table(c);
     switch (c->tune.lds_tile) {
-        case 2048: launch_lds_and_tiles<2048>(c, a, h); break;
-        case 4096: launch_lds_and_tiles<4096>(c, a, h); break;
-        case 8192: launch_lds_and_tiles<8192>(c, a, h); break;
-        default: launch_lds_and_tiles<16384>(c, a, h); break;
+        case 2048: launch_lds_and_tiles<2048>(c, a, h, t); break;
+        case 4096: launch_lds_and_tiles<4096>(c, a, h, t); break;
+        case 8192: launch_lds_and_tiles<8192>(c, a, h, t); break;
+        default: launch_lds_and_tiles<16384>(c, a, h, t); break;
     }
-    if (c->nh) {
+    if (c->nh && !ex_sorted && c->tune.hub_table) {
+        HubArgs ht = h;  // results per table chunk
+        ht.hub_chunk_begin = c->hub_tchunk_begin;
+        ht.chunk_bi = c->tchunk_bi;
+        ht.chunk_bl = c->tchunk_bl;
+        ht.chunk_rem = c->tchunk_rem;
+        ht.chunk_has_rem = c->tchunk_has_rem;
+        LAUNCH(c, "hub_eval", c->n_tchunks, 256, k_hub_table_eval, at(K_HUB), ht, t, c->tune.merge_chunk);
+        LAUNCH(c, "hub_finalize", cdiv(c->nh, 64), 64, k_hub_finalize, a, ht, c->nh);
+    } else if (c->nh) {
         const uint32_t C = c->tune.merge_chunk;
         uint32_t *src = c->hub_a, *dst = c->hub_b;
         for (uint64_t w = c->tune.lds_tile; w < c->max_hub_deg; w *= 2) {
@@ -1068,6 +1127,8 @@ extern "C" int scd_set_tuning(scd_ctx *c, const scd_tuning *t) {
     const uint32_t W = t->wave_max ? t->wave_max : 1024u;
     if (W != 256 && W != 512 && W != 1024)
         return fail(SCD_EINVAL, "scd_set_tuning: wave_max %u is not 256, 512 or 1024", W);
+    if (t->hub_table > 1) return fail(SCD_EINVAL, "scd_set_tuning: hub_table %u is not 0 or 1", t->hub_table);
+    if (c->tune.hub_table != t->hub_table) drop_graphs(c);
     c->tune = *t;
     c->tune.wave_max = W;
     if (c->have_graph) {
@@ -1681,12 +1742,12 @@ bool read_tune_file(const char *path, const std::string &key, TuneEntry &out) {
     FILE *f = fopen(path, "r");
     if (!f) return false;
     char k[256];
-    unsigned T, C, W;
+    unsigned T, C, W, H;
     double ms;
     bool found = false;
-    while (fscanf(f, "%255s %u %u %u %lf", k, &T, &C, &W, &ms) == 5) {
+    while (fscanf(f, "%255s %u %u %u %u %lf", k, &T, &C, &W, &H, &ms) == 6) {
         if (key == k) {  // the last line of a class wins
-            out.t = scd_tuning{T, C, W, {0, 0, 0, 0, 0}};
+            out.t = scd_tuning{T, C, W, H, {0, 0, 0, 0}};
             out.ms = ms;
             found = true;
         }
@@ -1738,8 +1799,13 @@ extern "C" int scd_autotune(scd_ctx *c, const char *cache_path, int reps, scd_tu
     for (uint32_t T : Ts)
         for (uint32_t W : Ws) {
             std::vector<uint32_t> Cs = {std::min<uint32_t>(2048, 2 * T)};
-            if (dmax > T) Cs = {1024, 2048, 4096};
-            for (uint32_t C : Cs) cand.push_back(scd_tuning{T, C, W, {0, 0, 0, 0, 0}});
+            std::vector<uint32_t> Hs = {0};
+            if (dmax > T) {
+                Cs = {1024, 2048, 4096};
+                Hs = {0, 1};
+            }
+            for (uint32_t C : Cs)
+                for (uint32_t H : Hs) cand.push_back(scd_tuning{T, C, W, H, {0, 0, 0, 0}});
         }
 
     // the labels, statistics and best labels to start every timed iteration from
@@ -1805,7 +1871,8 @@ extern "C" int scd_autotune(scd_ctx *c, const char *cache_path, int reps, scd_tu
     if (cache_path) {
         FILE *f = fopen(cache_path, "a");
         if (!f) return fail(SCD_EIO, "scd_autotune: cannot append to %s", cache_path);
-        fprintf(f, "%s %u %u %u %.6f\n", key, best_t.lds_tile, best_t.merge_chunk, best_t.wave_max, best_ms);
+        fprintf(f, "%s %u %u %u %u %.6f\n", key, best_t.lds_tile, best_t.merge_chunk, best_t.wave_max,
+                best_t.hub_table, best_ms);
         fclose(f);
     }
     if (chosen) *chosen = best_t;

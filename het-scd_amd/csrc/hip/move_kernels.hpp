@@ -458,6 +458,114 @@ __global__ __launch_bounds__(256) void k_hub_eval(MoveArgs a, HubArgs h,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Hub path of the move step without export: the candidates of a hub are its
+// distinct valid neighbour labels with their frequencies, and the argmax
+// order (improvement desc, label asc) is total, so they need no global sort.
+// Each tile of T neighbour labels is sorted in LDS and every run inserts
+// (label, run length) into the hub's hash table in global memory (one atomic
+// per run, so a frequent label is one hot slot per tile, not per entry);
+// then chunks of table slots are evaluated like sorted runs, and each slot is
+// reset for the next iteration.  Slot = label << 32 | count.
+// ---------------------------------------------------------------------------
+constexpr unsigned long long HUB_EMPTY = 0xFFFFFFFF00000000ull;
+
+struct HubTable {
+    unsigned long long *__restrict__ slots;
+    const uint64_t *__restrict__ tab_off;  // per hub
+    const uint32_t *__restrict__ tab_cap;  // per hub, a power of two
+    // chunks of table slots: hub, first slot; results per chunk
+    const uint32_t *__restrict__ tchunk_hub;
+    const uint32_t *__restrict__ tchunk_start;
+};
+
+__device__ __forceinline__ uint32_t hub_hash(uint32_t L, uint32_t mask) { return (L * 0x9E3779B1u) & mask; }
+
+__device__ __forceinline__ void hub_table_add(unsigned long long *tab, uint32_t mask, uint32_t L, uint32_t cnt) {
+    const unsigned long long key = (unsigned long long)L << 32;
+    for (uint32_t i = hub_hash(L, mask);; i = (i + 1) & mask) {
+        unsigned long long cur = tab[i];
+        if (cur == HUB_EMPTY) {
+            cur = atomicCAS(&tab[i], HUB_EMPTY, key | cnt);
+            if (cur == HUB_EMPTY) return;
+        }
+        if ((cur >> 32) == L) {
+            atomicAdd(&tab[i], (unsigned long long)cnt);
+            return;
+        }
+    }
+}
+
+template <int T>
+__global__ __launch_bounds__(256) void k_hub_tile_count(MoveArgs a, HubArgs h, HubTable t) {
+    __shared__ uint32_t keys[T];
+    const int tid = threadIdx.x;
+    const uint32_t hb = h.tile_hub[blockIdx.x];
+    const uint32_t st = h.tile_start[blockIdx.x];
+    const uint32_t p = h.hub_v[hb];
+    if (vtx_stopped(a, p)) return;  // uniform per block
+    const uint32_t d = h.hub_d[hb];
+    const uint32_t beg = a.indices[p] + st;
+    const uint32_t len = min((uint32_t)T, d - st);
+    const uint32_t n2 = max(next_pow2(len), 1024u);
+    gather_labels<T>(a, beg, len, n2, keys);
+    block_sort_lds<T>(keys, n2);
+    unsigned long long *tab = t.slots + t.tab_off[hb];
+    const uint32_t mask = t.tab_cap[hb] - 1u;
+    for (uint32_t i = tid; i < len; i += 256) {
+        const uint32_t L = keys[i];
+        if (L == SCD_INVALID || (i > 0 && keys[i - 1] == L)) continue;
+        hub_table_add(tab, mask, L, upper_bound_u32(keys, len, L) - i);
+    }
+}
+
+// Evaluation of a chunk of C table slots (every slot reset afterwards).
+__global__ __launch_bounds__(256) void k_hub_table_eval(MoveArgs a, HubArgs h, HubTable t, uint32_t C) {
+    __shared__ double s_bi[4];
+    __shared__ uint32_t s_bl[4];
+    __shared__ double s_rem;
+    __shared__ int s_has_rem;
+    const int tid = threadIdx.x;
+    const uint32_t c = blockIdx.x;
+    const uint32_t hb = t.tchunk_hub[c];
+    const uint32_t o = t.tchunk_start[c];
+    const uint32_t d = h.hub_d[hb];
+    const uint32_t p = h.hub_v[hb];
+    const uint32_t my = a.labels[p];
+    const double pext = vtx_pext(a, p);
+    unsigned long long *tab = t.slots + t.tab_off[hb];
+    const uint32_t end = min(o + C, t.tab_cap[hb]);
+    if (tid == 0) s_has_rem = 0;
+    __syncthreads();
+    double bi = 0.0;
+    uint32_t bl = SCD_INVALID;
+    uint32_t heads = 0;
+    for (uint32_t i = o + tid; i < end; i += 256) {
+        const unsigned long long e = tab[i];
+        if (e == HUB_EMPTY) continue;
+        tab[i] = HUB_EMPTY;
+        const uint32_t L = (uint32_t)(e >> 32), f = (uint32_t)e;
+        const double imp = candidate_improvement_s(a.stats[L], L, f, d, my, pext);
+        if (L == my) {
+            s_rem = -imp;
+            s_has_rem = 1;
+        }
+        if (imp > 0.0 && better(imp, L, bi, bl)) {
+            bi = imp;
+            bl = L;
+        }
+        heads++;
+    }
+    count_pairs(a.counters, heads);
+    block_argmax256(bi, bl, s_bi, s_bl);
+    if (tid == 0) {
+        h.chunk_bi[c] = bi;
+        h.chunk_bl[c] = bl;
+        h.chunk_has_rem[c] = (uint32_t)s_has_rem;
+        h.chunk_rem[c] = s_has_rem ? s_rem : 0.0;
+    }
+}
+
 // One thread per hub: combine the hub's chunks and apply the move rule.
 __global__ __launch_bounds__(64) void k_hub_finalize(MoveArgs a, HubArgs h, uint32_t nh) {
     const uint32_t hb = blockIdx.x * 64u + threadIdx.x;

@@ -122,6 +122,11 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t n) {
     if (i < n) p[i] = 0;
 }
 
+__global__ void k_fill_u64v(unsigned long long *__restrict__ p, uint64_t n, unsigned long long v) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 __global__ void k_fill_u64(unsigned long long *__restrict__ p, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0ull;

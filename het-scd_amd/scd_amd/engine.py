@@ -25,10 +25,10 @@ def device_count():
 def _get_tuning(ctx):
     t = _lib.ScdTuningC()
     check_hip(_lib.hip().scd_get_tuning(ctx, ctypes.byref(t)))
-    return {"lds_tile": t.lds_tile, "merge_chunk": t.merge_chunk, "wave_max": t.wave_max}
+    return {"lds_tile": t.lds_tile, "merge_chunk": t.merge_chunk, "wave_max": t.wave_max, "hub_table": t.hub_table}
 
 
-def _set_tuning(ctx, lds_tile=None, merge_chunk=None, wave_max=None):
+def _set_tuning(ctx, lds_tile=None, merge_chunk=None, wave_max=None, hub_table=None):
     t = _lib.ScdTuningC()
     check_hip(_lib.hip().scd_get_tuning(ctx, ctypes.byref(t)))
     if lds_tile is not None:
@@ -37,6 +37,8 @@ def _set_tuning(ctx, lds_tile=None, merge_chunk=None, wave_max=None):
         t.merge_chunk = merge_chunk
     if wave_max is not None:
         t.wave_max = wave_max
+    if hub_table is not None:
+        t.hub_table = hub_table
     check_hip(_lib.hip().scd_set_tuning(ctx, ctypes.byref(t)))
 
 
@@ -45,7 +47,8 @@ def _autotune(ctx, cache_path, reps):
     ms = ctypes.c_double()
     path = None if cache_path is None else str(cache_path).encode()
     check_hip(_lib.hip().scd_autotune(ctx, path, reps, ctypes.byref(t), ctypes.byref(ms)))
-    return {"lds_tile": t.lds_tile, "merge_chunk": t.merge_chunk, "wave_max": t.wave_max}, ms.value
+    return {"lds_tile": t.lds_tile, "merge_chunk": t.merge_chunk, "wave_max": t.wave_max,
+            "hub_table": t.hub_table}, ms.value
 
 
 class Engine:
@@ -165,8 +168,8 @@ class Engine:
     def get_tuning(self):
         return _get_tuning(self._ctx)
 
-    def set_tuning(self, lds_tile=None, merge_chunk=None, wave_max=None):
-        _set_tuning(self._ctx, lds_tile, merge_chunk, wave_max)
+    def set_tuning(self, lds_tile=None, merge_chunk=None, wave_max=None, hub_table=None):
+        _set_tuning(self._ctx, lds_tile, merge_chunk, wave_max, hub_table)
 
     def autotune(self, cache_path=None, reps=3):
         """scd_autotune: fastest tuning for this graph's size class (cached);
@@ -293,8 +296,8 @@ class BatchEngine:
     def get_tuning(self):
         return _get_tuning(self._ctx)
 
-    def set_tuning(self, lds_tile=None, merge_chunk=None, wave_max=None):
-        _set_tuning(self._ctx, lds_tile, merge_chunk, wave_max)
+    def set_tuning(self, lds_tile=None, merge_chunk=None, wave_max=None, hub_table=None):
+        _set_tuning(self._ctx, lds_tile, merge_chunk, wave_max, hub_table)
 
     def current_labels(self):
         lab = np.empty(self.arena.graph.n, np.uint32)

@@ -175,12 +175,17 @@ int scd_neighbor_label_pairs(scd_ctx *ctx, uint32_t *sorted_out, uint32_t *run_o
  *   wave_max:    largest degree sorted by one wavefront in registers (256,
  *                512 or 1024; 0 means 1024); larger degrees up to lds_tile
  *                take one workgroup each.
+ *   hub_table:   0: hub candidates by merge passes over sorted tiles; 1: by
+ *                per-hub hash tables of (label, count) filled from sorted
+ *                tiles (no merge passes).  Export (scd_neighbor_label_pairs)
+ *                always sorts.
  */
 typedef struct scd_tuning {
     uint32_t lds_tile;
     uint32_t merge_chunk;
     uint32_t wave_max;
-    uint32_t reserved[5];
+    uint32_t hub_table;
+    uint32_t reserved[4];
 } scd_tuning;
 
 int scd_get_tuning(const scd_ctx *ctx, scd_tuning *out);
@@ -189,7 +194,8 @@ int scd_set_tuning(scd_ctx *ctx, const scd_tuning *t);
 /*
  * Autotuner (no counterpart in the reference): times one refinement
  * iteration from the current labels under each candidate tuning (the
- * candidates that change the shape for this graph's degrees; reps timed
+ * candidates that change the shape for this graph's degrees, hub modes
+ * included when hubs exist; reps timed
  * runs each, median), keeps the fastest and leaves the labels, statistics
  * and snapshot as they were.  The choice is cached per graph-size class
  * (device architecture and CU count, log2 of n, of the maximum and of the
