@@ -169,7 +169,7 @@ struct scd_ctx {
     uint32_t n_tchunks = 0;
     std::vector<DevBuf> hub_bufs;
 
-    scd_tuning tune = {4096, 2048, 1024, 0, {0, 0, 0, 0}};
+    scd_tuning tune = {4096, 2048, 1024, 0, 0, 1, {0, 0}};
     uint32_t last_ip_rounds = 0;
 
     // per-iteration hipGraphs, one per label-buffer parity (cur == lab_a or
@@ -184,6 +184,14 @@ struct scd_ctx {
     std::vector<PendingEv> pending;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
+
+    // independent per-bin kernels run on lanes: the context's stream plus
+    // three more (default priority), forked and joined with events, so
+    // small and tail-bound launches overlap (captured as parallel branches)
+    hipStream_t lane[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t fork_ev = nullptr, join_ev[3] = {nullptr, nullptr, nullptr};
+    hipStream_t ls = nullptr;  // the stream LAUNCH uses
+    bool use_lanes = true;
 
     std::vector<DevBuf> bufs;
     // buffers of the previous graph, reused by the next upload when a size
@@ -307,9 +315,33 @@ static int prof_collect(scd_ctx *c) {
     do {                                                                                 \
         if ((grid) > 0) {                                                                \
             ProfScope ps_(c, name);                                                      \
-            hipLaunchKernelGGL((kern), dim3(grid), dim3(block), 0, (c)->stream, __VA_ARGS__); \
+            hipLaunchKernelGGL((kern), dim3(grid), dim3(block), 0, (c)->ls, __VA_ARGS__);     \
         }                                                                                \
     } while (0)
+
+// Lanes are off while profiling (per-kernel events are recorded on the
+// context's stream).
+static inline bool lanes_on(const scd_ctx *c) {
+    return c->use_lanes && c->tune.lanes && c->E >= (1ull << 20) && !c->profile && c->lane[0];
+}
+
+static void fork_lanes(scd_ctx *c) {
+    if (!lanes_on(c)) return;
+    (void)hipEventRecord(c->fork_ev, c->stream);
+    for (int i = 0; i < 3; i++) (void)hipStreamWaitEvent(c->lane[i], c->fork_ev, 0);
+}
+
+// Subsequent LAUNCHes go to lane i (0 = the context's stream).
+static void on_lane(scd_ctx *c, int i) { c->ls = (lanes_on(c) && i > 0) ? c->lane[i - 1] : c->stream; }
+
+static void join_lanes(scd_ctx *c) {
+    c->ls = c->stream;
+    if (!lanes_on(c)) return;
+    for (int i = 0; i < 3; i++) {
+        (void)hipEventRecord(c->join_ev[i], c->lane[i]);
+        (void)hipStreamWaitEvent(c->stream, c->join_ev[i], 0);
+    }
+}
 
 static int check_launch(const char *what) {
     hipError_t e = hipGetLastError();
@@ -473,7 +505,16 @@ static inline const uint32_t *bin_v(const scd_ctx *c, int b) { return c->bin_ver
 // ---------------------------------------------------------------------------
 template <int T>
 static void launch_lds_and_tiles(scd_ctx *c, const MoveArgs &a, const HubArgs &h, const HubTable &t) {
-    LAUNCH(c, "move_lds", bin_n(c, K_LDS), 256, (k_move_lds<T>), a, bin_v(c, K_LDS), bin_n(c, K_LDS));
+    if constexpr (T <= 4096) {
+        if (!a.ex_sorted && c->tune.lds_hash) {
+            LAUNCH(c, "move_lds", bin_n(c, K_LDS), 256, (k_move_lds_hash<T>), a, bin_v(c, K_LDS), bin_n(c, K_LDS));
+        } else {
+            LAUNCH(c, "move_lds", bin_n(c, K_LDS), 256, (k_move_lds<T>), a, bin_v(c, K_LDS), bin_n(c, K_LDS));
+        }
+    } else {
+        LAUNCH(c, "move_lds", bin_n(c, K_LDS), 256, (k_move_lds<T>), a, bin_v(c, K_LDS), bin_n(c, K_LDS));
+    }
+    on_lane(c, 1);  // the hub chain: tiles, then merges / table, evaluation, finalize
     if (a.ex_sorted || !c->tune.hub_table)
         LAUNCH(c, "hub_tile_sort", c->n_tiles, 256, (k_hub_tile_sort<T>), a, h, c->hub_a);
     else
@@ -510,14 +551,19 @@ static int phase_move(scd_ctx *c, uint32_t *labels_out, uint32_t *ex_sorted, uin
         b.counters = c->counting ? c->counters + bin : nullptr;
         return b;
     };
+    // lanes: 3 small-degree bins | 2 middle bins | 1 hub chain | 0 lds bin
+    fork_lanes(c);
+    on_lane(c, 3);
     LAUNCH(c, "move_sub8", cdiv(bin_n(c, K_SUB8), 32), 256, k_move_sub<8>, at(K_SUB8), bin_v(c, K_SUB8), bin_n(c, K_SUB8));
     LAUNCH(c, "move_sub16", cdiv(bin_n(c, K_SUB16), 16), 256, k_move_sub<16>, at(K_SUB16), bin_v(c, K_SUB16), bin_n(c, K_SUB16));
     LAUNCH(c, "move_sub32", cdiv(bin_n(c, K_SUB32), 8), 256, k_move_sub<32>, at(K_SUB32), bin_v(c, K_SUB32), bin_n(c, K_SUB32));
     LAUNCH(c, "move_wave1", cdiv(bin_n(c, K_W1), 4), 256, k_move_wave<1>, at(K_W1), bin_v(c, K_W1), bin_n(c, K_W1));
+    on_lane(c, 2);
     LAUNCH(c, "move_wave2", cdiv(bin_n(c, K_W2), 4), 256, k_move_wave<2>, at(K_W2), bin_v(c, K_W2), bin_n(c, K_W2));
     LAUNCH(c, "move_wave4", cdiv(bin_n(c, K_W4), 4), 256, k_move_wave<4>, at(K_W4), bin_v(c, K_W4), bin_n(c, K_W4));
     LAUNCH(c, "move_wave8", cdiv(bin_n(c, K_W8), 4), 256, k_move_wave<8>, at(K_W8), bin_v(c, K_W8), bin_n(c, K_W8));
     LAUNCH(c, "move_wave16", cdiv(bin_n(c, K_W16), 4), 256, k_move_wave<16>, at(K_W16), bin_v(c, K_W16), bin_n(c, K_W16));
+    on_lane(c, 0);
     a.counters = c->counting ? c->counters + K_LDS : nullptr;
     HubArgs h = hub_args(c);
     const HubTable t = hub_table(c);
@@ -527,6 +573,7 @@ static int phase_move(scd_ctx *c, uint32_t *labels_out, uint32_t *ex_sorted, uin
         case 8192: launch_lds_and_tiles<8192>(c, a, h, t); break;
         default: launch_lds_and_tiles<16384>(c, a, h, t); break;
     }
+    on_lane(c, 1);
     if (c->nh && !ex_sorted && c->tune.hub_table) {
         HubArgs ht = h;  // results per table chunk
         ht.hub_chunk_begin = c->hub_tchunk_begin;
@@ -546,6 +593,7 @@ static int phase_move(scd_ctx *c, uint32_t *labels_out, uint32_t *ex_sorted, uin
         LAUNCH(c, "hub_eval", c->n_chunks, 256, k_hub_eval, at(K_HUB), h, src, C);
         LAUNCH(c, "hub_finalize", cdiv(c->nh, 64), 64, k_hub_finalize, a, h, c->nh);
     }
+    join_lanes(c);
     return check_launch("move");
 }
 
@@ -563,14 +611,19 @@ static int phase_stats(scd_ctx *c) {
     s.gid = c->G ? c->gid : nullptr;
     s.active_g = c->active_g;
     s.counters = c->counting ? c->counters + K_NB : nullptr;
+    // lanes: 3 small-degree bins | 2 middle bins | 1 hubs | 0 lds bin
+    fork_lanes(c);
+    on_lane(c, 3);
     LAUNCH(c, "filter_sub8", cdiv(bin_n(c, K_SUB8), 32), 256, k_filter_sub<8>, s, bin_v(c, K_SUB8), bin_n(c, K_SUB8));
     LAUNCH(c, "filter_sub16", cdiv(bin_n(c, K_SUB16), 16), 256, k_filter_sub<16>, s, bin_v(c, K_SUB16), bin_n(c, K_SUB16));
     LAUNCH(c, "filter_sub32", cdiv(bin_n(c, K_SUB32), 8), 256, k_filter_sub<32>, s, bin_v(c, K_SUB32), bin_n(c, K_SUB32));
     LAUNCH(c, "filter_wave1", cdiv(bin_n(c, K_W1), 4), 256, k_filter_wave<1>, s, bin_v(c, K_W1), bin_n(c, K_W1));
+    on_lane(c, 2);
     LAUNCH(c, "filter_wave2", cdiv(bin_n(c, K_W2), 4), 256, k_filter_wave<2>, s, bin_v(c, K_W2), bin_n(c, K_W2));
     LAUNCH(c, "filter_wave4", cdiv(bin_n(c, K_W4), 4), 256, k_filter_wave<4>, s, bin_v(c, K_W4), bin_n(c, K_W4));
     LAUNCH(c, "filter_wave8", cdiv(bin_n(c, K_W8), 4), 256, k_filter_wave<8>, s, bin_v(c, K_W8), bin_n(c, K_W8));
     LAUNCH(c, "filter_wave16", cdiv(bin_n(c, K_W16), 4), 256, k_filter_wave<16>, s, bin_v(c, K_W16), bin_n(c, K_W16));
+    on_lane(c, 0);
     switch (c->tune.lds_tile) {
         case 2048: LAUNCH(c, "filter_block", bin_n(c, K_LDS), 256, k_filter_block<2048>, s, bin_v(c, K_LDS), bin_n(c, K_LDS)); break;
         case 4096: LAUNCH(c, "filter_block", bin_n(c, K_LDS), 256, k_filter_block<4096>, s, bin_v(c, K_LDS), bin_n(c, K_LDS)); break;
@@ -578,12 +631,14 @@ static int phase_stats(scd_ctx *c) {
         default: LAUNCH(c, "filter_block", bin_n(c, K_LDS), 256, k_filter_block<16384>, s, bin_v(c, K_LDS), bin_n(c, K_LDS)); break;
     }
     if (c->nh) {
+        on_lane(c, 1);
         const uint32_t C = c->tune.merge_chunk;
         LAUNCH(c, "hub_filter_count", c->n_chunks, 256, k_hub_filter_count, s, c->hub_v, c->hub_d,
                c->chunk_hub, c->chunk_start, C, c->chunk_cnt);
         LAUNCH(c, "hub_filter_write", c->n_chunks, 256, k_hub_filter_write, s, c->hub_v, c->hub_d,
                c->hub_chunk_begin, c->chunk_hub, c->chunk_start, C, c->chunk_cnt);
     }
+    join_lanes(c);
     LAUNCH(c, "stats_agg", std::min<uint32_t>(cdiv(c->n, 1024), 2u * c->num_cus), 1024, k_stats_agg, c->indices,
            c->cur, c->int_deg, c->n, c->G ? c->gid : nullptr, c->active_g, c->stats);
     return check_launch("stats");
@@ -615,6 +670,9 @@ static int scan_u32_dev(scd_ctx *c, const uint32_t *in, uint32_t *out, uint64_t 
 // kernels per degree bin and the work queue of long out-lists.
 static int count_internal_triangles(scd_ctx *c, const WccArgs &w) {
     LAUNCH(c, "wcc_dplus", cdiv(c->n, 256), 256, k_wtc_dplus, w, c->n);
+    // lanes: 3 small-degree groups | 2 out-lists up to 64 | 0 work queue
+    fork_lanes(c);
+    on_lane(c, 3);
     LAUNCH(c, "wcc_tri_sub8", cdiv(bin_n(c, K_SUB8), 32), 256, k_wtc_group<8>, w, bin_v(c, K_SUB8),
            bin_n(c, K_SUB8));
     LAUNCH(c, "wcc_tri_sub16", cdiv(bin_n(c, K_SUB16), 16), 256, k_wtc_group<16>, w, bin_v(c, K_SUB16),
@@ -623,21 +681,31 @@ static int count_internal_triangles(scd_ctx *c, const WccArgs &w) {
            bin_n(c, K_SUB32));
     if (c->wq_M) {
         const uint32_t M = c->wq_M;
+        on_lane(c, 2);
+        LAUNCH(c, "wcc_tri_wave", cdiv(M, 4), 256, k_wtc_group<64>, w, bin_v(c, K_W1), M);
+        on_lane(c, 0);
         LAUNCH(c, "wcc_tri_chunks", cdiv(M, 256), 256, k_wtc_count_chunks, w.dplus, bin_v(c, K_W1), M, c->wq_nch,
                c->wq_next);
-        LAUNCH(c, "wcc_tri_wave", cdiv(M, 4), 256, k_wtc_group<64>, w, bin_v(c, K_W1), M);
         const int rc = scan_u32_dev(c, c->wq_nch, c->wq_off, M, c->wq_tot, 0);
         if (rc) return rc;
         LAUNCH(c, "wcc_tri_queue", 4 * c->num_cus, WTC_QT, (k_wtc_queue<2048, WTC_QT, 4>), w, bin_v(c, K_W1), M,
                c->wq_off, c->wq_tot, c->wq_next);
     }
+    join_lanes(c);
+    // edge marks are complete: count them, again on lanes
+    fork_lanes(c);
+    on_lane(c, 3);
     LAUNCH(c, "wcc_tri_final", cdiv(bin_n(c, K_SUB8), 32), 256, k_wtc_final<8>, w, bin_v(c, K_SUB8),
            bin_n(c, K_SUB8));
+    on_lane(c, 2);
     LAUNCH(c, "wcc_tri_final", cdiv(bin_n(c, K_SUB16), 16), 256, k_wtc_final<16>, w, bin_v(c, K_SUB16),
            bin_n(c, K_SUB16));
+    on_lane(c, 1);
     LAUNCH(c, "wcc_tri_final", cdiv(bin_n(c, K_SUB32), 8), 256, k_wtc_final<32>, w, bin_v(c, K_SUB32),
            bin_n(c, K_SUB32));
+    on_lane(c, 0);
     LAUNCH(c, "wcc_tri_final", cdiv(c->wq_M, 4), 256, k_wtc_final<64>, w, bin_v(c, K_W1), c->wq_M);
+    join_lanes(c);
     LAUNCH(c, "wcc_tri_hot", cdiv(w.hot_n, 256), 256, k_wtc_hot_sum, w);
     return SCD_OK;
 }
@@ -730,6 +798,17 @@ extern "C" int scd_init(int device_id, int work_group_size, int profile, scd_ctx
         delete c;
         return fail(SCD_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
     }
+    c->ls = c->stream;
+    {
+        bool ok = hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) == hipSuccess;
+        for (int i = 0; i < 3 && ok; i++)
+            ok = hipStreamCreateWithFlags(&c->lane[i], hipStreamNonBlocking) == hipSuccess &&
+                 hipEventCreateWithFlags(&c->join_ev[i], hipEventDisableTiming) == hipSuccess;
+        if (!ok) {  // run everything on the one stream
+            (void)hipGetLastError();
+            c->use_lanes = false;
+        }
+    }
     e = hipHostMalloc((void **)&c->wcc_host, 256, hipHostMallocDefault);
     if (e != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
@@ -749,6 +828,11 @@ extern "C" void scd_destroy(scd_ctx *c) {
     free_graph(c, false);
     free_list(c, c->cache);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    for (int i = 0; i < 3; i++) {
+        if (c->lane[i]) (void)hipStreamDestroy(c->lane[i]);
+        if (c->join_ev[i]) (void)hipEventDestroy(c->join_ev[i]);
+    }
+    if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
     (void)hipHostFree(c->wcc_host);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1128,7 +1212,11 @@ extern "C" int scd_set_tuning(scd_ctx *c, const scd_tuning *t) {
     if (W != 256 && W != 512 && W != 1024)
         return fail(SCD_EINVAL, "scd_set_tuning: wave_max %u is not 256, 512 or 1024", W);
     if (t->hub_table > 1) return fail(SCD_EINVAL, "scd_set_tuning: hub_table %u is not 0 or 1", t->hub_table);
-    if (c->tune.hub_table != t->hub_table) drop_graphs(c);
+    if (t->lds_hash > 1 || (t->lds_hash && T > 4096))
+        return fail(SCD_EINVAL, "scd_set_tuning: lds_hash %u needs lds_tile <= 4096", t->lds_hash);
+    if (t->lanes > 1) return fail(SCD_EINVAL, "scd_set_tuning: lanes %u is not 0 or 1", t->lanes);
+    if (c->tune.hub_table != t->hub_table || c->tune.lds_hash != t->lds_hash || c->tune.lanes != t->lanes)
+        drop_graphs(c);
     c->tune = *t;
     c->tune.wave_max = W;
     if (c->have_graph) {
@@ -1742,12 +1830,12 @@ bool read_tune_file(const char *path, const std::string &key, TuneEntry &out) {
     FILE *f = fopen(path, "r");
     if (!f) return false;
     char k[256];
-    unsigned T, C, W, H;
+    unsigned T, C, W, H, Lh, N;
     double ms;
     bool found = false;
-    while (fscanf(f, "%255s %u %u %u %u %lf", k, &T, &C, &W, &H, &ms) == 6) {
+    while (fscanf(f, "%255s %u %u %u %u %u %u %lf", k, &T, &C, &W, &H, &Lh, &N, &ms) == 8) {
         if (key == k) {  // the last line of a class wins
-            out.t = scd_tuning{T, C, W, H, {0, 0, 0, 0}};
+            out.t = scd_tuning{T, C, W, H, Lh, N, {0, 0}};
             out.ms = ms;
             found = true;
         }
@@ -1804,8 +1892,14 @@ extern "C" int scd_autotune(scd_ctx *c, const char *cache_path, int reps, scd_tu
                 Cs = {1024, 2048, 4096};
                 Hs = {0, 1};
             }
+            std::vector<uint32_t> Ls = {0};
+            if (T <= 4096 && dmax > W) Ls = {0, 1};
+            std::vector<uint32_t> Ns = {1};
+            if (c->E >= (1ull << 20)) Ns = {0, 1};
             for (uint32_t C : Cs)
-                for (uint32_t H : Hs) cand.push_back(scd_tuning{T, C, W, H, {0, 0, 0, 0}});
+                for (uint32_t H : Hs)
+                    for (uint32_t Lh : Ls)
+                        for (uint32_t N : Ns) cand.push_back(scd_tuning{T, C, W, H, Lh, N, {0, 0}});
         }
 
     // the labels, statistics and best labels to start every timed iteration from
@@ -1871,8 +1965,8 @@ extern "C" int scd_autotune(scd_ctx *c, const char *cache_path, int reps, scd_tu
     if (cache_path) {
         FILE *f = fopen(cache_path, "a");
         if (!f) return fail(SCD_EIO, "scd_autotune: cannot append to %s", cache_path);
-        fprintf(f, "%s %u %u %u %u %.6f\n", key, best_t.lds_tile, best_t.merge_chunk, best_t.wave_max,
-                best_t.hub_table, best_ms);
+        fprintf(f, "%s %u %u %u %u %u %u %.6f\n", key, best_t.lds_tile, best_t.merge_chunk, best_t.wave_max,
+                best_t.hub_table, best_t.lds_hash, best_t.lanes, best_ms);
         fclose(f);
     }
     if (chosen) *chosen = best_t;

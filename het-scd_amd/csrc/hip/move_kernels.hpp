@@ -355,6 +355,99 @@ __global__ __launch_bounds__(256) void k_move_lds(MoveArgs a, const uint32_t *__
 }
 
 // ---------------------------------------------------------------------------
+// LDS bin without export, by hashing instead of sorting: the candidates are
+// the distinct valid neighbour labels with their frequencies and the argmax
+// order is total, so a (label, count) table in LDS replaces the sort (a few
+// instructions per neighbour instead of the bitonic stages).  One workgroup
+// per vertex, table of next_pow2(2 deg) <= 2T slots, linear probing.
+// ---------------------------------------------------------------------------
+template <int T>
+__global__ __launch_bounds__(256) void k_move_lds_hash(MoveArgs a, const uint32_t *__restrict__ verts,
+                                                       uint32_t nv) {
+    __shared__ uint32_t s_key[2 * T], s_cnt[2 * T];
+    __shared__ double s_bi[4];
+    __shared__ uint32_t s_bl[4];
+    __shared__ double s_rem;
+    __shared__ int s_has_rem;
+    const int tid = threadIdx.x;
+    const uint32_t p = verts[blockIdx.x];
+    const uint32_t my = a.labels[p];
+    if (vtx_stopped(a, p)) {  // uniform per block, before any barrier
+        if (tid == 0 && a.labels_out) a.labels_out[p] = my;
+        return;
+    }
+    const uint32_t beg = a.indices[p];
+    const uint32_t deg = a.indices[p + 1] - beg;
+    const uint32_t cap = min(next_pow2(2 * deg), 2u * T), mask = cap - 1u;
+    for (uint32_t i = tid; i < cap; i += 256) {
+        s_key[i] = SCD_INVALID;
+        s_cnt[i] = 0;
+    }
+    if (tid == 0) s_has_rem = 0;
+    constexpr int U = T / 256;
+    uint32_t q[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const uint32_t i = u * 256 + tid;
+        q[u] = i < deg ? a.adj[beg + i] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) q[u] = (u * 256 + tid) < deg ? a.labels[q[u]] : SCD_INVALID;
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const uint32_t L = q[u];
+        if (L == SCD_INVALID) continue;
+        for (uint32_t h = (L * 0x9E3779B1u) & mask;; h = (h + 1) & mask) {
+            uint32_t k = s_key[h];
+            if (k == SCD_INVALID) k = atomicCAS(&s_key[h], SCD_INVALID, L);
+            if (k == SCD_INVALID || k == L) {
+                atomicAdd(&s_cnt[h], 1u);
+                break;
+            }
+        }
+    }
+    __syncthreads();
+    double bi = 0.0;
+    uint32_t bl = SCD_INVALID;
+    uint32_t heads = 0;
+    const double pext = vtx_pext(a, p);
+    constexpr int BATCH = 4;
+    for (uint32_t base = 0; base < cap; base += 256 * BATCH) {
+        uint32_t v[BATCH], fr[BATCH];
+        int4 st[BATCH];
+#pragma unroll
+        for (int k = 0; k < BATCH; k++) {
+            const uint32_t i = base + k * 256 + tid;
+            v[k] = i < cap ? s_key[i] : SCD_INVALID;
+            fr[k] = v[k] != SCD_INVALID ? s_cnt[i] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < BATCH; k++) st[k] = fr[k] ? a.stats[v[k]] : make_int4(0, 0, 0, 0);
+#pragma unroll
+        for (int k = 0; k < BATCH; k++) {
+            if (!fr[k]) continue;
+            const double imp = candidate_improvement_s(st[k], v[k], fr[k], deg, my, pext);
+            if (v[k] == my) {
+                s_rem = -imp;
+                s_has_rem = 1;
+            }
+            if (imp > 0.0 && better(imp, v[k], bi, bl)) {
+                bi = imp;
+                bl = v[k];
+            }
+            heads++;
+        }
+    }
+    count_pairs(a.counters, heads);
+    block_argmax256(bi, bl, s_bi, s_bl);
+    if (tid == 0 && a.labels_out) {
+        const double rem = s_has_rem ? s_rem : 0.0;
+        a.labels_out[p] = vtx_result(a, p, move_rule(bi, bl, rem, my), my);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Hub path (deg > T).  Hub h has vertex hub_v[h], degree hub_d[h] and a
 // scratch segment at hub_off[h] in two ping-pong buffers.  Work items:
 //   tiles  (tile_hub[t], tile_start[t]): T keys each, sorted in LDS;

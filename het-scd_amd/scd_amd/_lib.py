@@ -65,7 +65,8 @@ class ScdEdgesC(ctypes.Structure):
 
 class ScdTuningC(ctypes.Structure):
     _fields_ = [("lds_tile", ctypes.c_uint32), ("merge_chunk", ctypes.c_uint32), ("wave_max", ctypes.c_uint32),
-                ("hub_table", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 4)]
+                ("hub_table", ctypes.c_uint32), ("lds_hash", ctypes.c_uint32), ("lanes", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 2)]
 
 
 def host():

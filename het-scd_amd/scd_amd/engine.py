@@ -25,10 +25,11 @@ def device_count():
 def _get_tuning(ctx):
     t = _lib.ScdTuningC()
     check_hip(_lib.hip().scd_get_tuning(ctx, ctypes.byref(t)))
-    return {"lds_tile": t.lds_tile, "merge_chunk": t.merge_chunk, "wave_max": t.wave_max, "hub_table": t.hub_table}
+    return {"lds_tile": t.lds_tile, "merge_chunk": t.merge_chunk, "wave_max": t.wave_max, "hub_table": t.hub_table,
+            "lds_hash": t.lds_hash, "lanes": t.lanes}
 
 
-def _set_tuning(ctx, lds_tile=None, merge_chunk=None, wave_max=None, hub_table=None):
+def _set_tuning(ctx, lds_tile=None, merge_chunk=None, wave_max=None, hub_table=None, lds_hash=None, lanes=None):
     t = _lib.ScdTuningC()
     check_hip(_lib.hip().scd_get_tuning(ctx, ctypes.byref(t)))
     if lds_tile is not None:
@@ -39,6 +40,10 @@ def _set_tuning(ctx, lds_tile=None, merge_chunk=None, wave_max=None, hub_table=N
         t.wave_max = wave_max
     if hub_table is not None:
         t.hub_table = hub_table
+    if lds_hash is not None:
+        t.lds_hash = lds_hash
+    if lanes is not None:
+        t.lanes = lanes
     check_hip(_lib.hip().scd_set_tuning(ctx, ctypes.byref(t)))
 
 
@@ -48,7 +53,7 @@ def _autotune(ctx, cache_path, reps):
     path = None if cache_path is None else str(cache_path).encode()
     check_hip(_lib.hip().scd_autotune(ctx, path, reps, ctypes.byref(t), ctypes.byref(ms)))
     return {"lds_tile": t.lds_tile, "merge_chunk": t.merge_chunk, "wave_max": t.wave_max,
-            "hub_table": t.hub_table}, ms.value
+            "hub_table": t.hub_table, "lds_hash": t.lds_hash, "lanes": t.lanes}, ms.value
 
 
 class Engine:
@@ -168,8 +173,9 @@ class Engine:
     def get_tuning(self):
         return _get_tuning(self._ctx)
 
-    def set_tuning(self, lds_tile=None, merge_chunk=None, wave_max=None, hub_table=None):
-        _set_tuning(self._ctx, lds_tile, merge_chunk, wave_max, hub_table)
+    def set_tuning(self, lds_tile=None, merge_chunk=None, wave_max=None, hub_table=None, lds_hash=None,
+                   lanes=None):
+        _set_tuning(self._ctx, lds_tile, merge_chunk, wave_max, hub_table, lds_hash, lanes)
 
     def autotune(self, cache_path=None, reps=3):
         """scd_autotune: fastest tuning for this graph's size class (cached);
@@ -296,8 +302,9 @@ class BatchEngine:
     def get_tuning(self):
         return _get_tuning(self._ctx)
 
-    def set_tuning(self, lds_tile=None, merge_chunk=None, wave_max=None, hub_table=None):
-        _set_tuning(self._ctx, lds_tile, merge_chunk, wave_max, hub_table)
+    def set_tuning(self, lds_tile=None, merge_chunk=None, wave_max=None, hub_table=None, lds_hash=None,
+                   lanes=None):
+        _set_tuning(self._ctx, lds_tile, merge_chunk, wave_max, hub_table, lds_hash, lanes)
 
     def current_labels(self):
         lab = np.empty(self.arena.graph.n, np.uint32)

@@ -177,15 +177,22 @@ int scd_neighbor_label_pairs(scd_ctx *ctx, uint32_t *sorted_out, uint32_t *run_o
  *                take one workgroup each.
  *   hub_table:   0: hub candidates by merge passes over sorted tiles; 1: by
  *                per-hub hash tables of (label, count) filled from sorted
- *                tiles (no merge passes).  Export (scd_neighbor_label_pairs)
- *                always sorts.
+ *                tiles (no merge passes).
+ *   lds_hash:    0: the lds bin sorts each vertex's neighbour labels; 1: it
+ *                counts them in an LDS hash table (lds_tile 2048 or 4096).
+ *   Export (scd_neighbor_label_pairs) always sorts.
+ *   lanes:       1: the per-bin kernels of a phase run concurrently on four
+ *                streams (graphs of at least 2^20 adjacency entries); 0: one
+ *                stream.
  */
 typedef struct scd_tuning {
     uint32_t lds_tile;
     uint32_t merge_chunk;
     uint32_t wave_max;
     uint32_t hub_table;
-    uint32_t reserved[4];
+    uint32_t lds_hash;
+    uint32_t lanes;
+    uint32_t reserved[2];
 } scd_tuning;
 
 int scd_get_tuning(const scd_ctx *ctx, scd_tuning *out);

@@ -147,16 +147,18 @@ def test_tuning_invariance(engine, manifest):
     assert g.degree().max() > 2048
     outs = []
     engine.set_counting(True)
-    for tile, chunk, wmax, table in ((2048, 256, 1024, 0), (2048, 4096, 256, 1), (4096, 1024, 512, 1),
-                                     (16384, 2048, 1024, 0), (8192, 2048, 256, 0), (2048, 1024, 512, 1)):
+    for tile, chunk, wmax, table, lh, ln in ((2048, 256, 1024, 0, 0, 1), (2048, 4096, 256, 1, 1, 0),
+                                             (4096, 1024, 512, 1, 0, 1), (16384, 2048, 1024, 0, 0, 0),
+                                             (8192, 2048, 256, 0, 0, 1), (2048, 1024, 512, 1, 1, 1),
+                                             (4096, 2048, 256, 0, 1, 0)):
         engine.transfer_graph(g)
-        engine.set_tuning(lds_tile=tile, merge_chunk=chunk, wave_max=wmax, hub_table=table)
+        engine.set_tuning(lds_tile=tile, merge_chunk=chunk, wave_max=wmax, hub_table=table, lds_hash=lh, lanes=ln)
         engine.labels_to_device(initial_labels(name, ent, g))
         wcc, lab = engine.improve_labels(return_labels=True)
         hubs = engine.last_counters()["hubs"]
         assert (hubs > 0) == (tile < g.degree().max())
         outs.append((wcc, lab))
-    engine.set_tuning(lds_tile=4096, merge_chunk=2048, wave_max=1024, hub_table=0)
+    engine.set_tuning(lds_tile=4096, merge_chunk=2048, wave_max=1024, hub_table=0, lds_hash=0, lanes=1)
     engine.set_counting(False)
     for wcc, lab in outs:
         assert wcc == outs[0][0]
@@ -189,7 +191,7 @@ def test_graph_replays_match_eager(engine, manifest, name, hub_table):
     ent = manifest["graphs"][name]
     g = graph(name)
     engine.transfer_graph(g)
-    engine.set_tuning(lds_tile=2048 if hub_table else 4096, hub_table=hub_table)
+    engine.set_tuning(lds_tile=2048 if hub_table else 4096, hub_table=hub_table, lds_hash=hub_table)
     engine.labels_to_device(initial_labels(name, ent, g))
     engine.snapshot()
     engine.set_graph_mode(False)
@@ -205,7 +207,7 @@ def test_graph_replays_match_eager(engine, manifest, name, hub_table):
             assert np.array_equal(lab, eager[it][1]), (rep, it)
             s, e, i = engine.stats()
             assert s.sum() == (lab != INVALID).sum()
-    engine.set_tuning(lds_tile=4096, hub_table=0)
+    engine.set_tuning(lds_tile=4096, hub_table=0, lds_hash=0)
 
 
 def _steps(engine, g, lab0, k, full):

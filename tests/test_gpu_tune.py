@@ -8,7 +8,7 @@ from golden_graphs import graph
 
 pytestmark = pytest.mark.gpu
 
-DEFAULT = {"lds_tile": 4096, "merge_chunk": 2048, "wave_max": 1024, "hub_table": 0}
+DEFAULT = {"lds_tile": 4096, "merge_chunk": 2048, "wave_max": 1024, "hub_table": 0, "lds_hash": 0, "lanes": 1}
 
 
 def refine_all(eng):
@@ -52,7 +52,7 @@ def test_cache_file_is_reused(tmp_path):
     # too, so write a different choice for the class and read it back
     key = lines[0].split()[0]
     with open(path, "a") as f:
-        f.write(f"{key} 2048 1024 256 1 1.5\n")
+        f.write(f"{key} 2048 1024 256 1 1 0 1.5\n")
     with scd_amd.Engine(device=0) as e2:
         e2.transfer_graph(g)
         e2.initial_partition(want_labels=False)
