@@ -61,6 +61,8 @@ struct Req {
 };
 
 const double kThreshold = 0.01;  // src/main-gpu.cpp:88, stop below best * (1 + 0.01)
+const size_t kIntakeGraphs = 256;         // graphs per intake batch
+const uint64_t kIntakeBytes = 8ull << 30;  // footprint per intake batch
 
 }  // namespace
 
@@ -76,8 +78,9 @@ struct scd_server {
     scd_server_stats st{};
     std::chrono::steady_clock::time_point t0;
 
-    // uploads run on their own thread and stream, overlapping the
-    // refinement on the device; the main loop drains finished ones
+    // uploads and preparation run on the intake thread (own stream, the
+    // preparation context), overlapping the refinement on the device; the
+    // main loop drains finished ones
     std::thread up_thread;
     std::mutex mu;
     std::condition_variable cv, cv_done;
@@ -85,8 +88,12 @@ struct scd_server {
     std::vector<std::pair<size_t, int>> up_done;  // (request, status) (guarded by mu)
     bool up_stop = false;                        // guarded by mu
     hipStream_t up_stream = nullptr;
+    // with graph_mode the main context captures iterations, and device
+    // allocations or copies issued by another thread during a capture make
+    // it fail: then intake work and iterations exclude each other
+    std::mutex capture_mu;
     size_t uploading = 0;  // main thread only
-    double upload_ms = 0;  // uploader thread only
+    double upload_ms = 0, prep_ms = 0;  // intake thread only
 
     double now() const {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -214,24 +221,78 @@ static int admit(scd_server *s, Req &r) {
     return SCD_OK;
 }
 
-// The uploader thread: device buffers and copies of each admitted request.
-static void uploader(scd_server *s) {
+static scd_part part_of(const Req &r);
+
+// Initial partition and initial WCC of a batch of resident graphs in the
+// preparation context; their labels go to their own buffers.
+static int prepare(scd_server *s, const std::vector<size_t> &batch) {
+    const uint32_t P = (uint32_t)batch.size();
+    std::vector<scd_part> parts(P);
+    std::vector<uint32_t *> cur(P), best(P);
+    for (uint32_t i = 0; i < P; i++) {
+        const Req &r = s->reqs[batch[i]];
+        parts[i] = part_of(r);
+        cur[i] = r.d_cur;
+        best[i] = r.d_best;
+    }
+    std::vector<double> w0(P);
+    int rc;
+    if ((rc = scd_batch_upload_parts(s->prep, P, parts.data(), nullptr, nullptr)) ||
+        (rc = scd_batch_initial_partition(s->prep, nullptr, nullptr)) ||
+        (rc = scd_batch_compute_wcc(s->prep, w0.data())) ||
+        (rc = scd_batch_export_labels(s->prep, cur.data(), best.data())))
+        return engine_fail(rc, "initial partition of joining graphs");
+    for (uint32_t i = 0; i < P; i++) {
+        Req &r = s->reqs[batch[i]];
+        r.best = w0[i];
+        r.res.initial_wcc = w0[i];
+    }
+    return SCD_OK;
+}
+
+// The intake thread: takes every admitted request queued so far, copies
+// each to the device (its own stream), prepares them together in the
+// preparation context, and hands them to the serving loop — all of it
+// overlapping the refinement of the joined graphs on the device.
+static void intake(scd_server *s) {
     (void)hipSetDevice(s->o.device);
     while (true) {
-        size_t i;
+        std::vector<size_t> batch;
         {
             std::unique_lock<std::mutex> lk(s->mu);
             s->cv.wait(lk, [&] { return s->up_stop || !s->up_queue.empty(); });
             if (s->up_queue.empty()) return;  // stop requested, nothing left
-            i = s->up_queue.front();
-            s->up_queue.pop_front();
+            // bounded batches: the first graphs start refining while later
+            // ones are still being copied and prepared (each batch costs one
+            // preparation and one arena assembly, so batches stay large)
+            uint64_t bytes = 0;
+            while (!s->up_queue.empty() && batch.size() < kIntakeGraphs && bytes < kIntakeBytes) {
+                bytes += s->reqs[s->up_queue.front()].bytes;
+                batch.push_back(s->up_queue.front());
+                s->up_queue.pop_front();
+            }
         }
-        const auto t = std::chrono::steady_clock::now();
-        const int rc = admit(s, s->reqs[i]);
-        s->upload_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+        std::vector<std::pair<size_t, int>> done;
+        std::vector<size_t> ok;
+        std::unique_lock<std::mutex> cap(s->capture_mu, std::defer_lock);
+        if (s->o.graph_mode) cap.lock();
+        auto t = std::chrono::steady_clock::now();
+        for (size_t i : batch) {
+            const int rc = admit(s, s->reqs[i]);
+            if (rc) done.emplace_back(i, rc);
+            else ok.push_back(i);
+        }
+        auto t2 = std::chrono::steady_clock::now();
+        s->upload_ms += std::chrono::duration<double, std::milli>(t2 - t).count();
+        if (!ok.empty()) {
+            const int rc = prepare(s, ok);
+            s->prep_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t2).count();
+            for (size_t i : ok) done.emplace_back(i, rc);
+        }
+        if (cap.owns_lock()) cap.unlock();
         {
             std::lock_guard<std::mutex> lk(s->mu);
-            s->up_done.emplace_back(i, rc);
+            s->up_done.insert(s->up_done.end(), done.begin(), done.end());
         }
         s->cv_done.notify_one();
     }
@@ -262,31 +323,9 @@ static scd_part part_of(const Req &r) {
 // Initial labels of the waiting graphs, then a new main arena from the
 // still-refining graphs (labels carried over) and the waiting ones.
 static int join(scd_server *s) {
-    const uint32_t P = (uint32_t)s->waiting.size();
-    std::vector<scd_part> parts(P);
-    std::vector<uint32_t *> cur(P), best(P);
-    for (uint32_t i = 0; i < P; i++) {
-        const Req &r = s->reqs[s->waiting[i]];
-        parts[i] = part_of(r);
-        cur[i] = r.d_cur;
-        best[i] = r.d_best;
-    }
-    std::vector<double> w0(P);
     int rc;
-    const double tp = s->now();
-    if ((rc = scd_batch_upload_parts(s->prep, P, parts.data(), nullptr, nullptr)) ||
-        (rc = scd_batch_initial_partition(s->prep, nullptr, nullptr)) ||
-        (rc = scd_batch_compute_wcc(s->prep, w0.data())) ||
-        (rc = scd_batch_export_labels(s->prep, cur.data(), best.data())))
-        return engine_fail(rc, "initial partition of joining graphs");
     const double t = s->now();
-    s->st.prep_ms += t - tp;
-    for (uint32_t i = 0; i < P; i++) {
-        Req &r = s->reqs[s->waiting[i]];
-        r.best = w0[i];
-        r.res.initial_wcc = w0[i];
-        r.res.start_ms = t;
-    }
+    for (size_t i : s->waiting) s->reqs[i].res.start_ms = t;
     // carry the refining graphs' labels out of the old arena
     if (!s->arena.empty()) {
         const size_t G = s->arena.size();
@@ -371,7 +410,8 @@ static int step(scd_server *s) {
     return SCD_OK;
 }
 
-// Collects finished uploads; waits for one when block is set.
+// Collects requests the intake thread finished (uploaded and prepared, or
+// failed); waits for some when block is set.
 static void drain_uploads(scd_server *s, bool block) {
     std::vector<std::pair<size_t, int>> done;
     {
@@ -431,6 +471,8 @@ static int serve_loop(scd_server *s, const std::vector<size_t> &order) {
             const double oldest = s->reqs[s->waiting.front()].in.arrival_ms;
             if (s->n_active == 0 || now - oldest >= s->o.join_wait_ms || 2 * s->n_active <= s->arena.size()) {
                 const double t = s->now();
+                std::unique_lock<std::mutex> cap(s->capture_mu, std::defer_lock);
+                if (s->o.graph_mode) cap.lock();
                 const int rc = join(s);
                 s->st.busy_ms += s->now() - t;
                 if (rc) return rc;
@@ -448,7 +490,12 @@ static int serve_loop(scd_server *s, const std::vector<size_t> &order) {
             continue;
         }
         const double t = s->now();
-        const int rc = step(s);
+        int rc;
+        {
+            std::unique_lock<std::mutex> cap(s->capture_mu, std::defer_lock);
+            if (s->o.graph_mode) cap.lock();
+            rc = step(s);
+        }
         s->st.busy_ms += s->now() - t;
         s->st.step_ms += s->now() - t;
         if (rc) return rc;
@@ -466,10 +513,12 @@ extern "C" int scd_server_run(scd_server *s) {
     s->t0 = std::chrono::steady_clock::now();
     s->up_stop = false;
     s->upload_ms = 0;
-    s->up_thread = std::thread(uploader, s);
+    s->prep_ms = 0;
+    s->up_thread = std::thread(intake, s);
     const int rc = serve_loop(s, order);
     stop_uploader(s);
     s->st.admit_ms += s->upload_ms;
+    s->st.prep_ms += s->prep_ms;
     if (rc) {  // requests left mid-way keep their device buffers until destroy
         drain_uploads(s, false);
         return rc;
