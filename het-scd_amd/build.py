@@ -56,7 +56,7 @@ def src(*parts):
 
 
 def build_host():
-    sources = [src("host", f) for f in ("graph_io.cpp", "gen.cpp")]
+    sources = [src("host", f) for f in ("graph_io.cpp", "gen.cpp", "labels_io.cpp")]
     run([CXX, *HOST_FLAGS, "-shared", "-o", os.path.join(LIB, "libscd_host.so"), *sources])
 
 

@@ -3,9 +3,12 @@
 // The reference's single-GPU driver (src/main-gpu.cpp:30-113) on this engine:
 // same arguments, same stdout lines and loop rule.  The initial partition and
 // the initial WCC run on the device (the reference computes them on the host,
-// src/main-gpu.cpp:54-55; the labels are identical).  With the environment
-// variable SCD_LABELS_OUT set, the best labels (uint32, n entries) are also
-// written to that file; the reference writes none.
+// src/main-gpu.cpp:54-55; the labels are identical).  Beyond the reference
+// (which writes neither):
+//   SCD_LABELS_OUT=<file>  the best labels as a labels file (scd_labels_write:
+//                          [u32 n][u32 communities][u32 compact labels n])
+//   SCD_RUN_LOG=<file>     a machine-readable run log, one JSON object per
+//                          line: set-up, every iteration, the result.
 #include <cstdio>
 #include <cstdlib>
 #include <iostream>
@@ -36,6 +39,9 @@ static void timer_end() {
     std::cout << t.first << ": " << (now_ms() - t.second) << " ms" << std::endl;
 }
 
+// JSON-lines run log (SCD_RUN_LOG); doubles with 17 significant digits.
+static FILE *g_log = nullptr;
+
 static void must(int rc, const char *what) {
     if (rc != SCD_OK) {
         std::cerr << what << ": " << scd_last_error() << std::endl;
@@ -52,6 +58,14 @@ int main(int argc, char *argv[]) {
     const int work_group_size = argc >= 4 ? atoi(argv[3]) : 256;
     if (!cuda_init(argc >= 3 ? atoi(argv[2]) : -1, work_group_size, true)) return EXIT_FAILURE;
 
+    if (const char *lp = getenv("SCD_RUN_LOG")) {
+        g_log = fopen(lp, "w");
+        if (!g_log) {
+            std::cerr << "failed to open " << lp << std::endl;
+            return EXIT_FAILURE;
+        }
+    }
+    double t_phase = now_ms();
     timer_start("load graph");
     scd_graph *hg = nullptr;
     if (scd_graph_read(argv[1], &hg) != SCD_OK) {
@@ -65,23 +79,36 @@ int main(int argc, char *argv[]) {
     }
     const Graph g(hg->n, hg->m, scd_graph_clustering_coef(hg), hg->tri_deg, hg->tri, hg->indices, hg->adj);
 
+    const double load_ms = now_ms() - t_phase;
+    t_phase = now_ms();
     timer_start("transfer graph to device");
     if (!cuda_transfer_graph(&g, alpha)) return EXIT_FAILURE;
     timer_end();
+    const double transfer_ms = now_ms() - t_phase;
 
     // the engine's context is the shim's; reach it through the C ABI calls
     // that take the labels directly
     const unsigned n = g.n;
     std::vector<unsigned> best_labels(n);
     scd_ctx *ctx = scd_shim_context();
+    t_phase = now_ms();
     timer_start("create initial partition");
     uint32_t k = 0;
     must(scd_initial_partition(ctx, best_labels.data(), &k), "create initial partition");
     timer_end();
+    const double ip_ms = now_ms() - t_phase;
+    t_phase = now_ms();
     timer_start("compute WCC");
     double best_wcc = 0.0;
     must(scd_compute_wcc(ctx, &best_wcc, nullptr, nullptr, nullptr), "compute WCC");
     timer_end();
+    const double wcc_ms = now_ms() - t_phase;
+    if (g_log)
+        fprintf(g_log,
+                "{\"event\": \"setup\", \"graph\": \"%s\", \"nodes\": %u, \"edges\": %u, \"avg_cc\": %.17g, "
+                "\"initial_labels\": %u, \"initial_wcc\": %.17g, \"load_ms\": %.3f, \"transfer_ms\": %.3f, "
+                "\"initial_partition_ms\": %.3f, \"compute_wcc_ms\": %.3f}\n",
+                argv[1], g.n, g.m, g.clustering_coef, k, best_wcc, load_ms, transfer_ms, ip_ms, wcc_ms);
 
     std::cout << std::endl;
     std::cout << "nodes: " << g.n << std::endl;
@@ -95,11 +122,16 @@ int main(int argc, char *argv[]) {
     int iters = 0;
     while (true) {
         iters++;
+        const double t_it = now_ms();
         timer_start("iteration");
         double wcc = 0;
         cuda_improve_labels(&wcc);
-        if (wcc > best_wcc) cuda_store_labels();
+        const bool stored = wcc > best_wcc;
+        if (stored) cuda_store_labels();
         timer_end();
+        if (g_log)
+            fprintf(g_log, "{\"event\": \"iteration\", \"iteration\": %d, \"wcc\": %.17g, \"stored\": %s, \"ms\": %.3f}\n",
+                    iters, wcc, stored ? "true" : "false", now_ms() - t_it);
         std::cout << "WCC: " << wcc << std::endl << std::endl;
         if (wcc < best_wcc * (1 + threshold)) break;
         best_wcc = wcc;
@@ -110,27 +142,32 @@ int main(int argc, char *argv[]) {
     cuda_labels_from_device(best_labels.data());
     timer_end();
 
-    // printed count of the reference's compress_labels (src/scd.cpp:32-55):
-    // n minus the number of distinct valid labels
-    std::vector<char> seen(n, 0);
-    unsigned distinct = 0;
-    for (unsigned v = 0; v < n; v++) {
-        const unsigned l = best_labels[v];
-        if (l != SCD_INVALID_LABEL && l < n && !seen[l]) {
-            seen[l] = 1;
-            distinct++;
+    // the count the reference prints after its compress_labels
+    // (src/scd.cpp:32-55), and the real one
+    const uint32_t printed = scd_labels_printed_count(n, best_labels.data());
+    uint32_t communities = 0;
+    {
+        std::vector<uint32_t> compact(n);
+        if (scd_labels_compact(n, best_labels.data(), compact.data(), &communities) != SCD_OK) {
+            std::cerr << scd_host_last_error() << std::endl;
+            return EXIT_FAILURE;
         }
     }
     if (const char *out = getenv("SCD_LABELS_OUT")) {
-        FILE *f = fopen(out, "wb");
-        if (!f || fwrite(best_labels.data(), 4, n, f) != n) {
-            std::cerr << "failed to write " << out << std::endl;
+        if (scd_labels_write(out, n, best_labels.data()) != SCD_OK) {
+            std::cerr << scd_host_last_error() << std::endl;
             return EXIT_FAILURE;
         }
-        fclose(f);
+    }
+    if (g_log) {
+        fprintf(g_log,
+                "{\"event\": \"done\", \"best_wcc\": %.17g, \"iterations\": %d, \"time_per_iteration_ms\": %.3f, "
+                "\"printed_communities\": %u, \"communities\": %u}\n",
+                best_wcc, iters, (after - before) / iters, printed, communities);
+        fclose(g_log);
     }
     std::cout << "----------------------------------------" << std::endl;
-    std::cout << "num. communities: " << (n - distinct) << std::endl;
+    std::cout << "num. communities: " << printed << std::endl;
     std::cout << "WCC: " << best_wcc << std::endl;
     std::cout << "iterations: " << iters << std::endl;
     std::cout << "time per iteration: " << (after - before) / iters << " ms" << std::endl;

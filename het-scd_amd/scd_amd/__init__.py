@@ -8,5 +8,5 @@ the HIP library is missing or no device is present.
 from ._lib import INVALID_LABEL, ScdError  # noqa: F401
 from .engine import BatchEngine, Engine, device_count  # noqa: F401
 from .server import Server  # noqa: F401
-from .graph import (Arena, Edges, Graph, compress_count, from_edge_pairs, planted_batch,  # noqa: F401
-                    planted_partition, planted_random, rmat)
+from .graph import (Arena, Edges, Graph, compact_labels, compress_count, from_edge_pairs,  # noqa: F401
+                    planted_batch, planted_partition, planted_random, read_labels, rmat, write_labels)

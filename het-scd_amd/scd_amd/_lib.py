@@ -96,6 +96,11 @@ def host():
         "scd_edges_write_text": (ctypes.c_int, [Ed, ctypes.c_char_p]),
         "scd_convert": (ctypes.c_int, [Ed, ctypes.POINTER(G)]),
         "scd_graph_concat": (ctypes.c_int, [ctypes.c_uint32, ctypes.POINTER(G), ctypes.POINTER(G), u64p]),
+        "scd_labels_printed_count": (ctypes.c_uint32, [ctypes.c_uint32, u32p]),
+        "scd_labels_compact": (ctypes.c_int, [ctypes.c_uint32, u32p, u32p, u32p]),
+        "scd_labels_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint32, u32p]),
+        "scd_labels_read": (ctypes.c_int, [ctypes.c_char_p, u32p, u32p, ctypes.POINTER(u32p)]),
+        "scd_labels_free": (None, [u32p]),
         "scd_gen_planted_batch": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                                  ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_double,
                                                  ctypes.POINTER(G), u64p, f64p, u64p]),

@@ -161,10 +161,39 @@ def from_edge_pairs(pairs):
 def compress_count(labels):
     """The community count the reference prints (compress_labels,
     src/scd.cpp:32-55, whose inverted map test makes it n minus the number
-    of distinct valid labels)."""
-    labels = np.asarray(labels, dtype=np.uint32)
-    valid = labels[labels != _lib.INVALID_LABEL]
-    return int(len(labels) - len(np.unique(valid)))
+    of distinct valid labels); scd_labels_printed_count."""
+    labels = np.ascontiguousarray(labels, dtype=np.uint32)
+    return int(_lib.host().scd_labels_printed_count(len(labels), ptr(labels, u32p)))
+
+
+def compact_labels(labels):
+    """The compaction compress_labels intends (scd_labels_compact): distinct
+    valid labels numbered 0.. in order of first appearance, every INVALID
+    vertex a community of its own.  Returns (labels, number of communities)."""
+    labels = np.ascontiguousarray(labels, dtype=np.uint32)
+    out = np.empty_like(labels)
+    k = ctypes.c_uint32()
+    check_host(_lib.host().scd_labels_compact(len(labels), ptr(labels, u32p), ptr(out, u32p), ctypes.byref(k)))
+    return out, k.value
+
+
+def write_labels(path, labels):
+    """Labels file [u32 n][u32 communities][u32 compact labels n]."""
+    labels = np.ascontiguousarray(labels, dtype=np.uint32)
+    check_host(_lib.host().scd_labels_write(str(path).encode(), len(labels), ptr(labels, u32p)))
+
+
+def read_labels(path):
+    """(labels, number of communities) from a labels file."""
+    n = ctypes.c_uint32()
+    k = ctypes.c_uint32()
+    lp = u32p()
+    check_host(_lib.host().scd_labels_read(str(path).encode(), ctypes.byref(n), ctypes.byref(k), ctypes.byref(lp)))
+    try:
+        out = np.ctypeslib.as_array(lp, (n.value,)).copy() if n.value else np.zeros(0, np.uint32)
+    finally:
+        _lib.host().scd_labels_free(lp)
+    return out, k.value
 
 
 class Arena:

@@ -30,7 +30,9 @@
 extern "C" {
 #endif
 
+#ifndef SCD_INVALID_LABEL
 #define SCD_INVALID_LABEL 0xFFFFFFFFu
+#endif
 
 typedef struct scd_ctx scd_ctx;
 

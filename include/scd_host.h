@@ -116,6 +116,27 @@ int scd_gen_planted_batch(uint64_t seed_base, uint32_t count, uint32_t n_min, ui
                           uint32_t k_min, uint32_t k_max, double p_in, double p_out,
                           scd_graph **out, uint64_t *vtx_off, double *cc, uint64_t *seeds);
 
+/*
+ * Community labels (no counterpart in the reference, which writes none).
+ * scd_labels_printed_count: the count the reference prints after
+ * compress_labels (src/scd.cpp:32-55, whose inverted map test makes it n
+ * minus the number of distinct valid labels).
+ * scd_labels_compact: the compaction compress_labels intends: distinct valid
+ * labels numbered 0.. in order of first appearance, every INVALID vertex a
+ * community of its own; *num_communities the number of communities.
+ * Labels file: [u32 n][u32 num_communities][u32 compact labels n],
+ * little-endian; scd_labels_write compacts first, scd_labels_read checks
+ * the size and the range (free with scd_labels_free).
+ */
+#ifndef SCD_INVALID_LABEL
+#define SCD_INVALID_LABEL 0xFFFFFFFFu /* a vertex without a community (src/scd.hpp:23) */
+#endif
+uint32_t scd_labels_printed_count(uint32_t n, const uint32_t *labels);
+int scd_labels_compact(uint32_t n, const uint32_t *labels, uint32_t *out, uint32_t *num_communities);
+int scd_labels_write(const char *path, uint32_t n, const uint32_t *labels);
+int scd_labels_read(const char *path, uint32_t *n_out, uint32_t *k_out, uint32_t **labels_out);
+void scd_labels_free(uint32_t *labels);
+
 #ifdef __cplusplus
 }
 #endif
