@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstring>
 #include <ctime>
+#include <limits>
 #include <map>
 #include <mutex>
 #include <string>
@@ -32,6 +33,7 @@
 #include "ip_kernels.hpp"
 #include "wcc_kernels.hpp"
 #include "arena_kernels.hpp"
+#include "convert_kernels.hpp"
 
 using namespace scd_dev;
 
@@ -1347,7 +1349,10 @@ extern "C" uint64_t scd_device_bytes(const scd_ctx *c) { return c ? c->dev_bytes
 // ---------------------------------------------------------------------------
 // t / vt (remove_edges_without_triangles, src/convert.cpp:271-351): the
 // internal-triangle kernels over the whole adjacency, tri = 2 x triangles.
-static int triangles_dev(scd_ctx *c, int32_t *d_tri, uint32_t *d_td, std::vector<DevBuf> &tmp) {
+// Triangles at every vertex of the context's graph (T), triangle edges at
+// every vertex (td) and the flag of the oriented slot
+// u -> w (w < u) of every edge in a triangle.
+static int triangles_core(scd_ctx *c, uint32_t **T_out, uint32_t *d_td, uint8_t **flag_out, std::vector<DevBuf> &tmp) {
     const uint32_t n = c->n;
     const uint64_t E = c->E;
     uint32_t *dplus = nullptr, *T = nullptr;
@@ -1369,7 +1374,16 @@ static int triangles_dev(scd_ctx *c, int32_t *d_tri, uint32_t *d_td, std::vector
     w.hot_n = c->hot_n;
     LAUNCH(c, "tc_clear", cdiv(E, 256), 256, k_fill_u8, flag, E);
     if ((rc = count_internal_triangles(c, w))) return rc;
-    LAUNCH(c, "tc_double", cdiv(n, 256), 256, k_wtc_double, T, n, d_tri);
+    *T_out = T;
+    if (flag_out) *flag_out = flag;
+    return check_launch("triangles");
+}
+
+static int triangles_dev(scd_ctx *c, int32_t *d_tri, uint32_t *d_td, std::vector<DevBuf> &tmp) {
+    uint32_t *T = nullptr;
+    int rc = triangles_core(c, &T, d_td, nullptr, tmp);
+    if (rc) return rc;
+    LAUNCH(c, "tc_double", cdiv(c->n, 256), 256, k_wtc_double, T, c->n, d_tri);
     return check_launch("triangles");
 }
 
@@ -1396,6 +1410,239 @@ extern "C" int scd_count_triangles(scd_ctx *c, int32_t *tri_out, int32_t *tri_de
     if (!rc) rc = finish(c);
     free_list(c, tmp);
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Convert (src/convert.cpp:466-537) on the device: an edge list becomes the
+// binary graph layout, byte-identical to the host scd_convert, and stays
+// uploaded in the context.
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort of n 64-bit keys with 8-bit digits; digits equal in
+// every key are skipped.  *keys and *alt swap with each pass: the sorted
+// keys end in *keys.
+static int sort_u64(scd_ctx *c, uint64_t **keys, uint64_t **alt, uint64_t n, std::vector<DevBuf> &tmp) {
+    if (n < 2) return SCD_OK;
+    unsigned long long *bits = nullptr;
+    uint32_t *hist = nullptr, *hoff = nullptr;
+    const uint32_t ntiles = cdiv(n, CV_TILE);
+    int rc;
+    if ((rc = dalloc(c, &bits, 2, &tmp)) || (rc = dalloc(c, &hist, 256ull * ntiles, &tmp)) ||
+        (rc = dalloc(c, &hoff, 256ull * ntiles, &tmp)))
+        return rc;
+    LAUNCH(c, "cv_sort_bits", 1, 64, k_cv_bits_init, bits);
+    LAUNCH(c, "cv_sort_bits", cdiv(n, 256), 256, k_cv_bits, *keys, n, bits);
+    if ((rc = check_launch("convert sort"))) return rc;
+    unsigned long long hb[2];
+    HIP_TRY(hipMemcpyAsync(hb, bits, sizeof(hb), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const uint64_t diff = hb[0] ^ hb[1];
+    for (int pass = 0; pass < 8; pass++) {
+        if (((diff >> (8 * pass)) & 0xFFu) == 0) continue;
+        LAUNCH(c, "cv_sort_hist", ntiles, 256, k_cv_hist, *keys, n, 8 * pass, ntiles, hist);
+        if ((rc = check_launch("convert sort"))) return rc;
+        uint32_t total = 0;
+        if ((rc = scan_u32(c, hist, hoff, 256ull * ntiles, &total, tmp))) return rc;
+        LAUNCH(c, "cv_sort_scatter", ntiles, 256, k_cv_scatter, *keys, n, 8 * pass, ntiles, hoff, *alt);
+        std::swap(*keys, *alt);
+    }
+    return check_launch("convert sort");
+}
+
+// CSR (indices of n + 1 entries, adj) of E sorted directed keys over n
+// vertices.  Returns the host copy of indices.
+static int csr_of_keys(scd_ctx *c, const uint64_t *dk, uint64_t E, uint32_t n, uint32_t *indices, uint32_t *adj,
+                       std::vector<uint32_t> &h_idx, std::vector<DevBuf> &tmp) {
+    uint32_t *deg = nullptr;
+    int rc;
+    if ((rc = dalloc(c, &deg, n, &tmp))) return rc;
+    LAUNCH(c, "cv_csr", cdiv(n, 256), 256, k_fill_u32, deg, n);
+    LAUNCH(c, "cv_csr", cdiv(E, 256), 256, k_cv_csr, dk, E, adj, deg);
+    if ((rc = check_launch("convert csr"))) return rc;
+    uint32_t total = 0;
+    if ((rc = scan_u32(c, deg, indices, n, &total, tmp))) return rc;
+    if (total != E) return fail(SCD_EDEVICE, "convert: CSR holds %u of %llu entries", total, (unsigned long long)E);
+    h_idx.resize((size_t)n + 1);
+    h_idx[n] = (uint32_t)E;
+    HIP_TRY(hipMemcpyAsync(indices + n, &h_idx[n], 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(h_idx.data(), indices, 4ull * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SCD_OK;
+}
+
+static int convert_impl(scd_ctx *c, uint64_t count, const uint32_t *u, const uint32_t *v, int on_device,
+                        uint32_t *n_out, uint32_t *m_out, double *cc_out, std::vector<DevBuf> &tmp) {
+    int rc;
+    *n_out = 0;
+    *m_out = 0;
+    *cc_out = std::numeric_limits<double>::quiet_NaN();  // no vertices: 0.0 / 0
+    if (count == 0) return SCD_OK;
+    if (!on_device) {
+        uint32_t *du = nullptr, *dv = nullptr;
+        if ((rc = dalloc(c, &du, count, &tmp)) || (rc = dalloc(c, &dv, count, &tmp))) return rc;
+        HIP_TRY(hipMemcpyAsync(du, u, 4ull * count, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(dv, v, 4ull * count, hipMemcpyHostToDevice, c->stream));
+        u = du;
+        v = dv;
+    }
+    // 1. ids by first appearance (src/convert.cpp:141-170)
+    uint32_t *mx = nullptr, hmx = 0;
+    if ((rc = dalloc(c, &mx, 1, &tmp))) return rc;
+    LAUNCH(c, "cv_ids", 1, 64, k_fill_u32, mx, 1u);
+    LAUNCH(c, "cv_ids", cdiv(count, 256), 256, k_cv_max, u, v, count, mx);
+    if ((rc = check_launch("convert"))) return rc;
+    HIP_TRY(hipMemcpyAsync(&hmx, mx, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const uint64_t nid = (uint64_t)hmx + 1;
+    unsigned long long *first = nullptr;
+    uint32_t *seq = nullptr;
+    if ((rc = dalloc(c, &first, nid, &tmp)) || (rc = dalloc(c, &seq, 2 * count, &tmp))) return rc;
+    LAUNCH(c, "cv_ids", cdiv(nid, 256), 256, k_fill_u64v, first, nid, ~0ull);
+    LAUNCH(c, "cv_ids", cdiv(count, 256), 256, k_cv_first, u, v, count, first);
+    LAUNCH(c, "cv_ids", cdiv(2 * count, 256), 256, k_fill_u32, seq, (uint32_t)(2 * count));
+    LAUNCH(c, "cv_ids", cdiv(nid, 256), 256, k_cv_mark, first, nid, seq);
+    if ((rc = check_launch("convert"))) return rc;
+    uint32_t nv = 0;
+    if ((rc = scan_u32(c, seq, seq, 2 * count, &nv, tmp))) return rc;
+    // 2. undirected keys sorted, loops and duplicates dropped
+    //    (src/convert.cpp:252-268, :501)
+    uint64_t *ka = nullptr, *kb = nullptr;
+    uint32_t *keep = nullptr, *pos = nullptr;
+    if ((rc = dalloc(c, &ka, count, &tmp)) || (rc = dalloc(c, &kb, count, &tmp)) ||
+        (rc = dalloc(c, &keep, count, &tmp)) || (rc = dalloc(c, &pos, count, &tmp)))
+        return rc;
+    LAUNCH(c, "cv_keys", cdiv(count, 256), 256, k_cv_keys, u, v, count, first, seq, ka);
+    if ((rc = sort_u64(c, &ka, &kb, count, tmp))) return rc;
+    LAUNCH(c, "cv_unique", cdiv(count, 256), 256, k_cv_unique, ka, count, keep);
+    if ((rc = check_launch("convert"))) return rc;
+    uint32_t mu = 0;
+    if ((rc = scan_u32(c, keep, pos, count, &mu, tmp))) return rc;
+    if (mu == 0) return SCD_OK;  // loops only: nothing survives
+    // 3. the simple graph in degree order (hubs get the low ids the triangle
+    //    kernels orient toward); both directions sorted: the CSR
+    uint32_t *deg = nullptr, *rank = nullptr, *perm = nullptr;
+    uint64_t *oa = nullptr, *ob = nullptr;
+    if ((rc = dalloc(c, &deg, nv, &tmp)) || (rc = dalloc(c, &rank, nv, &tmp)) || (rc = dalloc(c, &perm, nv, &tmp)) ||
+        (rc = dalloc(c, &oa, nv, &tmp)) || (rc = dalloc(c, &ob, nv, &tmp)))
+        return rc;
+    LAUNCH(c, "cv_degree", cdiv(nv, 256), 256, k_fill_u32, deg, nv);
+    LAUNCH(c, "cv_degree", cdiv(count, 256), 256, k_cv_deg, ka, count, keep, deg);
+    LAUNCH(c, "cv_degree", cdiv(nv, 256), 256, k_cv_rank_keys, deg, nv, oa);
+    if ((rc = sort_u64(c, &oa, &ob, nv, tmp))) return rc;
+    LAUNCH(c, "cv_degree", cdiv(nv, 256), 256, k_cv_rank, oa, nv, rank, perm);
+    const uint64_t E1 = 2ull * mu;
+    uint64_t *da = nullptr, *db = nullptr;
+    if ((rc = dalloc(c, &da, E1, &tmp)) || (rc = dalloc(c, &db, E1, &tmp))) return rc;
+    LAUNCH(c, "cv_directed", cdiv(count, 256), 256, k_cv_directed, ka, count, keep, pos, rank, da);
+    if ((rc = sort_u64(c, &da, &db, E1, tmp))) return rc;
+    if ((rc = alloc_graph(c, nv, mu, 0.0))) return rc;
+    std::vector<uint32_t> h_idx;
+    if ((rc = csr_of_keys(c, da, E1, nv, c->indices, c->adj, h_idx, tmp))) return rc;
+    LAUNCH(c, "cv_csr", cdiv(nv, 256), 256, k_fill_u32, (uint32_t *)c->tri, nv);
+    LAUNCH(c, "cv_csr", cdiv(nv, 256), 256, k_fill_u32, (uint32_t *)c->tri_deg, nv);
+    if ((rc = finish_upload(c, h_idx.data()))) return rc;
+    // 4. triangles per vertex and triangle edges (src/convert.cpp:321-340)
+    uint32_t *T = nullptr, *td = nullptr;
+    uint8_t *flag = nullptr;
+    if ((rc = dalloc(c, &td, nv, &tmp)) || (rc = triangles_core(c, &T, td, &flag, tmp))) return rc;
+    // 5. relabel by (triangle edges, id) descending; vertices without a
+    //    triangle edge and triangle-free edges dropped (src/convert.cpp:354-414)
+    uint32_t *newid = nullptr, *cnt = nullptr;
+    if ((rc = dalloc(c, &newid, nv, &tmp)) || (rc = dalloc(c, &cnt, 1, &tmp))) return rc;
+    LAUNCH(c, "cv_relabel", cdiv(nv, 256), 256, k_cv_order, td, perm, nv, oa);
+    if ((rc = sort_u64(c, &oa, &ob, nv, tmp))) return rc;
+    LAUNCH(c, "cv_relabel", cdiv(nv, 256), 256, k_cv_newid, oa, nv, rank, newid);
+    LAUNCH(c, "cv_relabel", 1, 64, k_fill_u32, cnt, 1u);
+    LAUNCH(c, "cv_relabel", cdiv(nv, 256), 256, k_cv_count_pos, td, nv, cnt);
+    uint32_t *kept = nullptr;
+    if ((rc = dalloc(c, &kept, E1, &tmp))) return rc;
+    LAUNCH(c, "cv_relabel", cdiv(E1, 256), 256, k_cv_flag32, flag, E1, kept);
+    if ((rc = check_launch("convert"))) return rc;
+    uint32_t n_new = 0, mk = 0;
+    HIP_TRY(hipMemcpyAsync(&n_new, cnt, 4, hipMemcpyDeviceToHost, c->stream));
+    if ((rc = scan_u32(c, kept, kept, E1, &mk, tmp))) return rc;  // synchronizes
+    if (n_new == 0 || mk == 0) {
+        free_graph(c, false);
+        return SCD_OK;
+    }
+    const uint64_t E2 = 2ull * mk;
+    uint64_t *fa = nullptr, *fb = nullptr;
+    uint32_t *f_idx = nullptr, *f_adj = nullptr;
+    int32_t *f_tri = nullptr, *f_td = nullptr;
+    if ((rc = dalloc(c, &fa, E2, &tmp)) || (rc = dalloc(c, &fb, E2, &tmp)) ||
+        (rc = dalloc(c, &f_idx, (size_t)n_new + 1, &tmp)) || (rc = dalloc(c, &f_adj, E2, &tmp)) ||
+        (rc = dalloc(c, &f_tri, n_new, &tmp)) || (rc = dalloc(c, &f_td, n_new, &tmp)))
+        return rc;
+    LAUNCH(c, "cv_final", cdiv(E1, 256), 256, k_cv_final_keys, da, flag, E1, newid, kept, fa);
+    LAUNCH(c, "cv_final", cdiv(nv, 256), 256, k_cv_final_tri, T, td, newid, nv, n_new, f_tri, f_td);
+    if ((rc = sort_u64(c, &fa, &fb, E2, tmp))) return rc;
+    if ((rc = csr_of_keys(c, fa, E2, n_new, f_idx, f_adj, h_idx, tmp))) return rc;
+    // clustering coefficient in vertex order (src/graph.cpp:57-68)
+    std::vector<int32_t> h_tri(n_new);
+    HIP_TRY(hipMemcpyAsync(h_tri.data(), f_tri, 4ull * n_new, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double cc = 0.0;
+    for (uint32_t p = 0; p < n_new; p++) {
+        const uint32_t d = h_idx[p + 1] - h_idx[p];
+        if (d > 1) cc += h_tri[p] / (d * (d - 1.0));
+    }
+    cc /= n_new;
+    // 6. the converted graph replaces the intermediate one
+    if ((rc = alloc_graph(c, n_new, mk, cc))) return rc;
+    HIP_TRY(hipMemcpyAsync(c->indices, f_idx, 4ull * (n_new + 1), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->adj, f_adj, 4ull * E2, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->tri, f_tri, 4ull * n_new, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->tri_deg, f_td, 4ull * n_new, hipMemcpyDeviceToDevice, c->stream));
+    if ((rc = finish_upload(c, h_idx.data()))) return rc;
+    *n_out = n_new;
+    *m_out = mk;
+    *cc_out = cc;
+    return finish(c);
+}
+
+extern "C" int scd_convert_edges(scd_ctx *c, uint64_t count, const uint32_t *u, const uint32_t *v,
+                                 int edges_on_device, uint32_t *n_out, uint32_t *m_out, double *cc_out) {
+    if (!c) return fail(SCD_EINVAL, "scd_convert_edges: null context");
+    if (count && (!u || !v)) return fail(SCD_EINVAL, "scd_convert_edges: null edge array");
+    if (count >= (1ull << 31) - (1ull << 16))
+        return fail(SCD_ELIMIT, "scd_convert_edges: %llu edges, at most 2^31 - 2^16 - 1",
+                    (unsigned long long)count);
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    free_graph(c, false);
+    std::vector<DevBuf> tmp;
+    uint32_t n = 0, m = 0;
+    double cc = 0.0;
+    int rc = convert_impl(c, count, u, v, edges_on_device, &n, &m, &cc, tmp);
+    (void)hipStreamSynchronize(c->stream);
+    free_list(c, tmp);
+    if (rc) {
+        free_graph(c);
+        return rc;
+    }
+    if (n_out) *n_out = n;
+    if (m_out) *m_out = m;
+    if (cc_out) *cc_out = cc;
+    return SCD_OK;
+}
+
+extern "C" int scd_download_graph(scd_ctx *c, uint32_t *indices, uint32_t *adj, int32_t *tri, int32_t *tri_deg) {
+    int rc = need_graph(c, "scd_download_graph");
+    if (rc) return rc;
+    if (indices) HIP_TRY(hipMemcpyAsync(indices, c->indices, 4ull * (c->n + 1), hipMemcpyDeviceToHost, c->stream));
+    if (adj && c->E) HIP_TRY(hipMemcpyAsync(adj, c->adj, 4ull * c->E, hipMemcpyDeviceToHost, c->stream));
+    if (tri) HIP_TRY(hipMemcpyAsync(tri, c->tri, 4ull * c->n, hipMemcpyDeviceToHost, c->stream));
+    if (tri_deg) HIP_TRY(hipMemcpyAsync(tri_deg, c->tri_deg, 4ull * c->n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SCD_OK;
+}
+
+extern "C" int scd_graph_shape(const scd_ctx *c, uint32_t *n, uint32_t *m, double *clustering_coef) {
+    if (!c) return fail(SCD_EINVAL, "scd_graph_shape: null context");
+    const bool g = c->have_graph;
+    if (n) *n = g ? c->n : 0;
+    if (m) *m = g ? c->m : 0;
+    if (clustering_coef) *clustering_coef = g ? c->cc : 0.0;
+    return SCD_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -128,6 +128,10 @@ def hip():
         "scd_upload": (ctypes.c_int, [C, ctypes.c_uint32, ctypes.c_uint32, u32p, u32p, i32p, i32p,
                                       ctypes.c_double]),
         "scd_count_triangles": (ctypes.c_int, [C, i32p, i32p, ctypes.c_int]),
+        "scd_convert_edges": (ctypes.c_int, [C, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                             u32p, u32p, f64p]),
+        "scd_download_graph": (ctypes.c_int, [C, u32p, u32p, i32p, i32p]),
+        "scd_graph_shape": (ctypes.c_int, [C, u32p, u32p, f64p]),
         "scd_initial_partition": (ctypes.c_int, [C, u32p, u32p]),
         "scd_labels_to_device": (ctypes.c_int, [C, u32p]),
         "scd_improve": (ctypes.c_int, [C, f64p, u32p]),

@@ -16,6 +16,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import check_hip, f64p, i32p, ptr, u32p, u64p
+from .graph import Graph
 
 
 def device_count():
@@ -61,6 +62,7 @@ class Engine:
         self._ctx = ctypes.c_void_p()
         check_hip(_lib.hip().scd_init(device, work_group_size, 1 if profile else 0, ctypes.byref(self._ctx)))
         self.n = 0
+        self.m = 0
         self.graph = None
 
     def close(self):
@@ -86,6 +88,7 @@ class Engine:
         check_hip(_lib.hip().scd_upload(self._ctx, g.n, g.m, ptr(g.indices, u32p), ptr(g.adj, u32p),
                                         ptr(g.tri, i32p), ptr(g.tri_deg, i32p), cc))
         self.n = g.n
+        self.m = g.m
         self.graph = g
 
     def labels_to_device(self, labels):
@@ -120,6 +123,49 @@ class Engine:
         check_hip(_lib.hip().scd_count_triangles(self._ctx, ptr(tri, i32p), ptr(td, i32p), 1 if replace else 0))
         return tri, td
 
+    def convert(self, edges, download=True):
+        """The convert tool (src/convert.cpp:466-537) on the device for an
+        Edges list: the result replaces the engine's graph, as
+        transfer_graph would.  Returns the converted Graph (downloaded, equal
+        byte for byte to edges.convert()) or, with download=False, its
+        (n, m, clustering_coef)."""
+        return self._convert(len(edges), edges.u.ctypes.data, edges.v.ctypes.data, 0, download)
+
+    def convert_device(self, count, u_ptr, v_ptr, download=False):
+        """As convert, for an edge list already in device memory: u_ptr and
+        v_ptr address count uint32 endpoints each on this engine's device."""
+        return self._convert(count, u_ptr, v_ptr, 1, download)
+
+    def _convert(self, count, u_ptr, v_ptr, on_device, download):
+        n, m, cc = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_double()
+        check_hip(_lib.hip().scd_convert_edges(self._ctx, count, ctypes.c_void_p(u_ptr), ctypes.c_void_p(v_ptr),
+                                               on_device, ctypes.byref(n), ctypes.byref(m), ctypes.byref(cc)))
+        self.n = n.value
+        self.m = m.value
+        self.graph = None
+        if not download:
+            return n.value, m.value, cc.value
+        g = self.download_graph() if n.value else Graph(0, 0, np.zeros(1, np.uint32), np.zeros(0, np.uint32),
+                                                          np.zeros(0, np.int32), np.zeros(0, np.int32))
+        g._cc = cc.value
+        self.graph = g if n.value else None
+        return g
+
+    def download_graph(self):
+        """The engine's graph copied back to the host as a Graph."""
+        n, m, cc = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_double()
+        check_hip(_lib.hip().scd_graph_shape(self._ctx, ctypes.byref(n), ctypes.byref(m), ctypes.byref(cc)))
+        n, m = n.value, m.value
+        idx = np.empty(n + 1, np.uint32)
+        adj = np.empty(2 * m, np.uint32)
+        tri = np.empty(n, np.int32)
+        td = np.empty(n, np.int32)
+        check_hip(_lib.hip().scd_download_graph(self._ctx, ptr(idx, u32p), ptr(adj, u32p), ptr(tri, i32p),
+                                                ptr(td, i32p)))
+        g = Graph(n, m, idx, adj, tri, td)
+        g._cc = cc.value
+        return g
+
     def initial_partition(self, want_labels=True):
         out = np.empty(self.n, np.uint32) if want_labels else None
         k = ctypes.c_uint32()
@@ -149,7 +195,7 @@ class Engine:
 
     def neighbor_label_pairs(self):
         """(sorted[2m], run_offsets[n+1], run_label[P], run_count[P])."""
-        E = 2 * self.graph.m
+        E = 2 * self.m
         srt = np.empty(max(E, 1), np.uint32)
         off = np.empty(self.n + 1, np.uint32)
         rl = np.empty(max(E, 1), np.uint32)

@@ -79,6 +79,31 @@ int scd_upload(scd_ctx *ctx, uint32_t n, uint32_t m, const uint32_t *indices,
 int scd_count_triangles(scd_ctx *ctx, int32_t *tri_out, int32_t *tri_deg_out, int replace);
 
 /*
+ * The convert tool (src/convert.cpp:466-537; scd_convert in scd_host.h is
+ * its host restatement) on the device: an edge list (u[i], v[i]) in file
+ * order becomes the binary graph layout -- ids by first appearance, loops
+ * and duplicate edges dropped, triangles counted, vertices renumbered by
+ * (triangle edges, id) descending, triangle-free edges and vertices without
+ * a triangle edge dropped -- byte-identical to scd_convert.  The result
+ * replaces the context's graph (as scd_upload would), with its clustering
+ * coefficient.  u and v hold count entries each, in host memory
+ * (edges_on_device 0) or device memory of the context's device (1).
+ * n_out, m_out, cc_out (each may be NULL) receive the result's shape and
+ * clustering coefficient.  A result without vertices leaves no graph in the
+ * context and returns SCD_OK with n = m = 0 (cc NaN).  count must stay
+ * below 2^31 - 2^16 (SCD_ELIMIT).
+ */
+int scd_convert_edges(scd_ctx *ctx, uint64_t count, const uint32_t *u, const uint32_t *v, int edges_on_device,
+                      uint32_t *n_out, uint32_t *m_out, double *cc_out);
+
+/* Copies the context's graph to host arrays of n + 1, 2m, n and n entries
+ * (each may be NULL): the layout of class Graph, src/graph.hpp:23-33. */
+int scd_download_graph(scd_ctx *ctx, uint32_t *indices, uint32_t *adj, int32_t *tri, int32_t *tri_deg);
+
+/* Shape and clustering coefficient of the context's graph (0 without one). */
+int scd_graph_shape(const scd_ctx *ctx, uint32_t *n, uint32_t *m, double *clustering_coef);
+
+/*
  * Clustering-coefficient-ordered initial partition on the device
  * (create_initial_partition, src/scd.cpp:113-171): vertices ordered by
  * (tri/(deg*(deg-1.0)), deg, id) descending; in that order an unlabelled
