@@ -326,6 +326,76 @@ def run_served(args, scd_amd, np):
             "results_identical": mismatch == 0}
 
 
+def run_convert(args, scd_amd, np, edges, g):
+    """The convert tool on the device for the same R-MAT edge list: the edge
+    list resident in HBM becomes the converted graph resident in HBM (the
+    rate reported), plus the call from host memory (PCIe included) and the
+    reference's own ./convert (oracle/_ref/convert, built from the reference
+    sources) on the host cores: its "process graph" timer, which spans the
+    same steps (src/convert.cpp:484-527) without file reading and writing."""
+    import ctypes
+    hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+    count = len(edges)
+    nbytes = 4 * count
+    du, dv = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(du), ctypes.c_size_t(nbytes)) or \
+            hip.hipMalloc(ctypes.byref(dv), ctypes.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed for the edge list")
+    try:
+        hip.hipMemcpy(du, ctypes.c_void_p(edges.u.ctypes.data), ctypes.c_size_t(nbytes), 1)
+        hip.hipMemcpy(dv, ctypes.c_void_p(edges.v.ctypes.data), ctypes.c_size_t(nbytes), 1)
+        hip.hipDeviceSynchronize()
+        with scd_amd.Engine(device=0) as ce:
+            ce.convert_device(count, du.value, dv.value)  # warm-up: code objects, first allocations
+            reps = 3
+            times = []
+            for _ in range(reps):
+                t = time.perf_counter()
+                n, m, cc = ce.convert_device(count, du.value, dv.value)
+                times.append(time.perf_counter() - t)
+            ok = ce.download_graph().sha256() == g.sha256()
+            t = time.perf_counter()
+            ce.convert(edges, download=False)
+            host_ms = (time.perf_counter() - t) * 1e3
+            ce.set_profiling(True)
+            ce.profile_reset()
+            ce.convert_device(count, du.value, dv.value)
+            prof = ce.profile()
+    finally:
+        hip.hipFree(du)
+        hip.hipFree(dv)
+    times.sort()
+    ms = times[len(times) // 2] * 1e3
+    kernels = {name: round(kms, 3) for name, (la, kms) in sorted(prof.items(), key=lambda x: -x[1][1]) if la}
+    res = {"workload": f"R-MAT scale {args.scale} edge list, {count} edges -> n={n}, m={m}",
+           "edges": count, "ms": round(ms, 2), "edges_per_s": round(count / (ms * 1e-3), 1),
+           "ms_from_host_edges": round(host_ms, 2), "bytes_equal_to_host_convert": ok,
+           "kernel_ms": kernels, "kernel_ms_total": round(sum(kernels.values()), 3)}
+    log(f"[bench] device convert of {count} edges: {ms:.1f} ms ({count / (ms * 1e-3) / 1e9:.2f} G edges/s), "
+        f"{host_ms:.1f} ms from host memory, output equal: {ok}")
+    ref = os.path.join(ROOT, "oracle", "_ref", "convert")
+    if not args.no_cpu and os.path.exists(ref):
+        with tempfile.TemporaryDirectory() as tmp:
+            txt = os.path.join(tmp, "edges.txt")
+            edges.write_text(txt)
+            out = os.path.join(tmp, "out.bin")
+            cores = cpu_cores()
+            env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="spread", OMP_PLACES="cores")
+            r = subprocess.run([ref, txt, out], capture_output=True, text=True, env=env, timeout=600)
+            pm = re.search(r"process graph: ([0-9.eE+-]+) ms", r.stdout)
+            if r.returncode == 0 and pm:
+                cms = float(pm.group(1))
+                same = hashlib.sha256(open(out, "rb").read()).hexdigest() == g.sha256()
+                res["cpu_reference"] = {"ms": round(cms, 1), "edges_per_s": round(count / (cms * 1e-3), 1),
+                                        "cores": cores, "output_equal": same,
+                                        "sample": "one full ./convert run on the same edge list (text file); "
+                                                  "the process-graph timer"}
+                res["vs_cpu_reference"] = round(cms / ms, 1)
+                log(f"[bench] reference ./convert ({cores} threads): process graph {cms:.0f} ms, "
+                    f"output equal: {same}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -345,6 +415,8 @@ def main():
     ap.add_argument("--served", action="store_true", default=True, help=argparse.SUPPRESS)
     ap.add_argument("--served-graphs", type=int, default=200)
     ap.add_argument("--join-wait-ms", type=float, default=5.0)
+    ap.add_argument("--no-convert", dest="convert", action="store_false",
+                    help="skip the device convert of the same edge list (measured by default)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain copy kernel over MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -359,7 +431,8 @@ def main():
     t = time.perf_counter()
     edges = scd_amd.rmat(args.scale, 16, 0.57, 0.19, 0.19, 1)
     g = edges.convert()
-    del edges
+    if not args.convert:
+        del edges
     log(f"[bench] graph R-MAT s{args.scale}: n={g.n} m={g.m} ({time.perf_counter() - t:.1f} s to generate)")
     graph_sha = g.sha256()
 
@@ -502,6 +575,7 @@ def main():
     eng.close()
     batched = run_batched(args, scd_amd, np) if args.batched else None
     served = run_served(args, scd_amd, np) if args.served else None
+    convert = run_convert(args, scd_amd, np, edges, g) if args.convert else None
     result = {
         "metric": METRIC, "value": round(value, 1), "unit": "edges/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
@@ -520,6 +594,7 @@ def main():
         "triangle_counting": tc,
         "batched": batched,
         "served": served,
+        "convert": convert,
     }
     if args.calibrate_copy:
         with scd_amd.Engine(device=0) as ce:

@@ -24,14 +24,23 @@ namespace scd_dev {
 
 constexpr uint32_t CV_TILE = 4096;  // keys per radix-sort workgroup (256 x 16)
 
-__global__ void k_cv_max(const uint32_t *__restrict__ u, const uint32_t *__restrict__ v, uint64_t count,
-                         uint32_t *__restrict__ mx) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+constexpr uint32_t CV_REDUCE_BLOCKS = 2048;  // grid of the whole-array reductions
+
+// Largest endpoint: a grid-stride pass, one atomic per workgroup.
+__global__ __launch_bounds__(256) void k_cv_max(const uint32_t *__restrict__ u, const uint32_t *__restrict__ v,
+                                                uint64_t count, uint32_t *__restrict__ mx) {
+    __shared__ uint32_t sm[4];
     uint32_t m = 0;
-    if (i < count) m = max(u[i], v[i]);
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (uint64_t)gridDim.x * 256)
+        m = max(m, max(u[i], v[i]));
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor(m, o, SCD_WAVE));
-    if ((threadIdx.x & 63) == 0 && m) atomicMax(mx, m);
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = max(max(sm[0], sm[1]), max(sm[2], sm[3]));
+        if (m) atomicMax(mx, m);
+    }
 }
 
 // first[x] = min over occurrences of 2i (smaller endpoint of edge i) or
@@ -41,8 +50,12 @@ __global__ void k_cv_first(const uint32_t *__restrict__ u, const uint32_t *__res
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     const uint32_t a = min(u[i], v[i]), b = max(u[i], v[i]);
-    atomicMin(&first[a], (unsigned long long)(2 * i));
-    atomicMin(&first[b], (unsigned long long)(2 * i + 1));
+    // a plain read first: hubs occur in many edges, and once an earlier
+    // position is in place the atomic is skipped (a stale read only costs
+    // an atomic that changes nothing)
+    const unsigned long long pa = 2 * i, pb = 2 * i + 1;
+    if (first[a] > pa) atomicMin(&first[a], pa);
+    if (first[b] > pb) atomicMin(&first[b], pb);
 }
 
 // mark[first[x]] = 1 for every vertex that occurs.
@@ -64,19 +77,30 @@ __global__ void k_cv_keys(const uint32_t *__restrict__ u, const uint32_t *__rest
     keys[i] = ((uint64_t)min(x, y) << 32) | max(x, y);
 }
 
-// AND / OR of all keys (digits equal everywhere need no sort pass).
-__global__ void k_cv_bits(const uint64_t *__restrict__ keys, uint64_t n, unsigned long long *__restrict__ bits) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t a = ~0ull, o = 0ull;
-    if (i < n) a = o = keys[i];
+// AND / OR of all keys (digits equal everywhere need no sort pass): a
+// grid-stride pass, one atomic pair per workgroup.
+__global__ __launch_bounds__(256) void k_cv_bits(const uint64_t *__restrict__ keys, uint64_t n,
+                                                 unsigned long long *__restrict__ bits) {
+    __shared__ unsigned long long sa[4], so[4];
+    unsigned long long a = ~0ull, o = 0ull;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+        const unsigned long long k = keys[i];
+        a &= k;
+        o |= k;
+    }
 #pragma unroll
     for (int s = 32; s > 0; s >>= 1) {
-        a &= (uint64_t)__shfl_xor((unsigned long long)a, s, SCD_WAVE);
-        o |= (uint64_t)__shfl_xor((unsigned long long)o, s, SCD_WAVE);
+        a &= __shfl_xor(a, s, SCD_WAVE);
+        o |= __shfl_xor(o, s, SCD_WAVE);
     }
     if ((threadIdx.x & 63) == 0) {
-        atomicAnd(&bits[0], (unsigned long long)a);
-        atomicOr(&bits[1], (unsigned long long)o);
+        sa[threadIdx.x >> 6] = a;
+        so[threadIdx.x >> 6] = o;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAnd(&bits[0], sa[0] & sa[1] & sa[2] & sa[3]);
+        atomicOr(&bits[1], so[0] | so[1] | so[2] | so[3]);
     }
 }
 
@@ -153,14 +177,33 @@ __global__ void k_cv_unique(const uint64_t *__restrict__ keys, uint64_t n, uint3
     keep[i] = ((uint32_t)(k >> 32) != (uint32_t)k && (i == 0 || keys[i - 1] != k)) ? 1u : 0u;
 }
 
-// Degrees of the simple graph (first-appearance ids).
-__global__ void k_cv_deg(const uint64_t *__restrict__ keys, uint64_t n, const uint32_t *__restrict__ keep,
-                         uint32_t *__restrict__ deg) {
+// Degrees of the simple graph (first-appearance ids).  Keys are sorted by
+// their smaller end, so each wave adds the run of one smaller end with a
+// single atomic (hubs, numbered early, are mostly smaller ends); larger ends
+// are added one by one.
+__global__ __launch_bounds__(256) void k_cv_deg(const uint64_t *__restrict__ keys, uint64_t n,
+                                                const uint32_t *__restrict__ keep, uint32_t *__restrict__ deg) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !keep[i]) return;
-    const uint64_t k = keys[i];
-    atomicAdd(&deg[(uint32_t)(k >> 32)], 1u);
-    atomicAdd(&deg[(uint32_t)k], 1u);
+    const int lane = threadIdx.x & 63;
+    const bool act = i < n;
+    uint64_t k = 0;
+    uint32_t kp = 0;
+    if (act) {
+        k = keys[i];
+        kp = keep[i];
+    }
+    const uint32_t lo = (uint32_t)(k >> 32);
+    const uint32_t prev = __shfl_up(lo, 1, SCD_WAVE);
+    const bool head = act && (lane == 0 || prev != lo);
+    const uint64_t heads = __ballot(head), kept = __ballot(act && kp);
+    if (head) {
+        const uint64_t above = heads & ~((2ull << lane) - 1ull);  // later heads (lane 63: none)
+        const int end = above ? __builtin_ctzll(above) : 64;
+        const uint64_t run = (end == 64 ? ~0ull : ((1ull << end) - 1ull)) & ~((1ull << lane) - 1ull);
+        const uint32_t c = (uint32_t)__builtin_popcountll(kept & run);
+        if (c) atomicAdd(&deg[lo], c);
+    }
+    if (act && kp) atomicAdd(&deg[(uint32_t)k], 1u);
 }
 
 // Sort keys of the degree order: degree desc, id asc.

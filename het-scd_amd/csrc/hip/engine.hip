@@ -199,22 +199,33 @@ struct scd_ctx {
     // buffers of the previous graph, reused by the next upload when a size
     // fits (a server re-assembles arenas often); freed one upload later
     std::vector<DevBuf> cache;
+    // temporaries of the entry points, kept for later calls up to
+    // kTmpCacheBytes (hipMalloc and hipFree cost more than many kernels)
+    std::vector<DevBuf> tmp_cache;
+    size_t tmp_cache_bytes = 0;
 };
+
+constexpr size_t kTmpCacheBytes = 4ull << 30;
 
 template <typename T>
 static int dalloc(scd_ctx *c, T **p, size_t count, std::vector<DevBuf> *list = nullptr) {
     const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-    if (!list) {  // graph buffers: best fit among the cached ones, at most 2x + 1 MiB
-        size_t best = c->cache.size();
-        for (size_t i = 0; i < c->cache.size(); i++) {
-            const size_t b = c->cache[i].bytes;
-            if (b >= bytes && b <= 2 * bytes + (1u << 20) && (best == c->cache.size() || b < c->cache[best].bytes))
+    // graph buffers come from the previous graph's, temporaries and degree
+    // bin lists from earlier ones (best fit, at most 2x + 1 MiB)
+    std::vector<DevBuf> *pool = !list ? &c->cache : &c->tmp_cache;
+    if (pool) {
+        size_t best = pool->size();
+        for (size_t i = 0; i < pool->size(); i++) {
+            const size_t b = (*pool)[i].bytes;
+            if (b >= bytes && b <= 2 * bytes + (1u << 20) && (best == pool->size() || b < (*pool)[best].bytes))
                 best = i;
         }
-        if (best < c->cache.size()) {
-            *p = (T *)c->cache[best].p;
-            c->bufs.push_back(c->cache[best]);
-            c->cache.erase(c->cache.begin() + best);
+        if (best < pool->size()) {
+            const DevBuf d = (*pool)[best];
+            *p = (T *)d.p;
+            (list ? list : &c->bufs)->push_back(d);
+            pool->erase(pool->begin() + best);
+            if (pool == &c->tmp_cache) c->tmp_cache_bytes -= d.bytes;
             return SCD_OK;
         }
     }
@@ -236,6 +247,26 @@ static void free_list(scd_ctx *c, std::vector<DevBuf> &list) {
     list.clear();
 }
 
+// Temporaries go back to the context's pool (later work on the stream is
+// ordered after the kernels that used them); beyond kTmpCacheBytes the
+// largest are freed.
+static void release_tmp(scd_ctx *c, std::vector<DevBuf> &tmp) {
+    for (auto &b : tmp) {
+        c->tmp_cache.push_back(b);
+        c->tmp_cache_bytes += b.bytes;
+    }
+    tmp.clear();
+    while (c->tmp_cache_bytes > kTmpCacheBytes && !c->tmp_cache.empty()) {
+        size_t big = 0;
+        for (size_t i = 1; i < c->tmp_cache.size(); i++)
+            if (c->tmp_cache[i].bytes > c->tmp_cache[big].bytes) big = i;
+        (void)hipFree(c->tmp_cache[big].p);
+        c->dev_bytes -= c->tmp_cache[big].bytes;
+        c->tmp_cache_bytes -= c->tmp_cache[big].bytes;
+        c->tmp_cache.erase(c->tmp_cache.begin() + big);
+    }
+}
+
 static void drop_graphs(scd_ctx *c) {
     for (int i = 0; i < 2; i++)
         for (int j = 0; j < 2; j++) {
@@ -246,7 +277,8 @@ static void drop_graphs(scd_ctx *c) {
 
 static void free_graph(scd_ctx *c, bool keep_cache = true) {
     drop_graphs(c);
-    free_list(c, c->hub_bufs);
+    if (keep_cache) release_tmp(c, c->hub_bufs);
+    else free_list(c, c->hub_bufs);
     free_list(c, c->cache);
     if (keep_cache) c->cache.swap(c->bufs);
     else free_list(c, c->bufs);
@@ -372,8 +404,9 @@ static uint32_t bin_of(uint32_t d, uint32_t lds_tile, uint32_t wave_max) {
 }
 
 static int build_bins(scd_ctx *c) {
+    HIP_TRY(hipStreamSynchronize(c->stream));  // the old lists may still be read
     drop_graphs(c);
-    free_list(c, c->hub_bufs);
+    release_tmp(c, c->hub_bufs);
     const uint32_t n = c->n, T = c->tune.lds_tile, C = c->tune.merge_chunk, Wm = c->tune.wave_max;
     std::vector<uint32_t> cnt(K_NB + 1, 0);
     for (uint32_t v = 0; v < n; v++) cnt[bin_of(c->h_deg[v], T, Wm) + 1]++;
@@ -829,6 +862,7 @@ extern "C" void scd_destroy(scd_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     free_graph(c, false);
     free_list(c, c->cache);
+    free_list(c, c->tmp_cache);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     for (int i = 0; i < 3; i++) {
         if (c->lane[i]) (void)hipStreamDestroy(c->lane[i]);
@@ -1085,12 +1119,12 @@ extern "C" int scd_compute_wcc(scd_ctx *c, double *wcc, double *terms_out, int32
     std::vector<DevBuf> tmp;
     if (int_tri_out || int_tri_deg_out) {
         if ((rc = dalloc(c, &d_it, c->n, &tmp)) || (rc = dalloc(c, &d_itd, c->n, &tmp))) {
-            free_list(c, tmp);
+            release_tmp(c, tmp);
             return rc;
         }
     }
     if ((rc = clear_counters(c)) || (rc = phase_stats(c)) || (rc = phase_wcc(c, d_it, d_itd))) {
-        free_list(c, tmp);
+        release_tmp(c, tmp);
         return rc;
     }
     HIP_TRY(hipMemcpyAsync(c->wcc_host, c->wcc_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1101,7 +1135,7 @@ extern "C" int scd_compute_wcc(scd_ctx *c, double *wcc, double *terms_out, int32
     if (int_tri_deg_out)
         HIP_TRY(hipMemcpyAsync(int_tri_deg_out, d_itd, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
     rc = finish(c);
-    free_list(c, tmp);
+    release_tmp(c, tmp);
     if (rc) return rc;
     c->wcc_valid = true;
     if (wcc) *wcc = c->wcc_host[0];
@@ -1160,7 +1194,7 @@ extern "C" int scd_neighbor_label_pairs(scd_ctx *c, uint32_t *sorted_out, uint32
     const uint64_t E = c->E;
     if ((rc = dalloc(c, &d_sorted, E, &tmp)) || (rc = dalloc(c, &d_cnt, E, &tmp)) ||
         (rc = dalloc(c, &d_pos, E, &tmp)) || (rc = dalloc(c, &d_off, (size_t)c->n + 1, &tmp))) {
-        free_list(c, tmp);
+        release_tmp(c, tmp);
         return rc;
     }
     uint32_t total = 0;
@@ -1172,7 +1206,7 @@ extern "C" int scd_neighbor_label_pairs(scd_ctx *c, uint32_t *sorted_out, uint32
     }
     if (!rc) {
         if ((rc = dalloc(c, &d_rl, total, &tmp)) || (rc = dalloc(c, &d_rc, total, &tmp))) {
-            free_list(c, tmp);
+            release_tmp(c, tmp);
             return rc;
         }
         if (E) LAUNCH(c, "run_scatter", cdiv(E, 256), 256, k_run_scatter, d_sorted, d_cnt, d_pos, E, d_rl, d_rc);
@@ -1191,7 +1225,7 @@ extern "C" int scd_neighbor_label_pairs(scd_ctx *c, uint32_t *sorted_out, uint32
         if (e != hipSuccess) rc = fail(SCD_EDEVICE, "scd_neighbor_label_pairs: copy: %s", hipGetErrorString(e));
     }
     if (!rc) rc = finish(c);
-    free_list(c, tmp);
+    release_tmp(c, tmp);
     if (rc) return rc;
     if (num_pairs) *num_pairs = total;
     return SCD_OK;
@@ -1303,7 +1337,7 @@ extern "C" int scd_calibration_copy(scd_ctx *c, uint64_t bytes, int reps, double
     const uint64_t n = bytes / 16;
     int rc;
     if ((rc = dalloc(c, &a, n, &tmp)) || (rc = dalloc(c, &b, n, &tmp))) {
-        free_list(c, tmp);
+        release_tmp(c, tmp);
         return rc;
     }
     hipEvent_t e0, e1;
@@ -1318,7 +1352,7 @@ extern "C" int scd_calibration_copy(scd_ctx *c, uint64_t bytes, int reps, double
     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    free_list(c, tmp);
+    release_tmp(c, tmp);
     if (ms_per_copy) *ms_per_copy = reps > 0 ? ms / reps : 0.0;
     return check_launch("calibration copy");
 }
@@ -1395,7 +1429,7 @@ extern "C" int scd_count_triangles(scd_ctx *c, int32_t *tri_out, int32_t *tri_de
     uint32_t *d_td = nullptr;
     if ((rc = dalloc(c, &d_tri, c->n, &tmp)) || (rc = dalloc(c, &d_td, c->n, &tmp)) ||
         (rc = triangles_dev(c, d_tri, d_td, tmp))) {
-        free_list(c, tmp);
+        release_tmp(c, tmp);
         return rc;
     }
     hipError_t e = hipSuccess;
@@ -1408,7 +1442,7 @@ extern "C" int scd_count_triangles(scd_ctx *c, int32_t *tri_out, int32_t *tri_de
     }
     if (e != hipSuccess) rc = fail(SCD_EDEVICE, "scd_count_triangles: copy: %s", hipGetErrorString(e));
     if (!rc) rc = finish(c);
-    free_list(c, tmp);
+    release_tmp(c, tmp);
     return rc;
 }
 
@@ -1430,7 +1464,7 @@ static int sort_u64(scd_ctx *c, uint64_t **keys, uint64_t **alt, uint64_t n, std
         (rc = dalloc(c, &hoff, 256ull * ntiles, &tmp)))
         return rc;
     LAUNCH(c, "cv_sort_bits", 1, 64, k_cv_bits_init, bits);
-    LAUNCH(c, "cv_sort_bits", cdiv(n, 256), 256, k_cv_bits, *keys, n, bits);
+    LAUNCH(c, "cv_sort_bits", std::min(cdiv(n, 256), CV_REDUCE_BLOCKS), 256, k_cv_bits, *keys, n, bits);
     if ((rc = check_launch("convert sort"))) return rc;
     unsigned long long hb[2];
     HIP_TRY(hipMemcpyAsync(hb, bits, sizeof(hb), hipMemcpyDeviceToHost, c->stream));
@@ -1488,7 +1522,7 @@ static int convert_impl(scd_ctx *c, uint64_t count, const uint32_t *u, const uin
     uint32_t *mx = nullptr, hmx = 0;
     if ((rc = dalloc(c, &mx, 1, &tmp))) return rc;
     LAUNCH(c, "cv_ids", 1, 64, k_fill_u32, mx, 1u);
-    LAUNCH(c, "cv_ids", cdiv(count, 256), 256, k_cv_max, u, v, count, mx);
+    LAUNCH(c, "cv_ids", std::min(cdiv(count, 256), CV_REDUCE_BLOCKS), 256, k_cv_max, u, v, count, mx);
     if ((rc = check_launch("convert"))) return rc;
     HIP_TRY(hipMemcpyAsync(&hmx, mx, 4, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1608,13 +1642,13 @@ extern "C" int scd_convert_edges(scd_ctx *c, uint64_t count, const uint32_t *u, 
                     (unsigned long long)count);
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    free_graph(c, false);
+    free_graph(c);
     std::vector<DevBuf> tmp;
     uint32_t n = 0, m = 0;
     double cc = 0.0;
     int rc = convert_impl(c, count, u, v, edges_on_device, &n, &m, &cc, tmp);
     (void)hipStreamSynchronize(c->stream);
-    free_list(c, tmp);
+    release_tmp(c, tmp);
     if (rc) {
         free_graph(c);
         return rc;
@@ -1660,7 +1694,7 @@ extern "C" int scd_initial_partition(scd_ctx *c, uint32_t *labels_out, uint32_t 
     uint8_t *state = nullptr;
     const uint32_t ntiles = cdiv(n, RS_TILE);
     auto cleanup = [&](int r) {
-        free_list(c, tmp);
+        release_tmp(c, tmp);
         return r;
     };
     if ((rc = dalloc(c, &khi[0], n, &tmp)) || (rc = dalloc(c, &khi[1], n, &tmp)) ||
@@ -2156,7 +2190,7 @@ extern "C" int scd_autotune(scd_ctx *c, const char *cache_path, int reps, scd_tu
     const uint32_t nc = c->num_comms;
     if ((rc = dalloc(c, &s_cur, c->n, &tmp)) || (rc = dalloc(c, &s_best, c->n, &tmp)) ||
         (rc = dalloc(c, &s_stats, std::max<uint32_t>(nc, 1), &tmp))) {
-        free_list(c, tmp);
+        release_tmp(c, tmp);
         return rc;
     }
     auto copy_state = [&](bool save) -> int {
@@ -2178,7 +2212,7 @@ extern "C" int scd_autotune(scd_ctx *c, const char *cache_path, int reps, scd_tu
     scd_tuning best_t = orig;
     double best_ms = 1e300;
     if ((rc = copy_state(true))) {
-        free_list(c, tmp);
+        release_tmp(c, tmp);
         return rc;
     }
     for (const scd_tuning &t : cand) {
@@ -2202,7 +2236,7 @@ extern "C" int scd_autotune(scd_ctx *c, const char *cache_path, int reps, scd_tu
     if (rc == SCD_OK) rc = scd_set_tuning(c, &best_t);
     else (void)scd_set_tuning(c, &orig);
     const int rc2 = copy_state(false);
-    free_list(c, tmp);
+    release_tmp(c, tmp);
     if (rc) return rc;
     if (rc2) return rc2;
     {
