@@ -3,7 +3,7 @@
 Outputs (git-ignored, shipped with the working tree):
   het-scd_amd/lib/libscd_host.so   binary format, generators, convert (g++)
   het-scd_amd/lib/libscd_hip.so    the gfx950 HIP engine behind include/scd_hip.h
-  het-scd_amd/lib/libscd_cuda_shim.so  the reference's cuda_* names over the C ABI
+  het-scd_amd/lib/libscd_dropin.so  the reference's cuda_* names over the C ABI
   het-scd_amd/lib/libscd_serve.so  the serving loop behind include/scd_serve.h
   het-scd_amd/bin/main-gpu         `main-gpu <graph> <device> [group size]`
 and, through oracle/Makefile, the CPU oracle (and the reference's own CPU
@@ -65,16 +65,16 @@ def build_hip():
          src("hip", "engine.hip")])
 
 
-def build_shim_and_cli():
-    shim = os.path.join(LIB, "libscd_cuda_shim.so")
+def build_dropin_and_cli():
+    dropin = os.path.join(LIB, "libscd_dropin.so")
     hip_rt = ["-I", "/opt/rocm/include", "-D__HIP_PLATFORM_AMD__", "-L", "/opt/rocm/lib", "-lamdhip64",
               "-Wl,-rpath,/opt/rocm/lib"]
-    run([CXX, *HOST_FLAGS, "-shared", "-o", shim, src("shim", "cuda_shim.cpp"),
+    run([CXX, *HOST_FLAGS, "-shared", "-o", dropin, src("dropin", "dropin.cpp"),
          "-L", LIB, "-lscd_hip", "-lscd_host", "-Wl,-rpath,$ORIGIN", *hip_rt])
     run([CXX, *HOST_FLAGS, "-shared", "-o", os.path.join(LIB, "libscd_serve.so"), src("serve", "server.cpp"),
          "-L", LIB, "-lscd_hip", "-Wl,-rpath,$ORIGIN", *hip_rt])
     run([CXX, *HOST_FLAGS, "-o", os.path.join(BIN, "main-gpu"), src("cli", "main_gpu.cpp"),
-         "-L", LIB, "-lscd_cuda_shim", "-lscd_hip", "-lscd_host",
+         "-L", LIB, "-lscd_dropin", "-lscd_hip", "-lscd_host",
          "-Wl,-rpath,$ORIGIN/../lib"])
 
 
@@ -94,8 +94,8 @@ def build(oracle=True):
             futs.append(ex.submit(build_oracle))
         for f in futs:
             f.result()
-    if os.path.exists(src("shim", "cuda_shim.cpp")):
-        build_shim_and_cli()
+    if os.path.exists(src("dropin", "dropin.cpp")):
+        build_dropin_and_cli()
 
 
 if __name__ == "__main__":

@@ -18,7 +18,7 @@
 #include <utility>
 #include <vector>
 
-#include "scd_cuda_shim.hpp"
+#include "scd_dropin.hpp"
 #include "scd_graph_layout.hpp"
 #include "scd_hip.h"
 #include "scd_host.h"
@@ -86,11 +86,11 @@ int main(int argc, char *argv[]) {
     timer_end();
     const double transfer_ms = now_ms() - t_phase;
 
-    // the engine's context is the shim's; reach it through the C ABI calls
+    // the engine's context is the drop-in library's; reach it through the C ABI calls
     // that take the labels directly
     const unsigned n = g.n;
     std::vector<unsigned> best_labels(n);
-    scd_ctx *ctx = scd_shim_context();
+    scd_ctx *ctx = scd_dropin_context();
     t_phase = now_ms();
     timer_start("create initial partition");
     uint32_t k = 0;

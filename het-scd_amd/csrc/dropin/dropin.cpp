@@ -1,4 +1,4 @@
-// libscd_cuda_shim.so: the six functions of the reference's src/cuda.hpp
+// libscd_dropin.so: the six functions of the reference's src/cuda.hpp
 // over the engine's C ABI, with one process-wide context as the reference
 // keeps its device state in file-static globals (src/cuda.cu:29-52).
 #include <cstdio>
@@ -8,13 +8,13 @@
 
 #include <hip/hip_runtime_api.h>
 
-#include "scd_cuda_shim.hpp"
+#include "scd_dropin.hpp"
 #include "scd_graph_layout.hpp"
 #include "scd_hip.h"
 
 static scd_ctx *g_ctx = nullptr;
 
-scd_ctx *scd_shim_context() { return g_ctx; }
+scd_ctx *scd_dropin_context() { return g_ctx; }
 
 // Device errors end the process, like CUDA_CALL (src/cuda-common.cuh:67-77).
 static void must(int rc, const char *what) {

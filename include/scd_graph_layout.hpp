@@ -1,7 +1,7 @@
 /*
  * scd_graph_layout.hpp — a declaration of the reference's Graph class with
  * the same public data members in the same order (src/graph.hpp:23-33), so
- * that code built without the reference's graph.hpp (the shim library and
+ * that code built without the reference's graph.hpp (the drop-in library and
  * this project's main-gpu) shares its object layout.  Only the members and
  * the inline CSR accessors are declared; construction is left to the caller.
  */

@@ -10,7 +10,7 @@
  *   SCD_OK (0) on success, a negative SCD_E* code (scd_host.h) on failure,
  *   with a message for the calling thread in scd_last_error().
  * The reference's exact C++ names and exit-on-error behaviour are provided by
- * a thin shim over this ABI (scd_cuda_shim.hpp / libscd_cuda_shim.so).
+ * a thin layer over this ABI (scd_dropin.hpp / libscd_dropin.so).
  *
  * Types are plain C: uint32 vertex ids and labels, int32 triangle and
  * community counters, fp64 WCC.  INVALID label = 0xFFFFFFFF (src/scd.hpp:23).

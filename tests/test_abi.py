@@ -29,8 +29,8 @@ def test_library_exports_header(header, lib):
     assert not missing, f"{lib} lacks {missing}"
 
 
-def test_shim_exports_reference_names():
-    path = os.path.join(_lib.LIB_DIR, "libscd_cuda_shim.so")
+def test_dropin_exports_reference_names():
+    path = os.path.join(_lib.LIB_DIR, "libscd_dropin.so")
     L = ctypes.CDLL(path)
     # the six functions of src/cuda.hpp:23-28 with C++ linkage
     for sym in ("_Z9cuda_initiib", "_Z19cuda_transfer_graphPK5Graphd", "_Z21cuda_labels_to_devicePj",
