@@ -6,6 +6,7 @@ Outputs (git-ignored, shipped with the working tree):
   het-scd_amd/lib/libscd_dropin.so  the reference's cuda_* names over the C ABI
   het-scd_amd/lib/libscd_serve.so  the serving loop behind include/scd_serve.h
   het-scd_amd/bin/main-gpu         `main-gpu <graph> <device> [group size]`
+  het-scd_amd/bin/convert-gpu      `convert-gpu <input file> <output file> [device]`
 and, through oracle/Makefile, the CPU oracle (and the reference's own CPU
 programs when the reference tree is present).
 
@@ -76,6 +77,8 @@ def build_dropin_and_cli():
     run([CXX, *HOST_FLAGS, "-o", os.path.join(BIN, "main-gpu"), src("cli", "main_gpu.cpp"),
          "-L", LIB, "-lscd_dropin", "-lscd_hip", "-lscd_host",
          "-Wl,-rpath,$ORIGIN/../lib"])
+    run([CXX, *HOST_FLAGS, "-o", os.path.join(BIN, "convert-gpu"), src("cli", "convert_gpu.cpp"),
+         "-L", LIB, "-lscd_hip", "-lscd_host", "-Wl,-rpath,$ORIGIN/../lib"])
 
 
 def build_oracle():

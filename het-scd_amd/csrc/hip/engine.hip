@@ -203,6 +203,8 @@ struct scd_ctx {
     // kTmpCacheBytes (hipMalloc and hipFree cost more than many kernels)
     std::vector<DevBuf> tmp_cache;
     size_t tmp_cache_bytes = 0;
+
+    scd_convert_info cv_info{};  // counts and stage times of the last convert
 };
 
 constexpr size_t kTmpCacheBytes = 4ull << 30;
@@ -1503,13 +1505,31 @@ static int csr_of_keys(scd_ctx *c, const uint64_t *dk, uint64_t E, uint32_t n, u
     return SCD_OK;
 }
 
+static double wall_ms() {
+    timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
+
 static int convert_impl(scd_ctx *c, uint64_t count, const uint32_t *u, const uint32_t *v, int on_device,
                         uint32_t *n_out, uint32_t *m_out, double *cc_out, std::vector<DevBuf> &tmp) {
     int rc;
     *n_out = 0;
     *m_out = 0;
     *cc_out = std::numeric_limits<double>::quiet_NaN();  // no vertices: 0.0 / 0
+    scd_convert_info &info = c->cv_info;
+    info = scd_convert_info{};
+    info.edges_read = count;
     if (count == 0) return SCD_OK;
+    // stage times: wall clock between stream synchronizations
+    double t_last = wall_ms();
+    auto stage = [&](double &slot) -> int {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        const double t = wall_ms();
+        slot = t - t_last;
+        t_last = t;
+        return SCD_OK;
+    };
     if (!on_device) {
         uint32_t *du = nullptr, *dv = nullptr;
         if ((rc = dalloc(c, &du, count, &tmp)) || (rc = dalloc(c, &dv, count, &tmp))) return rc;
@@ -1527,6 +1547,7 @@ static int convert_impl(scd_ctx *c, uint64_t count, const uint32_t *u, const uin
     HIP_TRY(hipMemcpyAsync(&hmx, mx, 4, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     const uint64_t nid = (uint64_t)hmx + 1;
+    info.id_bound = (uint32_t)nid;  // 32-bit, as the reference's num_nodes
     unsigned long long *first = nullptr;
     uint32_t *seq = nullptr;
     if ((rc = dalloc(c, &first, nid, &tmp)) || (rc = dalloc(c, &seq, 2 * count, &tmp))) return rc;
@@ -1537,6 +1558,7 @@ static int convert_impl(scd_ctx *c, uint64_t count, const uint32_t *u, const uin
     if ((rc = check_launch("convert"))) return rc;
     uint32_t nv = 0;
     if ((rc = scan_u32(c, seq, seq, 2 * count, &nv, tmp))) return rc;
+    info.vertices = nv;
     // 2. undirected keys sorted, loops and duplicates dropped
     //    (src/convert.cpp:252-268, :501)
     uint64_t *ka = nullptr, *kb = nullptr;
@@ -1545,11 +1567,13 @@ static int convert_impl(scd_ctx *c, uint64_t count, const uint32_t *u, const uin
         (rc = dalloc(c, &keep, count, &tmp)) || (rc = dalloc(c, &pos, count, &tmp)))
         return rc;
     LAUNCH(c, "cv_keys", cdiv(count, 256), 256, k_cv_keys, u, v, count, first, seq, ka);
-    if ((rc = sort_u64(c, &ka, &kb, count, tmp))) return rc;
+    if ((rc = stage(info.ms_reindex)) || (rc = sort_u64(c, &ka, &kb, count, tmp)) || (rc = stage(info.ms_sort)))
+        return rc;
     LAUNCH(c, "cv_unique", cdiv(count, 256), 256, k_cv_unique, ka, count, keep);
     if ((rc = check_launch("convert"))) return rc;
     uint32_t mu = 0;
-    if ((rc = scan_u32(c, keep, pos, count, &mu, tmp))) return rc;
+    if ((rc = scan_u32(c, keep, pos, count, &mu, tmp)) || (rc = stage(info.ms_dedup))) return rc;
+    info.edges_simple = mu;
     if (mu == 0) return SCD_OK;  // loops only: nothing survives
     // 3. the simple graph in degree order (hubs get the low ids the triangle
     //    kernels orient toward); both directions sorted: the CSR
@@ -1593,7 +1617,9 @@ static int convert_impl(scd_ctx *c, uint64_t count, const uint32_t *u, const uin
     if ((rc = check_launch("convert"))) return rc;
     uint32_t n_new = 0, mk = 0;
     HIP_TRY(hipMemcpyAsync(&n_new, cnt, 4, hipMemcpyDeviceToHost, c->stream));
-    if ((rc = scan_u32(c, kept, kept, E1, &mk, tmp))) return rc;  // synchronizes
+    if ((rc = scan_u32(c, kept, kept, E1, &mk, tmp)) || (rc = stage(info.ms_triangles))) return rc;
+    info.edges_final = mk;
+    info.vertices_final = n_new;
     if (n_new == 0 || mk == 0) {
         free_graph(c, false);
         return SCD_OK;
@@ -1626,7 +1652,7 @@ static int convert_impl(scd_ctx *c, uint64_t count, const uint32_t *u, const uin
     HIP_TRY(hipMemcpyAsync(c->adj, f_adj, 4ull * E2, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->tri, f_tri, 4ull * n_new, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->tri_deg, f_td, 4ull * n_new, hipMemcpyDeviceToDevice, c->stream));
-    if ((rc = finish_upload(c, h_idx.data()))) return rc;
+    if ((rc = finish_upload(c, h_idx.data())) || (rc = stage(info.ms_relabel))) return rc;
     *n_out = n_new;
     *m_out = mk;
     *cc_out = cc;
@@ -1656,6 +1682,12 @@ extern "C" int scd_convert_edges(scd_ctx *c, uint64_t count, const uint32_t *u, 
     if (n_out) *n_out = n;
     if (m_out) *m_out = m;
     if (cc_out) *cc_out = cc;
+    return SCD_OK;
+}
+
+extern "C" int scd_convert_last_info(const scd_ctx *c, scd_convert_info *out) {
+    if (!c || !out) return fail(SCD_EINVAL, "scd_convert_last_info: null argument");
+    *out = c->cv_info;
     return SCD_OK;
 }
 
@@ -1898,12 +1930,6 @@ extern "C" int scd_batch_compute_wcc(scd_ctx *c, double *wcc_out) {
     c->wcc_valid = true;
     if (wcc_out) memcpy(wcc_out, c->wcc_g_host, 8ull * c->G);
     return SCD_OK;
-}
-
-static double wall_ms() {
-    struct timespec ts;
-    clock_gettime(CLOCK_MONOTONIC, &ts);
-    return ts.tv_sec * 1e3 + ts.tv_nsec / 1e6;
 }
 
 extern "C" int scd_batch_refine(scd_ctx *c, const double *initial_wcc, uint32_t max_iters, double *best_out,

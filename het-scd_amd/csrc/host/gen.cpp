@@ -86,6 +86,153 @@ extern "C" int scd_edges_write_text(const scd_edges *e, const char *path) {
     return SCD_OK;
 }
 
+// Text edge list with the reference reader's rules (src/convert.cpp:74-136):
+// per line, leading white space, then an empty line or '#' is skipped;
+// otherwise two decimal numbers (uint32 arithmetic, wrapping) separated by
+// white space, the rest of the line ignored.  A line of 1024 or more
+// characters is a read error (the reference's getline into char[1024] sets
+// failbit after processing its first 1023 characters).  The first problem in
+// line order is reported.  Lines are parsed in parallel over chunks cut at
+// line ends.
+namespace {
+struct TextChunk {
+    size_t begin = 0, end = 0;  // byte range, whole lines
+    uint64_t first_line = 0;    // 1-based number of its first line
+    std::vector<uint32_t> u, v;
+    uint64_t bad_line = 0;      // first problem in the chunk (0: none)
+    int bad_kind = 0;           // 1 corrupt, 2 read error
+};
+
+inline bool c_space(char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\v' || c == '\f' || c == '\r'; }
+inline bool c_digit(char c) { return c >= '0' && c <= '9'; }
+
+// One line [s, e) (no '\n'): 0 skipped, 1 edge, 2 corrupt.
+int parse_line(const char *s, size_t len, uint32_t *a_out, uint32_t *b_out) {
+    size_t i = 0;
+    auto at = [&](size_t k) -> char { return k < len ? s[k] : '\0'; };
+    while (c_space(at(i))) i++;
+    if (at(i) == '\0' || at(i) == '#') return 0;
+    if (!c_digit(at(i))) return 2;
+    uint32_t a = 0;
+    while (c_digit(at(i))) a = 10u * a + (uint32_t)(at(i++) - '0');
+    while (c_space(at(i))) i++;
+    if (!c_digit(at(i))) return 2;
+    uint32_t b = 0;
+    while (c_digit(at(i))) b = 10u * b + (uint32_t)(at(i++) - '0');
+    *a_out = a;
+    *b_out = b;
+    return 1;
+}
+}  // namespace
+
+extern "C" int scd_edges_read_text(const char *path, scd_edges **out) {
+    if (!path || !out) {
+        set_host_error("scd_edges_read_text: null argument");
+        return SCD_EINVAL;
+    }
+    FILE *f = fopen(path, "rb");
+    if (!f) {
+        set_host_error("an error occured while reading input file (cannot open %s)", path);
+        return SCD_EIO;
+    }
+    std::vector<char> buf;
+    {
+        std::vector<char> blk(1 << 24);
+        size_t got;
+        while ((got = fread(blk.data(), 1, blk.size(), f)) > 0) buf.insert(buf.end(), blk.begin(), blk.begin() + got);
+        const bool err = ferror(f) != 0;
+        fclose(f);
+        if (err) {
+            set_host_error("an error occured while reading input file %s", path);
+            return SCD_EIO;
+        }
+    }
+    const size_t size = buf.size();
+    const int T = std::max(1, std::min(omp_get_max_threads(), (int)(size >> 20) + 1));
+    std::vector<TextChunk> ch(T);
+    for (int t = 0; t < T; t++) {
+        size_t b = size * (size_t)t / T;
+        if (t > 0) {  // start after a line end
+            const char *nl = (const char *)memchr(buf.data() + b - 1, '\n', size - (b - 1));
+            b = nl ? (size_t)(nl - buf.data()) + 1 : size;
+            b = std::max(b, ch[t - 1].begin);
+        }
+        ch[t].begin = b;
+        if (t > 0) ch[t - 1].end = b;
+    }
+    ch[T - 1].end = size;
+    // line numbers: lines that start in earlier chunks
+    std::vector<uint64_t> nlines(T, 0);
+#pragma omp parallel for num_threads(T)
+    for (int t = 0; t < T; t++) {
+        uint64_t c = 0;
+        for (size_t k = ch[t].begin; k < ch[t].end; k++) c += buf[k] == '\n';
+        nlines[t] = c;
+    }
+    uint64_t line = 1;
+    for (int t = 0; t < T; t++) {
+        ch[t].first_line = line;
+        line += nlines[t];
+    }
+#pragma omp parallel for num_threads(T) schedule(static, 1)
+    for (int t = 0; t < T; t++) {
+        TextChunk &c = ch[t];
+        uint64_t ln = c.first_line;
+        size_t k = c.begin;
+        while (k < c.end) {
+            const char *nl = (const char *)memchr(buf.data() + k, '\n', c.end - k);
+            const size_t e = nl ? (size_t)(nl - buf.data()) : c.end;
+            const size_t len = e - k;
+            uint32_t a, b;
+            const int r = parse_line(buf.data() + k, std::min<size_t>(len, 1023), &a, &b);
+            if (r == 2) {
+                c.bad_line = ln;
+                c.bad_kind = 1;
+                break;
+            }
+            if (r == 1) {
+                c.u.push_back(a);
+                c.v.push_back(b);
+            }
+            if (len >= 1024) {
+                c.bad_line = ln;
+                c.bad_kind = 2;
+                break;
+            }
+            k = e + 1;
+            ln++;
+        }
+    }
+    uint64_t total = 0;
+    for (int t = 0; t < T; t++) {
+        if (ch[t].bad_kind == 1) {
+            set_host_error("line %llu is corrupt", (unsigned long long)ch[t].bad_line);
+            return SCD_EINVAL;
+        }
+        if (ch[t].bad_kind == 2) {
+            set_host_error("an error occured while reading input file (line %llu has 1024 or more characters)",
+                           (unsigned long long)ch[t].bad_line);
+            return SCD_EIO;
+        }
+        total += ch[t].u.size();
+    }
+    scd_edges *e = edges_alloc(total);
+    if (!e) {
+        set_host_error("scd_edges_read_text: out of host memory");
+        return SCD_ENOMEM;
+    }
+    uint64_t o = 0;
+    for (int t = 0; t < T; t++) {
+        if (!ch[t].u.empty()) {
+            memcpy(e->u + o, ch[t].u.data(), 4 * ch[t].u.size());
+            memcpy(e->v + o, ch[t].v.data(), 4 * ch[t].v.size());
+        }
+        o += ch[t].u.size();
+    }
+    *out = e;
+    return SCD_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Planted partition
 // ---------------------------------------------------------------------------

@@ -27,6 +27,13 @@ _NAMES = {
 }
 
 
+class ScdConvertInfoC(ctypes.Structure):
+    _fields_ = [("edges_read", ctypes.c_uint64), ("id_bound", ctypes.c_uint32), ("vertices", ctypes.c_uint32),
+                ("edges_simple", ctypes.c_uint64), ("edges_final", ctypes.c_uint32),
+                ("vertices_final", ctypes.c_uint32), ("ms_reindex", ctypes.c_double), ("ms_sort", ctypes.c_double),
+                ("ms_dedup", ctypes.c_double), ("ms_triangles", ctypes.c_double), ("ms_relabel", ctypes.c_double)]
+
+
 class ScdError(RuntimeError):
     def __init__(self, code, message):
         super().__init__(f"{_NAMES.get(code, code)}: {message}")
@@ -94,6 +101,7 @@ def host():
         "scd_edges_from_arrays": (ctypes.c_int, [ctypes.c_uint64, u32p, u32p, ctypes.POINTER(Ed)]),
         "scd_edges_free": (None, [Ed]),
         "scd_edges_write_text": (ctypes.c_int, [Ed, ctypes.c_char_p]),
+        "scd_edges_read_text": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(Ed)]),
         "scd_convert": (ctypes.c_int, [Ed, ctypes.POINTER(G)]),
         "scd_graph_concat": (ctypes.c_int, [ctypes.c_uint32, ctypes.POINTER(G), ctypes.POINTER(G), u64p]),
         "scd_labels_printed_count": (ctypes.c_uint32, [ctypes.c_uint32, u32p]),
@@ -132,6 +140,7 @@ def hip():
                                              u32p, u32p, f64p]),
         "scd_download_graph": (ctypes.c_int, [C, u32p, u32p, i32p, i32p]),
         "scd_graph_shape": (ctypes.c_int, [C, u32p, u32p, f64p]),
+        "scd_convert_last_info": (ctypes.c_int, [C, ctypes.POINTER(ScdConvertInfoC)]),
         "scd_initial_partition": (ctypes.c_int, [C, u32p, u32p]),
         "scd_labels_to_device": (ctypes.c_int, [C, u32p]),
         "scd_improve": (ctypes.c_int, [C, f64p, u32p]),

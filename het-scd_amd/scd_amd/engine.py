@@ -151,6 +151,12 @@ class Engine:
         self.graph = g if n.value else None
         return g
 
+    def convert_info(self):
+        """Counts and per-step device times of the last convert, as a dict."""
+        info = _lib.ScdConvertInfoC()
+        check_hip(_lib.hip().scd_convert_last_info(self._ctx, ctypes.byref(info)))
+        return {f: getattr(info, f) for f, _ in info._fields_}
+
     def download_graph(self):
         """The engine's graph copied back to the host as a Graph."""
         n, m, cc = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_double()

@@ -119,6 +119,15 @@ class Edges:
             _lib.host().scd_edges_free(ep)
         return Graph._from_c(gp)
 
+    @staticmethod
+    def read_text(path):
+        """A text edge list read with the reference convert's rules
+        (src/convert.cpp:74-136); raises ScdError("line <i> is corrupt")
+        like the reference's reader reports it."""
+        ep = ctypes.POINTER(_lib.ScdEdgesC)()
+        check_host(_lib.host().scd_edges_read_text(str(path).encode(), ctypes.byref(ep)))
+        return Edges._from_c(ep)
+
     def write_text(self, path):
         ep = self._as_c()
         try:

@@ -96,6 +96,21 @@ int scd_count_triangles(scd_ctx *ctx, int32_t *tri_out, int32_t *tri_deg_out, in
 int scd_convert_edges(scd_ctx *ctx, uint64_t count, const uint32_t *u, const uint32_t *v, int edges_on_device,
                       uint32_t *n_out, uint32_t *m_out, double *cc_out);
 
+/* Counts and stage times of the context's last scd_convert_edges, in the
+ * terms of the reference convert's stdout (src/convert.cpp:478-530). */
+typedef struct scd_convert_info {
+    uint64_t edges_read;     /* edges given ("found ... edges") */
+    uint32_t id_bound;       /* largest id + 1 in uint32 ("found ... nodes") */
+    uint32_t vertices;       /* vertices after reindexing by first appearance */
+    uint64_t edges_simple;   /* edges after loops and duplicates are dropped */
+    uint32_t edges_final;    /* edges in a triangle: the result's m */
+    uint32_t vertices_final; /* vertices with a triangle edge: the result's n */
+    /* wall time of each step on the device (host syncs at the step ends);
+     * reindex includes the copy of host edges */
+    double ms_reindex, ms_sort, ms_dedup, ms_triangles, ms_relabel;
+} scd_convert_info;
+int scd_convert_last_info(const scd_ctx *ctx, scd_convert_info *out);
+
 /* Copies the context's graph to host arrays of n + 1, 2m, n and n entries
  * (each may be NULL): the layout of class Graph, src/graph.hpp:23-33. */
 int scd_download_graph(scd_ctx *ctx, uint32_t *indices, uint32_t *adj, int32_t *tri, int32_t *tri_deg);

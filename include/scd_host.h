@@ -86,6 +86,16 @@ int scd_edges_from_arrays(uint64_t count, const uint32_t *u, const uint32_t *v,
 void scd_edges_free(scd_edges *e);
 /* Writes "u v" lines, the text input format of the reference convert. */
 int scd_edges_write_text(const scd_edges *e, const char *path);
+/*
+ * Reads a text edge list with the rules of the reference convert's reader
+ * (src/convert.cpp:74-136): blank lines and lines starting with '#' (after
+ * white space) are skipped, every other line starts with two decimal ids
+ * separated by white space (the rest of the line is ignored).  Errors, the
+ * first in line order: SCD_EINVAL "line <i> is corrupt" (1-based, every line
+ * counted), SCD_EIO for a file that cannot be read or a line of 1024 or more
+ * characters.  Edges keep file order; (a, b) as written.
+ */
+int scd_edges_read_text(const char *path, scd_edges **out);
 
 /*
  * Convert pipeline with the reference's semantics (src/convert.cpp:466-537):

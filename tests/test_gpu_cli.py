@@ -45,3 +45,47 @@ def test_main_gpu_outputs(manifest, name, tmp_path):
     labels, k = scd_amd.read_labels(tmp_path / "labels.bin")
     exp, ek = scd_amd.compact_labels(orc.run(g)["best_labels"])
     assert k == ek == done["communities"] and np.array_equal(labels, exp)
+
+
+CONVERT_BIN = os.path.join(os.path.dirname(BIN), "convert-gpu")
+REF_CONVERT = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref", "convert")
+TIMER = re.compile(r"^(.*): [-0-9.e+]+ ms$")
+
+
+def _shape(stdout):
+    """stdout lines with the timer values taken out (only they may differ)."""
+    return [TIMER.sub(r"\1: <t> ms", l) for l in stdout.splitlines()]
+
+
+@pytest.mark.parametrize("name", ["bowtie", "pp8x40_s5", "c1", "rmat14"])
+def test_convert_gpu_cli(manifest, name, tmp_path):
+    """convert-gpu on the text edge list: the reference convert's output file
+    (its SHA-256 in the fixtures) and, where the reference's ./convert is
+    built, its stdout line for line (timer values aside)."""
+    import hashlib
+    from golden_graphs import GRAPHS, make_edges
+    ent = manifest["graphs"][name]
+    txt = tmp_path / f"{name}.txt"
+    make_edges(GRAPHS[name]).write_text(txt)
+    out = tmp_path / "out.bin"
+    print(f"[gpu] convert-gpu {name}", flush=True)
+    r = subprocess.run([CONVERT_BIN, str(txt), str(out), "0"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert hashlib.sha256(out.read_bytes()).hexdigest() == ent["convert_sha256"]
+    lines = _shape(r.stdout)
+    assert f"final graph has {ent['n']} nodes and {ent['m']} edges" in lines
+    if os.path.exists(REF_CONVERT):
+        ref_out = tmp_path / "ref.bin"
+        rr = subprocess.run([REF_CONVERT, str(txt), str(ref_out)], capture_output=True, text=True, timeout=120,
+                            env=dict(os.environ, OMP_NUM_THREADS="4"))
+        assert rr.returncode == 0
+        assert lines == [l.replace(str(ref_out), str(out)) for l in _shape(rr.stdout)]
+
+
+def test_convert_gpu_cli_errors(tmp_path):
+    bad = tmp_path / "bad.txt"
+    bad.write_text("1 2\n2 3\n3 x\n")
+    r = subprocess.run([CONVERT_BIN, str(bad), str(tmp_path / "o.bin")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 255 and "line 3 is corrupt" in r.stderr
+    r = subprocess.run([CONVERT_BIN, str(bad)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 255 and "usage:" in r.stderr
