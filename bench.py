@@ -420,6 +420,9 @@ def main():
     ap.add_argument("--served", action="store_true", default=True, help=argparse.SUPPRESS)
     ap.add_argument("--served-graphs", type=int, default=200)
     ap.add_argument("--join-wait-ms", type=float, default=5.0)
+    ap.add_argument("--tuning", default=None, metavar="JSON",
+                    help='fixed tuning for the single-graph steps instead of the autotuner, e.g. '
+                         '\'{"lds_tile": 4096, "wave_max": 1024}\'')
     ap.add_argument("--no-convert", dest="convert", action="store_false",
                     help="skip the device convert of the same edge list (measured by default)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -452,10 +455,15 @@ def main():
     # timed steps; cached per class)
     eng.set_profiling(False)
     t = time.perf_counter()
-    tune, tune_ms = eng.autotune(reps=3)
+    if args.tuning:
+        eng.set_tuning(**json.loads(args.tuning))
+        tune = eng.get_tuning()
+        log(f"[bench] tuning set by --tuning: {tune}")
+    else:
+        tune, tune_ms = eng.autotune(reps=3)
+        log(f"[bench] autotune {time.perf_counter() - t:.1f} s -> {tune} ({tune_ms:.3f} ms per iteration)")
     t_tune = time.perf_counter() - t
     eng.set_profiling(True)
-    log(f"[bench] autotune {t_tune:.1f} s -> {tune} ({tune_ms:.3f} ms per iteration)")
     w0 = eng.compute_wcc()
     eng.snapshot()
     log(f"[bench] set-up: upload {t_up * 1e3:.1f} ms, initial partition {t_ip * 1e3:.1f} ms "
@@ -523,8 +531,11 @@ def main():
         import glob
         for fn in glob.glob(os.path.join(ROOT, "profiles", "*_traffic.json")):
             t = json.load(open(fn))
+            # the knobs that decide the kernel's vertex set and variant
+            knobs = ("lds_tile", "wave_max", "lds_hash")
+            cur = eng.get_tuning()
             if (t.get("kernel") == roof["kernel"] and t.get("workload_graph_sha256") == graph_sha
-                    and t.get("tuning") == eng.get_tuning()):
+                    and all(t.get("tuning", {}).get(k) == cur[k] for k in knobs)):
                 roof["traffic"] = t["traffic_bytes"]
                 roof["traffic_source"] = os.path.relpath(fn, ROOT)
     per_kernel = {name: {"launches": la, "avg_us": round(ms / la * 1e3, 2), "total_ms": round(ms, 3)}

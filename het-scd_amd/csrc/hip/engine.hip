@@ -845,6 +845,10 @@ extern "C" int scd_init(int device_id, int work_group_size, int profile, scd_ctx
             (void)hipGetLastError();
             c->use_lanes = false;
         }
+        // SCD_LANES=0: one stream whatever the tuning says (tools that trace
+        // kernels may not follow work forked across captured streams)
+        if (const char *env = getenv("SCD_LANES"))
+            if (atoi(env) == 0) c->use_lanes = false;
     }
     e = hipHostMalloc((void **)&c->wcc_host, 256, hipHostMallocDefault);
     if (e != hipSuccess) {

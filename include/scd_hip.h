@@ -225,7 +225,8 @@ int scd_neighbor_label_pairs(scd_ctx *ctx, uint32_t *sorted_out, uint32_t *run_o
  *   Export (scd_neighbor_label_pairs) always sorts.
  *   lanes:       1: the per-bin kernels of a phase run concurrently on four
  *                streams (graphs of at least 2^20 adjacency entries); 0: one
- *                stream.
+ *                stream.  The environment variable SCD_LANES=0 (read by
+ *                scd_init) keeps a context on one stream whatever the tuning.
  */
 typedef struct scd_tuning {
     uint32_t lds_tile;
