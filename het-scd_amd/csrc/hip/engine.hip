@@ -152,6 +152,10 @@ struct scd_ctx {
     // (device-only, scan levels preallocated), item total and counter
     uint32_t wq_M = 0;
     uint32_t *wq_nch = nullptr, *wq_off = nullptr, *wq_tot = nullptr, *wq_next = nullptr;
+    // items of the triangle pass counted from the longer out-list's side
+    uint32_t *vq_nch = nullptr, *vq_off = nullptr, *vq_tot = nullptr, *vq_next = nullptr;
+    uint32_t tc_heavy = 128;  // WccArgs::vc_min of graph triangle counting (0: off)
+    uint32_t wcc_heavy = 0;   // the same for the WCC's internal triangles
     uint32_t *wq_sum[4] = {nullptr}, *wq_scan[4] = {nullptr};
 
     // hubs
@@ -427,7 +431,10 @@ static int build_bins(scd_ctx *c) {
     if (c->wq_M) {
         if ((rc = dalloc(c, &c->wq_nch, c->wq_M, &c->hub_bufs)) ||
             (rc = dalloc(c, &c->wq_off, c->wq_M, &c->hub_bufs)) || (rc = dalloc(c, &c->wq_tot, 1, &c->hub_bufs)) ||
-            (rc = dalloc(c, &c->wq_next, 1, &c->hub_bufs)))
+            (rc = dalloc(c, &c->wq_next, 1, &c->hub_bufs)) ||
+            (rc = dalloc(c, &c->vq_nch, c->wq_M, &c->hub_bufs)) ||
+            (rc = dalloc(c, &c->vq_off, c->wq_M, &c->hub_bufs)) || (rc = dalloc(c, &c->vq_tot, 1, &c->hub_bufs)) ||
+            (rc = dalloc(c, &c->vq_next, 1, &c->hub_bufs)))
             return rc;
         uint64_t len = c->wq_M;
         for (int l = 0; l < 4; l++) {
@@ -721,12 +728,19 @@ static int count_internal_triangles(scd_ctx *c, const WccArgs &w) {
         on_lane(c, 2);
         LAUNCH(c, "wcc_tri_wave", cdiv(M, 4), 256, k_wtc_group<64>, w, bin_v(c, K_W1), M);
         on_lane(c, 0);
-        LAUNCH(c, "wcc_tri_chunks", cdiv(M, 256), 256, k_wtc_count_chunks, w.dplus, bin_v(c, K_W1), M, c->wq_nch,
+        LAUNCH(c, "wcc_tri_chunks", cdiv(M, 256), 256, k_wtc_count_chunks<false>, w, bin_v(c, K_W1), M, c->wq_nch,
                c->wq_next);
-        const int rc = scan_u32_dev(c, c->wq_nch, c->wq_off, M, c->wq_tot, 0);
+        int rc = scan_u32_dev(c, c->wq_nch, c->wq_off, M, c->wq_tot, 0);
         if (rc) return rc;
-        LAUNCH(c, "wcc_tri_queue", 4 * c->num_cus, WTC_QT, (k_wtc_queue<2048, WTC_QT, 4>), w, bin_v(c, K_W1), M,
-               c->wq_off, c->wq_tot, c->wq_next);
+        LAUNCH(c, "wcc_tri_queue", 4 * c->num_cus, WTC_QT, (k_wtc_queue<2048, WTC_QT, 4, false>), w, bin_v(c, K_W1),
+               M, c->wq_off, c->wq_tot, c->wq_next);
+        if (w.vc_min) {  // edges counted from the longer out-list's side
+            LAUNCH(c, "wcc_tri_chunks", cdiv(M, 256), 256, k_wtc_count_chunks<true>, w, bin_v(c, K_W1), M,
+                   c->vq_nch, c->vq_next);
+            if ((rc = scan_u32_dev(c, c->vq_nch, c->vq_off, M, c->vq_tot, 0))) return rc;
+            LAUNCH(c, "wcc_tri_vqueue", 4 * c->num_cus, WTC_QT, (k_wtc_queue<2048, WTC_QT, 4, true>), w,
+                   bin_v(c, K_W1), M, c->vq_off, c->vq_tot, c->vq_next);
+        }
     }
     join_lanes(c);
     // edge marks are complete: count them, again on lanes
@@ -766,6 +780,7 @@ static int phase_wcc(scd_ctx *c, int32_t *int_tri_out, int32_t *int_tri_deg_out,
     w.T_hot = c->hot_T;
     w.td_hot = c->hot_td;
     w.hot_n = c->hot_n;
+    w.vc_min = c->wcc_heavy;
     w.int_tri_out = int_tri_out;
     w.int_tri_deg_out = int_tri_deg_out;
     int rc = count_internal_triangles(c, w);
@@ -1270,6 +1285,17 @@ extern "C" int scd_set_tuning(scd_ctx *c, const scd_tuning *t) {
     return SCD_OK;
 }
 
+extern "C" int scd_set_triangle_split(scd_ctx *c, uint32_t tc_heavy, uint32_t wcc_heavy) {
+    if (!c) return fail(SCD_EINVAL, "scd_set_triangle_split: null context");
+    for (uint32_t h : {tc_heavy, wcc_heavy})
+        if (h && (h < WTC_GROUP_MAX || h >= WTC_VC_CAP))
+            return fail(SCD_EINVAL, "scd_set_triangle_split: %u is not 0 or in [64, 2048)", h);
+    if (c->wcc_heavy != wcc_heavy) drop_graphs(c);  // captured iterations hold the WCC's value
+    c->tc_heavy = tc_heavy;
+    c->wcc_heavy = wcc_heavy;
+    return SCD_OK;
+}
+
 extern "C" int scd_last_counters(scd_ctx *c, uint64_t out[4]) {
     if (!c || !out) return fail(SCD_EINVAL, "scd_last_counters: null argument");
     out[0] = 0;
@@ -1412,6 +1438,7 @@ static int triangles_core(scd_ctx *c, uint32_t **T_out, uint32_t *d_td, uint8_t 
     w.T_hot = c->hot_T;
     w.td_hot = c->hot_td;
     w.hot_n = c->hot_n;
+    w.vc_min = c->tc_heavy;
     LAUNCH(c, "tc_clear", cdiv(E, 256), 256, k_fill_u8, flag, E);
     if ((rc = count_internal_triangles(c, w))) return rc;
     *T_out = T;

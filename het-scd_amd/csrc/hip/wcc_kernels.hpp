@@ -52,10 +52,21 @@ struct WccArgs {
     uint32_t *__restrict__ T_hot;   // WTC_HOT_COPIES x hot_n
     uint32_t *__restrict__ td_hot;  // WTC_HOT_COPIES x hot_n
     uint32_t hot_n;
+    // Out-lists longer than vc_min (and at most WTC_VC_CAP) are probed from
+    // their own side: an edge u -> v whose v has the longer out-list is
+    // walked as N+(u) below v against the set of N+(v) (k_wtc_queue<.., true>)
+    // instead of N+(v) against N+(u).  0: every edge from u's side.
+    uint32_t vc_min;
 };
 
 constexpr uint32_t WTC_HOT_MAX = 8192;
 constexpr uint32_t WTC_HOT_COPIES = 16;
+constexpr uint32_t WTC_VC_CAP = 2048;  // the work queue's LDS set capacity
+
+// Edge u -> v (v < u, out-list lengths dpu, dpv) is counted from v's side.
+__device__ __forceinline__ bool wtc_from_v(const WccArgs &a, uint32_t dpv, uint32_t dpu) {
+    return a.vc_min && dpv > a.vc_min && dpv <= WTC_VC_CAP && dpv > dpu;
+}
 
 __device__ __forceinline__ void wtc_add_T(const WccArgs &a, uint32_t x, uint32_t c) {
     if (x < a.hot_n) atomicAdd(&a.T_hot[(blockIdx.x & (WTC_HOT_COPIES - 1)) * a.hot_n + x], c);
@@ -146,6 +157,7 @@ __global__ __launch_bounds__(256) void k_wtc_group(WccArgs a, const uint32_t *__
     if ((uint32_t)lig < dp) {
         x = a.int_adj[beg + lig];
         len = a.dplus[x];
+        if (wtc_from_v(a, len, dp)) len = 0;  // counted from x's side
         vb = a.indices[x];
     }
     uint32_t inc = len;  // inclusive prefix over the group
@@ -224,18 +236,33 @@ __device__ __forceinline__ int wtc_find(const uint32_t *hk, const uint16_t *hp, 
 // N+_int(u), walks the member lists of its chunk — one list per wave when
 // they average >= 32 entries, else flattened over the threads — counts per
 // out-list position in LDS and flushes one atomic per non-zero position.
-constexpr uint32_t WTC_VC = 32;  // out-list entries per work item
+constexpr uint32_t WTC_VC = 32;    // out-list entries per work item
+constexpr uint32_t WTC_VCV = 128;  // upper-list entries per work item counted from the owner's side
 constexpr int WTC_QT = 512;  // work-queue workgroup: 8 waves, 64 VGPRs, 4 workgroups per CU
 constexpr uint32_t WTC_CLAIM = 8;  // items claimed at once by a workgroup
 
-// nch[i] = chunks of verts[M-1-i]; also resets the item counter.
-__global__ void k_wtc_count_chunks(const uint32_t *__restrict__ dplus, const uint32_t *__restrict__ verts,
-                                   uint32_t M, uint32_t *__restrict__ nch, uint32_t *__restrict__ next) {
+// nch[i] = chunks of verts[M-1-i]; also resets the item counter.  From u's
+// side (VC false): chunks of the out-list when it is longer than 64 entries;
+// from v's side (VC true): chunks of the upper list N-(v) (entries past the
+// out-list) of every vertex whose out-list takes edges from its side.
+template <bool VC>
+__global__ void k_wtc_count_chunks(WccArgs a, const uint32_t *__restrict__ verts, uint32_t M,
+                                   uint32_t *__restrict__ nch, uint32_t *__restrict__ next) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) next[0] = 0;
     if (i >= M) return;
-    const uint32_t dp = dplus[verts[M - 1 - i]];
-    nch[i] = dp > WTC_GROUP_MAX ? (dp + WTC_VC - 1) / WTC_VC : 0u;
+    const uint32_t v = verts[M - 1 - i];
+    const uint32_t dp = a.dplus[v];
+    if (!VC) {
+        nch[i] = dp > WTC_GROUP_MAX ? (dp + WTC_VC - 1) / WTC_VC : 0u;
+        return;
+    }
+    uint32_t up = 0;
+    if (a.vc_min && dp > a.vc_min && dp <= WTC_VC_CAP) {
+        const uint32_t deg = a.int_deg ? a.int_deg[v] : a.indices[v + 1] - a.indices[v];
+        up = deg - dp;
+    }
+    nch[i] = (up + WTC_VCV - 1) / WTC_VCV;
 }
 
 // Membership set of N+_int(u) (sorted, dp entries) for one work item.  When
@@ -244,7 +271,7 @@ __global__ void k_wtc_count_chunks(const uint32_t *__restrict__ dplus, const uin
 // member iff its bit is set, and its position is that rank plus the bits
 // below it in the word — one LDS read per probe.  Wider ranges use the
 // linear-probing hash (slot -> vertex, position).
-constexpr uint32_t WTC_TAB = 4000;  // (bits, rank) words: id ranges up to 128000 (4 blocks per CU)
+constexpr uint32_t WTC_TAB = 3800;  // (bits, rank) words: id ranges up to 121600 (4 blocks per CU)
 
 struct WtcSet {
     uint2 *tab;
@@ -265,14 +292,24 @@ struct WtcSet {
     }
 };
 
-template <int CAP, int NT, int UW>
+// VC false: items are WTC_VC-entry chunks of N+_int(u), each entry v's list
+// N+_int(v) walked against the set of N+_int(u).  VC true: items are
+// WTC_VCV-entry chunks of the upper list N-_int(v) of a vertex v whose edges
+// are counted from its side; each entry u's out-list is walked against the
+// set of N+_int(v), the entries below v probed (the walk's length needs no
+// search up front; entries past v are read and skipped), and u's count and
+// the flag of u -> v collect per chunk entry.
+template <int CAP, int NT, int UW, bool VC>
 __global__ __launch_bounds__(NT, NT >= 512 ? 8 : 1) void k_wtc_queue(WccArgs a, const uint32_t *__restrict__ verts, uint32_t M,
                                                    const uint32_t *__restrict__ off,
                                                    const uint32_t *__restrict__ total, uint32_t *__restrict__ next) {
     static_assert(2 * CAP * 4 + 2 * CAP * 2 <= WTC_TAB * 8, "hash must fit in the bitmap storage");
+    static_assert(!VC || CAP == WTC_VC_CAP, "v-side sets hold up to WTC_VC_CAP entries");
     __shared__ uint2 s_tab[WTC_TAB];  // bitmap words, or the hash (keys, positions)
     __shared__ uint32_t s_cnt[CAP];
-    __shared__ uint32_t s_vb[WTC_VC], s_pre[WTC_VC + 1];
+    constexpr uint32_t CH = VC ? WTC_VCV : WTC_VC;  // entries per chunk
+    __shared__ uint32_t s_vb[CH], s_pre[CH + 1];
+    __shared__ uint32_t s_ucnt[VC ? CH : 1], s_upos[VC ? CH : 1];
     __shared__ uint32_t s_item;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     constexpr int NW = NT / 64;
@@ -331,25 +368,48 @@ __global__ __launch_bounds__(NT, NT >= 512 ? 8 : 1) void k_wtc_queue(WccArgs a, 
                         set.hp[h] = (uint16_t)k;
                     }
                 }
+                // chunks of the owner u's out-list (VC false) or of its upper
+                // list (VC true); per chunk entry x: the list walked is x's
+                // out-list (all of it when VC: entries below u are probed,
+                // u's own position is recorded, entries above u are skipped)
+                const uint32_t lst = VC ? (a.int_deg ? a.int_deg[u] : a.indices[u + 1] - beg) : dp;
+                const uint32_t base = VC ? dp : 0u;
                 for (uint32_t c = c0; c < c1; c++) {
-                    const uint32_t k0 = c * WTC_VC, nk = min(WTC_VC, dp - k0);
+                    const uint32_t k0 = base + c * CH, nk = min(CH, lst - k0);
                     if (tid < (int)nk) {
                         const uint32_t v = N[k0 + tid];
+                        const uint32_t dv = a.dplus[v];
                         s_vb[tid] = a.indices[v];
-                        s_pre[tid + 1] = a.dplus[v];
+                        if (VC) {
+                            s_pre[tid + 1] = wtc_from_v(a, dp, dv) ? dv : 0u;  // else counted from v's side
+                            s_ucnt[tid] = 0;
+                            s_upos[tid] = SCD_INVALID;
+                        } else {
+                            s_pre[tid + 1] = wtc_from_v(a, dv, dp) ? 0u : dv;  // else counted from v's side
+                        }
                     }
                     if (tid == 0) s_pre[0] = 0;
                     __syncthreads();
-                    if (wv == 0) {  // prefix of the chunk's list lengths
-                        uint32_t x = lane < (int)nk ? s_pre[lane + 1] : 0u;
+                    if (wv < (int)((CH + 63) / 64)) {  // prefix of the chunk's list lengths, 64 per wave
+                        const uint32_t e = 64u * wv + lane;
+                        uint32_t x = e < nk ? s_pre[e + 1] : 0u;
 #pragma unroll
                         for (int o = 1; o < 64; o <<= 1) {
                             const uint32_t t = __shfl_up(x, o, SCD_WAVE);
                             if (lane >= o) x += t;
                         }
-                        if (lane < (int)nk) s_pre[lane + 1] = x;
+                        if (e < nk) s_pre[e + 1] = x;
                     }
                     __syncthreads();
+                    if (CH > 64) {  // carry the first waves' totals
+                        for (uint32_t g = 1; g < CH / 64; g++) {
+                            const uint32_t e = 64u * g + tid;
+                            const uint32_t carry = s_pre[64u * g];
+                            __syncthreads();
+                            if (tid < 64 && e < nk) s_pre[e + 1] += carry;
+                            __syncthreads();
+                        }
+                    }
                     const uint32_t Wc = s_pre[nk];
                     if (Wc >= 32u * nk) {
                         // one list per wave; UW loads of 64 entries in flight
@@ -366,6 +426,10 @@ __global__ __launch_bounds__(NT, NT >= 512 ? 8 : 1) void k_wtc_queue(WccArgs a, 
 #pragma unroll
                                 for (int q = 0; q < UW; q++) {
                                     if (w[q] == SCD_INVALID) continue;
+                                    if (VC && w[q] >= u) {
+                                        if (w[q] == u) s_upos[kk] = vb + j0 + 64 * q + lane;
+                                        continue;
+                                    }
                                     const int pos = set.find(w[q]);
                                     if (pos >= 0) {
                                         atomicAdd(&s_cnt[pos], 1u);
@@ -373,11 +437,12 @@ __global__ __launch_bounds__(NT, NT >= 512 ? 8 : 1) void k_wtc_queue(WccArgs a, 
                                         cnt++;
                                     }
                                 }
+                                if (VC && __ballot(w[UW - 1] >= u)) break;  // uniform: past u
                             }
 #pragma unroll
                             for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, SCD_WAVE);
                             if (lane == 0 && cnt) {
-                                atomicAdd(&s_cnt[k0 + kk], cnt);
+                                atomicAdd(VC ? &s_ucnt[kk] : &s_cnt[k0 + kk], cnt);
                                 tot += cnt;
                             }
                         }
@@ -400,9 +465,13 @@ __global__ __launch_bounds__(NT, NT >= 512 ? 8 : 1) void k_wtc_queue(WccArgs a, 
 #pragma unroll
                             for (int q = 0; q < WTC_UF; q++) {
                                 if (w[q] == SCD_INVALID) continue;
+                                if (VC && w[q] >= u) {
+                                    if (w[q] == u) s_upos[own[q]] = at[q];
+                                    continue;
+                                }
                                 const int pos = set.find(w[q]);
                                 if (pos >= 0) {
-                                    atomicAdd(&s_cnt[k0 + own[q]], 1u);
+                                    atomicAdd(VC ? &s_ucnt[own[q]] : &s_cnt[k0 + own[q]], 1u);
                                     atomicAdd(&s_cnt[pos], 1u);
                                     a.flag[at[q]] = 1;
                                     tot++;
@@ -411,6 +480,14 @@ __global__ __launch_bounds__(NT, NT >= 512 ? 8 : 1) void k_wtc_queue(WccArgs a, 
                         }
                     }
                     __syncthreads();  // s_vb / s_pre reused by the next chunk
+                    if (VC) {
+                        // the walked vertices: their counts and the edge to the owner
+                        if (tid < (int)nk && s_ucnt[tid]) {
+                            wtc_add_T(a, N[k0 + tid], s_ucnt[tid]);
+                            if (s_upos[tid] != SCD_INVALID) a.flag[s_upos[tid]] = 1;
+                        }
+                        __syncthreads();
+                    }
                 }
                 for (uint32_t k = tid; k < dp; k += NT) {
                     const uint32_t c = s_cnt[k];

@@ -117,11 +117,19 @@ class Engine:
         check_hip(_lib.hip().scd_current_labels(self._ctx, ptr(out, u32p)))
         return out
 
-    def count_triangles(self, replace=False):
-        tri = np.empty(self.n, np.int32)
-        td = np.empty(self.n, np.int32)
+    def count_triangles(self, replace=False, outputs=True):
+        """t / vt of the uploaded graph (src/convert.cpp:271-340); with
+        outputs=False nothing is copied back (returns None)."""
+        tri = np.empty(self.n, np.int32) if outputs else None
+        td = np.empty(self.n, np.int32) if outputs else None
         check_hip(_lib.hip().scd_count_triangles(self._ctx, ptr(tri, i32p), ptr(td, i32p), 1 if replace else 0))
-        return tri, td
+        return (tri, td) if outputs else None
+
+    def set_triangle_split(self, tc_heavy=128, wcc_heavy=0):
+        """scd_set_triangle_split: out-list length above which an edge is
+        walked from its longer out-list's side (0: never), for graph triangle
+        counting / convert and for the WCC's internal triangles."""
+        check_hip(_lib.hip().scd_set_triangle_split(self._ctx, tc_heavy, wcc_heavy))
 
     def convert(self, edges, download=True):
         """The convert tool (src/convert.cpp:466-537) on the device for an

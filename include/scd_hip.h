@@ -79,6 +79,19 @@ int scd_upload(scd_ctx *ctx, uint32_t n, uint32_t m, const uint32_t *indices,
 int scd_count_triangles(scd_ctx *ctx, int32_t *tri_out, int32_t *tri_deg_out, int replace);
 
 /*
+ * Work split of the triangle kernels (no counterpart in the reference; the
+ * results never depend on it).  Edges are oriented toward the lower id and
+ * a triangle w < v < u is found on the edge u -> v, by default by walking
+ * v's out-list against the set of u's.  With a threshold h != 0, an edge
+ * whose v has the longer out-list, of more than h and at most 2048 entries,
+ * is walked the other way: u's out-list below v against the set of v's, so
+ * each edge costs the shorter walk.  tc_heavy applies to scd_count_triangles
+ * and scd_convert_edges (default 128), wcc_heavy to the WCC's internal
+ * triangles (default 0: off).  Each is 0 or in [64, 2048).
+ */
+int scd_set_triangle_split(scd_ctx *ctx, uint32_t tc_heavy, uint32_t wcc_heavy);
+
+/*
  * The convert tool (src/convert.cpp:466-537; scd_convert in scd_host.h is
  * its host restatement) on the device: an edge list (u[i], v[i]) in file
  * order becomes the binary graph layout -- ids by first appearance, loops

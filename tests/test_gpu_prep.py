@@ -106,3 +106,67 @@ def test_whole_device_path(engine, manifest, name):
     import struct
     final = struct.unpack("<d", bytes.fromhex(ent["final_wcc_hex"]))[0]
     assert best == pytest.approx(final, rel=1e-9, abs=0)
+
+
+@pytest.mark.parametrize("name", ["pp8x40_s5", "c1", "rmat14", "rmat16", "c2"])
+def test_triangle_split_invariance(engine, name):
+    """Walking an edge from either end's out-list (scd_set_triangle_split)
+    changes the work, never t / vt."""
+    g = graph(name)
+    engine.transfer_graph(g)
+    try:
+        for h in [0, 64, 100, 128, 512, 2047]:
+            print(f"[gpu] triangle split {name} tc_heavy={h}", flush=True)
+            engine.set_triangle_split(tc_heavy=h)
+            tri, td = engine.count_triangles()
+            assert np.array_equal(tri, g.tri) and np.array_equal(td, g.tri_deg), h
+    finally:
+        engine.set_triangle_split()
+
+
+@pytest.mark.parametrize("scale,seed", [(12, 2), (15, 4)])
+def test_triangle_split_raw_graph(engine, scale, seed):
+    g = raw_graph(scd_amd.rmat(scale, 16, 0.57, 0.19, 0.19, seed))
+    engine.transfer_graph(g, clustering_coef=0.0)
+    otri, otd = orc.triangles(g)
+    try:
+        for h in [0, 64, 256]:
+            engine.set_triangle_split(tc_heavy=h)
+            tri, td = engine.count_triangles()
+            assert np.array_equal(tri, otri) and np.array_equal(td, otd), h
+    finally:
+        engine.set_triangle_split()
+
+
+def test_triangle_split_limits(engine):
+    for bad in [1, 63, 2048, 5000]:
+        with pytest.raises(scd_amd.ScdError):
+            engine.set_triangle_split(tc_heavy=bad)
+    engine.set_triangle_split()
+
+
+@pytest.mark.parametrize("name,k", [("rmat14", 4), ("rmat16", 16), ("c2", 64)])
+def test_wcc_triangle_split_invariance(engine, name, k):
+    """The WCC's internal triangle counts (coarse random partitions, so that
+    internal out-lists are long) do not depend on the walk side, and equal
+    the oracle's."""
+    g = graph(name)
+    rng = np.random.default_rng(7)
+    labels = rng.integers(0, k, g.n).astype(np.uint32)
+    engine.transfer_graph(g)
+    outs = []
+    try:
+        for h in [0, 64, 256]:
+            print(f"[gpu] WCC triangle split {name} wcc_heavy={h}", flush=True)
+            engine.set_triangle_split(wcc_heavy=h)
+            engine.labels_to_device(labels)
+            outs.append(engine.compute_wcc(details=True))
+    finally:
+        engine.set_triangle_split()
+    for w, terms, it, itd in outs[1:]:
+        assert w == outs[0][0]
+        assert np.array_equal(terms, outs[0][1])
+        assert np.array_equal(it, outs[0][2]) and np.array_equal(itd, outs[0][3])
+    if g.n <= 100000:
+        _, _, rit, ritd = orc.compute_wcc(g, labels, np.bincount(labels, minlength=k))
+        assert np.array_equal(outs[0][2], rit) and np.array_equal(outs[0][3], ritd)
