@@ -212,6 +212,7 @@ struct scd_ctx {
 };
 
 constexpr size_t kTmpCacheBytes = 4ull << 30;
+constexpr double kTuneMargin = 0.03;  // autotuner: required gain over the best so far
 
 template <typename T>
 static int dalloc(scd_ctx *c, T **p, size_t count, std::vector<DevBuf> *list = nullptr) {
@@ -2216,11 +2217,13 @@ extern "C" int scd_autotune(scd_ctx *c, const char *cache_path, int reps, scd_tu
         return SCD_OK;
     }
 
-    // candidates: only knobs that change a bin or tile for these degrees
+    // candidates: only knobs that change a bin or tile for these degrees,
+    // the defaults first; a later candidate must be kTuneMargin faster to
+    // win, so timing noise does not flip the choice between near-equals
     std::vector<uint32_t> Ws = {1024};
-    if (dmax > 256) Ws = {256, 512, 1024};
+    if (dmax > 256) Ws = {1024, 512, 256};
     std::vector<uint32_t> Ts = {2048};
-    if (dmax > 2048) Ts = {2048, 4096, 8192, 16384};
+    if (dmax > 2048) Ts = {4096, 2048, 8192, 16384};
     std::vector<scd_tuning> cand;
     for (uint32_t T : Ts)
         for (uint32_t W : Ws) {
@@ -2285,7 +2288,7 @@ extern "C" int scd_autotune(scd_ctx *c, const char *cache_path, int reps, scd_tu
         if (rc) break;
         std::sort(ms.begin(), ms.end());
         const double med = ms[ms.size() / 2];
-        if (med < best_ms) {
+        if (med < best_ms * (1.0 - kTuneMargin)) {
             best_ms = med;
             best_t = c->tune;
         }
