@@ -207,6 +207,25 @@ __device__ __forceinline__ uint32_t upper_bound_u32(const uint32_t *s, uint32_t 
     return lo;
 }
 
+// upper_bound_u32 for a key known to sit at s[i]: a galloping search from
+// i (runs of equal neighbour labels are mostly short, so most calls read
+// s[i + 1] only), then a binary search within the last step.
+__device__ __forceinline__ uint32_t run_end_u32(const uint32_t *s, uint32_t n, uint32_t i, uint32_t x) {
+    uint32_t lo = i + 1, hi = i + 1, step = 1;
+    while (hi < n && s[hi] <= x) {  // everything below hi + 1 is <= x
+        lo = hi + 1;
+        hi += step;
+        step <<= 1;
+    }
+    if (hi > n) hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s[mid] <= x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // Merge path: the number of elements taken from A among the first `diag`
 // outputs of merge(A[0..la), B[0..lb)) (A first on ties).
 __device__ __forceinline__ uint32_t merge_path(const uint32_t *A, uint32_t la, const uint32_t *B,
@@ -221,16 +240,20 @@ __device__ __forceinline__ uint32_t merge_path(const uint32_t *A, uint32_t la, c
     return lo;
 }
 
-// Sort of n2 keys (a power of two, 1024 <= n2 <= T) held in LDS by a block
-// of 256 threads: every wave sorts 1024-key chunks in registers
-// (wave_bitonic_sort<16>), then merge-path passes in LDS double the sorted
+// Sort of n2 keys (a power of two, 1024 <= n2 <= T; n real keys, then
+// padding) held in LDS by a block of 256 threads: every wave sorts 1024-key
+// chunks in registers (wave_bitonic_sort<16>), then merge-path passes in LDS double the sorted
 // run width; each thread computes its n2/256 outputs of a pass in registers
 // and writes them back after a barrier.  Ends with a barrier.
 template <int T>
-__device__ __forceinline__ void block_sort_lds(uint32_t *s, uint32_t n2) {
+__device__ __forceinline__ void block_sort_lds(uint32_t *s, uint32_t n2, uint32_t n) {
+    // positions n .. n2-1 hold padding (SCD_INVALID, the largest key): a
+    // chunk of padding only is sorted already, and merging a run with a
+    // run of padding only leaves both in place
     constexpr int PER_MAX = T / 256;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (uint32_t c = w; c * 1024 < n2; c += 4) {
+        if (c * 1024 >= n) break;  // uniform per wave
         uint32_t x[16];
 #pragma unroll
         for (int r = 0; r < 16; r++) x[r] = s[c * 1024 + r * 64 + lane];
@@ -243,22 +266,27 @@ __device__ __forceinline__ void block_sort_lds(uint32_t *s, uint32_t n2) {
     const uint32_t o0 = threadIdx.x * per;
     for (uint32_t w2 = 1024; w2 < n2; w2 *= 2) {
         const uint32_t ps = (o0 / (2 * w2)) * (2 * w2);
+        const bool skip = ps + w2 >= n;  // B is padding only
         const uint32_t *A = s + ps;
         const uint32_t *B = A + w2;
-        uint32_t i = merge_path(A, w2, B, w2, o0 - ps);
-        uint32_t j = o0 - ps - i;
         uint32_t out[PER_MAX];
+        if (!skip) {
+            uint32_t i = merge_path(A, w2, B, w2, o0 - ps);
+            uint32_t j = o0 - ps - i;
 #pragma unroll
-        for (int k = 0; k < PER_MAX; k++) {
-            if ((uint32_t)k < per) {
-                const bool takeA = (j >= w2) || (i < w2 && A[i] <= B[j]);
-                out[k] = takeA ? A[i++] : B[j++];
+            for (int k = 0; k < PER_MAX; k++) {
+                if ((uint32_t)k < per) {
+                    const bool takeA = (j >= w2) || (i < w2 && A[i] <= B[j]);
+                    out[k] = takeA ? A[i++] : B[j++];
+                }
             }
         }
         __syncthreads();
+        if (!skip) {
 #pragma unroll
-        for (int k = 0; k < PER_MAX; k++)
-            if ((uint32_t)k < per) s[o0 + k] = out[k];
+            for (int k = 0; k < PER_MAX; k++)
+                if ((uint32_t)k < per) s[o0 + k] = out[k];
+        }
         __syncthreads();
     }
 }

@@ -264,7 +264,7 @@ __device__ __forceinline__ void eval_sorted_batched(const MoveArgs &a, const uin
             fr[k] = 0;
             if (i < hi) {
                 v[k] = S[i];
-                if ((i == 0 || S[i - 1] != v[k]) && v[k] != SCD_INVALID) fr[k] = upper_bound_u32(S, n, v[k]) - i;
+                if ((i == 0 || S[i - 1] != v[k]) && v[k] != SCD_INVALID) fr[k] = run_end_u32(S, n, i, v[k]) - i;
             }
         }
 #pragma unroll
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(256) void k_move_lds(MoveArgs a, const uint32_t *__
     const uint32_t n2 = max(next_pow2(deg), 1024u);
     if (tid == 0) s_has_rem = 0;
     gather_labels<T>(a, beg, deg, n2, keys);
-    block_sort_lds<T>(keys, n2);
+    block_sort_lds<T>(keys, n2, deg);
 
     double bi = 0.0;
     uint32_t bl = SCD_INVALID;
@@ -481,7 +481,7 @@ __global__ __launch_bounds__(256) void k_hub_tile_sort(MoveArgs a, HubArgs h,
     const uint32_t len = min((uint32_t)T, d - st);
     const uint32_t n2 = max(next_pow2(len), 1024u);
     gather_labels<T>(a, beg, len, n2, keys);
-    block_sort_lds<T>(keys, n2);
+    block_sort_lds<T>(keys, n2, len);
     uint32_t *dst = out + h.hub_off[hb] + st;
     for (uint32_t i = tid; i < len; i += 256) dst[i] = keys[i];
     (void)tid;
@@ -602,13 +602,13 @@ __global__ __launch_bounds__(256) void k_hub_tile_count(MoveArgs a, HubArgs h, H
     const uint32_t len = min((uint32_t)T, d - st);
     const uint32_t n2 = max(next_pow2(len), 1024u);
     gather_labels<T>(a, beg, len, n2, keys);
-    block_sort_lds<T>(keys, n2);
+    block_sort_lds<T>(keys, n2, len);
     unsigned long long *tab = t.slots + t.tab_off[hb];
     const uint32_t mask = t.tab_cap[hb] - 1u;
     for (uint32_t i = tid; i < len; i += 256) {
         const uint32_t L = keys[i];
         if (L == SCD_INVALID || (i > 0 && keys[i - 1] == L)) continue;
-        hub_table_add(tab, mask, L, upper_bound_u32(keys, len, L) - i);
+        hub_table_add(tab, mask, L, run_end_u32(keys, len, i, L) - i);
     }
 }
 
