@@ -359,17 +359,24 @@ __global__ __launch_bounds__(256) void k_move_lds(MoveArgs a, const uint32_t *__
 // the distinct valid neighbour labels with their frequencies and the argmax
 // order is total, so a (label, count) table in LDS replaces the sort (a few
 // instructions per neighbour instead of the bitonic stages).  One workgroup
-// per vertex, table of next_pow2(2 deg) <= 2T slots, linear probing.
+// per vertex; slots hold label << 32 | count (64-bit LDS atomics), at least
+// 1.5 deg of them (at most 1.5 T: 48 KB for T = 4096), linear probing from a
+// multiplicative hash scaled to the table size.  The occupied slots are then
+// compacted in place (ballot ranks, 256 slots a round), so the fp64
+// evaluations run on dense lanes.
 // ---------------------------------------------------------------------------
 template <int T>
 __global__ __launch_bounds__(256) void k_move_lds_hash(MoveArgs a, const uint32_t *__restrict__ verts,
                                                        uint32_t nv) {
-    __shared__ uint32_t s_key[2 * T], s_cnt[2 * T];
+    constexpr uint32_t CAPMAX = T + T / 2;
+    constexpr unsigned long long EMPTY = ~0ull;  // label SCD_INVALID is never inserted
+    __shared__ unsigned long long s_tab[CAPMAX];
     __shared__ double s_bi[4];
     __shared__ uint32_t s_bl[4];
+    __shared__ uint32_t s_wn[4];
     __shared__ double s_rem;
     __shared__ int s_has_rem;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const uint32_t p = verts[blockIdx.x];
     const uint32_t my = a.labels[p];
     if (vtx_stopped(a, p)) {  // uniform per block, before any barrier
@@ -378,11 +385,8 @@ __global__ __launch_bounds__(256) void k_move_lds_hash(MoveArgs a, const uint32_
     }
     const uint32_t beg = a.indices[p];
     const uint32_t deg = a.indices[p + 1] - beg;
-    const uint32_t cap = min(next_pow2(2 * deg), 2u * T), mask = cap - 1u;
-    for (uint32_t i = tid; i < cap; i += 256) {
-        s_key[i] = SCD_INVALID;
-        s_cnt[i] = 0;
-    }
+    const uint32_t cap = min(CAPMAX, (deg + deg / 2 + 255u) & ~255u);
+    for (uint32_t i = tid; i < cap; i += 256) s_tab[i] = EMPTY;
     if (tid == 0) s_has_rem = 0;
     constexpr int U = T / 256;
     uint32_t q[U];
@@ -398,29 +402,50 @@ __global__ __launch_bounds__(256) void k_move_lds_hash(MoveArgs a, const uint32_
     for (int u = 0; u < U; u++) {
         const uint32_t L = q[u];
         if (L == SCD_INVALID) continue;
-        for (uint32_t h = (L * 0x9E3779B1u) & mask;; h = (h + 1) & mask) {
-            uint32_t k = s_key[h];
-            if (k == SCD_INVALID) k = atomicCAS(&s_key[h], SCD_INVALID, L);
-            if (k == SCD_INVALID || k == L) {
-                atomicAdd(&s_cnt[h], 1u);
+        uint32_t h = (uint32_t)(((uint64_t)(L * 0x9E3779B1u) * cap) >> 32);
+        while (true) {
+            unsigned long long cur = s_tab[h];
+            if (cur == EMPTY) {
+                cur = atomicCAS(&s_tab[h], EMPTY, ((unsigned long long)L << 32) | 1ull);
+                if (cur == EMPTY) break;  // inserted with count 1
+            }
+            if ((uint32_t)(cur >> 32) == L) {
+                atomicAdd(&s_tab[h], 1ull);
                 break;
             }
+            h = h + 1 == cap ? 0u : h + 1;
         }
     }
     __syncthreads();
+    // compaction: round r reads slots [256 r, 256 r + 256) and writes them
+    // below, at the count of occupied slots so far (never past what was read)
+    uint32_t nu = 0;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (uint32_t r0 = 0; r0 < cap; r0 += 256) {
+        const unsigned long long e = s_tab[r0 + tid];  // cap is a multiple of 256
+        const uint64_t occ = __ballot(e != EMPTY);
+        if (lane == 0) s_wn[wv] = (uint32_t)__builtin_popcountll(occ);
+        __syncthreads();
+        uint32_t off = nu;
+        for (int k = 0; k < wv; k++) off += s_wn[k];
+        if (e != EMPTY) s_tab[off + (uint32_t)__builtin_popcountll(occ & lt)] = e;
+        nu += s_wn[0] + s_wn[1] + s_wn[2] + s_wn[3];
+        __syncthreads();
+    }
     double bi = 0.0;
     uint32_t bl = SCD_INVALID;
     uint32_t heads = 0;
     const double pext = vtx_pext(a, p);
     constexpr int BATCH = 4;
-    for (uint32_t base = 0; base < cap; base += 256 * BATCH) {
+    for (uint32_t base = 0; base < nu; base += 256 * BATCH) {
         uint32_t v[BATCH], fr[BATCH];
         int4 st[BATCH];
 #pragma unroll
         for (int k = 0; k < BATCH; k++) {
             const uint32_t i = base + k * 256 + tid;
-            v[k] = i < cap ? s_key[i] : SCD_INVALID;
-            fr[k] = v[k] != SCD_INVALID ? s_cnt[i] : 0u;
+            const unsigned long long e = i < nu ? s_tab[i] : EMPTY;
+            v[k] = (uint32_t)(e >> 32);
+            fr[k] = i < nu ? (uint32_t)e : 0u;
         }
 #pragma unroll
         for (int k = 0; k < BATCH; k++) st[k] = fr[k] ? a.stats[v[k]] : make_int4(0, 0, 0, 0);
