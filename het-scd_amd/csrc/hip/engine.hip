@@ -921,6 +921,11 @@ extern "C" int scd_upload(scd_ctx *c, uint32_t n, uint32_t m, const uint32_t *in
     return finish_upload(c, indices);
 }
 
+// Private counter copies of the triangle kernels (wtc_hot_slot).
+static uint32_t hot_words(uint32_t n) {
+    return WTC_HOT_COPIES * std::min(n, WTC_HOT_MAX) + WTC_HOT1_COPIES * WTC_HOT1_N;
+}
+
 // Frees the previous graph and allocates every per-vertex and per-entry
 // buffer for n vertices and m edges.
 static int alloc_graph(scd_ctx *c, uint32_t n, uint32_t m, double clustering_coef) {
@@ -940,8 +945,7 @@ static int alloc_graph(scd_ctx *c, uint32_t n, uint32_t m, double clustering_coe
         (rc = dalloc(c, &c->int_deg, n)) || (rc = dalloc(c, &c->terms, n)) ||
         (rc = dalloc(c, &c->iflag, E)) || (rc = dalloc(c, &c->wdplus, n)) || (rc = dalloc(c, &c->wT, n)) ||
         (rc = dalloc(c, &c->wtd, n)) || (rc = dalloc(c, &c->wdirty, n)) ||
-        (rc = dalloc(c, &c->hot_T, (size_t)WTC_HOT_COPIES * std::min(n, WTC_HOT_MAX))) ||
-        (rc = dalloc(c, &c->hot_td, (size_t)WTC_HOT_COPIES * std::min(n, WTC_HOT_MAX))) ||
+        (rc = dalloc(c, &c->hot_T, hot_words(n))) || (rc = dalloc(c, &c->hot_td, hot_words(n))) ||
         (rc = dalloc(c, &c->partial, cdiv(n, 4096))) || (rc = dalloc(c, &c->wcc_dev, 1)) ||
         (rc = dalloc(c, &c->counters, 16)) || (rc = dalloc(c, &c->snap_labels, n)) ||
         (rc = dalloc(c, &c->snap_stats, n))) {
@@ -951,10 +955,8 @@ static int alloc_graph(scd_ctx *c, uint32_t n, uint32_t m, double clustering_coe
     c->lab_a = c->cur;
     c->lab_b = c->next;
     c->hot_n = std::min(n, WTC_HOT_MAX);
-    LAUNCH(c, "hot_clear", cdiv(WTC_HOT_COPIES * c->hot_n, 256), 256, k_fill_u32, c->hot_T,
-           WTC_HOT_COPIES * c->hot_n);
-    LAUNCH(c, "hot_clear", cdiv(WTC_HOT_COPIES * c->hot_n, 256), 256, k_fill_u32, c->hot_td,
-           WTC_HOT_COPIES * c->hot_n);
+    LAUNCH(c, "hot_clear", cdiv(hot_words(n), 256), 256, k_fill_u32, c->hot_T, hot_words(n));
+    LAUNCH(c, "hot_clear", cdiv(hot_words(n), 256), 256, k_fill_u32, c->hot_td, hot_words(n));
     return check_launch("allocation");
 }
 

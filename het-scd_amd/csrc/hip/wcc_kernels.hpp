@@ -61,6 +61,11 @@ struct WccArgs {
 
 constexpr uint32_t WTC_HOT_MAX = 8192;
 constexpr uint32_t WTC_HOT_COPIES = 16;
+// the hottest ids (the largest hubs collect tens of thousands of updates
+// each) get more copies, after the WTC_HOT_COPIES x hot_n block
+constexpr uint32_t WTC_HOT1_N = 256;
+constexpr uint32_t WTC_HOT1_COPIES = 64;
+constexpr uint32_t WTC_HOT_WORDS = WTC_HOT_COPIES * WTC_HOT_MAX + WTC_HOT1_COPIES * WTC_HOT1_N;
 constexpr uint32_t WTC_VC_CAP = 2048;  // the work queue's LDS set capacity
 
 // Edge u -> v (v < u, out-list lengths dpu, dpv) is counted from v's side.
@@ -68,13 +73,19 @@ __device__ __forceinline__ bool wtc_from_v(const WccArgs &a, uint32_t dpv, uint3
     return a.vc_min && dpv > a.vc_min && dpv <= WTC_VC_CAP && dpv > dpu;
 }
 
+__device__ __forceinline__ uint32_t wtc_hot_slot(const WccArgs &a, uint32_t x) {
+    return x < WTC_HOT1_N
+               ? WTC_HOT_COPIES * a.hot_n + (blockIdx.x & (WTC_HOT1_COPIES - 1)) * WTC_HOT1_N + x
+               : (blockIdx.x & (WTC_HOT_COPIES - 1)) * a.hot_n + x;
+}
+
 __device__ __forceinline__ void wtc_add_T(const WccArgs &a, uint32_t x, uint32_t c) {
-    if (x < a.hot_n) atomicAdd(&a.T_hot[(blockIdx.x & (WTC_HOT_COPIES - 1)) * a.hot_n + x], c);
+    if (x < a.hot_n) atomicAdd(&a.T_hot[wtc_hot_slot(a, x)], c);
     else atomicAdd(&a.T[x], c);
 }
 
 __device__ __forceinline__ void wtc_add_td(const WccArgs &a, uint32_t x, uint32_t c) {
-    if (x < a.hot_n) atomicAdd(&a.td_hot[(blockIdx.x & (WTC_HOT_COPIES - 1)) * a.hot_n + x], c);
+    if (x < a.hot_n) atomicAdd(&a.td_hot[wtc_hot_slot(a, x)], c);
     else atomicAdd(&a.td[x], c);
 }
 
@@ -89,6 +100,15 @@ __global__ void k_wtc_hot_sum(WccArgs a) {
         d += a.td_hot[k * a.hot_n + x];
         a.T_hot[k * a.hot_n + x] = 0;
         a.td_hot[k * a.hot_n + x] = 0;
+    }
+    if (x < WTC_HOT1_N) {
+        const uint32_t b = WTC_HOT_COPIES * a.hot_n + x;
+        for (uint32_t k = 0; k < WTC_HOT1_COPIES; k++) {
+            t += a.T_hot[b + k * WTC_HOT1_N];
+            d += a.td_hot[b + k * WTC_HOT1_N];
+            a.T_hot[b + k * WTC_HOT1_N] = 0;
+            a.td_hot[b + k * WTC_HOT1_N] = 0;
+        }
     }
     if (t) a.T[x] += t;
     if (d) a.td[x] += d;
