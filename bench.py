@@ -573,6 +573,9 @@ def main():
         tc = {"ms": round(tc_ms, 3), "algo_bytes": btc, "achieved_GBps": round(btc / (tc_ms * 1e-3) / 1e9, 1),
               "frac_of_hbm_peak": round(btc / (tc_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
               "timing": f"wall time per call over {reps_w} calls (lanes {'on' if tuning_lanes(eng) else 'off'})",
+              "bytes_model": "algorithmic: both out-lists of every oriented edge (merge intersection); the kernels "
+                             "read each owner's list once into LDS and walk the cheaper side, so this is a work "
+                             "rate, not HBM bandwidth (measured traffic in profiles/triangles_rmat_s20.md)",
               "kernel_sum_ms_one_stream": round(kern_ms, 3),
               "kernels": {name: round(ms / reps, 3) for name, (la, ms) in p.items() if la > 0}}
         log(f"[bench] triangle counting {tc_ms:.3f} ms per call ({tc['achieved_GBps']} GB/s algorithmic); "
