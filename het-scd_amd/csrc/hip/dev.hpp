@@ -107,6 +107,46 @@ __device__ __forceinline__ uint32_t move_rule(double best_imp, uint32_t best_lab
     return my;
 }
 
+// The value of lane (lane ^ J), J < 64, without the LDS crossbar that
+// shuffles use: DPP permutations inside rows of 16 lanes for J = 1, 2 (one
+// quad_perm) and 4, 8 (a row or half-row mirror, then a second one), and
+// the gfx950 row swaps v_permlane16_swap / v_permlane32_swap for J = 16, 32
+// (called with the value as both operands, each half of the result holds
+// the partner rows of one row parity).
+template <int J>
+__device__ __forceinline__ uint32_t lane_xor(uint32_t v) {
+    if constexpr (J == 1) {
+        return __builtin_amdgcn_update_dpp(0u, v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+    } else if constexpr (J == 2) {
+        return __builtin_amdgcn_update_dpp(0u, v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    } else if constexpr (J == 4) {
+        const uint32_t m = __builtin_amdgcn_update_dpp(0u, v, 0x141, 0xF, 0xF, false);  // row_half_mirror: ^7
+        return __builtin_amdgcn_update_dpp(0u, m, 0x1B, 0xF, 0xF, false);             // quad_perm [3,2,1,0]: ^3
+    } else if constexpr (J == 8) {
+        const uint32_t m = __builtin_amdgcn_update_dpp(0u, v, 0x140, 0xF, 0xF, false);  // row_mirror: ^15
+        return __builtin_amdgcn_update_dpp(0u, m, 0x141, 0xF, 0xF, false);            // row_half_mirror: ^7
+    } else if constexpr (J == 16) {
+        const auto sw = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        return (__lane_id() & 16) ? sw[0] : sw[1];
+    } else {
+        static_assert(J == 32, "lane_xor: J is a power of two below 64");
+        const auto sw = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        return (__lane_id() & 32) ? sw[0] : sw[1];
+    }
+}
+
+// lane_xor for a distance that is a constant after unrolling.
+__device__ __forceinline__ uint32_t lane_xor_c(uint32_t v, int j) {
+    switch (j) {
+        case 1: return lane_xor<1>(v);
+        case 2: return lane_xor<2>(v);
+        case 4: return lane_xor<4>(v);
+        case 8: return lane_xor<8>(v);
+        case 16: return lane_xor<16>(v);
+        default: return lane_xor<32>(v);
+    }
+}
+
 // Argmax over the lanes of aligned groups of S lanes (S a power of two
 // <= 64); every lane of a group ends with the group's winner.
 template <int S>
@@ -130,7 +170,7 @@ __device__ __forceinline__ uint32_t group_bitonic_sort(uint32_t v, int lig) {
     for (int k = 2; k <= S; k <<= 1) {
 #pragma unroll
         for (int j = k >> 1; j > 0; j >>= 1) {
-            const uint32_t o = __shfl_xor(v, j, SCD_WAVE);
+            const uint32_t o = lane_xor_c(v, j);
             const bool up = (lig & k) == 0;
             const bool lower = (lig & j) == 0;
             v = (lower == up) ? min(v, o) : max(v, o);
@@ -141,7 +181,8 @@ __device__ __forceinline__ uint32_t group_bitonic_sort(uint32_t v, int lig) {
 
 // Bitonic sort (ascending) of 64*R keys held by one wave: key i lives in
 // register r = i / 64 of lane i % 64.  Stages with a partner distance of 64
-// or more swap registers inside a lane; shorter ones exchange via shuffles.
+// or more swap registers inside a lane; shorter ones exchange across lanes
+// (lane_xor).
 template <int R>
 __device__ __forceinline__ void wave_bitonic_sort(uint32_t (&x)[R], int lane) {
     constexpr int N = 64 * R;
@@ -165,7 +206,7 @@ __device__ __forceinline__ void wave_bitonic_sort(uint32_t (&x)[R], int lane) {
             } else {
 #pragma unroll
                 for (int r = 0; r < R; r++) {
-                    const uint32_t o = __shfl_xor(x[r], j, SCD_WAVE);
+                    const uint32_t o = lane_xor_c(x[r], j);
                     const int i = r * 64 + lane;
                     const bool up = (i & k) == 0;
                     const bool lower = (lane & j) == 0;
