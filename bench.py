@@ -535,13 +535,15 @@ def main():
         import glob
         for fn in glob.glob(os.path.join(ROOT, "profiles", "*_traffic.json")):
             t = json.load(open(fn))
-            # the knobs that decide the kernel's vertex set and variant
+            # per tuning: the knobs that decide the kernel's vertex set and variant
             knobs = ("lds_tile", "wave_max", "lds_hash")
             cur = eng.get_tuning()
-            if (t.get("kernel") == roof["kernel"] and t.get("workload_graph_sha256") == graph_sha
-                    and all(t.get("tuning", {}).get(k) == cur[k] for k in knobs)):
-                roof["traffic"] = t["traffic_bytes"]
-                roof["traffic_source"] = os.path.relpath(fn, ROOT)
+            if t.get("kernel") != roof["kernel"] or t.get("workload_graph_sha256") != graph_sha:
+                continue
+            for ent in t.get("entries", []):
+                if all(ent["tuning"].get(k) == cur[k] for k in knobs):
+                    roof["traffic"] = ent["traffic_bytes"]
+                    roof["traffic_source"] = os.path.relpath(fn, ROOT)
     per_kernel = {name: {"launches": la, "avg_us": round(ms / la * 1e3, 2), "total_ms": round(ms, 3)}
                   for ms, name, la in kern}
 
