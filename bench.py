@@ -254,7 +254,8 @@ def run_served(args, scd_amd, np):
         f"{N - kinds.count('planted')} R-MAT), {sum(g.m for g in graphs)} edges ({gen_s:.1f} s to generate)")
 
     def serve(arrivals, join_wait_ms):
-        with scd_amd.Server(device=0, join_wait_ms=join_wait_ms, graph_mode=False) as srv:
+        with scd_amd.Server(device=0, join_wait_ms=join_wait_ms, graph_mode=False,
+                            intake_graphs=args.intake_graphs, intake_bytes=args.intake_mb << 20) as srv:
             ids = [srv.submit(g, arrival_ms=float(a)) for g, a in zip(graphs, arrivals)]
             t0 = time.perf_counter()
             srv.run()
@@ -424,6 +425,8 @@ def main():
     ap.add_argument("--served", action="store_true", default=True, help=argparse.SUPPRESS)
     ap.add_argument("--served-graphs", type=int, default=200)
     ap.add_argument("--join-wait-ms", type=float, default=5.0)
+    ap.add_argument("--intake-graphs", type=int, default=0, help="server intake batch, graphs (0: 64)")
+    ap.add_argument("--intake-mb", type=int, default=0, help="server intake batch, MiB of footprint (0: 2 GiB)")
     ap.add_argument("--tuning", default=None, metavar="JSON",
                     help='fixed tuning for the single-graph steps instead of the autotuner, e.g. '
                          '\'{"lds_tile": 4096, "wave_max": 1024}\'')

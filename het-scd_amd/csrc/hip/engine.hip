@@ -1055,6 +1055,13 @@ static int capture_iteration(scd_ctx *c, int par, int inc) {
     return rc;
 }
 
+extern "C" int scd_next_step_captures(const scd_ctx *c) {
+    if (!c || !c->use_graph || c->profile || !c->have_labels) return 0;
+    const int inc = c->wcc_valid ? 1 : 0;
+    const int par = c->cur == c->lab_a ? 0 : 1;
+    return c->gexec[par][inc] ? 0 : 1;
+}
+
 extern "C" int scd_improve(scd_ctx *c, double *wcc, uint32_t *labels_out) {
     int rc = need_labels(c, "scd_improve");
     if (rc) return rc;

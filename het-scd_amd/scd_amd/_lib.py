@@ -211,7 +211,8 @@ def ptr(a, t):
 
 class ScdServerOptsC(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("hbm_budget", ctypes.c_uint64), ("max_graphs", ctypes.c_uint32),
-                ("join_wait_ms", ctypes.c_double), ("max_iters", ctypes.c_uint32), ("graph_mode", ctypes.c_int)]
+                ("join_wait_ms", ctypes.c_double), ("max_iters", ctypes.c_uint32), ("graph_mode", ctypes.c_int),
+                ("intake_graphs", ctypes.c_uint32), ("intake_bytes", ctypes.c_uint64)]
 
 
 class ScdRequestC(ctypes.Structure):

@@ -13,8 +13,10 @@ class Server:
     """Requests are queued with submit(graph, arrival_ms) and served by
     run(), which replays the arrival times in wall-clock time."""
 
-    def __init__(self, device=0, hbm_budget=0, max_graphs=0, join_wait_ms=5.0, max_iters=1000, graph_mode=False):
-        o = _lib.ScdServerOptsC(device, hbm_budget, max_graphs, join_wait_ms, max_iters, 1 if graph_mode else 0)
+    def __init__(self, device=0, hbm_budget=0, max_graphs=0, join_wait_ms=5.0, max_iters=1000, graph_mode=False,
+                 intake_graphs=0, intake_bytes=0):
+        o = _lib.ScdServerOptsC(device, hbm_budget, max_graphs, join_wait_ms, max_iters, 1 if graph_mode else 0,
+                                intake_graphs, intake_bytes)
         self._s = ctypes.c_void_p()
         check_serve(_lib.serve().scd_server_create(ctypes.byref(o), ctypes.byref(self._s)))
         self._keep = []  # host arrays must outlive run()

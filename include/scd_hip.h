@@ -166,6 +166,12 @@ int scd_improve(scd_ctx *ctx, double *wcc, uint32_t *labels_out);
  * While profiling is on, iterations launch eagerly (per-kernel HIP events
  * need stream-recorded events). */
 int scd_set_graph_mode(scd_ctx *ctx, int on);
+/* 1 if the next scd_improve / scd_batch_step of this context captures its
+ * iteration into a HIP graph (graph mode, not captured yet for the current
+ * label buffers and WCC mode), else 0.  No counterpart in the reference: for
+ * callers that run other device work on other threads and must keep it out
+ * of a capture. */
+int scd_next_step_captures(const scd_ctx *ctx);
 
 /* Turns per-kernel HIP event timing (scd_prof_*) on or off. */
 int scd_set_profiling(scd_ctx *ctx, int on);

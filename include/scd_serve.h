@@ -45,6 +45,12 @@ typedef struct scd_server_opts {
     uint32_t max_iters;
     /* capture iterations as HIP graphs (1) or launch eagerly (0) */
     int graph_mode;
+    /* an intake batch (graphs copied and prepared together before they can
+     * join) holds at most this many graphs and footprint bytes (0: 64
+     * graphs, 2 GiB); smaller batches start refining sooner, each batch costs
+     * one preparation and one arena assembly */
+    uint32_t intake_graphs;
+    uint64_t intake_bytes;
 } scd_server_opts;
 
 typedef struct scd_request {
