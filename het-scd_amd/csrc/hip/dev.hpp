@@ -281,31 +281,44 @@ __device__ __forceinline__ uint32_t merge_path(const uint32_t *A, uint32_t la, c
     return lo;
 }
 
+// Chunks of 64 R keys at s[c * 64 R], c = w, w + 4, ..., sorted in
+// registers by wave w; chunks past the n real keys are padding only.
+template <int R>
+__device__ __forceinline__ void wave_sort_chunks(uint32_t *s, uint32_t n2, uint32_t n, int lane, int w) {
+    constexpr uint32_t C = 64u * R;
+    for (uint32_t c = w; c * C < n2; c += 4) {
+        if (c * C >= n) break;  // uniform per wave
+        uint32_t x[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) x[r] = s[c * C + r * 64 + lane];
+        wave_bitonic_sort<R>(x, lane);
+#pragma unroll
+        for (int r = 0; r < R; r++) s[c * C + r * 64 + lane] = x[r];
+    }
+}
+
 // Sort of n2 keys (a power of two, 1024 <= n2 <= T; n real keys, then
-// padding) held in LDS by a block of 256 threads: every wave sorts 1024-key
-// chunks in registers (wave_bitonic_sort<16>), then merge-path passes in LDS double the sorted
-// run width; each thread computes its n2/256 outputs of a pass in registers
-// and writes them back after a barrier.  Ends with a barrier.
+// padding) held in LDS by a block of 256 threads: the waves sort chunks of
+// n2 / 4 (at most 1024) keys in registers, then merge-path passes in LDS
+// double the sorted run width; each thread computes its n2/256 outputs of a
+// pass in registers and writes them back after a barrier.  Ends with a
+// barrier.
 template <int T>
 __device__ __forceinline__ void block_sort_lds(uint32_t *s, uint32_t n2, uint32_t n) {
     // positions n .. n2-1 hold padding (SCD_INVALID, the largest key): a
     // chunk of padding only is sorted already, and merging a run with a
-    // run of padding only leaves both in place
+    // run of padding only leaves both in place.  Chunks of n2 / 4 keys (at
+    // most 1024), so all four waves sort in registers
     constexpr int PER_MAX = T / 256;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (uint32_t c = w; c * 1024 < n2; c += 4) {
-        if (c * 1024 >= n) break;  // uniform per wave
-        uint32_t x[16];
-#pragma unroll
-        for (int r = 0; r < 16; r++) x[r] = s[c * 1024 + r * 64 + lane];
-        wave_bitonic_sort<16>(x, lane);
-#pragma unroll
-        for (int r = 0; r < 16; r++) s[c * 1024 + r * 64 + lane] = x[r];
-    }
+    const uint32_t chunk = min(n2 / 4, 1024u);
+    if (chunk <= 256) wave_sort_chunks<4>(s, n2, n, lane, w);
+    else if (chunk == 512) wave_sort_chunks<8>(s, n2, n, lane, w);
+    else wave_sort_chunks<16>(s, n2, n, lane, w);
     __syncthreads();
     const uint32_t per = n2 / 256;
     const uint32_t o0 = threadIdx.x * per;
-    for (uint32_t w2 = 1024; w2 < n2; w2 *= 2) {
+    for (uint32_t w2 = chunk; w2 < n2; w2 *= 2) {
         const uint32_t ps = (o0 / (2 * w2)) * (2 * w2);
         const bool skip = ps + w2 >= n;  // B is padding only
         const uint32_t *A = s + ps;
