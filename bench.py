@@ -286,6 +286,16 @@ def run_served(args, scd_amd, np):
         f"{np.percentile(lat, 50):.1f} ms p99 {np.percentile(lat, 99):.1f} ms, {st['joins']} joins, "
         f"busy {st['busy_ms'] / (span_s * 1e3):.2f}")
 
+    def serve_subset(sub):
+        with scd_amd.Server(device=0, join_wait_ms=0.0, graph_mode=False, intake_graphs=args.intake_graphs,
+                            intake_bytes=args.intake_mb << 20) as srv:
+            ids = [srv.submit(g, arrival_ms=0.0) for g in sub]
+            t0 = time.perf_counter()
+            srv.run()
+            wall = time.perf_counter() - t0
+            return [srv.result(i) for i in ids], srv.stats(), wall
+
+    plain_res = []
     eng = scd_amd.Engine(device=0)
     eng.set_graph_mode(False)
     eng.transfer_graph(graphs[0])
@@ -303,6 +313,7 @@ def run_served(args, scd_amd, np):
         best, iters, _ = eng.refine(w0)
         plain_ms[i] = (time.perf_counter() - ts) * 1e3
         lab = eng.labels_from_device()
+        plain_res.append((iters, best, w0, lab))
         for r in (res[i], res_c[i]):
             if (r["status"] != 0 or r["iterations"] != iters or r["best_wcc"] != best or r["initial_wcc"] != w0
                     or not np.array_equal(r["labels"], lab)):
@@ -311,6 +322,21 @@ def run_served(args, scd_amd, np):
     plain_gps = N / (plain_ms.sum() / 1e3)
     log(f"[bench] plain: {plain_ms.sum():.1f} ms summed over {N} graphs ({plain_gps:.1f} graphs/s, uploads "
         f"{plain_up_ms.sum():.1f} ms); result mismatches {mismatch}")
+
+    # the mix's small graphs (the planted ones) served on their own, every
+    # request present at time 0, against their plain times
+    small = [i for i, k in enumerate(kinds) if k == "planted"]
+    res_s, st_s, wall_s = serve_subset([graphs[i] for i in small])
+    small_mismatch = 0
+    for r, i in zip(res_s, small):
+        iters, best, w0, lab = plain_res[i]
+        if (r["status"] != 0 or r["iterations"] != iters or r["best_wcc"] != best or r["initial_wcc"] != w0
+                or not np.array_equal(r["labels"], lab)):
+            small_mismatch += 1
+    small_cap = len(small) / wall_s
+    small_plain = len(small) / (plain_ms[small].sum() / 1e3)
+    log(f"[bench] served small graphs only ({len(small)} planted, all at t=0): {small_cap:.1f} graphs/s vs plain "
+        f"{small_plain:.1f} = {small_cap / small_plain:.2f}x; result mismatches {small_mismatch}")
     pct = lambda x, q: round(float(np.percentile(x, q)), 3)
     return {"workload": f"{N} graphs, Poisson arrivals at 80% of measured capacity (seed 7): 90% planted partition "
                         "(n log-uniform [1000, 20000], 4-64 communities Dirichlet(1), p_in 0.15, p_out 0.002), "
@@ -328,7 +354,11 @@ def run_served(args, scd_amd, np):
                       "upload_ms": round(plain_up_ms.sum(), 2),
                       "p50_ms": pct(plain_ms, 50), "p99_ms": pct(plain_ms, 99)},
             "speedup_at_capacity": round(cap / plain_gps, 2),
-            "results_identical": mismatch == 0}
+            "results_identical": mismatch == 0,
+            "small_graphs_only": {"graphs": len(small), "capacity_graphs_per_s": round(small_cap, 1),
+                                  "plain_graphs_per_s": round(small_plain, 1),
+                                  "speedup": round(small_cap / small_plain, 2), "joins": st_s["joins"],
+                                  "results_identical": small_mismatch == 0}}
 
 
 def run_convert(args, scd_amd, np, edges, g):
