@@ -37,10 +37,14 @@ __device__ __forceinline__ bool stat_stopped(const StatArgs &a, uint32_t p) {
     return a.gid && !a.active_g[a.gid[p]];
 }
 
-// Adds one vertex's (1, external, internal) counts to community L.
+// Adds (members, external, internal) counts to community L.  Size and
+// external count share one 64-bit atomic (x in the low word, y in the high
+// word of the 16-byte entry): a size never reaches 2^32, so no carry crosses
+// into y, and the 32-bit halves wrap as the int32 sums of the reference do.
 __device__ __forceinline__ void add_stats(int4 *stats, uint32_t L, uint32_t sz, uint32_t ext, uint32_t cnt) {
-    if (sz > 0) atomicAdd(&stats[L].x, (int)sz);
-    if (ext > 0) atomicAdd(&stats[L].y, (int)ext);
+    if (sz > 0 || ext > 0)
+        atomicAdd(reinterpret_cast<unsigned long long *>(&stats[L].x),
+                  ((unsigned long long)ext << 32) | (unsigned long long)sz);
     if (cnt > 0) atomicAdd(&stats[L].z, (int)cnt);
 }
 
