@@ -162,6 +162,7 @@ def hip():
         "scd_bin_info": (ctypes.c_int, [C, ctypes.c_int, ctypes.c_void_p, u64p, u64p, u64p]),
         "scd_snapshot": (ctypes.c_int, [C]),
         "scd_set_graph_mode": (ctypes.c_int, [C, ctypes.c_int]),
+        "scd_next_step_captures": (ctypes.c_int, [C]),
         "scd_set_profiling": (ctypes.c_int, [C, ctypes.c_int]),
         "scd_set_counting": (ctypes.c_int, [C, ctypes.c_int]),
         "scd_restore": (ctypes.c_int, [C]),

@@ -244,3 +244,23 @@ def test_incremental_wcc_matches_full_recount(engine, manifest, name):
         assert np.array_equal(li, lf), f"{name}: labels differ at iteration {it + 1}"
         assert wi == wf, f"{name}: WCC {wi!r} vs full recount {wf!r} at iteration {it + 1}"
     assert engine.compute_wcc() == full[-1][0]
+
+
+def test_next_step_captures(engine):
+    """scd_next_step_captures tells whether the coming iteration records a
+    HIP graph: once per label-buffer parity and WCC mode, then replays."""
+    from scd_amd import _lib
+    g = graph("c1")
+    engine.transfer_graph(g)
+    engine.set_graph_mode(True)
+    engine.initial_partition(want_labels=False)
+    seen = []
+    for _ in range(6):
+        seen.append(_lib.hip().scd_next_step_captures(engine._ctx))
+        engine.improve_labels()
+    # iterations 1-3 capture (full WCC after the initial partition, then
+    # incremental at both parities), later ones replay
+    assert seen[0] == 1 and seen[-1] == 0 and seen[-2] == 0
+    engine.set_graph_mode(False)
+    assert _lib.hip().scd_next_step_captures(engine._ctx) == 0
+    engine.set_graph_mode(True)

@@ -102,3 +102,18 @@ def test_submit_rejects_bad_graphs():
         bad.indices[-1] += 2
         with pytest.raises(scd_amd.ScdError):
             srv.submit(bad)
+
+
+@pytest.mark.parametrize("intake_graphs,intake_mb,graph_mode", [(1, 0, True), (3, 0, False), (0, 1, True)])
+def test_intake_batches_equal_plain_path(mix, plain, intake_graphs, intake_mb, graph_mode):
+    """Small intake batches (one graph, three graphs, 1 MiB of footprint):
+    many more preparations and arena assemblies, overlapping with captured
+    or eager iterations — still each graph's plain result."""
+    print(f"[gpu] serving with intake {intake_graphs} graphs / {intake_mb} MiB, graph mode {graph_mode}", flush=True)
+    with scd_amd.Server(device=0, join_wait_ms=0.0, graph_mode=graph_mode, intake_graphs=intake_graphs,
+                        intake_bytes=intake_mb << 20) as srv:
+        ids = {name: srv.submit(g, arrival_ms=0.0) for name, g in mix}
+        srv.run()
+        for name, rid in ids.items():
+            check(name, srv.result(rid), plain[name])
+        assert srv.stats()["joins"] >= 2
