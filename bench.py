@@ -466,7 +466,7 @@ def main():
                     help="also run a plain copy kernel over MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the single-graph SCD path "
+        print(f"bench.py: the engine runs on one GPU (asked for {args.gpus}); the single-graph SCD path "
               "has no multi-GPU mode here", file=sys.stderr)
         return 2
 
