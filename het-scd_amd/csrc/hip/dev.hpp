@@ -217,6 +217,44 @@ __device__ __forceinline__ void wave_bitonic_sort(uint32_t (&x)[R], int lane) {
     }
 }
 
+// Bitonic sort (ascending) of 64*R keys held by one wave in the blocked
+// layout: key i lives in register r = i % R of lane i / R.  Partner
+// distances below R swap registers inside a lane, longer ones exchange
+// across lanes (lane_xor): log2(R) of the network's distance levels stay in
+// registers, against none below 64 in the strided layout.
+template <int R>
+__device__ __forceinline__ void wave_bitonic_sort_blocked(uint32_t (&x)[R], int lane) {
+    constexpr int N = 64 * R;
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j < R) {
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int rr = r ^ j;
+                    if (rr > r) {
+                        const bool up = ((lane * R + r) & k) == 0;
+                        const uint32_t a = x[r], b = x[rr];
+                        const bool sw = (a > b) == up;
+                        x[r] = sw ? b : a;
+                        x[rr] = sw ? a : b;
+                    }
+                }
+            } else {
+                const int lj = j / R;
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const uint32_t o = lane_xor_c(x[r], lj);
+                    const bool up = ((lane * R + r) & k) == 0;
+                    const bool lower = (lane & lj) == 0;
+                    x[r] = (lower == up) ? min(x[r], o) : max(x[r], o);
+                }
+            }
+        }
+    }
+}
+
 // Bitonic sort (ascending) of n2 keys (a power of two) in LDS by a block of
 // `nthreads` threads; ends with a barrier.
 __device__ __forceinline__ void lds_bitonic_sort(uint32_t *s, uint32_t n2, int tid, int nthreads) {

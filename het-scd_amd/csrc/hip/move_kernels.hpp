@@ -132,7 +132,9 @@ __global__ __launch_bounds__(256) void k_move_sub(MoveArgs a, const uint32_t *__
 }
 
 // ---------------------------------------------------------------------------
-// One wave per vertex, deg <= 64 * R; key i in register i / 64 of lane i % 64.
+// One wave per vertex, deg <= 64 * R; key i in register i % R of lane i / R
+// (blocked: each lane gathers R consecutive neighbours and the short sort
+// distances stay in registers).
 // ---------------------------------------------------------------------------
 template <int R>
 __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const uint32_t *__restrict__ verts,
@@ -149,26 +151,36 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const uint32_t *_
     const uint32_t beg = a.indices[p];
     const uint32_t deg = a.indices[p + 1] - beg;
     const double pext = vtx_pext(a, p);
+    const uint32_t i0 = (uint32_t)(lane * R);  // index of the lane's first key
 
     uint32_t x[R];
+    {
+        uint32_t q[R];
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-        const uint32_t i = (uint32_t)(r * 64 + lane);
-        x[r] = (i < deg) ? a.labels[a.adj[beg + i]] : SCD_INVALID;
+        for (int r = 0; r < R; r++) q[r] = (i0 + r < deg) ? a.adj[beg + i0 + r] : 0u;
+#pragma unroll
+        for (int r = 0; r < R; r++) x[r] = (i0 + r < deg) ? a.labels[q[r]] : SCD_INVALID;
     }
-    wave_bitonic_sort<R>(x, lane);
+    wave_bitonic_sort_blocked<R>(x, lane);
 
-    uint64_t B[R];
+    // run heads: key i differs from key i - 1 (key 0 always)
+    const uint32_t prev_lane_last = __shfl_up(x[R - 1], 1, SCD_WAVE);
     bool bnd[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
-        const uint32_t same_reg = __shfl(x[r], (lane + 63) & 63, SCD_WAVE);
-        uint32_t prev_reg = 0;
-        if (r > 0) prev_reg = __shfl(x[r - 1], 63, SCD_WAVE);
-        const uint32_t prev = lane ? same_reg : prev_reg;
-        bnd[r] = (r == 0 && lane == 0) || (x[r] != prev);
-        B[r] = __ballot(bnd[r]);
+        const uint32_t prev = r ? x[r - 1] : prev_lane_last;
+        bnd[r] = (i0 + r == 0) || (x[r] != prev);
     }
+    // the first head of every lane (64 R: none), for runs ending in a later lane
+    uint32_t first = 64u * R;
+#pragma unroll
+    for (int r = R - 1; r >= 0; r--)
+        if (bnd[r]) first = i0 + r;
+    const uint64_t has = __ballot(first != 64u * R);
+    const uint64_t later = lane < 63 ? (has >> (lane + 1)) : 0ull;
+    const int nl = later ? lane + 1 + __builtin_ctzll(later) : lane;
+    const uint32_t nfirst = __shfl(first, nl, SCD_WAVE);
+    const uint32_t next_lane_head = later ? nfirst : 64u * R;
 
     double bi = 0.0, rem_lane = 0.0;
     uint32_t bl = SCD_INVALID;
@@ -176,22 +188,12 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const uint32_t *_
     uint32_t heads = 0;
     uint32_t freq[R];
     int4 st[R];
+    {
+        uint32_t next = next_lane_head;  // next head after register r, scanning down
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-        const uint32_t i = (uint32_t)(r * 64 + lane);
-        const bool head = bnd[r] && x[r] != SCD_INVALID;
-        freq[r] = 0;
-        if (head) {
-            const uint64_t above = lane < 63 ? (B[r] >> (lane + 1)) : 0ull;
-            uint32_t next = 64u * R;
-            if (above) {
-                next = i + 1 + (uint32_t)__builtin_ctzll(above);
-            } else {
-#pragma unroll
-                for (int rr = R - 1; rr > r; rr--)
-                    if (B[rr]) next = (uint32_t)(rr * 64 + __builtin_ctzll(B[rr]));
-            }
-            freq[r] = next - i;
+        for (int r = R - 1; r >= 0; r--) {
+            freq[r] = (bnd[r] && x[r] != SCD_INVALID) ? next - (i0 + r) : 0u;
+            if (bnd[r]) next = i0 + r;
         }
     }
     // all statistics loads before the first evaluation
@@ -199,7 +201,7 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const uint32_t *_
     for (int r = 0; r < R; r++) st[r] = freq[r] ? a.stats[x[r]] : make_int4(0, 0, 0, 0);
 #pragma unroll
     for (int r = 0; r < R; r++) {
-        const uint32_t i = (uint32_t)(r * 64 + lane);
+        const uint32_t i = i0 + r;
         if (freq[r]) {
             const double imp = candidate_improvement_s(st[r], x[r], freq[r], deg, my, pext);
             if (x[r] == my) {
