@@ -328,10 +328,10 @@ __device__ __forceinline__ void wave_sort_chunks(uint32_t *s, uint32_t n2, uint3
         if (c * C >= n) break;  // uniform per wave
         uint32_t x[R];
 #pragma unroll
-        for (int r = 0; r < R; r++) x[r] = s[c * C + r * 64 + lane];
-        wave_bitonic_sort<R>(x, lane);
+        for (int r = 0; r < R; r++) x[r] = s[c * C + lane * R + r];
+        wave_bitonic_sort_blocked<R>(x, lane);
 #pragma unroll
-        for (int r = 0; r < R; r++) s[c * C + r * 64 + lane] = x[r];
+        for (int r = 0; r < R; r++) s[c * C + lane * R + r] = x[r];
     }
 }
 
