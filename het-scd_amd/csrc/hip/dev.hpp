@@ -179,44 +179,6 @@ __device__ __forceinline__ uint32_t group_bitonic_sort(uint32_t v, int lig) {
     return v;
 }
 
-// Bitonic sort (ascending) of 64*R keys held by one wave: key i lives in
-// register r = i / 64 of lane i % 64.  Stages with a partner distance of 64
-// or more swap registers inside a lane; shorter ones exchange across lanes
-// (lane_xor).
-template <int R>
-__device__ __forceinline__ void wave_bitonic_sort(uint32_t (&x)[R], int lane) {
-    constexpr int N = 64 * R;
-#pragma unroll
-    for (int k = 2; k <= N; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            if (j >= 64) {
-                const int jr = j >> 6;
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const int rr = r ^ jr;
-                    if (rr > r) {
-                        const bool up = ((r * 64) & k) == 0;
-                        const uint32_t a = x[r], b = x[rr];
-                        const bool sw = (a > b) == up;
-                        x[r] = sw ? b : a;
-                        x[rr] = sw ? a : b;
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const uint32_t o = lane_xor_c(x[r], j);
-                    const int i = r * 64 + lane;
-                    const bool up = (i & k) == 0;
-                    const bool lower = (lane & j) == 0;
-                    x[r] = (lower == up) ? min(x[r], o) : max(x[r], o);
-                }
-            }
-        }
-    }
-}
-
 // Bitonic sort (ascending) of 64*R keys held by one wave in the blocked
 // layout: key i lives in register r = i % R of lane i / R.  Partner
 // distances below R swap registers inside a lane, longer ones exchange
@@ -255,25 +217,6 @@ __device__ __forceinline__ void wave_bitonic_sort_blocked(uint32_t (&x)[R], int 
     }
 }
 
-// Bitonic sort (ascending) of n2 keys (a power of two) in LDS by a block of
-// `nthreads` threads; ends with a barrier.
-__device__ __forceinline__ void lds_bitonic_sort(uint32_t *s, uint32_t n2, int tid, int nthreads) {
-    for (uint32_t k = 2; k <= n2; k <<= 1) {
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t t = tid; t < (n2 >> 1); t += nthreads) {
-                const uint32_t i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
-                const uint32_t l = i + j;
-                const uint32_t a = s[i], b = s[l];
-                const bool up = (i & k) == 0;
-                if ((a > b) == up) {
-                    s[i] = b;
-                    s[l] = a;
-                }
-            }
-            __syncthreads();
-        }
-    }
-}
 
 // First index in sorted s[0..n) whose key is greater than x.
 __device__ __forceinline__ uint32_t upper_bound_u32(const uint32_t *s, uint32_t n, uint32_t x) {

@@ -2,7 +2,8 @@
 //
 // For every vertex p the kernels gather the labels of its neighbours, sort
 // them (own segmented sort: register bitonic across a wave64 or a lane
-// group, LDS bitonic for a workgroup, tiles + merge passes for hubs), reduce
+// group, register chunks + merge-path passes in LDS for a workgroup, tiles +
+// merge passes in global memory for hubs), reduce
 // equal labels to (label, count) runs with ballots (reduce-by-key), evaluate
 // the improvement of moving p into each neighbouring community with the CPU
 // path's fp64 arithmetic, take the argmax and apply the move rule.  Every
@@ -13,7 +14,7 @@
 // below serves one bin shape:
 //   k_move_sub<S>   deg <= S (8, 16, 32): S lanes per vertex, one key a lane
 //   k_move_wave<R>  deg <= 64R (R = 1..16): one wave per vertex, R keys a lane
-//   k_move_lds<T>   deg <= T: one workgroup per vertex, bitonic sort in LDS
+//   k_move_lds<T>   deg <= T: one workgroup per vertex, sort in LDS
 //   hub path        deg > T: tiles of T sorted in LDS by separate workgroups,
 //                   merged in global memory, then evaluated in chunks.
 // In export mode (MoveArgs::ex_sorted != nullptr) each kernel also writes
