@@ -747,16 +747,20 @@ static int count_internal_triangles(scd_ctx *c, const WccArgs &w) {
     // edge marks are complete: count them, again on lanes
     fork_lanes(c);
     on_lane(c, 3);
-    LAUNCH(c, "wcc_tri_final", cdiv(bin_n(c, K_SUB8), 32), 256, k_wtc_final<8>, w, bin_v(c, K_SUB8),
-           bin_n(c, K_SUB8));
+    // the whole graph: bounded grids, hub counts summed in LDS per workgroup
+    const int hub_lds = w.int_deg == nullptr ? 1 : 0;
+    const uint32_t fin_max = hub_lds ? WTC_FIN_BLOCKS_PER_CU * c->num_cus : 0xFFFFFFFFu;
+    LAUNCH(c, "wcc_tri_final", std::min(cdiv(bin_n(c, K_SUB8), 32), fin_max), 256, k_wtc_final<8>, w,
+           bin_v(c, K_SUB8), bin_n(c, K_SUB8), hub_lds);
     on_lane(c, 2);
-    LAUNCH(c, "wcc_tri_final", cdiv(bin_n(c, K_SUB16), 16), 256, k_wtc_final<16>, w, bin_v(c, K_SUB16),
-           bin_n(c, K_SUB16));
+    LAUNCH(c, "wcc_tri_final", std::min(cdiv(bin_n(c, K_SUB16), 16), fin_max), 256, k_wtc_final<16>, w,
+           bin_v(c, K_SUB16), bin_n(c, K_SUB16), hub_lds);
     on_lane(c, 1);
-    LAUNCH(c, "wcc_tri_final", cdiv(bin_n(c, K_SUB32), 8), 256, k_wtc_final<32>, w, bin_v(c, K_SUB32),
-           bin_n(c, K_SUB32));
+    LAUNCH(c, "wcc_tri_final", std::min(cdiv(bin_n(c, K_SUB32), 8), fin_max), 256, k_wtc_final<32>, w,
+           bin_v(c, K_SUB32), bin_n(c, K_SUB32), hub_lds);
     on_lane(c, 0);
-    LAUNCH(c, "wcc_tri_final", cdiv(c->wq_M, 4), 256, k_wtc_final<64>, w, bin_v(c, K_W1), c->wq_M);
+    LAUNCH(c, "wcc_tri_final", std::min(cdiv(c->wq_M, 4), fin_max), 256, k_wtc_final<64>, w, bin_v(c, K_W1),
+           c->wq_M, hub_lds);
     join_lanes(c);
     LAUNCH(c, "wcc_tri_hot", cdiv(w.hot_n, 256), 256, k_wtc_hot_sum, w);
     return SCD_OK;

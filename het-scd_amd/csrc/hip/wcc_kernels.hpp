@@ -563,28 +563,50 @@ __global__ __launch_bounds__(NT, NT >= 512 ? 8 : 1) void k_wtc_queue(WccArgs a, 
 }
 
 // Flagged internal out-edges count at both ends.  S lanes per vertex of
-// one degree bin.
+// one degree bin.  With hub_lds (the whole graph's triangles, where the
+// lower ends of most out-edges are the hubs, the lowest ids) a bounded grid
+// strides over the bin and each workgroup sums the counts for ids below
+// WTC_FIN_HOT in LDS, flushing them once at its end: one device-scope
+// atomic per hub and workgroup instead of one per edge.  The WCC's sparse
+// internal subgraphs gain nothing from it (one pass per workgroup, direct
+// atomics).
+constexpr uint32_t WTC_FIN_HOT = 2048;
+constexpr uint32_t WTC_FIN_BLOCKS_PER_CU = 4;
+
 template <int S>
-__global__ __launch_bounds__(256) void k_wtc_final(WccArgs a, const uint32_t *__restrict__ verts, uint32_t nv) {
+__global__ __launch_bounds__(256) void k_wtc_final(WccArgs a, const uint32_t *__restrict__ verts, uint32_t nv,
+                                                   int hub_lds) {
+    __shared__ uint32_t s_td[WTC_FIN_HOT];
     const int lane = threadIdx.x & 63;
     const int lig = lane & (S - 1);
-    const uint32_t slot = (blockIdx.x * 256u + threadIdx.x) / S;
-    uint32_t u = 0, beg = 0, dp = 0;
-    if (slot < nv) {
-        u = verts[slot];
-        dp = a.dplus[u];
-        if (dp) beg = a.indices[u];
-    }
-    uint32_t mine = 0;
-    for (uint32_t k = lig; k < dp; k += S) {
-        if (a.flag[beg + k]) {
-            wtc_add_td(a, a.int_adj[beg + k], 1u);
-            mine++;
+    constexpr uint32_t G = 256 / S;  // vertices per workgroup pass
+    const uint32_t hot = hub_lds ? WTC_FIN_HOT : 0u;  // uniform
+    for (uint32_t i = threadIdx.x; i < hot; i += 256) s_td[i] = 0;
+    __syncthreads();
+    for (uint32_t base = blockIdx.x * G; base < nv; base += gridDim.x * G) {
+        const uint32_t slot = base + threadIdx.x / S;
+        uint32_t u = 0, beg = 0, dp = 0;
+        if (slot < nv) {
+            u = verts[slot];
+            dp = a.dplus[u];
+            if (dp) beg = a.indices[u];
         }
-    }
+        uint32_t mine = 0;
+        for (uint32_t k = lig; k < dp; k += S) {
+            if (a.flag[beg + k]) {
+                const uint32_t w = a.int_adj[beg + k];
+                if (w < hot) atomicAdd(&s_td[w], 1u);
+                else wtc_add_td(a, w, 1u);
+                mine++;
+            }
+        }
 #pragma unroll
-    for (int off = S / 2; off > 0; off >>= 1) mine += __shfl_xor(mine, off, SCD_WAVE);
-    if (slot < nv && lig == 0 && mine) atomicAdd(&a.td[u], mine);
+        for (int off = S / 2; off > 0; off >>= 1) mine += __shfl_xor(mine, off, SCD_WAVE);
+        if (slot < nv && lig == 0 && mine) atomicAdd(&a.td[u], mine);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < hot; i += 256)
+        if (s_td[i]) wtc_add_td(a, i, s_td[i]);
 }
 
 // tri[u] = 2 x triangles at u (the layout of Graph::triangles).
