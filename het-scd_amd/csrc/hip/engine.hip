@@ -33,6 +33,7 @@
 #include "ip_kernels.hpp"
 #include "wcc_kernels.hpp"
 #include "arena_kernels.hpp"
+#include "bins_kernels.hpp"
 #include "convert_kernels.hpp"
 
 using namespace scd_dev;
@@ -63,8 +64,6 @@ extern "C" const char *scd_last_error(void) { return g_err; }
 // ---------------------------------------------------------------------------
 // Context
 // ---------------------------------------------------------------------------
-enum BinKind : uint32_t { K_SUB8, K_SUB16, K_SUB32, K_W1, K_W2, K_W4, K_W8, K_W16, K_LDS, K_HUB, K_NB };
-
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
@@ -96,7 +95,7 @@ struct scd_ctx {
     double cc = 0.0;
     bool have_graph = false, have_labels = false;
     uint32_t num_comms = 0;
-    std::vector<uint32_t> h_deg;  // degrees, kept for re-binning
+    uint32_t dmax = 0;  // largest degree (set with the degree bins)
 
     uint32_t *indices = nullptr, *adj = nullptr;
     int32_t *tri = nullptr, *tri_deg = nullptr;
@@ -132,6 +131,9 @@ struct scd_ctx {
     std::vector<uint32_t> h_label_off;  // G: first global label of each graph
     uint32_t *gid = nullptr, *active_g = nullptr, *store_g = nullptr, *k_g = nullptr;
     double *cc_g = nullptr, *wcc_g = nullptr, *wcc_g_host = nullptr;
+    size_t wcc_g_host_cap = 0;  // doubles held by wcc_g_host (pinned, kept across arenas)
+    void *h_stage = nullptr;    // pinned staging for label downloads (kept, grown)
+    size_t h_stage_bytes = 0;
     uint64_t *vtx_off_d = nullptr;
     uint32_t *witem_start = nullptr, *witem_end = nullptr, *g_item_begin = nullptr;
     uint32_t n_witems = 0;
@@ -289,8 +291,14 @@ static void free_graph(scd_ctx *c, bool keep_cache = true) {
     free_list(c, c->cache);
     if (keep_cache) c->cache.swap(c->bufs);
     else free_list(c, c->bufs);
-    if (c->wcc_g_host) (void)hipHostFree(c->wcc_g_host);
-    c->wcc_g_host = nullptr;
+    if (!keep_cache) {
+        if (c->wcc_g_host) (void)hipHostFree(c->wcc_g_host);
+        c->wcc_g_host = nullptr;
+        c->wcc_g_host_cap = 0;
+        if (c->h_stage) (void)hipHostFree(c->h_stage);
+        c->h_stage = nullptr;
+        c->h_stage_bytes = 0;
+    }
     c->G = 0;
     c->gid = c->active_g = c->store_g = c->k_g = nullptr;
     c->cc_g = c->wcc_g = nullptr;
@@ -396,36 +404,57 @@ static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b -
 // Degree bins and hub work lists (host-built, uploaded once per graph and
 // again when the tuning changes).
 // ---------------------------------------------------------------------------
-static uint32_t bin_of(uint32_t d, uint32_t lds_tile, uint32_t wave_max) {
-    if (d <= 8) return K_SUB8;
-    if (d <= 16) return K_SUB16;
-    if (d <= 32) return K_SUB32;
-    if (d > wave_max) return d <= lds_tile ? K_LDS : K_HUB;
-    if (d <= 64) return K_W1;
-    if (d <= 128) return K_W2;
-    if (d <= 256) return K_W4;
-    if (d <= 512) return K_W8;
-    if (d <= 1024) return K_W16;
-    if (d <= lds_tile) return K_LDS;
-    return K_HUB;
-}
-
+// Degree bins of the resident graph (bins_kernels.hpp) from its CSR offsets
+// on the device; the host receives per-workgroup counts, the degree sums and
+// the largest degree, and sends back each workgroup's offsets.
 static int build_bins(scd_ctx *c) {
     HIP_TRY(hipStreamSynchronize(c->stream));  // the old lists may still be read
     drop_graphs(c);
     release_tmp(c, c->hub_bufs);
     const uint32_t n = c->n, T = c->tune.lds_tile, C = c->tune.merge_chunk, Wm = c->tune.wave_max;
-    std::vector<uint32_t> cnt(K_NB + 1, 0);
-    for (uint32_t v = 0; v < n; v++) cnt[bin_of(c->h_deg[v], T, Wm) + 1]++;
-    for (int b = 0; b < K_NB; b++) cnt[b + 1] += cnt[b];
-    for (int b = 0; b <= K_NB; b++) c->bin_begin[b] = cnt[b];
-    std::vector<uint32_t> order(n), pos(cnt.begin(), cnt.end() - 1);
-    for (uint32_t v = 0; v < n; v++) order[pos[bin_of(c->h_deg[v], T, Wm)]++] = v;
-    for (int b = 0; b < 16; b++) c->bin_edges[b] = 0;
-    for (uint32_t v = 0; v < n; v++) c->bin_edges[bin_of(c->h_deg[v], T, Wm)] += c->h_deg[v];
+    const uint32_t nblk = cdiv(n, BIN_CHUNK);
+    std::vector<DevBuf> tmp;
+    struct TmpGuard {
+        scd_ctx *c;
+        std::vector<DevBuf> &t;
+        ~TmpGuard() { release_tmp(c, t); }
+    } guard{c, tmp};
+    uint32_t *cnt = nullptr, *offs = nullptr, *dmax = nullptr;
+    unsigned long long *esum = nullptr;
     int rc;
+    if ((rc = dalloc(c, &cnt, (size_t)nblk * K_NB, &tmp)) || (rc = dalloc(c, &offs, (size_t)nblk * K_NB, &tmp)) ||
+        (rc = dalloc(c, &esum, K_NB, &tmp)) || (rc = dalloc(c, &dmax, 1, &tmp)))
+        return rc;
+    LAUNCH(c, "bins", 1, 64, k_fill_u64v, esum, (uint64_t)K_NB, 0ull);
+    LAUNCH(c, "bins", 1, 64, k_fill_u32, dmax, 1u);
+    LAUNCH(c, "bins", nblk, 256, k_bin_count, c->indices, n, T, Wm, cnt, esum, dmax);
+    if ((rc = check_launch("degree bins"))) return rc;
+    std::vector<uint32_t> h_cnt((size_t)nblk * K_NB), h_off((size_t)nblk * K_NB);
+    unsigned long long h_esum[K_NB];
+    uint32_t h_dmax = 0;
+    HIP_TRY(hipMemcpyAsync(h_cnt.data(), cnt, 4ull * nblk * K_NB, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(h_esum, esum, sizeof(h_esum), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&h_dmax, dmax, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::vector<uint32_t> tot(K_NB, 0);
+    for (uint32_t k = 0; k < nblk; k++)
+        for (int b = 0; b < K_NB; b++) tot[b] += h_cnt[(size_t)k * K_NB + b];
+    c->bin_begin[0] = 0;
+    for (int b = 0; b < K_NB; b++) c->bin_begin[b + 1] = c->bin_begin[b] + tot[b];
+    for (int b = 0; b < K_NB; b++) {
+        uint32_t o = c->bin_begin[b];
+        for (uint32_t k = 0; k < nblk; k++) {
+            h_off[(size_t)k * K_NB + b] = o;
+            o += h_cnt[(size_t)k * K_NB + b];
+        }
+    }
+    for (int b = 0; b < 16; b++) c->bin_edges[b] = b < K_NB ? h_esum[b] : 0;
+    c->dmax = h_dmax;
     if ((rc = dalloc(c, &c->bin_verts, n, &c->hub_bufs))) return rc;
-    HIP_TRY(hipMemcpy(c->bin_verts, order.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync(offs, h_off.data(), 4ull * nblk * K_NB, hipMemcpyHostToDevice, c->stream));
+    LAUNCH(c, "bins", nblk, 256, k_bin_scatter, c->indices, n, T, Wm, offs, c->bin_verts);
+    if ((rc = check_launch("degree bins"))) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
 
     // internal triangle work queue
     c->wq_M = c->bin_begin[K_NB] - c->bin_begin[K_W1];
@@ -455,11 +484,18 @@ static int build_bins(scd_ctx *c) {
     c->n_tiles = c->n_chunks = c->max_hub_deg = 0;
     if (nh == 0) return SCD_OK;
     std::vector<uint32_t> hv(nh), hd(nh), hoff(nh), hcb(nh + 1), th, ts, ch, cs;
+    {
+        uint32_t *d_hd = nullptr;
+        if ((rc = dalloc(c, &d_hd, nh, &tmp))) return rc;
+        LAUNCH(c, "bins", cdiv(nh, 256), 256, k_bin_degrees, c->indices, c->bin_verts + hb0, nh, d_hd);
+        if ((rc = check_launch("degree bins"))) return rc;
+        HIP_TRY(hipMemcpyAsync(hv.data(), c->bin_verts + hb0, 4ull * nh, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(hd.data(), d_hd, 4ull * nh, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
     uint64_t off = 0;
     for (uint32_t h = 0; h < nh; h++) {
-        const uint32_t v = order[hb0 + h], d = c->h_deg[v];
-        hv[h] = v;
-        hd[h] = d;
+        const uint32_t d = hd[h];
         hoff[h] = (uint32_t)off;
         off += d;
         c->max_hub_deg = std::max(c->max_hub_deg, d);
@@ -901,7 +937,7 @@ extern "C" void scd_destroy(scd_ctx *c) {
 }
 
 static int alloc_graph(scd_ctx *c, uint32_t n, uint32_t m, double clustering_coef);
-static int finish_upload(scd_ctx *c, const uint32_t *h_indices);
+static int finish_upload(scd_ctx *c);
 
 extern "C" int scd_upload(scd_ctx *c, uint32_t n, uint32_t m, const uint32_t *indices,
                           const uint32_t *adj, const int32_t *tri, const int32_t *tri_deg,
@@ -922,7 +958,7 @@ extern "C" int scd_upload(scd_ctx *c, uint32_t n, uint32_t m, const uint32_t *in
     if (E) HIP_TRY(hipMemcpyAsync(c->adj, adj, sizeof(uint32_t) * E, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->tri, tri, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->tri_deg, tri_deg, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
-    return finish_upload(c, indices);
+    return finish_upload(c);
 }
 
 // Private counter copies of the triangle kernels (wtc_hot_slot).
@@ -966,10 +1002,8 @@ static int alloc_graph(scd_ctx *c, uint32_t n, uint32_t m, double clustering_coe
 
 // After the graph arrays are on the device (enqueued on the stream): host
 // degrees from the host copy of indices, degree bins.
-static int finish_upload(scd_ctx *c, const uint32_t *h_indices) {
+static int finish_upload(scd_ctx *c) {
     HIP_TRY(hipStreamSynchronize(c->stream));
-    c->h_deg.resize(c->n);
-    for (uint32_t v = 0; v < c->n; v++) c->h_deg[v] = h_indices[v + 1] - h_indices[v];
     int rc = build_bins(c);
     if (rc) {
         free_graph(c);
@@ -1530,9 +1564,9 @@ static int sort_u64(scd_ctx *c, uint64_t **keys, uint64_t **alt, uint64_t n, std
 }
 
 // CSR (indices of n + 1 entries, adj) of E sorted directed keys over n
-// vertices.  Returns the host copy of indices.
+// vertices; h_idx (may be null) receives the host copy of indices.
 static int csr_of_keys(scd_ctx *c, const uint64_t *dk, uint64_t E, uint32_t n, uint32_t *indices, uint32_t *adj,
-                       std::vector<uint32_t> &h_idx, std::vector<DevBuf> &tmp) {
+                       std::vector<uint32_t> *h_idx, std::vector<DevBuf> &tmp) {
     uint32_t *deg = nullptr;
     int rc;
     if ((rc = dalloc(c, &deg, n, &tmp))) return rc;
@@ -1542,10 +1576,13 @@ static int csr_of_keys(scd_ctx *c, const uint64_t *dk, uint64_t E, uint32_t n, u
     uint32_t total = 0;
     if ((rc = scan_u32(c, deg, indices, n, &total, tmp))) return rc;
     if (total != E) return fail(SCD_EDEVICE, "convert: CSR holds %u of %llu entries", total, (unsigned long long)E);
-    h_idx.resize((size_t)n + 1);
-    h_idx[n] = (uint32_t)E;
-    HIP_TRY(hipMemcpyAsync(indices + n, &h_idx[n], 4, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(h_idx.data(), indices, 4ull * n, hipMemcpyDeviceToHost, c->stream));
+    const uint32_t last = (uint32_t)E;
+    HIP_TRY(hipMemcpyAsync(indices + n, &last, 4, hipMemcpyHostToDevice, c->stream));
+    if (h_idx) {
+        h_idx->resize((size_t)n + 1);
+        (*h_idx)[n] = last;
+        HIP_TRY(hipMemcpyAsync(h_idx->data(), indices, 4ull * n, hipMemcpyDeviceToHost, c->stream));
+    }
     HIP_TRY(hipStreamSynchronize(c->stream));
     return SCD_OK;
 }
@@ -1638,11 +1675,10 @@ static int convert_impl(scd_ctx *c, uint64_t count, const uint32_t *u, const uin
     LAUNCH(c, "cv_directed", cdiv(count, 256), 256, k_cv_directed, ka, count, keep, pos, rank, da);
     if ((rc = sort_u64(c, &da, &db, E1, tmp))) return rc;
     if ((rc = alloc_graph(c, nv, mu, 0.0))) return rc;
-    std::vector<uint32_t> h_idx;
-    if ((rc = csr_of_keys(c, da, E1, nv, c->indices, c->adj, h_idx, tmp))) return rc;
+    if ((rc = csr_of_keys(c, da, E1, nv, c->indices, c->adj, nullptr, tmp))) return rc;
     LAUNCH(c, "cv_csr", cdiv(nv, 256), 256, k_fill_u32, (uint32_t *)c->tri, nv);
     LAUNCH(c, "cv_csr", cdiv(nv, 256), 256, k_fill_u32, (uint32_t *)c->tri_deg, nv);
-    if ((rc = finish_upload(c, h_idx.data()))) return rc;
+    if ((rc = finish_upload(c))) return rc;
     // 4. triangles per vertex and triangle edges (src/convert.cpp:321-340)
     uint32_t *T = nullptr, *td = nullptr;
     uint8_t *flag = nullptr;
@@ -1680,7 +1716,8 @@ static int convert_impl(scd_ctx *c, uint64_t count, const uint32_t *u, const uin
     LAUNCH(c, "cv_final", cdiv(E1, 256), 256, k_cv_final_keys, da, flag, E1, newid, kept, fa);
     LAUNCH(c, "cv_final", cdiv(nv, 256), 256, k_cv_final_tri, T, td, newid, nv, n_new, f_tri, f_td);
     if ((rc = sort_u64(c, &fa, &fb, E2, tmp))) return rc;
-    if ((rc = csr_of_keys(c, fa, E2, n_new, f_idx, f_adj, h_idx, tmp))) return rc;
+    std::vector<uint32_t> h_idx;
+    if ((rc = csr_of_keys(c, fa, E2, n_new, f_idx, f_adj, &h_idx, tmp))) return rc;
     // clustering coefficient in vertex order (src/graph.cpp:57-68)
     std::vector<int32_t> h_tri(n_new);
     HIP_TRY(hipMemcpyAsync(h_tri.data(), f_tri, 4ull * n_new, hipMemcpyDeviceToHost, c->stream));
@@ -1697,7 +1734,7 @@ static int convert_impl(scd_ctx *c, uint64_t count, const uint32_t *u, const uin
     HIP_TRY(hipMemcpyAsync(c->adj, f_adj, 4ull * E2, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->tri, f_tri, 4ull * n_new, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->tri_deg, f_td, 4ull * n_new, hipMemcpyDeviceToDevice, c->stream));
-    if ((rc = finish_upload(c, h_idx.data())) || (rc = stage(info.ms_relabel))) return rc;
+    if ((rc = finish_upload(c)) || (rc = stage(info.ms_relabel))) return rc;
     *n_out = n_new;
     *m_out = mk;
     *cc_out = cc;
@@ -1864,7 +1901,7 @@ extern "C" int scd_initial_partition(scd_ctx *c, uint32_t *labels_out, uint32_t 
 // preserving each graph's label order and hence its tie-breaks) and local
 // (0-based per graph) at the API.
 // ---------------------------------------------------------------------------
-static int setup_batch(scd_ctx *c, uint32_t G, const uint64_t *vtx_off, const double *cc);
+static int setup_batch(scd_ctx *c, uint32_t G, const uint64_t *vtx_off, const double *cc, bool host_gid);
 
 extern "C" int scd_batch_upload(scd_ctx *c, uint32_t G, const uint64_t *vtx_off, const uint32_t *indices,
                                 const uint32_t *adj, const int32_t *tri, const int32_t *tri_deg, const double *cc) {
@@ -1880,18 +1917,18 @@ extern "C" int scd_batch_upload(scd_ctx *c, uint32_t G, const uint64_t *vtx_off,
     if (E & 1u) return fail(SCD_EINVAL, "scd_batch_upload: odd adjacency length %u", E);
     int rc = scd_upload(c, n, E / 2, indices, adj, tri, tri_deg, 0.0);
     if (rc) return rc;
-    return setup_batch(c, G, vtx_off, cc);
+    return setup_batch(c, G, vtx_off, cc, true);
 }
 
 // Per-graph state of an arena whose union graph is uploaded: graph of each
-// vertex, clustering coefficients, activity and store flags, WCC reduction
-// items.  Labels offsets start compact (set by the initial partition).
-static int setup_batch(scd_ctx *c, uint32_t G, const uint64_t *vtx_off, const double *cc) {
+// vertex (written here when host_gid is set, else by the caller's kernel),
+// clustering coefficients, activity and store flags, WCC reduction items.
+// Labels offsets start compact (set by the initial partition).
+static int setup_batch(scd_ctx *c, uint32_t G, const uint64_t *vtx_off, const double *cc, bool host_gid) {
     const uint32_t n = c->n;
     int rc;
-    std::vector<uint32_t> gid(n), ws, we, gib(G + 1);
+    std::vector<uint32_t> ws, we, gib(G + 1);
     for (uint32_t g = 0; g < G; g++) {
-        for (uint64_t v = vtx_off[g]; v < vtx_off[g + 1]; v++) gid[v] = g;
         gib[g] = (uint32_t)ws.size();
         for (uint64_t s = vtx_off[g]; s < vtx_off[g + 1]; s += 4096) {
             ws.push_back((uint32_t)s);
@@ -1908,24 +1945,70 @@ static int setup_batch(scd_ctx *c, uint32_t G, const uint64_t *vtx_off, const do
         free_graph(c);
         return rc;
     }
-    hipError_t e = hipHostMalloc((void **)&c->wcc_g_host, 8ull * G, hipHostMallocDefault);
-    if (e != hipSuccess) {
+    if (c->wcc_g_host_cap < G) {
+        const size_t cap = std::max<size_t>(G, 2 * c->wcc_g_host_cap);
+        if (c->wcc_g_host) (void)hipHostFree(c->wcc_g_host);
         c->wcc_g_host = nullptr;
-        free_graph(c);
-        return fail(SCD_ENOMEM, "hipHostMalloc: %s", hipGetErrorString(e));
+        c->wcc_g_host_cap = 0;
+        hipError_t e = hipHostMalloc((void **)&c->wcc_g_host, 8ull * cap, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            c->wcc_g_host = nullptr;
+            free_graph(c);
+            return fail(SCD_ENOMEM, "hipHostMalloc: %s", hipGetErrorString(e));
+        }
+        c->wcc_g_host_cap = cap;
     }
-    std::vector<uint32_t> ones(G, 1u), zeros(G, 0u);
-    HIP_TRY(hipMemcpy(c->gid, gid.data(), 4ull * n, hipMemcpyHostToDevice));
+    if (host_gid) {
+        std::vector<uint32_t> gid(n);
+        for (uint32_t g = 0; g < G; g++)
+            for (uint64_t v = vtx_off[g]; v < vtx_off[g + 1]; v++) gid[v] = g;
+        HIP_TRY(hipMemcpy(c->gid, gid.data(), 4ull * n, hipMemcpyHostToDevice));
+    }
+    // the small per-graph tables in one copy
+    std::vector<uint32_t> pack;
+    pack.reserve(4ull * G + 2 * ws.size() + G + 1);
+    pack.insert(pack.end(), G, 1u);  // active
+    pack.insert(pack.end(), G, 0u);  // store
+    pack.insert(pack.end(), ws.begin(), ws.end());
+    pack.insert(pack.end(), we.begin(), we.end());
+    pack.insert(pack.end(), gib.begin(), gib.end());
     HIP_TRY(hipMemcpy(c->cc_g, cc, 8ull * G, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->active_g, ones.data(), 4ull * G, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->store_g, zeros.data(), 4ull * G, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->vtx_off_d, vtx_off, 8ull * (G + 1), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->witem_start, ws.data(), 4ull * ws.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->witem_end, we.data(), 4ull * we.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->g_item_begin, gib.data(), 4ull * (G + 1), hipMemcpyHostToDevice));
+    std::vector<DevBuf> tmp;
+    uint32_t *d_pack = nullptr;
+    if ((rc = dalloc(c, &d_pack, pack.size(), &tmp))) return rc;
+    HIP_TRY(hipMemcpy(d_pack, pack.data(), 4ull * pack.size(), hipMemcpyHostToDevice));
+    size_t o = 0;
+    HIP_TRY(hipMemcpyAsync(c->active_g, d_pack + o, 4ull * G, hipMemcpyDeviceToDevice, c->stream));
+    o += G;
+    HIP_TRY(hipMemcpyAsync(c->store_g, d_pack + o, 4ull * G, hipMemcpyDeviceToDevice, c->stream));
+    o += G;
+    HIP_TRY(hipMemcpyAsync(c->witem_start, d_pack + o, 4ull * ws.size(), hipMemcpyDeviceToDevice, c->stream));
+    o += ws.size();
+    HIP_TRY(hipMemcpyAsync(c->witem_end, d_pack + o, 4ull * we.size(), hipMemcpyDeviceToDevice, c->stream));
+    o += we.size();
+    HIP_TRY(hipMemcpyAsync(c->g_item_begin, d_pack + o, 4ull * (G + 1), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    release_tmp(c, tmp);
     c->G = G;
     c->h_vtx_off.assign(vtx_off, vtx_off + G + 1);
     c->h_label_off.assign(G, 0);
+    return SCD_OK;
+}
+
+// Pinned host staging of at least `bytes` (kept and grown).
+static int host_stage(scd_ctx *c, size_t bytes) {
+    if (c->h_stage_bytes >= bytes) return SCD_OK;
+    if (c->h_stage) (void)hipHostFree(c->h_stage);
+    c->h_stage = nullptr;
+    const size_t want = std::max(bytes, 2 * c->h_stage_bytes);
+    c->h_stage_bytes = 0;
+    hipError_t e = hipHostMalloc(&c->h_stage, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        c->h_stage = nullptr;
+        return fail(SCD_ENOMEM, "hipHostMalloc: %s", hipGetErrorString(e));
+    }
+    c->h_stage_bytes = want;
     return SCD_OK;
 }
 
@@ -2070,69 +2153,112 @@ extern "C" int scd_batch_upload_parts(scd_ctx *c, uint32_t G, const scd_part *pa
     if (vtx_off[G] >= 0xFFFFFFFFull) return fail(SCD_ELIMIT, "scd_batch_upload_parts: too many vertices");
     if (eoff[G] >= (1ull << 32)) return fail(SCD_ELIMIT, "scd_batch_upload_parts: 2m >= 2^32");
     const uint32_t n = (uint32_t)vtx_off[G];
-    int rc = alloc_graph(c, n, (uint32_t)(eoff[G] / 2), 0.0);
+    const uint32_t Eall = (uint32_t)eoff[G];
+    int rc = alloc_graph(c, n, Eall / 2, 0.0);
     if (rc) return rc;
+    std::vector<ArenaPart> table(G);
     for (uint32_t g = 0; g < G; g++) {
         const scd_part &p = parts[g];
-        const uint32_t vo = (uint32_t)vtx_off[g], eo = (uint32_t)eoff[g];
-        LAUNCH(c, "arena_indices", cdiv(p.n, 256), 256, k_arena_shift, p.indices, (uint64_t)p.n, eo,
-               c->indices + vo);
-        if (p.m)
-            LAUNCH(c, "arena_adj", cdiv(2ull * p.m, 256), 256, k_arena_shift, p.adj, 2ull * p.m, vo, c->adj + eo);
-        HIP_TRY(hipMemcpyAsync(c->tri + vo, p.tri, 4ull * p.n, hipMemcpyDeviceToDevice, c->stream));
-        HIP_TRY(hipMemcpyAsync(c->tri_deg + vo, p.tri_deg, 4ull * p.n, hipMemcpyDeviceToDevice, c->stream));
+        const uint32_t *cl = cur_labels ? cur_labels[g] : nullptr;
+        const uint32_t *bl = cur_labels && best_labels ? best_labels[g] : nullptr;
+        table[g] = {p.indices, p.adj, p.tri, p.tri_deg, cl, bl, p.n, (uint32_t)vtx_off[g], 2u * p.m, (uint32_t)eoff[g]};
     }
-    const uint32_t Eall = (uint32_t)eoff[G];
-    HIP_TRY(hipMemcpyAsync(c->indices + n, &Eall, 4, hipMemcpyHostToDevice, c->stream));
-    if ((rc = check_launch("arena assembly"))) return rc;
-    std::vector<uint32_t> h_ind((size_t)n + 1);
-    HIP_TRY(hipMemcpyAsync(h_ind.data(), c->indices, 4ull * (n + 1), hipMemcpyDeviceToHost, c->stream));
-    if ((rc = finish_upload(c, h_ind.data())) || (rc = setup_batch(c, G, vtx_off.data(), cc.data()))) return rc;
+    std::vector<DevBuf> tmp;
+    ArenaPart *d_table = nullptr;
+    if ((rc = dalloc(c, &d_table, G, &tmp))) return rc;
+    HIP_TRY(hipMemcpyAsync(d_table, table.data(), sizeof(ArenaPart) * G, hipMemcpyHostToDevice, c->stream));
+    if ((rc = setup_batch(c, G, vtx_off.data(), cc.data(), false))) {
+        release_tmp(c, tmp);
+        return rc;
+    }
+    LAUNCH(c, "arena_vertices", cdiv(n + 1ull, ARENA_CHUNK), 256, k_arena_vertices, d_table, G, n, Eall,
+           cur_labels ? 1 : 0, c->indices, c->tri, c->tri_deg, c->gid, c->cur, c->best);
+    if (Eall) LAUNCH(c, "arena_entries", cdiv(Eall, ARENA_CHUNK), 256, k_arena_entries, d_table, G, Eall, c->adj);
+    if ((rc = check_launch("arena assembly"))) {
+        release_tmp(c, tmp);
+        return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    rc = finish_upload(c);
+    release_tmp(c, tmp);
+    if (rc) return rc;
     if (!cur_labels) return SCD_OK;
     // labels of graph g live at [vtx_off[g], vtx_off[g+1]) of the label space
-    for (uint32_t g = 0; g < G; g++) {
-        const uint32_t vo = (uint32_t)vtx_off[g];
-        LAUNCH(c, "arena_labels", cdiv(parts[g].n, 256), 256, k_arena_labels, cur_labels[g], parts[g].n, vo, 1,
-               c->cur + vo);
-        const uint32_t *b = best_labels && best_labels[g] ? best_labels[g] : cur_labels[g];
-        LAUNCH(c, "arena_labels", cdiv(parts[g].n, 256), 256, k_arena_labels, b, parts[g].n, vo, 1, c->best + vo);
-        c->h_label_off[g] = vo;
-    }
+    for (uint32_t g = 0; g < G; g++) c->h_label_off[g] = (uint32_t)vtx_off[g];
     c->num_comms = n;
-    if ((rc = check_launch("arena labels")) || (rc = clear_counters(c)) || (rc = phase_stats(c)) ||
-        (rc = finish(c)))
-        return rc;
+    if ((rc = clear_counters(c)) || (rc = phase_stats(c)) || (rc = finish(c))) return rc;
     c->have_labels = true;
     c->wcc_valid = false;
     return SCD_OK;
 }
 
+// Labels of the graphs listed in outs (already holding vo, n and label
+// offsets, outputs and positions) leave the arena in one launch.
+static int export_table(scd_ctx *c, const std::vector<ArenaOut> &outs, uint32_t total) {
+    if (outs.empty() || total == 0) return SCD_OK;
+    std::vector<DevBuf> tmp;
+    ArenaOut *d_outs = nullptr;
+    int rc = dalloc(c, &d_outs, outs.size(), &tmp);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(d_outs, outs.data(), sizeof(ArenaOut) * outs.size(), hipMemcpyHostToDevice, c->stream));
+    LAUNCH(c, "arena_export", cdiv(total, ARENA_CHUNK), 256, k_arena_export, d_outs, (uint32_t)outs.size(), total,
+           c->cur, c->best);
+    rc = check_launch("label export");
+    if (!rc) rc = finish(c);
+    release_tmp(c, tmp);
+    return rc;
+}
+
 extern "C" int scd_batch_export_labels(scd_ctx *c, uint32_t *const *cur_out, uint32_t *const *best_out) {
     int rc = need_batch(c, "scd_batch_export_labels", true);
     if (rc) return rc;
+    std::vector<ArenaOut> outs;
+    uint32_t total = 0;
     for (uint32_t g = 0; g < c->G; g++) {
+        uint32_t *co = cur_out ? cur_out[g] : nullptr, *bo = best_out ? best_out[g] : nullptr;
+        if (!co && !bo) continue;
         const uint32_t vo = (uint32_t)c->h_vtx_off[g], ng = (uint32_t)(c->h_vtx_off[g + 1] - vo);
-        if (cur_out && cur_out[g])
-            LAUNCH(c, "arena_export", cdiv(ng, 256), 256, k_arena_labels, c->cur + vo, ng, c->h_label_off[g], 0,
-                   cur_out[g]);
-        if (best_out && best_out[g])
-            LAUNCH(c, "arena_export", cdiv(ng, 256), 256, k_arena_labels, c->best + vo, ng, c->h_label_off[g], 0,
-                   best_out[g]);
+        outs.push_back({co, bo, vo, ng, c->h_label_off[g], total});
+        total += ng;
     }
-    if ((rc = check_launch("label export"))) return rc;
-    return finish(c);
+    return export_table(c, outs, total);
 }
 
 extern "C" int scd_batch_graph_labels(scd_ctx *c, uint32_t g, int best, uint32_t *labels_out) {
-    int rc = need_batch(c, "scd_batch_graph_labels", true);
+    return scd_batch_graphs_labels(c, 1, &g, best, &labels_out);
+}
+
+extern "C" int scd_batch_graphs_labels(scd_ctx *c, uint32_t count, const uint32_t *graphs, int best,
+                                       uint32_t *const *labels_out) {
+    int rc = need_batch(c, "scd_batch_graphs_labels", true);
     if (rc) return rc;
-    if (g >= c->G || !labels_out) return fail(SCD_EINVAL, "scd_batch_graph_labels: bad graph or null output");
-    const uint64_t vo = c->h_vtx_off[g], ng = c->h_vtx_off[g + 1] - vo;
-    HIP_TRY(hipMemcpyAsync(labels_out, (best ? c->best : c->cur) + vo, 4ull * ng, hipMemcpyDeviceToHost, c->stream));
-    if ((rc = finish(c))) return rc;
-    const uint32_t off = c->h_label_off[g];
-    for (uint64_t v = 0; v < ng; v++)
-        if (labels_out[v] != SCD_INVALID_LABEL) labels_out[v] -= off;
+    if (count && (!graphs || !labels_out)) return fail(SCD_EINVAL, "scd_batch_graphs_labels: null argument");
+    std::vector<ArenaOut> outs(count);
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        const uint32_t g = graphs[i];
+        if (g >= c->G || !labels_out[i])
+            return fail(SCD_EINVAL, "scd_batch_graphs_labels: bad graph or null output (entry %u)", i);
+        const uint32_t vo = (uint32_t)c->h_vtx_off[g], ng = (uint32_t)(c->h_vtx_off[g + 1] - vo);
+        outs[i] = {nullptr, nullptr, vo, ng, c->h_label_off[g], (uint32_t)total};
+        total += ng;
+    }
+    if (total == 0) return SCD_OK;
+    // one device staging area and one copy to pinned host memory for all
+    std::vector<DevBuf> tmp;
+    uint32_t *stage = nullptr;
+    if ((rc = dalloc(c, &stage, total, &tmp))) return rc;
+    for (uint32_t i = 0; i < count; i++) (best ? outs[i].best : outs[i].cur) = stage + outs[i].so;
+    if ((rc = export_table(c, outs, (uint32_t)total)) || (rc = host_stage(c, 4 * total))) {
+        release_tmp(c, tmp);
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(c->h_stage, stage, 4 * total, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    release_tmp(c, tmp);
+    if (e != hipSuccess) return fail(SCD_EDEVICE, "scd_batch_graphs_labels: %s", hipGetErrorString(e));
+    const uint32_t *h = (const uint32_t *)c->h_stage;
+    for (uint32_t i = 0; i < count; i++) memcpy(labels_out[i], h + outs[i].so, 4ull * outs[i].n);
     return SCD_OK;
 }
 
@@ -2202,8 +2328,7 @@ extern "C" int scd_autotune(scd_ctx *c, const char *cache_path, int reps, scd_tu
     int rc = need_labels(c, "scd_autotune");
     if (rc) return rc;
     if (reps < 1) reps = 3;
-    uint32_t dmax = 0;
-    for (uint32_t v = 0; v < c->n; v++) dmax = std::max(dmax, c->h_deg[v]);
+    const uint32_t dmax = c->dmax;
     const uint64_t dsum = c->E;
     char key[256];
     snprintf(key, sizeof(key), "%s:%ucu:n%u:dmax%u:davg%u", c->arch.c_str(), c->num_cus, ilog2u(c->n),

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstdarg>
@@ -49,9 +50,18 @@ int engine_fail(int rc, const char *what) {
 
 enum State { QUEUED = 0, RESIDENT = 1, REFINING = 2, DONE = 3, FAILED = 4, UPLOADING = 5 };
 
+// One device allocation shared by the graphs of an intake batch; it goes
+// back to the pool when the last of them leaves.
+struct Block {
+    void *p = nullptr;
+    uint64_t bytes = 0;
+    uint32_t refs = 0;
+};
+
 struct Req {
     scd_request in{};
     int state = QUEUED;
+    struct Block *blk = nullptr;  // the device allocation holding the arrays below
     uint32_t *d_ind = nullptr, *d_adj = nullptr, *d_cur = nullptr, *d_best = nullptr;
     int32_t *d_tri = nullptr, *d_td = nullptr;
     uint64_t bytes = 0;
@@ -67,6 +77,15 @@ const double kThreshold = 0.01;  // src/main-gpu.cpp:88, stop below best * (1 + 
 // graphs and R-MAT s18-22: 64 / 2 GiB beat 256 / 8 GiB by 19% in capacity)
 const size_t kIntakeGraphs = 64;
 const uint64_t kIntakeBytes = 2ull << 30;
+// uploads go through two pinned staging buffers in turn: host threads fill
+// one while the other is copied to the device
+const size_t kStageBytes = 64ull << 20;
+// device blocks of finished graphs are kept for later admissions (hipFree
+// waits for the device, so the serving loop never calls it): at most
+// kPoolBytes (or an eighth of the HBM budget) stay pooled, the intake
+// thread frees the rest
+const uint64_t kPoolBytes = 4ull << 30;
+const int kCopyThreads = 12;
 
 }  // namespace
 
@@ -90,14 +109,29 @@ struct scd_server {
     std::condition_variable cv, cv_done;
     std::deque<size_t> up_queue;                 // requests to upload (guarded by mu)
     std::vector<std::pair<size_t, int>> up_done;  // (request, status) (guarded by mu)
+    // uploaded, unprepared batches: the intake thread hands a batch over
+    // as it is when the main loop has nothing to refine (main_idle), and
+    // the main loop prepares it in its own arena, which then needs no
+    // second assembly (guarded by mu)
+    std::vector<std::vector<size_t>> up_raw;
+    std::atomic<bool> main_idle{false};
+    std::mutex prep_mu;  // the preparation context, used by either thread
     bool up_stop = false;                        // guarded by mu
     hipStream_t up_stream = nullptr;
+    void *pin[2] = {nullptr, nullptr};        // pinned staging (intake thread)
+    hipEvent_t pin_ev[2] = {nullptr, nullptr};  // last copy out of each
+    bool pin_busy[2] = {false, false};
+    int pin_next = 0;
+    std::mutex pool_mu;
+    std::vector<std::pair<void *, uint64_t>> pool;  // free device blocks (guarded by pool_mu)
+    uint64_t pool_bytes = 0;                        // guarded by pool_mu
     // with graph_mode the main context captures iterations, and device
     // allocations or copies issued by another thread during a capture make
     // it fail: intake batches and capturing steps exclude each other
     std::mutex capture_mu;
     size_t uploading = 0;  // main thread only
     double upload_ms = 0, prep_ms = 0;  // intake thread only
+    double main_prep_ms = 0;            // main thread only
 
     double now() const {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -135,6 +169,13 @@ extern "C" int scd_server_create(const scd_server_opts *opts, scd_server **out) 
         scd_server_destroy(s);
         return sfail(SCD_EDEVICE, "scd_server_create: cannot create the upload stream");
     }
+    for (int b = 0; b < 2; b++)
+        if (hipHostMalloc(&s->pin[b], kStageBytes, hipHostMallocDefault) != hipSuccess ||
+            hipEventCreateWithFlags(&s->pin_ev[b], hipEventDisableTiming) != hipSuccess) {
+            s->pin[b] = nullptr;
+            scd_server_destroy(s);
+            return sfail(SCD_ENOMEM, "scd_server_create: pinned staging allocation failed");
+        }
     if (s->o.hbm_budget == 0) {
         size_t fr = 0, tot = 0;
         if (hipSetDevice(opts->device) != hipSuccess || hipMemGetInfo(&fr, &tot) != hipSuccess) {
@@ -149,28 +190,67 @@ extern "C" int scd_server_create(const scd_server_opts *opts, scd_server **out) 
 
 static void stop_uploader(scd_server *s);
 
-static void free_device(Req &r) {
-    (void)hipFree(r.d_ind);
-    (void)hipFree(r.d_adj);
-    (void)hipFree(r.d_tri);
-    (void)hipFree(r.d_td);
-    (void)hipFree(r.d_cur);
-    (void)hipFree(r.d_best);
+static uint64_t pool_cap(const scd_server *s) { return std::min(kPoolBytes, s->o.hbm_budget / 8); }
+
+// Drops a request's share of its device block (the block returns to the
+// pool with its last graph) and of the budget.
+static void release(scd_server *s, Req &r) {
+    if (r.blk && --r.blk->refs == 0) {
+        std::lock_guard<std::mutex> lk(s->pool_mu);
+        s->pool.emplace_back(r.blk->p, r.blk->bytes);
+        s->pool_bytes += r.blk->bytes;
+        delete r.blk;
+    }
+    r.blk = nullptr;
     r.d_ind = r.d_adj = r.d_cur = r.d_best = nullptr;
     r.d_tri = r.d_td = nullptr;
-}
-
-// Frees a request's device buffers and its share of the budget.
-static void release(scd_server *s, Req &r) {
-    free_device(r);
     if (r.bytes) s->resident -= r.bytes;
     r.bytes = 0;
+}
+
+// A pooled block of at least `bytes` (at most twice that plus 1 MiB), or null.
+static void *pool_take(scd_server *s, uint64_t bytes, uint64_t *got) {
+    std::lock_guard<std::mutex> lk(s->pool_mu);
+    size_t best = s->pool.size();
+    for (size_t i = 0; i < s->pool.size(); i++) {
+        const uint64_t b = s->pool[i].second;
+        if (b >= bytes && b <= 2 * bytes + (1u << 20) && (best == s->pool.size() || b < s->pool[best].second))
+            best = i;
+    }
+    if (best == s->pool.size()) return nullptr;
+    void *p = s->pool[best].first;
+    *got = s->pool[best].second;
+    s->pool_bytes -= *got;
+    s->pool[best] = s->pool.back();
+    s->pool.pop_back();
+    return p;
+}
+
+// Frees pooled blocks, largest first, down to `keep` bytes (intake thread
+// and destruction only).
+static void pool_trim(scd_server *s, uint64_t keep) {
+    std::vector<void *> drop;
+    {
+        std::lock_guard<std::mutex> lk(s->pool_mu);
+        std::sort(s->pool.begin(), s->pool.end(), [](const auto &a, const auto &b) { return a.second < b.second; });
+        while (s->pool_bytes > keep && !s->pool.empty()) {
+            drop.push_back(s->pool.back().first);
+            s->pool_bytes -= s->pool.back().second;
+            s->pool.pop_back();
+        }
+    }
+    for (void *p : drop) (void)hipFree(p);
 }
 
 extern "C" void scd_server_destroy(scd_server *s) {
     if (!s) return;
     stop_uploader(s);
     for (auto &r : s->reqs) release(s, r);
+    pool_trim(s, 0);
+    for (int b = 0; b < 2; b++) {
+        if (s->pin[b]) (void)hipHostFree(s->pin[b]);
+        if (s->pin_ev[b]) (void)hipEventDestroy(s->pin_ev[b]);
+    }
     if (s->up_stream) (void)hipStreamDestroy(s->up_stream);
     if (s->main) scd_destroy(s->main);
     if (s->prep) scd_destroy(s->prep);
@@ -200,35 +280,126 @@ extern "C" int scd_server_submit(scd_server *s, const scd_request *req, uint64_t
     return SCD_OK;
 }
 
-// Copies a request's graph to the device.
-static int admit(scd_server *s, Req &r) {
-    const scd_request &q = r.in;
-    const uint64_t N = q.n, E = 2ull * q.m;
-    auto dmalloc = [](void **p, uint64_t bytes) {
-        return hipMalloc(p, std::max<uint64_t>(bytes, 4)) == hipSuccess;
+// Copies the graphs of an intake batch to the device: one allocation for
+// all of them (each graph's indices, adj, tri, tri_deg together, 256-byte
+// aligned, then every graph's two label arrays), filled through the two
+// pinned staging buffers in turn, kStageBytes at a time: host threads pack
+// a piece from the graphs' host arrays while the previous piece is copied
+// asynchronously on the upload stream.  Returns each request's status in
+// rc (the batch fails or succeeds as a whole).
+static void admit_batch(scd_server *s, const std::vector<size_t> &batch, std::vector<int> &rc) {
+    struct Seg {
+        const char *src;
+        uint64_t bytes, off;
     };
-    if (!dmalloc((void **)&r.d_ind, 4 * (N + 1)) || !dmalloc((void **)&r.d_adj, 4 * E) ||
-        !dmalloc((void **)&r.d_tri, 4 * N) || !dmalloc((void **)&r.d_td, 4 * N) ||
-        !dmalloc((void **)&r.d_cur, 4 * N) || !dmalloc((void **)&r.d_best, 4 * N)) {
-        free_device(r);
-        return sfail(SCD_ENOMEM, "admission: device allocation failed");
+    auto al = [](uint64_t b) { return (std::max<uint64_t>(b, 4) + 255) & ~255ull; };
+    std::vector<Seg> segs;
+    std::vector<uint64_t> off_csr(batch.size()), off_lab(batch.size());
+    uint64_t csr = 0;
+    for (size_t k = 0; k < batch.size(); k++) {
+        const scd_request &q = s->reqs[batch[k]].in;
+        const uint64_t N = q.n, E = 2ull * q.m;
+        off_csr[k] = csr;
+        segs.push_back({(const char *)q.indices, 4 * (N + 1), csr});
+        csr += al(4 * (N + 1));
+        segs.push_back({(const char *)q.adj, 4 * E, csr});
+        csr += al(4 * E);
+        segs.push_back({(const char *)q.tri, 4 * N, csr});
+        csr += al(4 * N);
+        segs.push_back({(const char *)q.tri_deg, 4 * N, csr});
+        csr += al(4 * N);
     }
-    hipStream_t st = s->up_stream;
-    if (hipMemcpyAsync(r.d_ind, q.indices, 4 * (N + 1), hipMemcpyHostToDevice, st) != hipSuccess ||
-        (E && hipMemcpyAsync(r.d_adj, q.adj, 4 * E, hipMemcpyHostToDevice, st) != hipSuccess) ||
-        hipMemcpyAsync(r.d_tri, q.tri, 4 * N, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(r.d_td, q.tri_deg, 4 * N, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess) {
-        free_device(r);
-        return sfail(SCD_EDEVICE, "admission: copy to the device failed");
+    uint64_t total = csr;
+    for (size_t k = 0; k < batch.size(); k++) {
+        off_lab[k] = total;
+        total += 2 * al(4ull * s->reqs[batch[k]].in.n);
     }
-    return SCD_OK;
+    rc.assign(batch.size(), SCD_OK);
+    auto fail_all = [&](int code, const char *msg) {
+        sfail(code, "%s", msg);
+        rc.assign(batch.size(), code);
+    };
+    uint64_t got = 0;
+    void *p = pool_take(s, total, &got);
+    if (!p) {
+        hipError_t e = hipMalloc(&p, total);
+        if (e != hipSuccess) {  // pooled blocks may hold the memory: free them all and try once more
+            (void)hipGetLastError();
+            pool_trim(s, 0);
+            e = hipMalloc(&p, total);
+        }
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail_all(SCD_ENOMEM, "admission: device allocation failed");
+        }
+        got = total;
+    }
+    char *base = (char *)p;
+    bool ok = true;
+    size_t first = 0;  // first segment that may overlap the piece
+    for (uint64_t a = 0; ok && a < csr; a += kStageBytes) {
+        const uint64_t b = std::min<uint64_t>(csr, a + kStageBytes);
+        const int k = s->pin_next;
+        s->pin_next ^= 1;
+        if (s->pin_busy[k] && hipEventSynchronize(s->pin_ev[k]) != hipSuccess) {
+            ok = false;
+            break;
+        }
+        char *buf = (char *)s->pin[k];
+        // the piece's copies, cut into 256 KiB jobs shared by the threads
+        struct Job {
+            char *dst;
+            const char *src;
+            uint64_t bytes;
+        };
+        std::vector<Job> jobs;
+        while (first < segs.size() && segs[first].off + segs[first].bytes <= a) first++;
+        for (size_t i = first; i < segs.size() && segs[i].off < b; i++) {
+            const Seg &g = segs[i];
+            const uint64_t lo = std::max(a, g.off), hi = std::min(b, g.off + g.bytes);
+            for (uint64_t x = lo; x < hi; x += 256u << 10)
+                jobs.push_back({buf + (x - a), g.src + (x - g.off), std::min<uint64_t>(256u << 10, hi - x)});
+        }
+#pragma omp parallel for num_threads(kCopyThreads) schedule(dynamic, 4)
+        for (long j = 0; j < (long)jobs.size(); j++) memcpy(jobs[j].dst, jobs[j].src, jobs[j].bytes);
+        if (hipMemcpyAsync(base + a, buf, b - a, hipMemcpyHostToDevice, s->up_stream) != hipSuccess ||
+            hipEventRecord(s->pin_ev[k], s->up_stream) != hipSuccess)
+            ok = false;
+        else
+            s->pin_busy[k] = true;
+    }
+    if (!ok || hipStreamSynchronize(s->up_stream) != hipSuccess) {
+        (void)hipStreamSynchronize(s->up_stream);
+        (void)hipFree(p);
+        return fail_all(SCD_EDEVICE, "admission: copy to the device failed");
+    }
+    Block *blk = new Block{p, got, (uint32_t)batch.size()};
+    for (size_t k = 0; k < batch.size(); k++) {
+        Req &r = s->reqs[batch[k]];
+        const uint64_t N = r.in.n, E = 2ull * r.in.m, o = off_csr[k];
+        r.blk = blk;
+        r.d_ind = (uint32_t *)(base + o);
+        r.d_adj = (uint32_t *)(base + o + al(4 * (N + 1)));
+        r.d_tri = (int32_t *)(base + o + al(4 * (N + 1)) + al(4 * E));
+        r.d_td = (int32_t *)(base + o + al(4 * (N + 1)) + al(4 * E) + al(4 * N));
+        r.d_cur = (uint32_t *)(base + off_lab[k]);
+        r.d_best = (uint32_t *)(base + off_lab[k] + al(4 * N));
+    }
 }
 
 static scd_part part_of(const Req &r);
 
+static void set_initial(scd_server *s, const std::vector<size_t> &batch, const std::vector<double> &w0) {
+    for (size_t i = 0; i < batch.size(); i++) {
+        Req &r = s->reqs[batch[i]];
+        r.best = w0[i];
+        r.res.initial_wcc = w0[i];
+    }
+}
+
 // Initial partition and initial WCC of a batch of resident graphs in the
-// preparation context; their labels go to their own buffers.
+// preparation context (caller holds prep_mu); their labels go to their own
+// buffers.
 static int prepare(scd_server *s, const std::vector<size_t> &batch) {
     const uint32_t P = (uint32_t)batch.size();
     std::vector<scd_part> parts(P);
@@ -246,11 +417,7 @@ static int prepare(scd_server *s, const std::vector<size_t> &batch) {
         (rc = scd_batch_compute_wcc(s->prep, w0.data())) ||
         (rc = scd_batch_export_labels(s->prep, cur.data(), best.data())))
         return engine_fail(rc, "initial partition of joining graphs");
-    for (uint32_t i = 0; i < P; i++) {
-        Req &r = s->reqs[batch[i]];
-        r.best = w0[i];
-        r.res.initial_wcc = w0[i];
-    }
+    set_initial(s, batch, w0);
     return SCD_OK;
 }
 
@@ -278,19 +445,25 @@ static void intake(scd_server *s) {
                 s->up_queue.pop_front();
             }
         }
+        pool_trim(s, pool_cap(s));
         std::vector<std::pair<size_t, int>> done;
         std::vector<size_t> ok;
         std::unique_lock<std::mutex> cap(s->capture_mu, std::defer_lock);
         if (s->o.graph_mode) cap.lock();
         auto t = std::chrono::steady_clock::now();
-        for (size_t i : batch) {
-            const int rc = admit(s, s->reqs[i]);
-            if (rc) done.emplace_back(i, rc);
-            else ok.push_back(i);
+        std::vector<int> arc;
+        admit_batch(s, batch, arc);
+        for (size_t k = 0; k < batch.size(); k++) {
+            if (arc[k]) done.emplace_back(batch[k], arc[k]);
+            else ok.push_back(batch[k]);
         }
         auto t2 = std::chrono::steady_clock::now();
         s->upload_ms += std::chrono::duration<double, std::milli>(t2 - t).count();
-        if (!ok.empty()) {
+        std::vector<size_t> raw;
+        if (!ok.empty() && s->main_idle.load()) {
+            raw.swap(ok);  // nothing refining: the main loop prepares it in its arena
+        } else if (!ok.empty()) {
+            std::lock_guard<std::mutex> pl(s->prep_mu);
             const int rc = prepare(s, ok);
             s->prep_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t2).count();
             for (size_t i : ok) done.emplace_back(i, rc);
@@ -299,6 +472,7 @@ static void intake(scd_server *s) {
         {
             std::lock_guard<std::mutex> lk(s->mu);
             s->up_done.insert(s->up_done.end(), done.begin(), done.end());
+            if (!raw.empty()) s->up_raw.push_back(std::move(raw));
         }
         s->cv_done.notify_one();
     }
@@ -400,12 +574,21 @@ static int step(scd_server *s) {
     }
     if (any_store && (rc = scd_batch_store_best(s->main, store.data())))
         return engine_fail(rc, "store best labels");
+    // every graph that stopped hands back its best labels in one download
+    std::vector<uint32_t> fin;
+    std::vector<uint32_t *> outs;
     for (size_t g = 0; g < G; g++) {
         if (!stop[g]) continue;
         Req &r = s->reqs[s->arena[g]];
         r.labels.resize(r.in.n);
-        if ((rc = scd_batch_graph_labels(s->main, (uint32_t)g, 1, r.labels.data())))
-            return engine_fail(rc, "result labels");
+        fin.push_back((uint32_t)g);
+        outs.push_back(r.labels.data());
+    }
+    if (!fin.empty() && (rc = scd_batch_graphs_labels(s->main, (uint32_t)fin.size(), fin.data(), 1, outs.data())))
+        return engine_fail(rc, "result labels");
+    for (size_t g = 0; g < G; g++) {
+        if (!stop[g]) continue;
+        Req &r = s->reqs[s->arena[g]];
         r.res.best_wcc = r.best;
         r.res.done_ms = s->now();
         r.state = DONE;
@@ -418,13 +601,21 @@ static int step(scd_server *s) {
 
 // Collects requests the intake thread finished (uploaded and prepared, or
 // failed); waits for some when block is set.
-static void drain_uploads(scd_server *s, bool block) {
+static void drain_uploads(scd_server *s, bool block, std::vector<size_t> *raw) {
     std::vector<std::pair<size_t, int>> done;
+    std::vector<std::vector<size_t>> rb;
     {
         std::unique_lock<std::mutex> lk(s->mu);
-        if (block) s->cv_done.wait(lk, [&] { return !s->up_done.empty(); });
+        if (block) s->cv_done.wait(lk, [&] { return !s->up_done.empty() || !s->up_raw.empty(); });
         done.swap(s->up_done);
+        rb.swap(s->up_raw);
     }
+    for (auto &b : rb)
+        for (size_t i : b) {
+            s->uploading--;
+            s->reqs[i].state = RESIDENT;
+            raw->push_back(i);
+        }
     for (auto &d : done) {
         Req &r = s->reqs[d.first];
         s->uploading--;
@@ -440,8 +631,65 @@ static void drain_uploads(scd_server *s, bool block) {
     }
 }
 
+// A batch handed over unprepared while nothing refines: assembled once, in
+// the main arena, given its initial partition and initial WCC there, and
+// refined on from there (the same per-graph results as the preparation
+// context gives: arena composition never changes a graph's result).
+static int prepare_main(scd_server *s, const std::vector<size_t> &batch) {
+    const double t = s->now();
+    const uint32_t P = (uint32_t)batch.size();
+    std::vector<scd_part> parts(P);
+    for (uint32_t i = 0; i < P; i++) parts[i] = part_of(s->reqs[batch[i]]);
+    std::vector<double> w0(P);
+    int rc;
+    if ((rc = scd_batch_upload_parts(s->main, P, parts.data(), nullptr, nullptr)) ||
+        (rc = scd_batch_initial_partition(s->main, nullptr, nullptr)) ||
+        (rc = scd_batch_compute_wcc(s->main, w0.data())))
+        return engine_fail(rc, "initial partition of joining graphs");
+    set_initial(s, batch, w0);
+    const double t2 = s->now();
+    for (size_t i : batch) {
+        Req &r = s->reqs[i];
+        r.res.start_ms = t2;
+        r.state = REFINING;
+        r.res.joined_arena_graphs = P;
+    }
+    s->arena = batch;
+    s->active.assign(P, 1u);
+    s->n_active = P;
+    s->st.joins++;
+    s->st.peak_graphs = std::max(s->st.peak_graphs, P);
+    s->main_prep_ms += t2 - t;
+    s->st.busy_ms += t2 - t;
+    return SCD_OK;
+}
+
+// Unprepared batches that arrived while the arena was busy after all: the
+// preparation context takes them here; they wait to join like the others.
+static void prepare_late(scd_server *s, const std::vector<size_t> &batch) {
+    const double t = s->now();
+    int rc;
+    {
+        std::lock_guard<std::mutex> pl(s->prep_mu);
+        rc = prepare(s, batch);
+    }
+    s->main_prep_ms += s->now() - t;
+    for (size_t i : batch) {
+        Req &r = s->reqs[i];
+        if (rc) {
+            release(s, r);
+            r.state = FAILED;
+            r.res.status = rc;
+            r.res.done_ms = s->now();
+        } else {
+            s->waiting.push_back(i);
+        }
+    }
+}
+
 static int serve_loop(scd_server *s, const std::vector<size_t> &order) {
     size_t qi = 0;
+    std::vector<size_t> raw;  // unprepared batches handed over by the intake thread
     while (true) {
         const double now = s->now();
         // admission, first come first served, within the footprint budget;
@@ -458,7 +706,8 @@ static int serve_loop(scd_server *s, const std::vector<size_t> &order) {
                 continue;
             }
             if (s->resident + fp > s->o.hbm_budget) break;
-            if (s->o.max_graphs && s->n_active + s->waiting.size() + s->uploading >= s->o.max_graphs) break;
+            if (s->o.max_graphs && s->n_active + s->waiting.size() + s->uploading + raw.size() >= s->o.max_graphs)
+                break;
             r.bytes = fp;
             s->resident += fp;
             s->st.peak_bytes = std::max(s->st.peak_bytes, s->resident);
@@ -472,7 +721,17 @@ static int serve_loop(scd_server *s, const std::vector<size_t> &order) {
             s->cv.notify_one();
             qi++;
         }
-        drain_uploads(s, false);
+        drain_uploads(s, false, &raw);
+        if (!raw.empty()) {
+            if (s->n_active == 0) {
+                const int rc = prepare_main(s, raw);
+                if (rc) return rc;
+            } else {
+                prepare_late(s, raw);
+            }
+            raw.clear();
+        }
+        s->main_idle.store(s->n_active == 0 && s->waiting.empty());
         if (!s->waiting.empty()) {
             const double oldest = s->reqs[s->waiting.front()].in.arrival_ms;
             if (s->n_active == 0 || now - oldest >= s->o.join_wait_ms || 2 * s->n_active <= s->arena.size()) {
@@ -485,7 +744,7 @@ static int serve_loop(scd_server *s, const std::vector<size_t> &order) {
         if (s->n_active == 0) {
             if (!s->waiting.empty()) continue;
             if (s->uploading) {
-                drain_uploads(s, true);
+                drain_uploads(s, true, &raw);
                 continue;
             }
             if (qi >= order.size()) break;
@@ -520,13 +779,16 @@ extern "C" int scd_server_run(scd_server *s) {
     s->up_stop = false;
     s->upload_ms = 0;
     s->prep_ms = 0;
+    s->main_prep_ms = 0;
+    s->main_idle.store(false);
     s->up_thread = std::thread(intake, s);
     const int rc = serve_loop(s, order);
     stop_uploader(s);
     s->st.admit_ms += s->upload_ms;
-    s->st.prep_ms += s->prep_ms;
+    s->st.prep_ms += s->prep_ms + s->main_prep_ms;
     if (rc) {  // requests left mid-way keep their device buffers until destroy
-        drain_uploads(s, false);
+        std::vector<size_t> raw;
+        drain_uploads(s, false, &raw);
         return rc;
     }
     return SCD_OK;

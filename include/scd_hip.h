@@ -382,6 +382,15 @@ int scd_batch_export_labels(scd_ctx *ctx, uint32_t *const *cur_out, uint32_t *co
 /* Best (best != 0) or current local labels of graph g into host memory. */
 int scd_batch_graph_labels(scd_ctx *ctx, uint32_t g, int best, uint32_t *labels_out);
 /*
+ * Best (best != 0) or current local labels of graphs[0..count) into the host
+ * arrays labels_out[i] (n of graphs[i] entries each), with one launch and
+ * one device-to-host copy for all of them (a serving loop hands back every
+ * graph that finished in an iteration this way).  No counterpart in the
+ * reference, which refines one graph per process.
+ */
+int scd_batch_graphs_labels(scd_ctx *ctx, uint32_t count, const uint32_t *graphs, int best,
+                            uint32_t *const *labels_out);
+/*
  * One iteration (cuda_improve_labels, src/cuda.cu:176-351) of the graphs
  * whose active[g] != 0 (host array, G entries); the others keep their labels.
  * wcc_out (G entries, may be NULL): the WCC of each active graph.
