@@ -164,6 +164,16 @@ extern "C" int scd_server_create(const scd_server_opts *opts, scd_server **out) 
     }
     scd_set_graph_mode(s->main, opts->graph_mode ? 1 : 0);
     scd_set_graph_mode(s->prep, 0);
+    // arenas of many graphs fill the device bin by bin: the per-bin kernels
+    // run on one stream (concurrent bin lanes only added synchronization:
+    // 8% slower iterations on a 185-graph planted arena)
+    for (scd_ctx *c : {s->main, s->prep}) {
+        scd_tuning t;
+        if (scd_get_tuning(c, &t) == SCD_OK) {
+            t.lanes = 0;
+            (void)scd_set_tuning(c, &t);
+        }
+    }
     if (hipSetDevice(opts->device) != hipSuccess ||
         hipStreamCreateWithFlags(&s->up_stream, hipStreamNonBlocking) != hipSuccess) {
         scd_server_destroy(s);
