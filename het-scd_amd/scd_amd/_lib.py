@@ -177,6 +177,7 @@ def hip():
                                                   ctypes.c_void_p]),
         "scd_batch_export_labels": (ctypes.c_int, [C, ctypes.c_void_p, ctypes.c_void_p]),
         "scd_batch_graph_labels": (ctypes.c_int, [C, ctypes.c_uint32, ctypes.c_int, u32p]),
+        "scd_batch_graphs_labels": (ctypes.c_int, [C, ctypes.c_uint32, u32p, ctypes.c_int, ctypes.POINTER(u32p)]),
         "scd_batch_step": (ctypes.c_int, [C, u32p, f64p]),
         "scd_autotune": (ctypes.c_int, [C, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ScdTuningC), f64p]),
         "scd_batch_store_best": (ctypes.c_int, [C, u32p]),

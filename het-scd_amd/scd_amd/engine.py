@@ -370,3 +370,13 @@ class BatchEngine:
         lab = np.empty(self.arena.graph.n, np.uint32)
         check_hip(_lib.hip().scd_batch_current_labels(self._ctx, ptr(lab, u32p)))
         return lab
+
+    def graph_labels(self, graphs, best=True):
+        """Local best (or current) labels of the listed graphs, one array
+        each, through scd_batch_graphs_labels (one download for all)."""
+        gs = np.ascontiguousarray(graphs, np.uint32)
+        off = self.arena.vtx_off
+        outs = [np.empty(int(off[g + 1] - off[g]) if g < self.arena.count else 1, np.uint32) for g in gs]
+        arr = (u32p * max(len(outs), 1))(*[ptr(o, u32p) for o in outs])
+        check_hip(_lib.hip().scd_batch_graphs_labels(self._ctx, len(gs), ptr(gs, u32p), 1 if best else 0, arr))
+        return outs

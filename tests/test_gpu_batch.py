@@ -68,3 +68,25 @@ def test_batch_rejects_empty_member(engine):
     with scd_amd.BatchEngine(device=0) as be:
         with pytest.raises(scd_amd.ScdError):
             be.upload(arena)
+
+
+def test_graphs_labels_one_download(small_batch):
+    """scd_batch_graphs_labels hands back the labels of any list of graphs
+    (best or current, unordered and repeated ids, the empty list) equal to
+    the slices of the whole-arena download; an id past the arena is an
+    error."""
+    a = small_batch
+    off = a.vtx_off
+    with scd_amd.BatchEngine(device=0) as be:
+        be.upload(a)
+        be.initial_partition(want_labels=False)
+        be.refine(be.compute_wcc(), max_iters=3)
+        full = {True: be.labels(), False: be.current_labels()}
+        for sel in ([], [0], [a.count - 1, 3, 3, 17, 0], list(range(a.count))):
+            for best in (True, False):
+                got = be.graph_labels(sel, best=best)
+                assert len(got) == len(sel)
+                for g, lab in zip(sel, got):
+                    assert np.array_equal(lab, full[best][off[g]:off[g + 1]]), (sel, g, best)
+        with pytest.raises(scd_amd.ScdError):
+            be.graph_labels([a.count])

@@ -117,3 +117,31 @@ def test_intake_batches_equal_plain_path(mix, plain, intake_graphs, intake_mb, g
         for name, rid in ids.items():
             check(name, srv.result(rid), plain[name])
         assert srv.stats()["joins"] >= 2
+
+
+def test_runs_again_with_pooled_blocks(mix, plain):
+    """One server, two runs: the second run's graphs take the device blocks
+    that the first run's graphs returned to the pool.  Every result equals
+    the plain path's."""
+    with scd_amd.Server(device=0, join_wait_ms=0.0) as srv:
+        first = {name: srv.submit(g, arrival_ms=0.0) for name, g in mix}
+        srv.run()
+        second = {name: srv.submit(g, arrival_ms=0.0) for name, g in mix}
+        srv.run()
+        for ids in (first, second):
+            for name, rid in ids.items():
+                check(name, srv.result(rid), plain[name])
+
+
+def test_one_graph_intake_batches_over_time(mix, plain):
+    """Graphs arriving one by one as one-graph intake batches: the ones that
+    find nothing refining are prepared in the main arena, the others in the
+    preparation context while the arena refines; every result equals the
+    plain path's."""
+    with scd_amd.Server(device=0, join_wait_ms=0.0, intake_graphs=1) as srv:
+        ids = {name: srv.submit(g, arrival_ms=0.5 * i) for i, (name, g) in enumerate(mix)}
+        srv.run()
+        st = srv.stats()
+        for name, rid in ids.items():
+            check(name, srv.result(rid), plain[name])
+    assert st["joins"] >= 2
