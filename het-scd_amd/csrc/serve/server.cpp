@@ -179,13 +179,15 @@ extern "C" int scd_server_create(const scd_server_opts *opts, scd_server **out) 
         scd_server_destroy(s);
         return sfail(SCD_EDEVICE, "scd_server_create: cannot create the upload stream");
     }
-    for (int b = 0; b < 2; b++)
-        if (hipHostMalloc(&s->pin[b], kStageBytes, hipHostMallocDefault) != hipSuccess ||
-            hipEventCreateWithFlags(&s->pin_ev[b], hipEventDisableTiming) != hipSuccess) {
-            s->pin[b] = nullptr;
+    for (int b = 0; b < 2; b++) {
+        if (hipHostMalloc(&s->pin[b], kStageBytes, hipHostMallocDefault) != hipSuccess) s->pin[b] = nullptr;
+        if (s->pin[b] && hipEventCreateWithFlags(&s->pin_ev[b], hipEventDisableTiming) != hipSuccess)
+            s->pin_ev[b] = nullptr;
+        if (!s->pin[b] || !s->pin_ev[b]) {
             scd_server_destroy(s);
             return sfail(SCD_ENOMEM, "scd_server_create: pinned staging allocation failed");
         }
+    }
     if (s->o.hbm_budget == 0) {
         size_t fr = 0, tot = 0;
         if (hipSetDevice(opts->device) != hipSuccess || hipMemGetInfo(&fr, &tot) != hipSuccess) {
@@ -455,11 +457,11 @@ static void intake(scd_server *s) {
                 s->up_queue.pop_front();
             }
         }
-        pool_trim(s, pool_cap(s));
         std::vector<std::pair<size_t, int>> done;
         std::vector<size_t> ok;
         std::unique_lock<std::mutex> cap(s->capture_mu, std::defer_lock);
         if (s->o.graph_mode) cap.lock();
+        pool_trim(s, pool_cap(s));  // hipFree: not while the main context captures
         auto t = std::chrono::steady_clock::now();
         std::vector<int> arc;
         admit_batch(s, batch, arc);
