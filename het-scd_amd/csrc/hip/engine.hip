@@ -1849,24 +1849,29 @@ extern "C" int scd_initial_partition(scd_ctx *c, uint32_t *labels_out, uint32_t 
     }
     LAUNCH(c, "ip_rank", cdiv(n, 256), 256, k_ip_rank, val[cur], n, rank, state);
     const uint32_t nbig = c->bin_begin[K_NB] - c->bin_begin[K_W1];  // degree > 32: a wave each
+    // a round after the last decision changes nothing, so several rounds go
+    // out per host check of the open flag (which the last of them sets)
+    constexpr uint32_t kRoundsPerCheck = 4;
     uint32_t rounds = 0;
     while (true) {
-        rounds++;
-        LAUNCH(c, "ip_clear", 1, 64, k_fill_u32, flags, 4u);
-        LAUNCH(c, "ip_round", cdiv(bin_n(c, K_SUB8), 32), 256, k_ip_round<8>, c->indices, c->adj, rank, state,
-               bin_v(c, K_SUB8), bin_n(c, K_SUB8), flags);
-        LAUNCH(c, "ip_round", cdiv(bin_n(c, K_SUB16), 16), 256, k_ip_round<16>, c->indices, c->adj, rank, state,
-               bin_v(c, K_SUB16), bin_n(c, K_SUB16), flags);
-        LAUNCH(c, "ip_round", cdiv(bin_n(c, K_SUB32), 8), 256, k_ip_round<32>, c->indices, c->adj, rank, state,
-               bin_v(c, K_SUB32), bin_n(c, K_SUB32), flags);
-        LAUNCH(c, "ip_round", cdiv(nbig, 4), 256, k_ip_round<64>, c->indices, c->adj, rank, state, bin_v(c, K_W1),
-               nbig, flags);
+        for (uint32_t r = 0; r < kRoundsPerCheck; r++) {
+            rounds++;
+            LAUNCH(c, "ip_clear", 1, 64, k_fill_u32, flags, 4u);
+            LAUNCH(c, "ip_round", cdiv(bin_n(c, K_SUB8), 32), 256, k_ip_round<8>, c->indices, c->adj, rank, state,
+                   bin_v(c, K_SUB8), bin_n(c, K_SUB8), flags);
+            LAUNCH(c, "ip_round", cdiv(bin_n(c, K_SUB16), 16), 256, k_ip_round<16>, c->indices, c->adj, rank,
+                   state, bin_v(c, K_SUB16), bin_n(c, K_SUB16), flags);
+            LAUNCH(c, "ip_round", cdiv(bin_n(c, K_SUB32), 8), 256, k_ip_round<32>, c->indices, c->adj, rank,
+                   state, bin_v(c, K_SUB32), bin_n(c, K_SUB32), flags);
+            LAUNCH(c, "ip_round", cdiv(nbig, 4), 256, k_ip_round<64>, c->indices, c->adj, rank, state,
+                   bin_v(c, K_W1), nbig, flags);
+        }
         if ((rc = check_launch("initial partition"))) return cleanup(rc);
         if (hipMemcpyAsync(hflags, flags, 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
             hipStreamSynchronize(c->stream) != hipSuccess)
             return cleanup(fail(SCD_EDEVICE, "scd_initial_partition: readback failed"));
         if (hflags[0] == 0) break;
-        if (rounds > n + 1) return cleanup(fail(SCD_EDEVICE, "scd_initial_partition: no progress"));
+        if (rounds > n + kRoundsPerCheck) return cleanup(fail(SCD_EDEVICE, "scd_initial_partition: no progress"));
     }
     if (c->G) LAUNCH(c, "ip_clear", cdiv(c->G, 256), 256, k_fill_u32, c->k_g, c->G);
     LAUNCH(c, "ip_seedflags", cdiv(n, 256), 256, k_ip_seedflags, val[cur], state, n, c->G ? c->gid : nullptr, c->k_g,
@@ -2242,6 +2247,7 @@ extern "C" int scd_batch_graphs_labels(scd_ctx *c, uint32_t count, const uint32_
         const uint32_t vo = (uint32_t)c->h_vtx_off[g], ng = (uint32_t)(c->h_vtx_off[g + 1] - vo);
         outs[i] = {nullptr, nullptr, vo, ng, c->h_label_off[g], (uint32_t)total};
         total += ng;
+        if (total >= (1ull << 32)) return fail(SCD_ELIMIT, "scd_batch_graphs_labels: 2^32 labels or more at once");
     }
     if (total == 0) return SCD_OK;
     // one device staging area and one copy to pinned host memory for all
