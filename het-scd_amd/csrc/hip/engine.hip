@@ -1849,12 +1849,14 @@ extern "C" int scd_initial_partition(scd_ctx *c, uint32_t *labels_out, uint32_t 
     }
     LAUNCH(c, "ip_rank", cdiv(n, 256), 256, k_ip_rank, val[cur], n, rank, state);
     const uint32_t nbig = c->bin_begin[K_NB] - c->bin_begin[K_W1];  // degree > 32: a wave each
-    // a round after the last decision changes nothing, so several rounds go
-    // out per host check of the open flag (which the last of them sets)
-    constexpr uint32_t kRoundsPerCheck = 4;
+    // a round after the last decision changes nothing, so on small graphs,
+    // where a round costs less than a host check, four rounds go out per
+    // check of the open flag (which the last of them sets); a round over 2^24
+    // adjacency entries or more costs more than the check it would save
+    const uint32_t per_check = c->E < (1ull << 24) ? 4u : 1u;
     uint32_t rounds = 0;
     while (true) {
-        for (uint32_t r = 0; r < kRoundsPerCheck; r++) {
+        for (uint32_t r = 0; r < per_check; r++) {
             rounds++;
             LAUNCH(c, "ip_clear", 1, 64, k_fill_u32, flags, 4u);
             LAUNCH(c, "ip_round", cdiv(bin_n(c, K_SUB8), 32), 256, k_ip_round<8>, c->indices, c->adj, rank, state,
@@ -1871,7 +1873,7 @@ extern "C" int scd_initial_partition(scd_ctx *c, uint32_t *labels_out, uint32_t 
             hipStreamSynchronize(c->stream) != hipSuccess)
             return cleanup(fail(SCD_EDEVICE, "scd_initial_partition: readback failed"));
         if (hflags[0] == 0) break;
-        if (rounds > n + kRoundsPerCheck) return cleanup(fail(SCD_EDEVICE, "scd_initial_partition: no progress"));
+        if (rounds > n + per_check) return cleanup(fail(SCD_EDEVICE, "scd_initial_partition: no progress"));
     }
     if (c->G) LAUNCH(c, "ip_clear", cdiv(c->G, 256), 256, k_fill_u32, c->k_g, c->G);
     LAUNCH(c, "ip_seedflags", cdiv(n, 256), 256, k_ip_seedflags, val[cur], state, n, c->G ? c->gid : nullptr, c->k_g,
