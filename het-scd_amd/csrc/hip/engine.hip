@@ -1803,7 +1803,7 @@ extern "C" int scd_initial_partition(scd_ctx *c, uint32_t *labels_out, uint32_t 
     std::vector<DevBuf> tmp;
     uint64_t *khi[2] = {nullptr, nullptr};
     uint32_t *klo[2] = {nullptr, nullptr}, *val[2] = {nullptr, nullptr}, *kg[2] = {nullptr, nullptr};
-    uint32_t *flags = nullptr, *hist = nullptr, *hoff = nullptr, *rank = nullptr, *sf = nullptr;
+    uint32_t *flags = nullptr, *hist = nullptr, *hoff = nullptr, *rank = nullptr, *sf = nullptr, *resume = nullptr;
     unsigned long long *bits = nullptr;
     uint8_t *state = nullptr;
     const uint32_t ntiles = cdiv(n, RS_TILE);
@@ -1817,7 +1817,8 @@ extern "C" int scd_initial_partition(scd_ctx *c, uint32_t *labels_out, uint32_t 
         (rc = dalloc(c, &kg[0], n, &tmp)) || (rc = dalloc(c, &kg[1], n, &tmp)) ||
         (rc = dalloc(c, &flags, 4, &tmp)) || (rc = dalloc(c, &bits, 6, &tmp)) ||
         (rc = dalloc(c, &hist, 256ull * ntiles, &tmp)) || (rc = dalloc(c, &hoff, 256ull * ntiles, &tmp)) ||
-        (rc = dalloc(c, &rank, n, &tmp)) || (rc = dalloc(c, &sf, n, &tmp)) || (rc = dalloc(c, &state, n, &tmp)))
+        (rc = dalloc(c, &rank, n, &tmp)) || (rc = dalloc(c, &sf, n, &tmp)) || (rc = dalloc(c, &state, n, &tmp)) ||
+        (rc = dalloc(c, &resume, n, &tmp)))
         return cleanup(rc);
     LAUNCH(c, "ip_clear", 1, 64, k_fill_u32, flags, 4u);
     LAUNCH(c, "ip_clear", 1, 64, k_ip_bits_init, bits);
@@ -1854,19 +1855,20 @@ extern "C" int scd_initial_partition(scd_ctx *c, uint32_t *labels_out, uint32_t 
     // check of the open flag (which the last of them sets); a round over 2^24
     // adjacency entries or more costs more than the check it would save
     const uint32_t per_check = c->E < (1ull << 24) ? 4u : 1u;
+    LAUNCH(c, "ip_clear", cdiv(n, 256), 256, k_fill_u32, resume, n);
     uint32_t rounds = 0;
     while (true) {
         for (uint32_t r = 0; r < per_check; r++) {
             rounds++;
             LAUNCH(c, "ip_clear", 1, 64, k_fill_u32, flags, 4u);
             LAUNCH(c, "ip_round", cdiv(bin_n(c, K_SUB8), 32), 256, k_ip_round<8>, c->indices, c->adj, rank, state,
-                   bin_v(c, K_SUB8), bin_n(c, K_SUB8), flags);
+                   bin_v(c, K_SUB8), bin_n(c, K_SUB8), flags, resume);
             LAUNCH(c, "ip_round", cdiv(bin_n(c, K_SUB16), 16), 256, k_ip_round<16>, c->indices, c->adj, rank,
-                   state, bin_v(c, K_SUB16), bin_n(c, K_SUB16), flags);
+                   state, bin_v(c, K_SUB16), bin_n(c, K_SUB16), flags, resume);
             LAUNCH(c, "ip_round", cdiv(bin_n(c, K_SUB32), 8), 256, k_ip_round<32>, c->indices, c->adj, rank,
-                   state, bin_v(c, K_SUB32), bin_n(c, K_SUB32), flags);
+                   state, bin_v(c, K_SUB32), bin_n(c, K_SUB32), flags, resume);
             LAUNCH(c, "ip_round", cdiv(nbig, 4), 256, k_ip_round<64>, c->indices, c->adj, rank, state,
-                   bin_v(c, K_W1), nbig, flags);
+                   bin_v(c, K_W1), nbig, flags, resume);
         }
         if ((rc = check_launch("initial partition"))) return cleanup(rc);
         if (hipMemcpyAsync(hflags, flags, 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||

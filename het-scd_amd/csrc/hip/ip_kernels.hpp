@@ -188,12 +188,15 @@ __device__ __forceinline__ uint64_t ip_group_bits(uint64_t ballot, int gbase) {
 // when an earlier-ranked neighbour is a seed, a seed when no earlier-ranked
 // neighbour is seed or undecided; otherwise it stays undecided and sets
 // *open_flag.  Decisions are final, so reading states that other vertices
-// set in the same round only speeds the rounds up.
+// set in the same round only speeds the rounds up.  An undecided vertex
+// resumes its walk at resume[p], the first neighbour that was still an
+// undecided earlier one in its last round: every neighbour before it was
+// later in the order or a non-seed, which stays so.
 template <int S>
 __global__ __launch_bounds__(256) void k_ip_round(const uint32_t *__restrict__ indices, const uint32_t *__restrict__ adj,
                                                   const uint32_t *__restrict__ rank, uint8_t *state,
                                                   const uint32_t *__restrict__ verts, uint32_t nv,
-                                                  uint32_t *__restrict__ open_flag) {
+                                                  uint32_t *__restrict__ open_flag, uint32_t *__restrict__ resume) {
     const int lane = threadIdx.x & 63;
     const int lig = lane & (S - 1), gbase = lane & ~(S - 1);
     const uint32_t slot = (blockIdx.x * 256u + threadIdx.x) / S;
@@ -203,13 +206,16 @@ __global__ __launch_bounds__(256) void k_ip_round(const uint32_t *__restrict__ i
         p = verts[slot];
         act = __atomic_load_n(&state[p], __ATOMIC_RELAXED) == 0;
     }
+    uint32_t s0 = 0;
     if (act) {
         b = indices[p];
         e = indices[p + 1];
         rp = rank[p];
+        s0 = resume[p];
     }
     bool pending = false, seed_before = false;
-    for (uint32_t s = 0;; s += S) {
+    uint32_t first_open = 0;
+    for (uint32_t s = s0;; s += S) {
         const bool more = act && !seed_before && b + s < e;
         if (!__ballot(more)) break;  // wave-uniform exit
         bool es = false, eo = false;
@@ -226,14 +232,19 @@ __global__ __launch_bounds__(256) void k_ip_round(const uint32_t *__restrict__ i
         }
         const uint64_t bs = ip_group_bits<S>(__ballot(es), gbase), bo = ip_group_bits<S>(__ballot(eo), gbase);
         if (more) {
-            if (bs) seed_before = true;
-            else if (bo) pending = true;
+            if (bs) {
+                seed_before = true;
+            } else if (bo) {
+                if (!pending) first_open = s + (uint32_t)__builtin_ctzll(bo);
+                pending = true;
+            }
         }
     }
     const bool still_open = act && !seed_before && pending;
     if (act && lig == 0) {
         if (seed_before) __atomic_store_n(&state[p], (uint8_t)2, __ATOMIC_RELAXED);
         else if (!pending) __atomic_store_n(&state[p], (uint8_t)1, __ATOMIC_RELAXED);
+        else resume[p] = first_open;
     }
     if (__ballot(still_open) && lane == 0) open_flag[0] = 1u;
 }
