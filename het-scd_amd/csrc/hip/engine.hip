@@ -632,18 +632,9 @@ static int phase_move(scd_ctx *c, uint32_t *labels_out, uint32_t *ex_sorted, uin
         b.counters = c->counting ? c->counters + bin : nullptr;
         return b;
     };
-    // lanes: 3 small-degree bins | 2 middle bins | 1 hub chain | 0 lds bin
+    // lanes: 3 small-degree bins | 2 middle bins | 1 hub chain | 0 lds bin;
+    // the lds bin and the hub chain (the longest) go out first
     fork_lanes(c);
-    on_lane(c, 3);
-    LAUNCH(c, "move_sub8", cdiv(bin_n(c, K_SUB8), 32), 256, k_move_sub<8>, at(K_SUB8), bin_v(c, K_SUB8), bin_n(c, K_SUB8));
-    LAUNCH(c, "move_sub16", cdiv(bin_n(c, K_SUB16), 16), 256, k_move_sub<16>, at(K_SUB16), bin_v(c, K_SUB16), bin_n(c, K_SUB16));
-    LAUNCH(c, "move_sub32", cdiv(bin_n(c, K_SUB32), 8), 256, k_move_sub<32>, at(K_SUB32), bin_v(c, K_SUB32), bin_n(c, K_SUB32));
-    LAUNCH(c, "move_wave1", cdiv(bin_n(c, K_W1), 4), 256, k_move_wave<1>, at(K_W1), bin_v(c, K_W1), bin_n(c, K_W1));
-    on_lane(c, 2);
-    LAUNCH(c, "move_wave2", cdiv(bin_n(c, K_W2), 4), 256, k_move_wave<2>, at(K_W2), bin_v(c, K_W2), bin_n(c, K_W2));
-    LAUNCH(c, "move_wave4", cdiv(bin_n(c, K_W4), 4), 256, k_move_wave<4>, at(K_W4), bin_v(c, K_W4), bin_n(c, K_W4));
-    LAUNCH(c, "move_wave8", cdiv(bin_n(c, K_W8), 4), 256, k_move_wave<8>, at(K_W8), bin_v(c, K_W8), bin_n(c, K_W8));
-    LAUNCH(c, "move_wave16", cdiv(bin_n(c, K_W16), 4), 256, k_move_wave<16>, at(K_W16), bin_v(c, K_W16), bin_n(c, K_W16));
     on_lane(c, 0);
     a.counters = c->counting ? c->counters + K_LDS : nullptr;
     HubArgs h = hub_args(c);
@@ -674,6 +665,16 @@ static int phase_move(scd_ctx *c, uint32_t *labels_out, uint32_t *ex_sorted, uin
         LAUNCH(c, "hub_eval", c->n_chunks, 256, k_hub_eval, at(K_HUB), h, src, C);
         LAUNCH(c, "hub_finalize", cdiv(c->nh, 64), 64, k_hub_finalize, a, h, c->nh);
     }
+    on_lane(c, 3);
+    LAUNCH(c, "move_sub8", cdiv(bin_n(c, K_SUB8), 32), 256, k_move_sub<8>, at(K_SUB8), bin_v(c, K_SUB8), bin_n(c, K_SUB8));
+    LAUNCH(c, "move_sub16", cdiv(bin_n(c, K_SUB16), 16), 256, k_move_sub<16>, at(K_SUB16), bin_v(c, K_SUB16), bin_n(c, K_SUB16));
+    LAUNCH(c, "move_sub32", cdiv(bin_n(c, K_SUB32), 8), 256, k_move_sub<32>, at(K_SUB32), bin_v(c, K_SUB32), bin_n(c, K_SUB32));
+    LAUNCH(c, "move_wave1", cdiv(bin_n(c, K_W1), 4), 256, k_move_wave<1>, at(K_W1), bin_v(c, K_W1), bin_n(c, K_W1));
+    on_lane(c, 2);
+    LAUNCH(c, "move_wave2", cdiv(bin_n(c, K_W2), 4), 256, k_move_wave<2>, at(K_W2), bin_v(c, K_W2), bin_n(c, K_W2));
+    LAUNCH(c, "move_wave4", cdiv(bin_n(c, K_W4), 4), 256, k_move_wave<4>, at(K_W4), bin_v(c, K_W4), bin_n(c, K_W4));
+    LAUNCH(c, "move_wave8", cdiv(bin_n(c, K_W8), 4), 256, k_move_wave<8>, at(K_W8), bin_v(c, K_W8), bin_n(c, K_W8));
+    LAUNCH(c, "move_wave16", cdiv(bin_n(c, K_W16), 4), 256, k_move_wave<16>, at(K_W16), bin_v(c, K_W16), bin_n(c, K_W16));
     join_lanes(c);
     return check_launch("move");
 }
