@@ -1878,11 +1878,10 @@ extern "C" int scd_initial_partition(scd_ctx *c, uint32_t *labels_out, uint32_t 
         if (hflags[0] == 0) break;
         if (rounds > n + per_check) return cleanup(fail(SCD_EDEVICE, "scd_initial_partition: no progress"));
     }
-    if (c->G) LAUNCH(c, "ip_clear", cdiv(c->G, 256), 256, k_fill_u32, c->k_g, c->G);
-    LAUNCH(c, "ip_seedflags", cdiv(n, 256), 256, k_ip_seedflags, val[cur], state, n, c->G ? c->gid : nullptr, c->k_g,
-           sf);
+    LAUNCH(c, "ip_seedflags", cdiv(n, 256), 256, k_ip_seedflags, val[cur], state, n, sf);
     uint32_t k = 0;
     if ((rc = scan_u32(c, sf, sf, n, &k, tmp))) return cleanup(rc);
+    if (c->G) LAUNCH(c, "ip_graph_seeds", cdiv(c->G, 256), 256, k_ip_graph_seeds, sf, c->vtx_off_d, c->G, n, k, c->k_g);
     LAUNCH(c, "ip_labels", cdiv(bin_n(c, K_SUB8), 32), 256, k_ip_labels<8>, c->indices, c->adj, rank, state, sf,
            bin_v(c, K_SUB8), bin_n(c, K_SUB8), c->cur);
     LAUNCH(c, "ip_labels", cdiv(bin_n(c, K_SUB16), 16), 256, k_ip_labels<16>, c->indices, c->adj, rank, state, sf,

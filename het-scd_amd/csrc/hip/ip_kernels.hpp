@@ -250,15 +250,24 @@ __global__ __launch_bounds__(256) void k_ip_round(const uint32_t *__restrict__ i
 }
 
 // sf[rank] = 1 for seeds (by position in the order).
-// k_g[g] += seeds of graph g (arenas only).
 __global__ void k_ip_seedflags(const uint32_t *__restrict__ val, const uint8_t *__restrict__ state, uint32_t n,
-                               const uint32_t *__restrict__ gid, uint32_t *__restrict__ k_g,
                                uint32_t *__restrict__ sf) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const bool seed = state[val[i]] == 1;
-    sf[i] = seed ? 1u : 0u;
-    if (seed && gid) atomicAdd(&k_g[gid[val[i]]], 1u);
+    sf[i] = state[val[i]] == 1 ? 1u : 0u;
+}
+
+// Arenas: k_g[g] = seeds of graph g.  The graph id leads the sort key, so
+// graph g holds the positions [vtx_off[g], vtx_off[g+1]) of the order and
+// its seed count is the difference of the exclusive seed scan (seedpos,
+// total seeds `total`) at the two ends.
+__global__ void k_ip_graph_seeds(const uint32_t *__restrict__ seedpos, const uint64_t *__restrict__ vtx_off,
+                                 uint32_t G, uint32_t n, uint32_t total, uint32_t *__restrict__ k_g) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= G) return;
+    const uint64_t a = vtx_off[g], b = vtx_off[g + 1];
+    const uint32_t sa = a < n ? seedpos[a] : total, sb = b < n ? seedpos[b] : total;
+    k_g[g] = sb - sa;
 }
 
 // Labels: a seed's label is its seed rank (seedpos = exclusive scan of sf);
