@@ -1823,7 +1823,8 @@ extern "C" int scd_initial_partition(scd_ctx *c, uint32_t *labels_out, uint32_t 
         return cleanup(rc);
     LAUNCH(c, "ip_clear", 1, 64, k_fill_u32, flags, 4u);
     LAUNCH(c, "ip_clear", 1, 64, k_ip_bits_init, bits);
-    LAUNCH(c, "ip_keys", cdiv(n, 256), 256, k_ip_keys, c->indices, c->tri, c->G ? c->gid : nullptr, n, kg[0],
+    LAUNCH(c, "ip_keys", std::min<uint32_t>(cdiv(n, 256), IP_KEYS_BLOCKS), 256, k_ip_keys, c->indices, c->tri,
+           c->G ? c->gid : nullptr, n, kg[0],
            khi[0], klo[0], val[0], flags, bits);
     uint32_t hflags[4];
     unsigned long long hbits[6];

@@ -40,17 +40,20 @@ __device__ __forceinline__ uint64_t orderable(double x) {
 // In an arena (gid != nullptr) the graph id is the most significant key
 // (kg), so each graph's vertices form one contiguous, correctly ordered run
 // and seed ranks are numbered graph by graph.
+constexpr uint32_t IP_KEYS_BLOCKS = 2048;  // bounded grid: few same-address atomics on bits
+
 __global__ __launch_bounds__(256) void k_ip_keys(const uint32_t *__restrict__ indices, const int32_t *__restrict__ tri,
                                                  const uint32_t *__restrict__ gid, uint32_t n,
                                                  uint32_t *__restrict__ kg, uint64_t *__restrict__ khi,
                                                  uint32_t *__restrict__ klo, uint32_t *__restrict__ val,
                                                  uint32_t *__restrict__ flags, unsigned long long *__restrict__ bits) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ unsigned long long s_red[4][6];
     uint64_t h_and = ~0ull, h_or = 0, l_and = ~0ull, l_or = 0, g_and = ~0ull, g_or = 0;
-    if (i < n) {
+    bool low_deg = false;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t p = n - 1 - i;
         const uint32_t deg = indices[p + 1] - indices[p];
-        if (deg < 2) atomicOr(flags, 1u);
+        low_deg |= deg < 2;
         const double cc = (double)tri[p] / ((double)deg * ((double)deg - 1.0));
         const uint64_t h = ~orderable(cc);
         const uint32_t l = ~(uint32_t)(int)deg;  // the reference compares deg as int
@@ -59,10 +62,14 @@ __global__ __launch_bounds__(256) void k_ip_keys(const uint32_t *__restrict__ in
         klo[i] = l;
         kg[i] = g;
         val[i] = p;
-        h_and = h_or = h;
-        l_and = l_or = l;
-        g_and = g_or = g;
+        h_and &= h;
+        h_or |= h;
+        l_and &= l;
+        l_or |= l;
+        g_and &= g;
+        g_or |= g;
     }
+    if (__ballot(low_deg) && (threadIdx.x & 63) == 0) atomicOr(flags, 1u);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         h_and &= (uint64_t)__shfl_xor((unsigned long long)h_and, off, SCD_WAVE);
@@ -72,13 +79,22 @@ __global__ __launch_bounds__(256) void k_ip_keys(const uint32_t *__restrict__ in
         g_and &= (uint64_t)__shfl_xor((unsigned long long)g_and, off, SCD_WAVE);
         g_or |= (uint64_t)__shfl_xor((unsigned long long)g_or, off, SCD_WAVE);
     }
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        atomicAnd(&bits[0], (unsigned long long)h_and);
-        atomicOr(&bits[1], (unsigned long long)h_or);
-        atomicAnd(&bits[2], (unsigned long long)l_and);
-        atomicOr(&bits[3], (unsigned long long)l_or);
-        atomicAnd(&bits[4], (unsigned long long)g_and);
-        atomicOr(&bits[5], (unsigned long long)g_or);
+        s_red[w][0] = h_and;
+        s_red[w][1] = h_or;
+        s_red[w][2] = l_and;
+        s_red[w][3] = l_or;
+        s_red[w][4] = g_and;
+        s_red[w][5] = g_or;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        unsigned long long x = s_red[0][k];
+        for (int q = 1; q < 4; q++) x = (k & 1) ? (x | s_red[q][k]) : (x & s_red[q][k]);
+        if (k & 1) atomicOr(&bits[k], x);
+        else atomicAnd(&bits[k], x);
     }
 }
 
