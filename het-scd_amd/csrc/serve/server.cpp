@@ -166,11 +166,15 @@ extern "C" int scd_server_create(const scd_server_opts *opts, scd_server **out) 
     scd_set_graph_mode(s->prep, 0);
     // arenas of many graphs fill the device bin by bin: the per-bin kernels
     // run on one stream (concurrent bin lanes only added synchronization:
-    // 8% slower iterations on a 185-graph planted arena)
+    // 8% slower iterations on a 185-graph planted arena), and the hubs of
+    // several R-MAT graphs count their labels in hash tables instead of
+    // merge passes over all hub chunks (15% faster arena iterations on a
+    // 200-graph mix of planted graphs and R-MAT s18-22)
     for (scd_ctx *c : {s->main, s->prep}) {
         scd_tuning t;
         if (scd_get_tuning(c, &t) == SCD_OK) {
             t.lanes = 0;
+            t.hub_table = 1;
             (void)scd_set_tuning(c, &t);
         }
     }
