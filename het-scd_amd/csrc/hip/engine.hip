@@ -1851,7 +1851,8 @@ extern "C" int scd_initial_partition(scd_ctx *c, uint32_t *labels_out, uint32_t 
         cur = 1 - cur;
     }
     LAUNCH(c, "ip_rank", cdiv(n, 256), 256, k_ip_rank, val[cur], n, rank, state);
-    const uint32_t nbig = c->bin_begin[K_NB] - c->bin_begin[K_W1];  // degree > 32: a wave each
+    const uint32_t nmid = c->bin_begin[K_LDS] - c->bin_begin[K_W1];  // 32 < degree <= wave_max: a wave each
+    const uint32_t nbig = c->bin_begin[K_NB] - c->bin_begin[K_LDS];  // degree > wave_max: a workgroup each
     // a round after the last decision changes nothing, so on small graphs,
     // where a round costs less than a host check, four rounds go out per
     // check of the open flag (which the last of them sets); a round over 2^24
@@ -1869,8 +1870,10 @@ extern "C" int scd_initial_partition(scd_ctx *c, uint32_t *labels_out, uint32_t 
                    state, bin_v(c, K_SUB16), bin_n(c, K_SUB16), flags, resume);
             LAUNCH(c, "ip_round", cdiv(bin_n(c, K_SUB32), 8), 256, k_ip_round<32>, c->indices, c->adj, rank,
                    state, bin_v(c, K_SUB32), bin_n(c, K_SUB32), flags, resume);
-            LAUNCH(c, "ip_round", cdiv(nbig, 4), 256, k_ip_round<64>, c->indices, c->adj, rank, state,
-                   bin_v(c, K_W1), nbig, flags, resume);
+            LAUNCH(c, "ip_round", cdiv(nmid, 4), 256, k_ip_round<64>, c->indices, c->adj, rank, state,
+                   bin_v(c, K_W1), nmid, flags, resume);
+            LAUNCH(c, "ip_round", nbig, 256, k_ip_round_block, c->indices, c->adj, rank, state, bin_v(c, K_LDS),
+                   nbig, flags, resume);
         }
         if ((rc = check_launch("initial partition"))) return cleanup(rc);
         if (hipMemcpyAsync(hflags, flags, 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
@@ -1889,8 +1892,10 @@ extern "C" int scd_initial_partition(scd_ctx *c, uint32_t *labels_out, uint32_t 
            bin_v(c, K_SUB16), bin_n(c, K_SUB16), c->cur);
     LAUNCH(c, "ip_labels", cdiv(bin_n(c, K_SUB32), 8), 256, k_ip_labels<32>, c->indices, c->adj, rank, state, sf,
            bin_v(c, K_SUB32), bin_n(c, K_SUB32), c->cur);
-    LAUNCH(c, "ip_labels", cdiv(nbig, 4), 256, k_ip_labels<64>, c->indices, c->adj, rank, state, sf, bin_v(c, K_W1),
-           nbig, c->cur);
+    LAUNCH(c, "ip_labels", cdiv(nmid, 4), 256, k_ip_labels<64>, c->indices, c->adj, rank, state, sf, bin_v(c, K_W1),
+           nmid, c->cur);
+    LAUNCH(c, "ip_labels", nbig, 256, k_ip_labels_block, c->indices, c->adj, rank, state, sf, bin_v(c, K_LDS), nbig,
+           c->cur);
     if ((rc = check_launch("initial partition"))) return cleanup(rc);
     HIP_TRY(hipMemcpyAsync(c->best, c->cur, 4ull * n, hipMemcpyDeviceToDevice, c->stream));
     if (c->num_comms != k) drop_graphs(c);

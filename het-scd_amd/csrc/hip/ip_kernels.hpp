@@ -265,6 +265,64 @@ __global__ __launch_bounds__(256) void k_ip_round(const uint32_t *__restrict__ i
     if (__ballot(still_open) && lane == 0) open_flag[0] = 1u;
 }
 
+// The decision round of k_ip_round for vertices of more than wave_max
+// neighbours: one workgroup per vertex, 256 neighbours at a time, the
+// chunk's seed / undecided findings combined through LDS.
+__global__ __launch_bounds__(256) void k_ip_round_block(const uint32_t *__restrict__ indices,
+                                                        const uint32_t *__restrict__ adj,
+                                                        const uint32_t *__restrict__ rank, uint8_t *state,
+                                                        const uint32_t *__restrict__ verts, uint32_t nv,
+                                                        uint32_t *__restrict__ open_flag,
+                                                        uint32_t *__restrict__ resume) {
+    __shared__ uint32_t s_any_seed[4], s_first[4];
+    __shared__ uint32_t s_open;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint32_t p = verts[blockIdx.x];
+    if (tid == 0) s_open = __atomic_load_n(&state[p], __ATOMIC_RELAXED) == 0;
+    __syncthreads();
+    if (!s_open) return;  // uniform: decided vertices are done
+    const uint32_t b = indices[p], e = indices[p + 1], rp = rank[p];
+    bool seed_before = false, pending = false;
+    uint32_t first_open = 0;
+    for (uint32_t s = resume[p]; b + s < e; s += 256) {  // uniform bounds
+        const uint32_t i = b + s + tid;
+        bool es = false, eo = false;
+        if (i < e) {
+            const uint32_t q = adj[i];
+            if (rank[q] < rp) {
+                const uint8_t sq = __atomic_load_n(&state[q], __ATOMIC_RELAXED);
+                es = sq == 1;
+                eo = sq == 0;
+            }
+        }
+        const uint64_t bs = __ballot(es), bo = __ballot(eo);
+        if (lane == 0) {
+            s_any_seed[w] = bs != 0;
+            s_first[w] = bo ? 64u * w + (uint32_t)__builtin_ctzll(bo) : 0xFFFFFFFFu;
+        }
+        __syncthreads();
+        const bool any_seed = s_any_seed[0] | s_any_seed[1] | s_any_seed[2] | s_any_seed[3];
+        const uint32_t fo = min(min(s_first[0], s_first[1]), min(s_first[2], s_first[3]));
+        __syncthreads();
+        if (any_seed) {
+            seed_before = true;
+            break;
+        }
+        if (fo != 0xFFFFFFFFu && !pending) {
+            first_open = s + fo;
+            pending = true;
+        }
+    }
+    if (tid == 0) {
+        if (seed_before) __atomic_store_n(&state[p], (uint8_t)2, __ATOMIC_RELAXED);
+        else if (!pending) __atomic_store_n(&state[p], (uint8_t)1, __ATOMIC_RELAXED);
+        else {
+            resume[p] = first_open;
+            open_flag[0] = 1u;
+        }
+    }
+}
+
 // sf[rank] = 1 for seeds (by position in the order).
 __global__ void k_ip_seedflags(const uint32_t *__restrict__ val, const uint8_t *__restrict__ state, uint32_t n,
                                uint32_t *__restrict__ sf) {
@@ -312,6 +370,37 @@ __global__ __launch_bounds__(256) void k_ip_labels(const uint32_t *__restrict__ 
 #pragma unroll
     for (int off = S / 2; off > 0; off >>= 1) best = min(best, (uint32_t)__shfl_xor(best, off, SCD_WAVE));
     if (act && lig == 0) labels[p] = seed ? seedpos[rank[p]] : seedpos[best];
+}
+
+// The labels of k_ip_labels for vertices of more than wave_max neighbours:
+// one workgroup per vertex, the earliest seed neighbour reduced through LDS.
+__global__ __launch_bounds__(256) void k_ip_labels_block(const uint32_t *__restrict__ indices,
+                                                         const uint32_t *__restrict__ adj,
+                                                         const uint32_t *__restrict__ rank,
+                                                         const uint8_t *__restrict__ state,
+                                                         const uint32_t *__restrict__ seedpos,
+                                                         const uint32_t *__restrict__ verts, uint32_t nv,
+                                                         uint32_t *__restrict__ labels) {
+    __shared__ uint32_t s_min[4];
+    const int tid = threadIdx.x, w = tid >> 6;
+    const uint32_t p = verts[blockIdx.x];
+    const bool seed = state[p] == 1;
+    uint32_t best = 0xFFFFFFFFu;
+    if (!seed) {
+        const uint32_t b = indices[p], e = indices[p + 1];
+        for (uint32_t i = b + tid; i < e; i += 256) {
+            const uint32_t q = adj[i];
+            if (state[q] == 1) best = min(best, rank[q]);
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) best = min(best, (uint32_t)__shfl_xor(best, off, SCD_WAVE));
+    if ((tid & 63) == 0) s_min[w] = best;
+    __syncthreads();
+    if (tid == 0) {
+        best = min(min(s_min[0], s_min[1]), min(s_min[2], s_min[3]));
+        labels[p] = seed ? seedpos[rank[p]] : seedpos[best];
+    }
 }
 
 }  // namespace scd_dev
