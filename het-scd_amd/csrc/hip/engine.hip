@@ -1870,8 +1870,12 @@ extern "C" int scd_initial_partition(scd_ctx *c, uint32_t *labels_out, uint32_t 
                    state, bin_v(c, K_SUB16), bin_n(c, K_SUB16), flags, resume);
             LAUNCH(c, "ip_round", cdiv(bin_n(c, K_SUB32), 8), 256, k_ip_round<32>, c->indices, c->adj, rank,
                    state, bin_v(c, K_SUB32), bin_n(c, K_SUB32), flags, resume);
-            LAUNCH(c, "ip_round", cdiv(nmid, 4), 256, k_ip_round<64>, c->indices, c->adj, rank, state,
-                   bin_v(c, K_W1), nmid, flags, resume);
+            if (nmid >= (1u << 19))  // enough vertices to fill the device at 64 per wave
+                LAUNCH(c, "ip_round", cdiv(nmid, 256), 256, k_ip_round_wave, c->indices, c->adj, rank, state,
+                       bin_v(c, K_W1), nmid, flags, resume);
+            else
+                LAUNCH(c, "ip_round", cdiv(nmid, 4), 256, k_ip_round<64>, c->indices, c->adj, rank, state,
+                       bin_v(c, K_W1), nmid, flags, resume);
             LAUNCH(c, "ip_round", nbig, 256, k_ip_round_block, c->indices, c->adj, rank, state, bin_v(c, K_LDS),
                    nbig, flags, resume);
         }

@@ -265,6 +265,64 @@ __global__ __launch_bounds__(256) void k_ip_round(const uint32_t *__restrict__ i
     if (__ballot(still_open) && lane == 0) open_flag[0] = 1u;
 }
 
+// The decision round of k_ip_round for vertices of 33 to wave_max
+// neighbours: a wave takes 64 entries of the list, checks their states
+// together (late rounds find most of them decided) and walks the undecided
+// ones one after another with all of its lanes.
+__global__ __launch_bounds__(256) void k_ip_round_wave(const uint32_t *__restrict__ indices,
+                                                       const uint32_t *__restrict__ adj,
+                                                       const uint32_t *__restrict__ rank, uint8_t *state,
+                                                       const uint32_t *__restrict__ verts, uint32_t nv,
+                                                       uint32_t *__restrict__ open_flag,
+                                                       uint32_t *__restrict__ resume) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t slot = (blockIdx.x * 4u + (threadIdx.x >> 6)) * 64u + lane;
+    uint32_t mine = 0;
+    bool und = false;
+    if (slot < nv) {
+        mine = verts[slot];
+        und = __atomic_load_n(&state[mine], __ATOMIC_RELAXED) == 0;
+    }
+    uint64_t todo = __ballot(und);
+    bool any_open = false;
+    while (todo) {  // wave-uniform
+        const int l = __builtin_ctzll(todo);
+        todo &= todo - 1;
+        const uint32_t p = __shfl(mine, l, SCD_WAVE);
+        const uint32_t b = indices[p], e = indices[p + 1], rp = rank[p];
+        bool pending = false, seed_before = false;
+        uint32_t first_open = 0;
+        for (uint32_t s = resume[p]; b + s < e; s += 64) {  // wave-uniform
+            const uint32_t i = b + s + lane;
+            bool es = false, eo = false;
+            if (i < e) {
+                const uint32_t q = adj[i];
+                if (rank[q] < rp) {
+                    const uint8_t sq = __atomic_load_n(&state[q], __ATOMIC_RELAXED);
+                    es = sq == 1;
+                    eo = sq == 0;
+                }
+            }
+            const uint64_t bs = __ballot(es), bo = __ballot(eo);
+            if (bs) {
+                seed_before = true;
+                break;
+            }
+            if (bo && !pending) {
+                first_open = s + (uint32_t)__builtin_ctzll(bo);
+                pending = true;
+            }
+        }
+        if (lane == 0) {
+            if (seed_before) __atomic_store_n(&state[p], (uint8_t)2, __ATOMIC_RELAXED);
+            else if (!pending) __atomic_store_n(&state[p], (uint8_t)1, __ATOMIC_RELAXED);
+            else resume[p] = first_open;
+        }
+        any_open |= !seed_before && pending;
+    }
+    if (any_open && lane == 0) open_flag[0] = 1u;
+}
+
 // The decision round of k_ip_round for vertices of more than wave_max
 // neighbours: one workgroup per vertex, 256 neighbours at a time, the
 // chunk's seed / undecided findings combined through LDS.
