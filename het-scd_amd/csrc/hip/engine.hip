@@ -796,8 +796,11 @@ static int count_internal_triangles(scd_ctx *c, const WccArgs &w) {
     LAUNCH(c, "wcc_tri_final", std::min(cdiv(bin_n(c, K_SUB32), 8), fin_max), 256, k_wtc_final<32>, w,
            bin_v(c, K_SUB32), bin_n(c, K_SUB32), hub_lds);
     on_lane(c, 0);
-    LAUNCH(c, "wcc_tri_final", std::min(cdiv(c->wq_M, 4), fin_max), 256, k_wtc_final<64>, w, bin_v(c, K_W1),
-           c->wq_M, hub_lds);
+    if (!hub_lds && c->wq_M >= (1u << 19))  // a large arena's WCC: 64 list entries per wave
+        LAUNCH(c, "wcc_tri_final", cdiv(c->wq_M, 256), 256, k_wtc_final_wave, w, bin_v(c, K_W1), c->wq_M);
+    else
+        LAUNCH(c, "wcc_tri_final", std::min(cdiv(c->wq_M, 4), fin_max), 256, k_wtc_final<64>, w, bin_v(c, K_W1),
+               c->wq_M, hub_lds);
     join_lanes(c);
     LAUNCH(c, "wcc_tri_hot", cdiv(w.hot_n, 256), 256, k_wtc_hot_sum, w);
     return SCD_OK;

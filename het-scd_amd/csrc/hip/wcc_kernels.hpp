@@ -173,6 +173,7 @@ __global__ __launch_bounds__(256) void k_wtc_group(WccArgs a, const uint32_t *__
         if (dp > (uint32_t)S) dp = 0;  // work queue
         if (dp) beg = a.indices[u];
     }
+    if (!__syncthreads_or(dp != 0)) return;  // nothing to count in this workgroup (uniform)
     uint32_t x = SCD_INVALID, len = 0, vb = 0;
     if ((uint32_t)lig < dp) {
         x = a.int_adj[beg + lig];
@@ -607,6 +608,41 @@ __global__ __launch_bounds__(256) void k_wtc_final(WccArgs a, const uint32_t *__
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < hot; i += 256)
         if (s_td[i]) wtc_add_td(a, i, s_td[i]);
+}
+
+// k_wtc_final for the out-lists of more than 32 entries of a large arena's
+// WCC (no LDS hub counters): a wave takes 64 entries of the list, finds
+// those with a non-empty out-list together (late iterations leave most
+// communities unchanged, their out-lists empty) and walks only those, one
+// after another with all of its lanes.
+__global__ __launch_bounds__(256) void k_wtc_final_wave(WccArgs a, const uint32_t *__restrict__ verts, uint32_t nv) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t step = gridDim.x * 256u;
+    for (uint32_t base = blockIdx.x * 256u + (threadIdx.x & ~63u); base < nv; base += step) {  // wave-uniform
+        const uint32_t slot = base + lane;
+        uint32_t u = 0, dp = 0;
+        if (slot < nv) {
+            u = verts[slot];
+            dp = a.dplus[u];
+        }
+        uint64_t todo = __ballot(dp != 0);
+        while (todo) {  // wave-uniform
+            const int l = __builtin_ctzll(todo);
+            todo &= todo - 1;
+            const uint32_t uu = __shfl(u, l, SCD_WAVE), d = __shfl(dp, l, SCD_WAVE);
+            const uint32_t beg = a.indices[uu];
+            uint32_t mine = 0;
+            for (uint32_t k = lane; k < d; k += 64) {
+                if (a.flag[beg + k]) {
+                    wtc_add_td(a, a.int_adj[beg + k], 1u);
+                    mine++;
+                }
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off, SCD_WAVE);
+            if (lane == 0 && mine) atomicAdd(&a.td[uu], mine);
+        }
+    }
 }
 
 // tri[u] = 2 x triangles at u (the layout of Graph::triangles).
