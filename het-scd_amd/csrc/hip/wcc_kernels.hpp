@@ -257,7 +257,7 @@ __device__ __forceinline__ int wtc_find(const uint32_t *hk, const uint16_t *hp, 
 // N+_int(u), walks the member lists of its chunk — one list per wave when
 // they average >= 32 entries, else flattened over the threads — counts per
 // out-list position in LDS and flushes one atomic per non-zero position.
-constexpr uint32_t WTC_VC = 32;    // out-list entries per work item
+constexpr uint32_t WTC_VC = 64;     // out-list entries per work item
 constexpr uint32_t WTC_VCV = 128;  // upper-list entries per work item counted from the owner's side
 constexpr int WTC_QT = 512;  // work-queue workgroup: 8 waves, 64 VGPRs, 4 workgroups per CU
 constexpr uint32_t WTC_CLAIM = 16;  // items claimed at once by a workgroup
