@@ -260,7 +260,7 @@ __device__ __forceinline__ int wtc_find(const uint32_t *hk, const uint16_t *hp, 
 constexpr uint32_t WTC_VC = 64;     // out-list entries per work item
 constexpr uint32_t WTC_VCV = 128;  // upper-list entries per work item counted from the owner's side
 constexpr int WTC_QT = 512;  // work-queue workgroup: 8 waves, 64 VGPRs, 4 workgroups per CU
-constexpr uint32_t WTC_CLAIM = 16;  // items claimed at once by a workgroup
+constexpr uint32_t WTC_CLAIM = 8;  // items claimed at once by a workgroup
 
 // nch[i] = chunks of verts[M-1-i]; also resets the item counter.  From u's
 // side (VC false): chunks of the out-list when it is longer than 64 entries;
