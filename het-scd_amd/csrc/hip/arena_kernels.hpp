@@ -68,8 +68,9 @@ __global__ __launch_bounds__(256) void k_arena_vertices(const ArenaPart *__restr
                                                         uint32_t E, int labels, uint32_t *__restrict__ indices,
                                                         int32_t *__restrict__ tri, int32_t *__restrict__ tri_deg,
                                                         uint32_t *__restrict__ gid, uint32_t *__restrict__ cur,
-                                                        uint32_t *__restrict__ best) {
+                                                        uint32_t *__restrict__ best, uint32_t *__restrict__ bad) {
     __shared__ uint32_t s_g;
+    bool ok = true;
     const uint32_t base = blockIdx.x * ARENA_CHUNK;
     if (threadIdx.x == 0) s_g = base < n ? arena_find(G, base, [&](uint32_t g) { return parts[g].vo; }) : 0u;
     __syncthreads();
@@ -89,22 +90,28 @@ __global__ __launch_bounds__(256) void k_arena_vertices(const ArenaPart *__restr
         }
         const ArenaPart &P = parts[g];
         const uint32_t v = i - lo;
-        indices[i] = P.indices[v] + P.eo;
+        const uint32_t b0 = P.indices[v], b1 = P.indices[v + 1];
+        // the part's CSR offsets: from 0 to 2m, not decreasing
+        ok &= b0 <= b1 && (v != 0 || b0 == 0) && (v + 1 != P.n || b1 == P.e2);
+        indices[i] = b0 + P.eo;
         tri[i] = P.tri[v];
         tri_deg[i] = P.tri_deg[v];
         gid[i] = g;
         if (labels) {
             const uint32_t c = P.cur[v], b = P.best ? P.best[v] : c;
+            ok &= (c == SCD_INVALID || c < P.n) && (b == SCD_INVALID || b < P.n);
             cur[i] = c == SCD_INVALID ? c : c + lo;
             best[i] = b == SCD_INVALID ? b : b + lo;
         }
     }
+    if (__any(!ok) && (threadIdx.x & 63) == 0) bad[0] = 1u;
 }
 
 // Arena adjacency [0, E): ids shifted by the graph's first vertex.
 __global__ __launch_bounds__(256) void k_arena_entries(const ArenaPart *__restrict__ parts, uint32_t G, uint32_t E,
-                                                       uint32_t *__restrict__ adj) {
+                                                       uint32_t *__restrict__ adj, uint32_t *__restrict__ bad) {
     __shared__ uint32_t s_g;
+    bool ok = true;
     const uint32_t base = blockIdx.x * ARENA_CHUNK;
     if (threadIdx.x == 0) s_g = arena_find(G, base, [&](uint32_t g) { return parts[g].eo; });
     __syncthreads();
@@ -118,8 +125,11 @@ __global__ __launch_bounds__(256) void k_arena_entries(const ArenaPart *__restri
             lo = parts[g].eo;
             hi = lo + parts[g].e2;
         }
-        adj[j] = parts[g].adj[j - lo] + parts[g].vo;
+        const uint32_t x = parts[g].adj[j - lo];
+        ok &= x < parts[g].n;
+        adj[j] = x + parts[g].vo;
     }
+    if (__any(!ok) && (threadIdx.x & 63) == 0) bad[0] = 1u;
 }
 
 // Labels of the graphs in outs[0..G) (output space [0, total), graph g at
@@ -152,6 +162,36 @@ __global__ __launch_bounds__(256) void k_arena_export(const ArenaOut *__restrict
             O.best[v] = l == SCD_INVALID ? l : l - O.label_off;
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Input checks before any kernel indexes by an adjacency entry or a label.
+// Each sets bad[0] = 1 when it finds a violation (bad cleared beforehand).
+// ---------------------------------------------------------------------------
+// Every adjacency entry below n.
+__global__ __launch_bounds__(256) void k_check_adj(const uint32_t *__restrict__ adj, uint64_t E, uint32_t n,
+                                                   uint32_t *__restrict__ bad) {
+    bool ok = true;
+    for (uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x; j < E; j += (uint64_t)gridDim.x * 256)
+        ok &= adj[j] < n;
+    if (__any(!ok) && (threadIdx.x & 63) == 0) bad[0] = 1u;
+}
+
+// Arena of G graphs uploaded as one union (graph g: vertices [vtx_off[g],
+// vtx_off[g+1])): every list points into its own graph's vertices.  One
+// wave per vertex, four per workgroup.
+__global__ __launch_bounds__(256) void k_check_arena(const uint32_t *__restrict__ indices,
+                                                     const uint32_t *__restrict__ adj,
+                                                     const uint32_t *__restrict__ gid,
+                                                     const uint64_t *__restrict__ vtx_off, uint32_t n,
+                                                     uint32_t *__restrict__ bad) {
+    const uint32_t v = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (v >= n) return;  // wave-uniform
+    const uint32_t g = gid[v];
+    const uint32_t lo = (uint32_t)vtx_off[g], hi = (uint32_t)vtx_off[g + 1];
+    bool ok = true;
+    for (uint32_t j = indices[v] + (threadIdx.x & 63); j < indices[v + 1]; j += 64) ok &= adj[j] >= lo && adj[j] < hi;
+    if (__any(!ok) && (threadIdx.x & 63) == 0) bad[0] = 1u;
 }
 
 }  // namespace scd_dev

@@ -116,6 +116,10 @@ struct scd_ctx {
     bool wcc_valid = false;
     double *terms = nullptr, *partial = nullptr, *wcc_dev = nullptr;
     double *wcc_host = nullptr;  // pinned: [0] mean WCC of the last pass
+    // input checks of the uploads: device word set by the check kernels,
+    // its pinned host copy
+    uint32_t *chk = nullptr;
+    uint32_t *chk_host = nullptr;
     // device counters of the last pass: [0, K_NB) evaluated pairs per degree
     // bin, [K_NB] internal adjacency entries, [K_NB + 1] sum of internal
     // degree squared
@@ -917,6 +921,7 @@ extern "C" int scd_init(int device_id, int work_group_size, int profile, scd_ctx
         return fail(SCD_ENOMEM, "hipHostMalloc: %s", hipGetErrorString(e));
     }
     c->counters_host = (unsigned long long *)((char *)c->wcc_host + 64);
+    c->chk_host = (uint32_t *)((char *)c->wcc_host + 192);
     for (int i = 0; i < 16; i++) c->counters_host[i] = 0;
     *out = c;
     return SCD_OK;
@@ -943,6 +948,36 @@ extern "C" void scd_destroy(scd_ctx *c) {
 static int alloc_graph(scd_ctx *c, uint32_t n, uint32_t m, double clustering_coef);
 static int finish_upload(scd_ctx *c);
 
+// Input checks on the device before any kernel indexes by adjacency entries
+// or labels: check_begin clears the check word, the check kernels follow on
+// the stream, check_end waits and turns a violation into SCD_EINVAL (the
+// half-uploaded graph is dropped).
+static int check_begin(scd_ctx *c) {
+    LAUNCH(c, "check_clear", 1, 64, k_fill_u32, c->chk, 1u);
+    return check_launch("input check");
+}
+
+static int check_end(scd_ctx *c, const char *what) {
+    int rc = check_launch(what);
+    if (rc == SCD_OK) {
+        hipError_t e = hipMemcpyAsync(c->chk_host, c->chk, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = fail(SCD_EDEVICE, "%s: %s", what, hipGetErrorString(e));
+        else if (c->chk_host[0]) rc = fail(SCD_EINVAL, "%s", what);
+    }
+    if (rc) free_graph(c);
+    return rc;
+}
+
+static int check_adjacency(scd_ctx *c, const char *what) {
+    int rc = check_begin(c);
+    if (rc) return rc;
+    if (c->E)
+        LAUNCH(c, "check_adj", std::min<uint32_t>(cdiv(c->E, 256), 64u * c->num_cus), 256, k_check_adj, c->adj, c->E,
+               c->n, c->chk);
+    return check_end(c, what);
+}
+
 extern "C" int scd_upload(scd_ctx *c, uint32_t n, uint32_t m, const uint32_t *indices,
                           const uint32_t *adj, const int32_t *tri, const int32_t *tri_deg,
                           double clustering_coef) {
@@ -962,6 +997,7 @@ extern "C" int scd_upload(scd_ctx *c, uint32_t n, uint32_t m, const uint32_t *in
     if (E) HIP_TRY(hipMemcpyAsync(c->adj, adj, sizeof(uint32_t) * E, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->tri, tri, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->tri_deg, tri_deg, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+    if ((rc = check_adjacency(c, "scd_upload: an adjacency entry is not a vertex id (>= n)"))) return rc;
     return finish_upload(c);
 }
 
@@ -992,7 +1028,7 @@ static int alloc_graph(scd_ctx *c, uint32_t n, uint32_t m, double clustering_coe
         (rc = dalloc(c, &c->hot_T, hot_words(n))) || (rc = dalloc(c, &c->hot_td, hot_words(n))) ||
         (rc = dalloc(c, &c->partial, cdiv(n, 4096))) || (rc = dalloc(c, &c->wcc_dev, 1)) ||
         (rc = dalloc(c, &c->counters, 16)) || (rc = dalloc(c, &c->snap_labels, n)) ||
-        (rc = dalloc(c, &c->snap_stats, n))) {
+        (rc = dalloc(c, &c->snap_stats, n)) || (rc = dalloc(c, &c->chk, 64))) {
         free_graph(c);
         return rc;
     }
@@ -1461,6 +1497,97 @@ extern "C" int scd_prof_reset(scd_ctx *c) {
 }
 
 extern "C" uint64_t scd_device_bytes(const scd_ctx *c) { return c ? c->dev_bytes : 0; }
+
+// ---------------------------------------------------------------------------
+// The move step's arithmetic on caller rows (dev.hpp's improvement and
+// candidate set-up, the functions every move kernel inlines).
+// ---------------------------------------------------------------------------
+__global__ void k_eval_improvement(uint32_t count, const int32_t *__restrict__ r, const int32_t *__restrict__ d_in,
+                                   const int32_t *__restrict__ d_out, const int32_t *__restrict__ c_out,
+                                   const double *__restrict__ p_in, const double *__restrict__ p_ext,
+                                   double *__restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) out[i] = improvement(r[i], d_in[i], d_out[i], c_out[i], p_in[i], p_ext[i], 1.0);
+}
+
+__global__ void k_eval_candidate(uint32_t count, const int4 *__restrict__ s, const uint32_t *__restrict__ L,
+                                 const uint32_t *__restrict__ freq, const uint32_t *__restrict__ deg,
+                                 const uint32_t *__restrict__ my, double p_ext, double *__restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) out[i] = candidate_improvement_s(s[i], L[i], freq[i], deg[i], my[i], p_ext);
+}
+
+// Copies `count` elements of each host array to a device temporary.
+template <typename T>
+static int upload_tmp(scd_ctx *c, const T *h, uint32_t count, T **d, std::vector<DevBuf> &tmp) {
+    int rc = dalloc(c, d, count, &tmp);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(*d, h, sizeof(T) * count, hipMemcpyHostToDevice, c->stream));
+    return SCD_OK;
+}
+
+extern "C" int scd_eval_improvement(scd_ctx *c, uint32_t count, const int32_t *r, const int32_t *d_in,
+                                    const int32_t *d_out, const int32_t *c_out, const double *p_in,
+                                    const double *p_ext, double *out) {
+    if (!c) return fail(SCD_EINVAL, "scd_eval_improvement: null context");
+    if (count == 0) return SCD_OK;
+    if (!r || !d_in || !d_out || !c_out || !p_in || !p_ext || !out)
+        return fail(SCD_EINVAL, "scd_eval_improvement: null array");
+    (void)hipSetDevice(c->device);
+    std::vector<DevBuf> tmp;
+    int32_t *dr = nullptr, *di = nullptr, *dd = nullptr, *dc = nullptr;
+    double *dpi = nullptr, *dpe = nullptr, *dout = nullptr;
+    int rc;
+    if ((rc = upload_tmp(c, r, count, &dr, tmp)) || (rc = upload_tmp(c, d_in, count, &di, tmp)) ||
+        (rc = upload_tmp(c, d_out, count, &dd, tmp)) || (rc = upload_tmp(c, c_out, count, &dc, tmp)) ||
+        (rc = upload_tmp(c, p_in, count, &dpi, tmp)) || (rc = upload_tmp(c, p_ext, count, &dpe, tmp)) ||
+        (rc = dalloc(c, &dout, count, &tmp))) {
+        release_tmp(c, tmp);
+        return rc;
+    }
+    hipLaunchKernelGGL(k_eval_improvement, dim3(cdiv(count, 256)), dim3(256), 0, c->stream, count, dr, di, dd, dc,
+                       dpi, dpe, dout);
+    if ((rc = check_launch("scd_eval_improvement")) == SCD_OK) {
+        hipError_t e = hipMemcpyAsync(out, dout, sizeof(double) * count, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = fail(SCD_EDEVICE, "scd_eval_improvement: %s", hipGetErrorString(e));
+    }
+    release_tmp(c, tmp);
+    return rc;
+}
+
+extern "C" int scd_eval_candidate(scd_ctx *c, uint32_t count, const int32_t *size, const int32_t *ext_edges,
+                                  const int32_t *int_edges, const uint32_t *label, const uint32_t *freq,
+                                  const uint32_t *degree, const uint32_t *my_label, double clustering_coef,
+                                  double *out) {
+    if (!c) return fail(SCD_EINVAL, "scd_eval_candidate: null context");
+    if (count == 0) return SCD_OK;
+    if (!size || !ext_edges || !int_edges || !label || !freq || !degree || !my_label || !out)
+        return fail(SCD_EINVAL, "scd_eval_candidate: null array");
+    (void)hipSetDevice(c->device);
+    std::vector<int4> s(count);
+    for (uint32_t i = 0; i < count; i++) s[i] = make_int4(size[i], ext_edges[i], int_edges[i], 0);
+    std::vector<DevBuf> tmp;
+    int4 *ds = nullptr;
+    uint32_t *dl = nullptr, *df = nullptr, *dg = nullptr, *dm = nullptr;
+    double *dout = nullptr;
+    int rc;
+    if ((rc = upload_tmp(c, s.data(), count, &ds, tmp)) || (rc = upload_tmp(c, label, count, &dl, tmp)) ||
+        (rc = upload_tmp(c, freq, count, &df, tmp)) || (rc = upload_tmp(c, degree, count, &dg, tmp)) ||
+        (rc = upload_tmp(c, my_label, count, &dm, tmp)) || (rc = dalloc(c, &dout, count, &tmp))) {
+        release_tmp(c, tmp);
+        return rc;
+    }
+    hipLaunchKernelGGL(k_eval_candidate, dim3(cdiv(count, 256)), dim3(256), 0, c->stream, count, ds, dl, df, dg, dm,
+                       clustering_coef, dout);
+    if ((rc = check_launch("scd_eval_candidate")) == SCD_OK) {
+        hipError_t e = hipMemcpyAsync(out, dout, sizeof(double) * count, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = fail(SCD_EDEVICE, "scd_eval_candidate: %s", hipGetErrorString(e));
+    }
+    release_tmp(c, tmp);
+    return rc;
+}
 
 // ---------------------------------------------------------------------------
 // Triangle statistics (src/convert.cpp:271-340) on the device.
@@ -1939,7 +2066,10 @@ extern "C" int scd_batch_upload(scd_ctx *c, uint32_t G, const uint64_t *vtx_off,
     if (E & 1u) return fail(SCD_EINVAL, "scd_batch_upload: odd adjacency length %u", E);
     int rc = scd_upload(c, n, E / 2, indices, adj, tri, tri_deg, 0.0);
     if (rc) return rc;
-    return setup_batch(c, G, vtx_off, cc, true);
+    if ((rc = setup_batch(c, G, vtx_off, cc, true))) return rc;
+    if ((rc = check_begin(c))) return rc;
+    LAUNCH(c, "check_arena", cdiv(n, 4), 256, k_check_arena, c->indices, c->adj, c->gid, c->vtx_off_d, n, c->chk);
+    return check_end(c, "scd_batch_upload: an adjacency list leaves its graph's vertex range");
 }
 
 // Per-graph state of an arena whose union graph is uploaded: graph of each
@@ -2193,14 +2323,21 @@ extern "C" int scd_batch_upload_parts(scd_ctx *c, uint32_t G, const scd_part *pa
         release_tmp(c, tmp);
         return rc;
     }
-    LAUNCH(c, "arena_vertices", cdiv(n + 1ull, ARENA_CHUNK), 256, k_arena_vertices, d_table, G, n, Eall,
-           cur_labels ? 1 : 0, c->indices, c->tri, c->tri_deg, c->gid, c->cur, c->best);
-    if (Eall) LAUNCH(c, "arena_entries", cdiv(Eall, ARENA_CHUNK), 256, k_arena_entries, d_table, G, Eall, c->adj);
-    if ((rc = check_launch("arena assembly"))) {
+    if ((rc = check_begin(c))) {
         release_tmp(c, tmp);
         return rc;
     }
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    // the assembly kernels check the parts as they copy them: CSR offsets
+    // from 0 to 2m, adjacency entries and labels below the part's n
+    LAUNCH(c, "arena_vertices", cdiv(n + 1ull, ARENA_CHUNK), 256, k_arena_vertices, d_table, G, n, Eall,
+           cur_labels ? 1 : 0, c->indices, c->tri, c->tri_deg, c->gid, c->cur, c->best, c->chk);
+    if (Eall)
+        LAUNCH(c, "arena_entries", cdiv(Eall, ARENA_CHUNK), 256, k_arena_entries, d_table, G, Eall, c->adj, c->chk);
+    if ((rc = check_end(c, "scd_batch_upload_parts: a part's CSR offsets, adjacency entries or labels are out "
+                           "of range"))) {
+        release_tmp(c, tmp);
+        return rc;
+    }
     rc = finish_upload(c);
     release_tmp(c, tmp);
     if (rc) return rc;

@@ -287,6 +287,15 @@ extern "C" int scd_server_submit(scd_server *s, const scd_request *req, uint64_t
     for (uint32_t v = 0; v < q.n; v++)
         if (q.indices[v + 1] < q.indices[v] + 2)
             return sfail(SCD_EINVAL, "scd_server_submit: vertex %u has degree < 2 (not a convert output)", v);
+    // every adjacency entry must be a vertex id: the kernels index by them
+    const int64_t E = 2ll * q.m;
+    int64_t bad = E;
+#pragma omp parallel for reduction(min : bad) if (E > (1 << 20))
+    for (int64_t j = 0; j < E; j++)
+        if (q.adj[j] >= q.n && j < bad) bad = j;
+    if (bad < E)
+        return sfail(SCD_EINVAL, "scd_server_submit: adjacency entry %lld is %u, not below n = %u", (long long)bad,
+                     q.adj[bad], q.n);
     Req r;
     r.in = q;
     r.res.status = SCD_OK;

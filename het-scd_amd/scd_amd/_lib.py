@@ -181,6 +181,9 @@ def hip():
         "scd_batch_step": (ctypes.c_int, [C, u32p, f64p]),
         "scd_autotune": (ctypes.c_int, [C, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ScdTuningC), f64p]),
         "scd_batch_store_best": (ctypes.c_int, [C, u32p]),
+        "scd_eval_improvement": (ctypes.c_int, [C, ctypes.c_uint32, i32p, i32p, i32p, i32p, f64p, f64p, f64p]),
+        "scd_eval_candidate": (ctypes.c_int, [C, ctypes.c_uint32, i32p, i32p, i32p, u32p, u32p, u32p, u32p,
+                                              ctypes.c_double, f64p]),
     }
     for k, (res, args) in sig.items():
         f = getattr(lib, k, None)

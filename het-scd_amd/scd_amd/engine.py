@@ -297,6 +297,26 @@ class Engine:
     def device_bytes(self):
         return int(_lib.hip().scd_device_bytes(self._ctx))
 
+    def eval_improvement(self, r, d_in, d_out, c_out, p_in, p_ext):
+        """calculate_improvement (src/scd.cpp:173-219, alpha 1.0) per row on
+        the device, by the move kernels' own device function."""
+        a = [np.ascontiguousarray(x, np.int32) for x in (r, d_in, d_out, c_out)]
+        f = [np.ascontiguousarray(x, np.float64) for x in (p_in, p_ext)]
+        out = np.empty(len(a[0]), np.float64)
+        check_hip(_lib.hip().scd_eval_improvement(self._ctx, len(out), *(ptr(x, i32p) for x in a),
+                                                  *(ptr(x, f64p) for x in f), ptr(out, f64p)))
+        return out
+
+    def eval_candidate(self, size, ext_edges, int_edges, label, freq, degree, my_label, clustering_coef):
+        """The candidate set-up of src/scd.cpp:280-307 plus the improvement,
+        per row on the device (scd_eval_candidate)."""
+        s = [np.ascontiguousarray(x, np.int32) for x in (size, ext_edges, int_edges)]
+        u = [np.ascontiguousarray(x, np.uint32) for x in (label, freq, degree, my_label)]
+        out = np.empty(len(s[0]), np.float64)
+        check_hip(_lib.hip().scd_eval_candidate(self._ctx, len(out), *(ptr(x, i32p) for x in s),
+                                                *(ptr(x, u32p) for x in u), clustering_coef, ptr(out, f64p)))
+        return out
+
 
 class BatchEngine:
     """Batched refinement of an Arena (scd_batch_*): every graph follows the

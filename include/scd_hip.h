@@ -320,6 +320,31 @@ int scd_prof_reset(scd_ctx *ctx);
 uint64_t scd_device_bytes(const scd_ctx *ctx);
 
 /*
+ * The move step's arithmetic on caller rows, evaluated on the device by the
+ * same device functions the move kernels inline (for bit-level tests of the
+ * fp64 formula; none of them needs a graph).
+ *
+ * scd_eval_improvement: out[i] = calculate_improvement(r[i], d_in[i],
+ * d_out[i], c_out[i], p_in[i], p_ext[i], alpha = 1.0) (src/scd.cpp:173-219):
+ * the int32 products wrap and every fp64 operation rounds on its own.
+ *
+ * scd_eval_candidate: out[i] = the improvement of moving a vertex of degree
+ * degree[i] and label my_label[i] into community label[i], seen freq[i]
+ * times among its neighbours, where that community has size[i] members,
+ * ext_edges[i] boundary and int_edges[i] internal (directed) edge ends, and
+ * p_ext = clustering_coef: the candidate set-up of src/scd.cpp:280-307
+ * (own community without the vertex, int_edges halved, p_in from the size)
+ * followed by calculate_improvement.
+ */
+int scd_eval_improvement(scd_ctx *ctx, uint32_t count, const int32_t *r, const int32_t *d_in,
+                         const int32_t *d_out, const int32_t *c_out, const double *p_in,
+                         const double *p_ext, double *out);
+int scd_eval_candidate(scd_ctx *ctx, uint32_t count, const int32_t *size, const int32_t *ext_edges,
+                       const int32_t *int_edges, const uint32_t *label, const uint32_t *freq,
+                       const uint32_t *degree, const uint32_t *my_label, double clustering_coef,
+                       double *out);
+
+/*
  * Batched arenas (no counterpart in the reference, which refines one graph
  * per process): G independent graphs refined together, every kernel launch
  * covering all of them.  The arena is the disjoint union of the graphs

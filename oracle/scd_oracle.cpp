@@ -16,6 +16,8 @@
 //   orc_collect_stats        golden stats; sum invariants
 //   orc_improvement          golden table from the reference's own
 //                            calculate_improvement, incl. int32-overflow rows
+//   orc_candidate_improvement the candidate step of orc_improve: golden
+//                            per-iteration labels; brute-force move rule
 //   orc_improve              golden per-iteration labels; exhaustive T2 check
 //   orc_compute_wcc          golden WCC (1e-9); bowtie = 0.5; brute-force
 //                            internal triangle counts
@@ -139,6 +141,25 @@ double orc_improvement(int r, int d_in, int d_out, int c_out, double p_in, doubl
     return A + (double)d_in * BMinus + (double)wsub(r, d_in) * CMinus;
 }
 
+// src/scd.cpp:280-307: the improvement of moving a vertex of degree deg and
+// label my_label into community label (size / ext / int edges of that
+// community), seen freq times among its neighbours.
+double orc_candidate_improvement(int size, int ext_edge, int int_edges, uint32_t label, uint32_t freq,
+                                 uint32_t deg, uint32_t my_label, double clustering_coef, double alpha) {
+    int int_edge = int_edges / 2;
+    if (label == my_label) {
+        size--;
+        ext_edge = wadd(ext_edge, (int)(freq - (deg - freq)));
+        int_edge = wsub(int_edge, (int)freq);
+    }
+    const int d_in = (int)freq;
+    const int d_out = wsub((int)deg, d_in);
+    const int c_out = ext_edge;
+    double p_int = 0.0;
+    if (size > 1) p_int = (2.0 * int_edge) / (size * (size - 1.0));
+    return orc_improvement(size, d_in, d_out, c_out, p_int, clustering_coef, alpha);
+}
+
 // src/scd.cpp:221-341: synchronous best-move step, old labels -> new labels.
 void orc_improve(uint32_t n, const uint32_t *indices, const uint32_t *adj,
                  const uint32_t *old_labels, const int32_t *sizes, const int32_t *ext_edges,
@@ -168,21 +189,9 @@ void orc_improve(uint32_t n, const uint32_t *indices, const uint32_t *adj,
                 }
                 while (index + freq < deg && buffer[index + freq] == other_label) freq++;
 
-                int size = sizes[other_label];
-                int ext_edge = ext_edges[other_label];
-                int int_edge = int_edges[other_label] / 2;
-                if (other_label == my_label) {
-                    size--;
-                    ext_edge = wadd(ext_edge, (int)(freq - (deg - freq)));
-                    int_edge = wsub(int_edge, (int)freq);
-                }
-                const int d_in = (int)freq;
-                const int d_out = wsub((int)deg, d_in);
-                const int c_out = ext_edge;
-                double p_int = 0.0;
-                if (size > 1) p_int = (2.0 * int_edge) / (size * (size - 1.0));
-                const double improvement =
-                    orc_improvement(size, d_in, d_out, c_out, p_int, clustering_coef, alpha);
+                const double improvement = orc_candidate_improvement(
+                    sizes[other_label], ext_edges[other_label], int_edges[other_label], other_label, freq,
+                    deg, my_label, clustering_coef, alpha);
                 if (other_label == my_label) remove_improvement = -improvement;
                 if (improvement > insert_improvement) {
                     insert_improvement = improvement;

@@ -39,6 +39,9 @@ def lib():
             "orc_initial_partition": (ctypes.c_uint32, [ctypes.c_uint32, u32p, u32p, i32p, u32p]),
             "orc_improvement": (ctypes.c_double, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                   ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+            "orc_candidate_improvement": (ctypes.c_double, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                            ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                                            ctypes.c_uint32, ctypes.c_double, ctypes.c_double]),
             "orc_improve": (None, [ctypes.c_uint32, u32p, u32p, u32p, i32p, i32p, i32p, ctypes.c_double,
                                    ctypes.c_double, u32p]),
             "orc_compute_wcc": (ctypes.c_double, [ctypes.c_uint32, u32p, u32p, i32p, i32p, u32p, i32p,
@@ -87,6 +90,10 @@ def collect_stats(g, labels, num_comms):
 
 def improvement(r, d_in, d_out, c_out, p_in, p_ext, alpha=1.0):
     return lib().orc_improvement(r, d_in, d_out, c_out, p_in, p_ext, alpha)
+
+
+def candidate_improvement(size, ext_edges, int_edges, label, freq, degree, my_label, cc, alpha=1.0):
+    return lib().orc_candidate_improvement(size, ext_edges, int_edges, label, freq, degree, my_label, cc, alpha)
 
 
 def improve(g, labels, sizes, ext, inte, cc):
