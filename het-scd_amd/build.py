@@ -99,6 +99,9 @@ def build(oracle=True):
             f.result()
     if os.path.exists(src("dropin", "dropin.cpp")):
         build_dropin_and_cli()
+    if oracle and os.path.isdir(REF_SRC):
+        # the reference's own main-gpu.cpp, unchanged, against libscd_dropin.so
+        run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref-dropin"])
 
 
 if __name__ == "__main__":

@@ -179,6 +179,9 @@ struct scd_ctx {
     uint32_t *tchunk_bl = nullptr, *tchunk_has_rem = nullptr;
     double *tchunk_bi = nullptr, *tchunk_rem = nullptr;
     uint32_t n_tchunks = 0;
+    // chunks launched by each merge pass (a prefix of the chunk list: hubs
+    // are ordered by degree, largest first)
+    std::vector<uint32_t> merge_pass_chunks;
     std::vector<DevBuf> hub_bufs;
 
     scd_tuning tune = {4096, 2048, 1024, 0, 0, 1, {0, 0}};
@@ -201,9 +204,22 @@ struct scd_ctx {
     // three more (default priority), forked and joined with events, so
     // small and tail-bound launches overlap (captured as parallel branches)
     hipStream_t lane[3] = {nullptr, nullptr, nullptr};
-    hipEvent_t fork_ev = nullptr, join_ev[3] = {nullptr, nullptr, nullptr};
+    // fork and join events: every fork or join point of a launch sequence
+    // records its own events (pool reset when the stream is idle, and at the
+    // start of each capture; captures draw from a pool of their own)
+    std::vector<hipEvent_t> lane_ev[2];  // [0] eager, [1] captured
+    size_t lane_ev_used[2] = {0, 0};
     hipStream_t ls = nullptr;  // the stream LAUNCH uses
     bool use_lanes = true;
+
+    // phase timing of each iteration (cuda_improve_labels' four lines,
+    // src/cuda.cu:347-360): device wall-clock stamps written by one-thread
+    // kernels at the phase boundaries, inside captured iterations too
+    bool phase_timing = false;
+    unsigned long long *stamps = nullptr;       // device, kStamps entries
+    unsigned long long *stamps_host = nullptr;  // pinned copy
+    double wall_clock_khz = 0.0;
+    double phase_ms[4] = {0, 0, 0, 0};
 
     std::vector<DevBuf> bufs;
     // buffers of the previous graph, reused by the next upload when a size
@@ -378,10 +394,28 @@ static inline bool lanes_on(const scd_ctx *c) {
     return c->use_lanes && c->tune.lanes && c->E >= (1ull << 20) && !c->profile && c->lane[0];
 }
 
+// The next unused fork / join event of the current pool (nullptr if the
+// runtime cannot create one).
+static hipEvent_t lane_event(scd_ctx *c) {
+    const int k = c->capturing ? 1 : 0;
+    auto &pool = c->lane_ev[k];
+    if (c->lane_ev_used[k] == pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+        pool.push_back(e);
+    }
+    return pool[c->lane_ev_used[k]++];
+}
+
 static void fork_lanes(scd_ctx *c) {
     if (!lanes_on(c)) return;
-    (void)hipEventRecord(c->fork_ev, c->stream);
-    for (int i = 0; i < 3; i++) (void)hipStreamWaitEvent(c->lane[i], c->fork_ev, 0);
+    hipEvent_t e = lane_event(c);
+    if (!e) {  // no event: stay on the context's stream from here on
+        c->use_lanes = false;
+        return;
+    }
+    (void)hipEventRecord(e, c->stream);
+    for (int i = 0; i < 3; i++) (void)hipStreamWaitEvent(c->lane[i], e, 0);
 }
 
 // Subsequent LAUNCHes go to lane i (0 = the context's stream).
@@ -391,9 +425,26 @@ static void join_lanes(scd_ctx *c) {
     c->ls = c->stream;
     if (!lanes_on(c)) return;
     for (int i = 0; i < 3; i++) {
-        (void)hipEventRecord(c->join_ev[i], c->lane[i]);
-        (void)hipStreamWaitEvent(c->stream, c->join_ev[i], 0);
+        hipEvent_t e = lane_event(c);
+        if (!e) {  // join through a synchronization instead (eager launches only)
+            (void)hipDeviceSynchronize();
+            c->use_lanes = false;
+            return;
+        }
+        (void)hipEventRecord(e, c->lane[i]);
+        (void)hipStreamWaitEvent(c->stream, e, 0);
     }
+}
+
+// Phase boundary i of an iteration (0: start, 1: after the move, 2: after
+// the filter kernels, 3: after the statistics, 4: after the WCC).
+constexpr int kStamps = 8;
+__global__ void k_stamp(unsigned long long *__restrict__ t, int i) {
+    if (threadIdx.x == 0) t[i] = wall_clock64();
+}
+
+static void stamp(scd_ctx *c, int i) {
+    if (c->phase_timing && c->stamps) hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, c->stream, c->stamps, i);
 }
 
 static int check_launch(const char *what) {
@@ -497,6 +548,20 @@ static int build_bins(scd_ctx *c) {
         HIP_TRY(hipMemcpyAsync(hd.data(), d_hd, 4ull * nh, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
     }
+    // hubs by degree, largest first: the hubs that a merge pass of width w
+    // still sorts (degree > w) then own a prefix of the chunk list
+    {
+        std::vector<uint32_t> ord(nh);
+        for (uint32_t h = 0; h < nh; h++) ord[h] = h;
+        std::stable_sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) { return hd[x] > hd[y]; });
+        std::vector<uint32_t> v2(nh), d2(nh);
+        for (uint32_t h = 0; h < nh; h++) {
+            v2[h] = hv[ord[h]];
+            d2[h] = hd[ord[h]];
+        }
+        hv.swap(v2);
+        hd.swap(d2);
+    }
     uint64_t off = 0;
     for (uint32_t h = 0; h < nh; h++) {
         const uint32_t d = hd[h];
@@ -515,6 +580,13 @@ static int build_bins(scd_ctx *c) {
     }
     hcb[nh] = (uint32_t)ch.size();
     if (off >= (1ull << 32)) return fail(SCD_ELIMIT, "hub segments exceed 2^32 entries");
+    // chunks of each merge pass: those of the hubs with degree above its width
+    c->merge_pass_chunks.clear();
+    for (uint64_t w = T; w < c->max_hub_deg; w *= 2) {
+        uint32_t k = 0;
+        while (k < nh && hd[k] > w) k++;
+        c->merge_pass_chunks.push_back(hcb[k]);
+    }
     c->n_tiles = (uint32_t)th.size();
     c->n_chunks = (uint32_t)ch.size();
     auto up = [&](uint32_t **dst, const std::vector<uint32_t> &src) -> int {
@@ -660,13 +732,18 @@ static int phase_move(scd_ctx *c, uint32_t *labels_out, uint32_t *ex_sorted, uin
         LAUNCH(c, "hub_eval", c->n_tchunks, 256, k_hub_table_eval, at(K_HUB), ht, t, c->tune.merge_chunk);
         LAUNCH(c, "hub_finalize", cdiv(c->nh, 64), 64, k_hub_finalize, a, ht, c->nh);
     } else if (c->nh) {
+        // merge passes over the hubs still longer than the pass width; each
+        // hub's sorted keys end in hub_a or hub_b by the parity of its own
+        // pass count (k_hub_eval picks)
         const uint32_t C = c->tune.merge_chunk;
         uint32_t *src = c->hub_a, *dst = c->hub_b;
-        for (uint64_t w = c->tune.lds_tile; w < c->max_hub_deg; w *= 2) {
-            LAUNCH(c, "hub_merge", c->n_chunks, 256, k_hub_merge, h, src, dst, (uint32_t)w, C);
+        uint64_t w = c->tune.lds_tile;
+        for (uint32_t chunks : c->merge_pass_chunks) {
+            LAUNCH(c, "hub_merge", chunks, 256, k_hub_merge, h, src, dst, (uint32_t)w, C);
             std::swap(src, dst);
+            w *= 2;
         }
-        LAUNCH(c, "hub_eval", c->n_chunks, 256, k_hub_eval, at(K_HUB), h, src, C);
+        LAUNCH(c, "hub_eval", c->n_chunks, 256, k_hub_eval, at(K_HUB), h, c->hub_a, c->hub_b, c->tune.lds_tile, C);
         LAUNCH(c, "hub_finalize", cdiv(c->nh, 64), 64, k_hub_finalize, a, h, c->nh);
     }
     on_lane(c, 3);
@@ -683,8 +760,9 @@ static int phase_move(scd_ctx *c, uint32_t *labels_out, uint32_t *ex_sorted, uin
     return check_launch("move");
 }
 
-// Statistics + internal adjacency of the current labels.
-static int phase_stats(scd_ctx *c) {
+// Statistics + internal adjacency of the current labels.  stamps: write the
+// phase boundary after the filter kernels (iterations only).
+static int phase_stats(scd_ctx *c, bool stamps = false) {
     LAUNCH(c, "clear_stats", cdiv(c->num_comms, 256), 256, k_fill_int4, c->stats, c->num_comms);
     StatArgs s;
     s.indices = c->indices;
@@ -725,6 +803,7 @@ static int phase_stats(scd_ctx *c) {
                c->hub_chunk_begin, c->chunk_hub, c->chunk_start, C, c->chunk_cnt);
     }
     join_lanes(c);
+    if (stamps) stamp(c, 2);
     LAUNCH(c, "stats_agg", std::min<uint32_t>(cdiv(c->n, 1024), 2u * c->num_cus), 1024, k_stats_agg, c->indices,
            c->cur, c->int_deg, c->n, c->G ? c->gid : nullptr, c->active_g, c->stats);
     return check_launch("stats");
@@ -755,7 +834,7 @@ static int scan_u32_dev(scd_ctx *c, const uint32_t *in, uint32_t *out, uint64_t 
 // graph): T[u] = triangles at u, td[u] = edges at u in a triangle; the
 // kernels per degree bin and the work queue of long out-lists.
 static int count_internal_triangles(scd_ctx *c, const WccArgs &w) {
-    LAUNCH(c, "wcc_dplus", cdiv(c->n, 256), 256, k_wtc_dplus, w, c->n);
+    LAUNCH(c, "wcc_dplus", cdiv(c->n, c->block_size), c->block_size, k_wtc_dplus, w, c->n);
     // lanes: 3 small-degree groups | 2 out-lists up to 64 | 0 work queue
     fork_lanes(c);
     on_lane(c, 3);
@@ -834,7 +913,7 @@ static int phase_wcc(scd_ctx *c, int32_t *int_tri_out, int32_t *int_tri_deg_out,
     w.int_tri_deg_out = int_tri_deg_out;
     int rc = count_internal_triangles(c, w);
     if (rc) return rc;
-    LAUNCH(c, "wcc_terms", cdiv(c->n, 256), 256, k_wcc_terms, w, c->n);
+    LAUNCH(c, "wcc_terms", cdiv(c->n, c->block_size), c->block_size, k_wcc_terms, w, c->n);
     if (c->G) {
         LAUNCH(c, "wcc_sum_partial", c->n_witems, 256, k_reduce_partial_g, c->terms, c->witem_start, c->witem_end,
                c->partial_g);
@@ -854,6 +933,7 @@ static int clear_counters(scd_ctx *c) {
 
 static int finish(scd_ctx *c) {
     HIP_TRY(hipStreamSynchronize(c->stream));
+    c->lane_ev_used[0] = 0;  // every recorded fork / join event has completed
     return prof_collect(c);
 }
 
@@ -871,10 +951,8 @@ extern "C" int scd_device_count(void) {
 extern "C" int scd_init(int device_id, int work_group_size, int profile, scd_ctx **out) {
     if (!out) return fail(SCD_EINVAL, "scd_init: null output");
     *out = nullptr;
-    if (work_group_size != 64 && work_group_size != 128 && work_group_size != 256 &&
-        work_group_size != 512 && work_group_size != 1024)
-        return fail(SCD_EINVAL, "scd_init: work group size %d is not 64, 128, 256, 512 or 1024",
-                    work_group_size);
+    if (work_group_size < 1 || work_group_size > 1024)
+        return fail(SCD_EINVAL, "scd_init: work group size %d is not in [1, 1024]", work_group_size);
     int count = scd_device_count();
     if (count < 0) return count;
     if (device_id < 0 || device_id >= count)
@@ -901,10 +979,8 @@ extern "C" int scd_init(int device_id, int work_group_size, int profile, scd_ctx
     }
     c->ls = c->stream;
     {
-        bool ok = hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) == hipSuccess;
-        for (int i = 0; i < 3 && ok; i++)
-            ok = hipStreamCreateWithFlags(&c->lane[i], hipStreamNonBlocking) == hipSuccess &&
-                 hipEventCreateWithFlags(&c->join_ev[i], hipEventDisableTiming) == hipSuccess;
+        bool ok = true;
+        for (int i = 0; i < 3 && ok; i++) ok = hipStreamCreateWithFlags(&c->lane[i], hipStreamNonBlocking) == hipSuccess;
         if (!ok) {  // run everything on the one stream
             (void)hipGetLastError();
             c->use_lanes = false;
@@ -914,7 +990,7 @@ extern "C" int scd_init(int device_id, int work_group_size, int profile, scd_ctx
         if (const char *env = getenv("SCD_LANES"))
             if (atoi(env) == 0) c->use_lanes = false;
     }
-    e = hipHostMalloc((void **)&c->wcc_host, 256, hipHostMallocDefault);
+    e = hipHostMalloc((void **)&c->wcc_host, 512, hipHostMallocDefault);
     if (e != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
         delete c;
@@ -922,6 +998,12 @@ extern "C" int scd_init(int device_id, int work_group_size, int profile, scd_ctx
     }
     c->counters_host = (unsigned long long *)((char *)c->wcc_host + 64);
     c->chk_host = (uint32_t *)((char *)c->wcc_host + 192);
+    c->stamps_host = (unsigned long long *)((char *)c->wcc_host + 256);
+    {
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device_id) == hipSuccess && khz > 0)
+            c->wall_clock_khz = khz;
+    }
     for (int i = 0; i < 16; i++) c->counters_host[i] = 0;
     *out = c;
     return SCD_OK;
@@ -935,11 +1017,11 @@ extern "C" void scd_destroy(scd_ctx *c) {
     free_list(c, c->cache);
     free_list(c, c->tmp_cache);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
-    for (int i = 0; i < 3; i++) {
+    for (int i = 0; i < 3; i++)
         if (c->lane[i]) (void)hipStreamDestroy(c->lane[i]);
-        if (c->join_ev[i]) (void)hipEventDestroy(c->join_ev[i]);
-    }
-    if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+    for (auto &pool : c->lane_ev)
+        for (auto e : pool) (void)hipEventDestroy(e);
+    if (c->stamps) (void)hipFree(c->stamps);
     (void)hipHostFree(c->wcc_host);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1093,13 +1175,22 @@ extern "C" int scd_labels_to_device(scd_ctx *c, const uint32_t *labels) {
 static int enqueue_iteration(scd_ctx *c, bool incremental) {
     int rc;
     if (c->counting && (rc = clear_counters(c))) return rc;
+    stamp(c, 0);
     if ((rc = phase_move(c, c->next, nullptr, nullptr))) return rc;
+    stamp(c, 1);
     if (incremental) {
+        const uint32_t B = c->block_size;
         LAUNCH(c, "wcc_dirty_clear", cdiv(c->n, 256), 256, k_fill_u8, c->wdirty, (uint64_t)c->n);
-        LAUNCH(c, "wcc_dirty_mark", cdiv(c->n, 256), 256, k_mark_dirty, c->cur, c->next, c->n, c->wdirty);
+        LAUNCH(c, "wcc_dirty_mark", cdiv(c->n, B), B, k_mark_dirty, c->cur, c->next, c->n, c->wdirty);
     }
     std::swap(c->cur, c->next);
-    if ((rc = phase_stats(c)) || (rc = phase_wcc(c, nullptr, nullptr, incremental))) return rc;
+    if ((rc = phase_stats(c, true))) return rc;
+    stamp(c, 3);
+    if ((rc = phase_wcc(c, nullptr, nullptr, incremental))) return rc;
+    stamp(c, 4);
+    if (c->phase_timing && c->stamps)
+        HIP_TRY(hipMemcpyAsync(c->stamps_host, c->stamps, sizeof(unsigned long long) * kStamps,
+                               hipMemcpyDeviceToHost, c->stream));
     if (c->G) HIP_TRY(hipMemcpyAsync(c->wcc_g_host, c->wcc_g, 8ull * c->G, hipMemcpyDeviceToHost, c->stream));
     else HIP_TRY(hipMemcpyAsync(c->wcc_host, c->wcc_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     if (c->counting)
@@ -1114,6 +1205,7 @@ static int capture_iteration(scd_ctx *c, int par, int inc) {
     uint32_t *cur0 = c->cur, *next0 = c->next;
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     c->capturing = true;
+    c->lane_ev_used[1] = 0;
     int rc = enqueue_iteration(c, inc != 0);
     c->capturing = false;
     hipGraph_t g = nullptr;
@@ -1129,8 +1221,33 @@ static int capture_iteration(scd_ctx *c, int par, int inc) {
         }
     }
     if (g) (void)hipGraphDestroy(g);
+    // move the executable graph to the device now, not at its first launch
+    if (rc == SCD_OK) (void)hipGraphUpload(c->gexec[par][inc], c->stream);
     if (rc != SCD_OK) (void)hipGetLastError();
     return rc;
+}
+
+extern "C" int scd_capture_iterations(scd_ctx *c) {
+    int rc = need_labels(c, "scd_capture_iterations");
+    if (rc) return rc;
+    if (!c->use_graph || c->profile) return SCD_OK;
+    // the next iteration (its WCC mode as it stands), then both label
+    // parities with the incremental WCC every later iteration uses
+    const int par = c->cur == c->lab_a ? 0 : 1;
+    const int want[3][2] = {{par, c->wcc_valid ? 1 : 0}, {1 - par, 1}, {par, 1}};
+    for (const auto &w : want) {
+        const int p = w[0], inc = w[1];
+        if (c->gexec[p][inc]) continue;
+        // capture with the label buffers of that parity in place
+        uint32_t *cur0 = c->cur, *next0 = c->next;
+        c->cur = p == 0 ? c->lab_a : c->lab_b;
+        c->next = p == 0 ? c->lab_b : c->lab_a;
+        rc = capture_iteration(c, p, inc);
+        c->cur = cur0;
+        c->next = next0;
+        if (rc) return rc;
+    }
+    return finish(c);
 }
 
 extern "C" int scd_next_step_captures(const scd_ctx *c) {
@@ -1160,6 +1277,35 @@ extern "C" int scd_improve(scd_ctx *c, double *wcc, uint32_t *labels_out) {
         HIP_TRY(hipMemcpyAsync(labels_out, c->cur, sizeof(uint32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
     if ((rc = finish(c))) return rc;
     if (wcc) *wcc = c->wcc_host[0];
+    if (c->phase_timing && c->stamps && c->wall_clock_khz > 0) {
+        // reference order: move, statistics, removal of external edges, WCC
+        const unsigned long long *t = c->stamps_host;
+        const int from[4] = {0, 2, 1, 3}, to[4] = {1, 3, 2, 4};
+        for (int i = 0; i < 4; i++) c->phase_ms[i] = (double)(t[to[i]] - t[from[i]]) / c->wall_clock_khz;
+    }
+    return SCD_OK;
+}
+
+extern "C" int scd_set_phase_timing(scd_ctx *c, int on) {
+    if (!c) return fail(SCD_EINVAL, "scd_set_phase_timing: null context");
+    (void)hipSetDevice(c->device);
+    if (on && !c->stamps) {
+        hipError_t e = hipMalloc((void **)&c->stamps, sizeof(unsigned long long) * kStamps);
+        if (e != hipSuccess) {
+            c->stamps = nullptr;
+            return fail(SCD_ENOMEM, "scd_set_phase_timing: hipMalloc: %s", hipGetErrorString(e));
+        }
+    }
+    if (on && c->wall_clock_khz <= 0) return fail(SCD_EDEVICE, "scd_set_phase_timing: no device wall clock rate");
+    if (c->phase_timing != (on != 0)) drop_graphs(c);  // captured iterations hold the stamp kernels or not
+    c->phase_timing = on != 0;
+    for (double &x : c->phase_ms) x = 0.0;
+    return SCD_OK;
+}
+
+extern "C" int scd_last_phase_ms(const scd_ctx *c, double ms[4]) {
+    if (!c || !ms) return fail(SCD_EINVAL, "scd_last_phase_ms: null argument");
+    for (int i = 0; i < 4; i++) ms[i] = c->phase_ms[i];
     return SCD_OK;
 }
 

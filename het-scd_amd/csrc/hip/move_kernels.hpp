@@ -546,9 +546,11 @@ __global__ __launch_bounds__(256) void k_hub_merge(HubArgs h, const uint32_t *__
 
 // Reduce-by-key + evaluation over a chunk of a hub's sorted segment; the
 // chunk's best candidate and (if present) the own-label improvement go to
-// per-chunk slots.
-__global__ __launch_bounds__(256) void k_hub_eval(MoveArgs a, HubArgs h,
-                                                  const uint32_t *__restrict__ sorted, uint32_t C) {
+// per-chunk slots.  The tiles of T keys were sorted into buf_a; a hub of
+// degree d then took one merge pass per width T, 2T, ... below d, each
+// moving its keys to the other buffer.
+__global__ __launch_bounds__(256) void k_hub_eval(MoveArgs a, HubArgs h, const uint32_t *__restrict__ buf_a,
+                                                  const uint32_t *__restrict__ buf_b, uint32_t T, uint32_t C) {
     __shared__ double s_bi[4];
     __shared__ uint32_t s_bl[4];
     __shared__ double s_rem;
@@ -561,7 +563,9 @@ __global__ __launch_bounds__(256) void k_hub_eval(MoveArgs a, HubArgs h,
     const uint32_t p = h.hub_v[hb];
     const uint32_t my = a.labels[p];
     const uint32_t beg = a.indices[p];
-    const uint32_t *S = sorted + h.hub_off[hb];
+    uint32_t passes = 0;
+    for (uint64_t w = T; w < d; w *= 2) passes++;
+    const uint32_t *S = ((passes & 1u) ? buf_b : buf_a) + h.hub_off[hb];
     if (tid == 0) s_has_rem = 0;
     __syncthreads();
     double bi = 0.0;

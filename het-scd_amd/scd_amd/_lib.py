@@ -164,6 +164,9 @@ def hip():
         "scd_set_graph_mode": (ctypes.c_int, [C, ctypes.c_int]),
         "scd_next_step_captures": (ctypes.c_int, [C]),
         "scd_set_profiling": (ctypes.c_int, [C, ctypes.c_int]),
+        "scd_set_phase_timing": (ctypes.c_int, [C, ctypes.c_int]),
+        "scd_capture_iterations": (ctypes.c_int, [C]),
+        "scd_last_phase_ms": (ctypes.c_int, [C, f64p]),
         "scd_set_counting": (ctypes.c_int, [C, ctypes.c_int]),
         "scd_restore": (ctypes.c_int, [C]),
         "scd_calibration_copy": (ctypes.c_int, [C, ctypes.c_uint64, ctypes.c_int, f64p]),

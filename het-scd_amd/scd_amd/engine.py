@@ -277,6 +277,22 @@ class Engine:
     def set_profiling(self, on=True):
         check_hip(_lib.hip().scd_set_profiling(self._ctx, 1 if on else 0))
 
+    def capture_iterations(self):
+        """scd_capture_iterations: record the next iterations' hipGraphs now."""
+        check_hip(_lib.hip().scd_capture_iterations(self._ctx))
+
+    def set_phase_timing(self, on=True):
+        """scd_set_phase_timing: device clock stamps at the phase boundaries
+        of every iteration (captured iterations included)."""
+        check_hip(_lib.hip().scd_set_phase_timing(self._ctx, 1 if on else 0))
+
+    def last_phase_ms(self):
+        """(move, statistics, removal of external edges, WCC) of the last
+        iteration in ms (src/cuda.cu:347-360's split)."""
+        ms = np.zeros(4, np.float64)
+        check_hip(_lib.hip().scd_last_phase_ms(self._ctx, ptr(ms, f64p)))
+        return tuple(float(x) for x in ms)
+
     def set_graph_mode(self, on=True):
         check_hip(_lib.hip().scd_set_graph_mode(self._ctx, 1 if on else 0))
 

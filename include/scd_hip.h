@@ -47,9 +47,14 @@ int scd_device_count(void);
  * (src/cuda.hpp:24, src/cuda.cu:55-82).  An ordinal outside [0, count)
  * fails with SCD_EDEVICE, as the reference returns false after listing the
  * devices.  work_group_size is the reference's "group size" argument
- * (src/main-gpu.cpp:38); it sets the workgroup of the per-vertex kernels and
- * must be 64, 128, 256, 512 or 1024.  profile != 0 records a HIP event pair
- * around every kernel launch (scd_prof_*).
+ * (src/main-gpu.cpp:38), the block size of its kernels
+ * (src/cuda-common.cuh:38-44); any value in [1, 1024] is accepted, as a CUDA
+ * launch would.  Here it sets the workgroup of the element-wise per-vertex
+ * kernels (WCC terms, internal out-degrees, moved-community marks); the
+ * sort, reduce-by-key and intersection kernels keep the workgroup shapes
+ * their tiling is built for (64-lane waves, 256- and 512-thread groups).
+ * profile != 0 records a HIP event pair around every kernel launch
+ * (scd_prof_*) and launches iterations eagerly on one stream.
  */
 int scd_init(int device_id, int work_group_size, int profile, scd_ctx **out);
 void scd_destroy(scd_ctx *ctx);
@@ -62,7 +67,10 @@ void scd_destroy(scd_ctx *ctx);
  * src/graph.cpp:57-68; scd_graph_clustering_coef computes it).  Adjacency
  * lists must be sorted ascending without self loops or duplicates, which the
  * binary format guarantees.  Replaces any graph uploaded before.  n == 0 is
- * rejected (the reference would loop forever, src/main.cpp:77).
+ * rejected (the reference would loop forever, src/main.cpp:77), and so are
+ * offsets that do not run from 0 to 2m without decreasing and adjacency
+ * entries that are not vertex ids (>= n): SCD_EINVAL, checked before any
+ * kernel indexes by them.
  */
 int scd_upload(scd_ctx *ctx, uint32_t n, uint32_t m, const uint32_t *indices,
                const uint32_t *adj, const int32_t *tri, const int32_t *tri_deg,
@@ -173,8 +181,29 @@ int scd_set_graph_mode(scd_ctx *ctx, int on);
  * of a capture. */
 int scd_next_step_captures(const scd_ctx *ctx);
 
+/* Captures (and uploads to the device) the hipGraphs of the next
+ * iterations of the current labels -- the next one, then both label-buffer
+ * parities with the incremental WCC -- so that the first scd_improve calls
+ * only launch them.  No counterpart in the reference; the drop-in library
+ * calls it in cuda_labels_to_device, before the driver's timed loop.  No
+ * effect in eager mode or while profiling. */
+int scd_capture_iterations(scd_ctx *ctx);
+
 /* Turns per-kernel HIP event timing (scd_prof_*) on or off. */
 int scd_set_profiling(scd_ctx *ctx, int on);
+
+/*
+ * Phase timing of each scd_improve, as cuda_improve_labels prints it
+ * (src/cuda.cu:347-360): with on != 0, one-thread kernels write the device
+ * wall clock at the phase boundaries of every iteration -- inside captured
+ * iterations too, so iterations keep their graph replay and concurrent
+ * lanes -- and scd_last_phase_ms gives the last iteration's phases in the
+ * reference's order: best move, community statistics, removal of external
+ * edges (the internal adjacency), WCC.  Milliseconds of wall time between
+ * boundaries; the four add up to the iteration's device time.
+ */
+int scd_set_phase_timing(scd_ctx *ctx, int on);
+int scd_last_phase_ms(const scd_ctx *ctx, double ms[4]);
 
 /* Replaces cuda_store_labels() (src/cuda.cu:363-365): best := current. */
 int scd_store_best(scd_ctx *ctx);
@@ -354,7 +383,8 @@ int scd_eval_candidate(scd_ctx *ctx, uint32_t count, const int32_t *size, const 
  * src/main-gpu.cpp:71-96 with its own stopping rule; its labels, statistics,
  * per-iteration WCC (same summation order as a single-graph context) and
  * best labels equal the single-graph path's.  Labels at this API are local
- * (0-based per graph).
+ * (0-based per graph).  A list that leaves its graph's vertex range is
+ * rejected with SCD_EINVAL.
  */
 int scd_batch_upload(scd_ctx *ctx, uint32_t G, const uint64_t *vtx_off, const uint32_t *indices,
                      const uint32_t *adj, const int32_t *tri, const int32_t *tri_deg,
@@ -397,7 +427,10 @@ typedef struct scd_part {
  * cur_labels (G device pointers to local labels, n entries each) the arena
  * starts from those labels (best_labels likewise, NULL entries or a NULL
  * array: best = current), statistics recomputed, ready for scd_batch_step;
- * without, an initial partition must follow.
+ * without, an initial partition must follow.  The assembly checks each part
+ * as it copies it -- CSR offsets from 0 to 2m without decreasing, adjacency
+ * entries below the part's n, labels below n or INVALID -- and fails with
+ * SCD_EINVAL (no graph left in the context) when one is out of range.
  */
 int scd_batch_upload_parts(scd_ctx *ctx, uint32_t G, const scd_part *parts,
                            const uint32_t *const *cur_labels, const uint32_t *const *best_labels);

@@ -18,7 +18,18 @@ Outputs:
                      including rows whose int32 products wrap
 
 Run in the build container (it needs the reference build):
-  make -C oracle ref && python tests/golden/make_golden.py
+  make -C oracle ref && python tests/golden/make_golden.py [names...]
+Graphs of golden_graphs.HUGE (R-MAT scale 24) are written only when named.
+SCD_GOLDEN_CACHE=<dir>: a file <dir>/<name>.ref.bin (an earlier output of
+_ref/convert on the recipe's edge list) replaces the convert run, and
+<dir>/<name>.dump.bin (an earlier output of _ref/ref_dump on that file, one
+thread) the dump run; both are still the reference programs' outputs, and
+the first is still compared byte for byte with scd_convert's.
+SCD_GOLDEN_NO_REF_CONVERT=1: for a HUGE graph without a cached reference
+convert output, the graph file is scd_convert's (byte-equal to the reference
+convert on every other recipe) and the entry says so ("convert_source");
+everything else still comes from the reference's own code.  Temporary files
+go to TMPDIR.
 """
 import hashlib
 import json
@@ -36,7 +47,7 @@ sys.path.insert(0, os.path.join(ROOT, "het-scd_amd"))
 sys.path.insert(0, os.path.dirname(HERE))
 
 import scd_amd  # noqa: E402
-from golden_graphs import GRAPHS, make_edges  # noqa: E402
+from golden_graphs import GRAPHS, HUGE, make_edges  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref")
 FULL_MAX_N = 20000
@@ -50,7 +61,21 @@ def hexf(x):
     return struct.pack("<d", x).hex()
 
 
-def ref_convert(edges, tmp):
+CACHE = os.environ.get("SCD_GOLDEN_CACHE")
+
+
+def cached(name, suffix):
+    if not CACHE:
+        return None
+    path = os.path.join(CACHE, f"{name}.{suffix}")
+    return path if os.path.exists(path) else None
+
+
+def ref_convert(edges, tmp, name=None):
+    pre = cached(name, "ref.bin")
+    if pre:
+        with open(pre, "rb") as f:
+            return f.read(), pre
     txt = os.path.join(tmp, "g.txt")
     out = os.path.join(tmp, "g.bin")
     edges.write_text(txt)
@@ -59,13 +84,15 @@ def ref_convert(edges, tmp):
         return f.read(), out
 
 
-def ref_dump(path, tmp):
-    out = os.path.join(tmp, "dump.bin")
-    # one thread: the reference's OpenMP sums (clustering coefficient,
-    # src/graph.cpp:59; WCC, src/wcc.cpp:49) then run in vertex order, the
-    # canonical order whose bits the oracle and the engine reproduce
-    subprocess.run([os.path.join(REF, "ref_dump"), path, out], check=True, stdout=subprocess.DEVNULL,
-                   env=dict(os.environ, OMP_NUM_THREADS="1"))
+def ref_dump(path, tmp, name=None):
+    out = cached(name, "dump.bin")
+    if not out:
+        out = os.path.join(tmp, "dump.bin")
+        # one thread: the reference's OpenMP sums (clustering coefficient,
+        # src/graph.cpp:59; WCC, src/wcc.cpp:49) then run in vertex order, the
+        # canonical order whose bits the oracle and the engine reproduce
+        subprocess.run([os.path.join(REF, "ref_dump"), path, out], check=True, stdout=subprocess.DEVNULL,
+                       env=dict(os.environ, OMP_NUM_THREADS="1"))
     b = open(out, "rb").read()
     n, nc, iters, printed = struct.unpack_from("<4I", b, 0)
     cc, init, final = struct.unpack_from("<3d", b, 16)
@@ -131,17 +158,30 @@ def main(names=None):
     manifest = json.load(open(man_path)) if os.path.exists(man_path) else {"graphs": {}}
     with tempfile.TemporaryDirectory() as tmp:
         for name, recipe in GRAPHS.items():
-            if names and name not in names:
+            if (names and name not in names) or (not names and name in HUGE):
                 continue
             print(f"[golden] {name}", flush=True)
             edges = make_edges(recipe)
-            ref_bytes, path = ref_convert(edges, tmp)
-            ours = edges.convert()
-            assert ours.to_bytes() == ref_bytes, f"{name}: scd_convert differs from the reference convert"
-            d = ref_dump(path, tmp)
+            source = "reference convert (oracle/_ref/convert)"
+            if name in HUGE and not cached(name, "ref.bin") and os.environ.get("SCD_GOLDEN_NO_REF_CONVERT"):
+                ours = edges.convert()
+                del edges
+                path = os.path.join(tmp, "g.bin")
+                ours.write(path)
+                ref_bytes = ours.to_bytes()
+                source = ("scd_convert (host restatement, byte-equal to the reference convert on every smaller "
+                          "recipe; the reference convert was not run at this size)")
+            else:
+                ref_bytes, path = ref_convert(edges, tmp, name)
+                ours = edges.convert()
+                del edges
+                assert ours.to_bytes() == ref_bytes, f"{name}: scd_convert differs from the reference convert"
+            d = ref_dump(path, tmp, name)
+            assert d["n"] == ours.n, f"{name}: the dump is not of this graph"
             ent = {
                 "recipe": recipe,
                 "convert_sha256": hashlib.sha256(ref_bytes).hexdigest(),
+                "convert_source": source,
                 "n": d["n"], "m": ours.m, "num_comms": d["num_comms"], "iterations": d["iterations"],
                 "printed_num_communities": d["printed"],
                 "cc_hex": hexf(d["cc"]), "initial_wcc_hex": hexf(d["initial_wcc"]),

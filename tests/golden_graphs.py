@@ -4,7 +4,8 @@ the same edge sequence is produced here, in the fixtures and on any host.
 Sizes:  tiny (bowtie, K_n, n <= 64 random), small (planted partitions of a
 few hundred vertices, R-MAT scale 10-14), C1 (planted partition 100 x 100,
 p_in 0.1, p_out 0.0005), R-MAT scale 16/18 and C2 (R-MAT scale 20, edge
-factor 16, a=0.57 b=0.19 c=0.19).
+factor 16, a=0.57 b=0.19 c=0.19); C3 (R-MAT scale 24, same parameters), whose
+fixture keeps hashes only and whose GPU tests convert it on the device.
 """
 import os
 import sys
@@ -34,12 +35,14 @@ GRAPHS = {
     "rmat16": {"kind": "rmat", "scale": 16, "ef": 16, "seed": 1},
     "rmat18": {"kind": "rmat", "scale": 18, "ef": 16, "seed": 1},
     "c2": {"kind": "rmat", "scale": 20, "ef": 16, "seed": 1},
+    "c3": {"kind": "rmat", "scale": 24, "ef": 16, "seed": 1},
 }
 
 SMALL = ["bowtie", "k5", "k9", "er24_s1", "er40_s2", "er64_s3", "pp4x16_s4", "pp8x40_s5", "pprand_s6",
          "rmat10", "rmat12"]
 MEDIUM = ["pprand_s7", "c1", "rmat14", "rmat16"]
 LARGE = ["rmat18", "c2"]
+HUGE = ["c3"]  # written only when named (make_golden.py c3)
 
 
 def make_edges(recipe):

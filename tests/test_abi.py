@@ -46,9 +46,10 @@ def test_init_without_device_fails_cleanly():
     assert ei.value.code == -4  # SCD_EDEVICE, like cuda_init returning false
 
 
-def test_init_rejects_bad_group_size():
+@pytest.mark.parametrize("size", [0, -64, 1025])
+def test_init_rejects_bad_group_size(size):
     with pytest.raises(scd_amd.ScdError) as ei:
-        scd_amd.Engine(device=0, work_group_size=100)
+        scd_amd.Engine(device=0, work_group_size=size)
     assert ei.value.code == -1
 
 
