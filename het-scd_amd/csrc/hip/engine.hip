@@ -211,6 +211,7 @@ struct scd_ctx {
     size_t lane_ev_used[2] = {0, 0};
     hipStream_t ls = nullptr;  // the stream LAUNCH uses
     bool use_lanes = true;
+    bool one_stream = false;  // set for the length of a call that must not create events
 
     // phase timing of each iteration (cuda_improve_labels' four lines,
     // src/cuda.cu:347-360): device wall-clock stamps written by one-thread
@@ -391,7 +392,7 @@ static int prof_collect(scd_ctx *c) {
 // Lanes are off while profiling (per-kernel events are recorded on the
 // context's stream).
 static inline bool lanes_on(const scd_ctx *c) {
-    return c->use_lanes && c->tune.lanes && c->E >= (1ull << 20) && !c->profile && c->lane[0];
+    return c->use_lanes && c->tune.lanes && c->E >= (1ull << 20) && !c->profile && c->lane[0] && !c->one_stream;
 }
 
 // The next unused fork / join event of the current pool (nullptr if the
@@ -938,6 +939,37 @@ static int finish(scd_ctx *c) {
 }
 
 // ---------------------------------------------------------------------------
+// Static scratch for scd_count_triangles_noalloc: one array per GPU, held
+// until the process ends.  Host code never uses the array's name as an
+// address: scd_init looks up each device's copy once, with that device
+// current, and the entry point works with that pointer.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = 64ull << 20;
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kScratchDevices];
+static hipError_t g_scratch_err[kScratchDevices];
+
+// Device `dev` is current.
+static void scratch_setup(int dev) {
+    if (dev < 0 || dev >= kScratchDevices) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_err[dev] = e;
+    if (e == hipSuccess) g_scratch_ptr[dev] = p;
+    else (void)hipGetLastError();
+}
+
+static uint8_t *scratch_of(int dev) {
+    if (dev < 0 || dev >= kScratchDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return (uint8_t *)g_scratch_ptr[dev];
+}
+
+// ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
 extern "C" int scd_device_count(void) {
@@ -959,6 +991,7 @@ extern "C" int scd_init(int device_id, int work_group_size, int profile, scd_ctx
         return fail(SCD_EDEVICE, "scd_init: device %d is not one of the %d visible HIP devices",
                     device_id, count);
     HIP_TRY(hipSetDevice(device_id));
+    scratch_setup(device_id);
     scd_ctx *c = new scd_ctx();
     c->device = device_id;
     c->block_size = work_group_size;
@@ -1801,6 +1834,62 @@ extern "C" int scd_count_triangles(scd_ctx *c, int32_t *tri_out, int32_t *tri_de
     if (e != hipSuccess) rc = fail(SCD_EDEVICE, "scd_count_triangles: copy: %s", hipGetErrorString(e));
     if (!rc) rc = finish(c);
     release_tmp(c, tmp);
+    return rc;
+}
+
+// scd_count_triangles with device outputs and no allocation: the
+// out-degrees, triangle counts and edge flags of the pass live in the static
+// scratch array (every call rewrites what it reads: k_wtc_dplus zeroes the
+// counts, tc_clear the flags); the per-bin kernels run on the context's
+// stream only (lane fork / join events are not created here).
+extern "C" int scd_count_triangles_noalloc(scd_ctx *c, int32_t *tri_out, int32_t *tri_deg_out) {
+    int rc = need_graph(c, "scd_count_triangles_noalloc");
+    if (rc) return rc;
+    if (c->profile)
+        return fail(SCD_ESTATE, "scd_count_triangles_noalloc: profiling creates an event per launch; turn it off");
+    uint8_t *base = scratch_of(c->device);
+    if (!base) {
+        const hipError_t e = c->device >= 0 && c->device < kScratchDevices ? g_scratch_err[c->device] : hipSuccess;
+        return fail(SCD_EDEVICE, "scd_count_triangles_noalloc: no scratch address for device %d (%s)", c->device,
+                    hipGetErrorString(e));
+    }
+    const uint32_t n = c->n;
+    const uint64_t E = c->E;
+    auto al = [](uint64_t b) { return (b + 255) & ~255ull; };
+    const uint64_t need = 2 * al(4ull * n) + (tri_deg_out ? 0 : al(4ull * n)) + al(E);
+    if (need > kScratchBytes)
+        return fail(SCD_ELIMIT,
+                    "scd_count_triangles_noalloc: n = %u, 2m = %llu need %llu bytes of scratch; the static array "
+                    "holds %llu (use scd_count_triangles)",
+                    n, (unsigned long long)E, (unsigned long long)need, (unsigned long long)kScratchBytes);
+    uint64_t o = 0;
+    uint32_t *dplus = (uint32_t *)(base + o);
+    o += al(4ull * n);
+    uint32_t *T = (uint32_t *)(base + o);
+    o += al(4ull * n);
+    uint32_t *td = tri_deg_out ? (uint32_t *)tri_deg_out : (uint32_t *)(base + o);
+    if (!tri_deg_out) o += al(4ull * n);
+    uint8_t *flag = base + o;
+    WccArgs w;
+    memset(&w, 0, sizeof(w));
+    w.indices = c->indices;
+    w.int_adj = c->adj;
+    w.int_deg = nullptr;
+    w.dplus = dplus;
+    w.T = T;
+    w.td = td;
+    w.flag = flag;
+    w.T_hot = c->hot_T;
+    w.td_hot = c->hot_td;
+    w.hot_n = c->hot_n;
+    w.vc_min = c->tc_heavy;
+    c->one_stream = true;
+    LAUNCH(c, "tc_clear", cdiv(E, 256), 256, k_fill_u8, flag, E);
+    rc = count_internal_triangles(c, w);
+    if (!rc && tri_out) LAUNCH(c, "tc_double", cdiv(n, 256), 256, k_wtc_double, T, n, tri_out);
+    c->one_stream = false;
+    if (!rc) rc = check_launch("scd_count_triangles_noalloc");
+    if (!rc) rc = finish(c);
     return rc;
 }
 

@@ -136,6 +136,7 @@ def hip():
         "scd_upload": (ctypes.c_int, [C, ctypes.c_uint32, ctypes.c_uint32, u32p, u32p, i32p, i32p,
                                       ctypes.c_double]),
         "scd_count_triangles": (ctypes.c_int, [C, i32p, i32p, ctypes.c_int]),
+        "scd_count_triangles_noalloc": (ctypes.c_int, [C, ctypes.c_void_p, ctypes.c_void_p]),
         "scd_set_triangle_split": (ctypes.c_int, [C, ctypes.c_uint32, ctypes.c_uint32]),
         "scd_convert_edges": (ctypes.c_int, [C, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                              u32p, u32p, f64p]),

@@ -87,6 +87,21 @@ int scd_upload(scd_ctx *ctx, uint32_t n, uint32_t m, const uint32_t *indices,
 int scd_count_triangles(scd_ctx *ctx, int32_t *tri_out, int32_t *tri_deg_out, int replace);
 
 /*
+ * scd_count_triangles without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  tri_out and
+ * tri_deg_out (n entries each, either may be NULL) are DEVICE pointers of
+ * the context's device; the pass's temporaries (out-degrees, triangle
+ * counts, one flag per adjacency entry) live in a static device array of
+ * 64 MiB that each GPU the library runs on holds until the process ends.
+ * Graphs needing more than 8n + 2m (+ 4n without tri_deg_out) bytes of it
+ * are rejected with SCD_ELIMIT; profiling must be off (SCD_ESTATE).  Every
+ * call on one GPU shares that GPU's array, so calls that overlap on one GPU
+ * -- from any context -- race.  Results equal scd_count_triangles'.
+ */
+int scd_count_triangles_noalloc(scd_ctx *ctx, int32_t *tri_out, int32_t *tri_deg_out);
+
+/*
  * Work split of the triangle kernels (no counterpart in the reference; the
  * results never depend on it).  Edges are oriented toward the lower id and
  * a triangle w < v < u is found on the edge u -> v, by default by walking
