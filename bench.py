@@ -46,9 +46,9 @@ METRIC = "refine edges/s + WCC (bit-exact) at 1 GPU; batched graphs/s & p99 vs p
 HBM_PEAK_GBPS = 8000.0
 BIN_OF_KERNEL = {"move_sub8": "sub8", "move_sub16": "sub16", "move_sub32": "sub32", "move_wave1": "wave1",
                  "move_wave2": "wave2", "move_wave4": "wave4", "move_wave8": "wave8", "move_wave16": "wave16",
-                 "move_lds": "lds", "filter_sub8": "sub8", "filter_sub16": "sub16", "filter_sub32": "sub32",
+                 "move_lds": "lds", "move_lds2": "lds2", "filter_sub8": "sub8", "filter_sub16": "sub16", "filter_sub32": "sub32",
                  "filter_wave1": "wave1", "filter_wave2": "wave2", "filter_wave4": "wave4", "filter_wave8": "wave8",
-                 "filter_wave16": "wave16", "filter_block": "lds"}
+                 "filter_wave16": "wave16", "filter_block": "lds", "filter_block2": "lds2"}
 GOLDEN_MANIFEST = os.path.join(ROOT, "tests", "golden", "manifest.json")
 
 
@@ -72,11 +72,11 @@ def stats_bytes(v, e):
 
 def kernel_algo_bytes(name, bins, nwave_bins):
     if name in ("move_sub8", "move_sub16", "move_sub32", "move_wave1", "move_wave2", "move_wave4", "move_wave8",
-                "move_wave16", "move_lds"):
+                "move_wave16", "move_lds", "move_lds2"):
         v, e, p = bins[BIN_OF_KERNEL[name]]
         return move_bytes(v, e, p)
     if name in ("filter_sub8", "filter_sub16", "filter_sub32", "filter_wave1", "filter_wave2", "filter_wave4",
-                "filter_wave8", "filter_wave16", "filter_block"):
+                "filter_wave8", "filter_wave16", "filter_block", "filter_block2"):
         v, e, _ = bins[BIN_OF_KERNEL[name]]
         return stats_bytes(v, e)
     if name == "filter_wave":

@@ -13,22 +13,31 @@
 
 namespace scd_dev {
 
-enum BinKind : uint32_t { K_SUB8, K_SUB16, K_SUB32, K_W1, K_W2, K_W4, K_W8, K_W16, K_LDS, K_HUB, K_NB };
+enum BinKind : uint32_t { K_SUB8, K_SUB16, K_SUB32, K_W1, K_W2, K_W4, K_W8, K_W16, K_LDS, K_LDS2, K_HUB, K_NB };
+
+// Largest degree of the K_LDS bin: with lds_tile >= 8192 the workgroup-per-
+// vertex range is split at lds_tile / 2, so that the vertices up to there
+// run the kernel shape sized for lds_tile / 2 keys (99 -> 66 registers at
+// 8192: 4 -> 7 waves per SIMD) and only the longer lists the one sized for
+// lds_tile.  Below 8192 both shapes fit 7 waves and one launch is better.
+__host__ __device__ inline uint32_t lds_split(uint32_t lds_tile) { return lds_tile >= 8192 ? lds_tile / 2 : lds_tile; }
 
 // The bin of a vertex of degree d: sub-wave groups up to 32 neighbours,
-// 1-16 waves up to wave_max, an LDS tile up to lds_tile, hub beyond.
+// 1-16 waves up to wave_max, an LDS tile up to lds_tile (two bins, split at
+// lds_split), hub beyond.
 __host__ __device__ inline uint32_t bin_of(uint32_t d, uint32_t lds_tile, uint32_t wave_max) {
     if (d <= 8) return K_SUB8;
     if (d <= 16) return K_SUB16;
     if (d <= 32) return K_SUB32;
-    if (d > wave_max) return d <= lds_tile ? K_LDS : K_HUB;
+    if (d > wave_max || d > 1024) {
+        if (d <= lds_split(lds_tile)) return K_LDS;
+        return d <= lds_tile ? K_LDS2 : K_HUB;
+    }
     if (d <= 64) return K_W1;
     if (d <= 128) return K_W2;
     if (d <= 256) return K_W4;
     if (d <= 512) return K_W8;
-    if (d <= 1024) return K_W16;
-    if (d <= lds_tile) return K_LDS;
-    return K_HUB;
+    return K_W16;
 }
 
 constexpr uint32_t BIN_CHUNK = 4096;  // vertices per workgroup (256 threads x 16)

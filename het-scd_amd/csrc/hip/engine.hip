@@ -661,16 +661,21 @@ static inline const uint32_t *bin_v(const scd_ctx *c, int b) { return c->bin_ver
 // ---------------------------------------------------------------------------
 // Phases
 // ---------------------------------------------------------------------------
+// The workgroup-per-vertex bins: K_LDS (degrees up to lds_split(T)) with
+// the kernel shape for lds_split(T) keys, K_LDS2 with the one for T.
 template <int T>
 static void launch_lds_and_tiles(scd_ctx *c, const MoveArgs &a, const HubArgs &h, const HubTable &t) {
-    if constexpr (T <= 4096) {
-        if (!a.ex_sorted && c->tune.lds_hash) {
-            LAUNCH(c, "move_lds", bin_n(c, K_LDS), 256, (k_move_lds_hash<T>), a, bin_v(c, K_LDS), bin_n(c, K_LDS));
-        } else {
-            LAUNCH(c, "move_lds", bin_n(c, K_LDS), 256, (k_move_lds<T>), a, bin_v(c, K_LDS), bin_n(c, K_LDS));
-        }
+    constexpr int TH = T >= 8192 ? T / 2 : T;  // lds_split(T)
+    MoveArgs a2 = a;
+    a2.counters = c->counting ? c->counters + K_LDS2 : nullptr;
+    if (!a.ex_sorted && c->tune.lds_hash && T <= 4096) {
+        LAUNCH(c, "move_lds2", bin_n(c, K_LDS2), 256, (k_move_lds_hash<T <= 4096 ? T : 4096>), a2, bin_v(c, K_LDS2),
+               bin_n(c, K_LDS2));
+        LAUNCH(c, "move_lds", bin_n(c, K_LDS), 256, (k_move_lds_hash<TH <= 4096 ? TH : 4096>), a, bin_v(c, K_LDS),
+               bin_n(c, K_LDS));
     } else {
-        LAUNCH(c, "move_lds", bin_n(c, K_LDS), 256, (k_move_lds<T>), a, bin_v(c, K_LDS), bin_n(c, K_LDS));
+        LAUNCH(c, "move_lds2", bin_n(c, K_LDS2), 256, (k_move_lds<T>), a2, bin_v(c, K_LDS2), bin_n(c, K_LDS2));
+        LAUNCH(c, "move_lds", bin_n(c, K_LDS), 256, (k_move_lds<TH>), a, bin_v(c, K_LDS), bin_n(c, K_LDS));
     }
     on_lane(c, 1);  // the hub chain: tiles, then merges / table, evaluation, finalize
     if (a.ex_sorted || !c->tune.hub_table)
@@ -789,11 +794,23 @@ static int phase_stats(scd_ctx *c, bool stamps = false) {
     LAUNCH(c, "filter_wave8", cdiv(bin_n(c, K_W8), 4), 256, k_filter_wave<8>, s, bin_v(c, K_W8), bin_n(c, K_W8));
     LAUNCH(c, "filter_wave16", cdiv(bin_n(c, K_W16), 4), 256, k_filter_wave<16>, s, bin_v(c, K_W16), bin_n(c, K_W16));
     on_lane(c, 0);
+    // K_LDS2 (degrees above lds_split) with the shape for lds_tile keys,
+    // K_LDS with the one for lds_split
     switch (c->tune.lds_tile) {
-        case 2048: LAUNCH(c, "filter_block", bin_n(c, K_LDS), 256, k_filter_block<2048>, s, bin_v(c, K_LDS), bin_n(c, K_LDS)); break;
-        case 4096: LAUNCH(c, "filter_block", bin_n(c, K_LDS), 256, k_filter_block<4096>, s, bin_v(c, K_LDS), bin_n(c, K_LDS)); break;
-        case 8192: LAUNCH(c, "filter_block", bin_n(c, K_LDS), 256, k_filter_block<8192>, s, bin_v(c, K_LDS), bin_n(c, K_LDS)); break;
-        default: LAUNCH(c, "filter_block", bin_n(c, K_LDS), 256, k_filter_block<16384>, s, bin_v(c, K_LDS), bin_n(c, K_LDS)); break;
+        case 2048:
+            LAUNCH(c, "filter_block", bin_n(c, K_LDS), 256, k_filter_block<2048>, s, bin_v(c, K_LDS), bin_n(c, K_LDS));
+            break;
+        case 4096:
+            LAUNCH(c, "filter_block", bin_n(c, K_LDS), 256, k_filter_block<4096>, s, bin_v(c, K_LDS), bin_n(c, K_LDS));
+            break;
+        case 8192:
+            LAUNCH(c, "filter_block2", bin_n(c, K_LDS2), 256, k_filter_block<8192>, s, bin_v(c, K_LDS2), bin_n(c, K_LDS2));
+            LAUNCH(c, "filter_block", bin_n(c, K_LDS), 256, k_filter_block<4096>, s, bin_v(c, K_LDS), bin_n(c, K_LDS));
+            break;
+        default:
+            LAUNCH(c, "filter_block2", bin_n(c, K_LDS2), 256, k_filter_block<16384>, s, bin_v(c, K_LDS2), bin_n(c, K_LDS2));
+            LAUNCH(c, "filter_block", bin_n(c, K_LDS), 256, k_filter_block<8192>, s, bin_v(c, K_LDS), bin_n(c, K_LDS));
+            break;
     }
     if (c->nh) {
         on_lane(c, 1);
@@ -1576,7 +1593,7 @@ extern "C" int scd_last_counters(scd_ctx *c, uint64_t out[4]) {
 extern "C" int scd_bin_info(scd_ctx *c, int max_bins, char (*names)[16], uint64_t *vertices,
                             uint64_t *entries, uint64_t *pairs) {
     static const char *bin_names[K_NB] = {"sub8", "sub16", "sub32", "wave1", "wave2",
-                                          "wave4", "wave8", "wave16", "lds", "hub"};
+                                          "wave4", "wave8", "wave16", "lds", "lds2", "hub"};
     if (!c) return fail(SCD_EINVAL, "scd_bin_info: null context");
     if (!c->have_graph) return fail(SCD_ESTATE, "scd_bin_info: no graph uploaded");
     const int k = std::min<int>(max_bins, K_NB);

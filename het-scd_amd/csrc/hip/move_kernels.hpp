@@ -188,7 +188,6 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const uint32_t *_
     bool has_rem = false;
     uint32_t heads = 0;
     uint32_t freq[R];
-    int4 st[R];
     {
         uint32_t next = next_lane_head;  // next head after register r, scanning down
 #pragma unroll
@@ -197,27 +196,35 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const uint32_t *_
             if (bnd[r]) next = i0 + r;
         }
     }
-    // all statistics loads before the first evaluation
+    // the statistics loads of SB candidates before their evaluations (all R
+    // at once for R <= 8; two batches at R = 16 keep the kernel at 4 waves
+    // per SIMD instead of 3)
+    constexpr int SB = R < 4 ? R : 4;
 #pragma unroll
-    for (int r = 0; r < R; r++) st[r] = freq[r] ? a.stats[x[r]] : make_int4(0, 0, 0, 0);
+    for (int r0 = 0; r0 < R; r0 += SB) {
+        int4 st[SB];
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-        const uint32_t i = i0 + r;
-        if (freq[r]) {
-            const double imp = candidate_improvement_s(st[r], x[r], freq[r], deg, my, pext);
-            if (x[r] == my) {
-                rem_lane = -imp;
-                has_rem = true;
+        for (int q = 0; q < SB; q++) st[q] = freq[r0 + q] ? a.stats[x[r0 + q]] : make_int4(0, 0, 0, 0);
+#pragma unroll
+        for (int q = 0; q < SB; q++) {
+            const int r = r0 + q;
+            const uint32_t i = i0 + r;
+            if (freq[r]) {
+                const double imp = candidate_improvement_s(st[q], x[r], freq[r], deg, my, pext);
+                if (x[r] == my) {
+                    rem_lane = -imp;
+                    has_rem = true;
+                }
+                if (imp > 0.0 && better(imp, x[r], bi, bl)) {
+                    bi = imp;
+                    bl = x[r];
+                }
+                heads++;
             }
-            if (imp > 0.0 && better(imp, x[r], bi, bl)) {
-                bi = imp;
-                bl = x[r];
+            if (a.ex_sorted && i < deg) {
+                a.ex_sorted[beg + i] = x[r];
+                a.ex_count[beg + i] = freq[r];
             }
-            heads++;
-        }
-        if (a.ex_sorted && i < deg) {
-            a.ex_sorted[beg + i] = x[r];
-            a.ex_count[beg + i] = freq[r];
         }
     }
     group_argmax<64>(bi, bl);
