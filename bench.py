@@ -610,10 +610,11 @@ def run_served(args, scd_amd, np):
         f"{np.percentile(lat, 50):.1f} ms p99 {np.percentile(lat, 99):.1f} ms, {st['joins']} joins, "
         f"busy {st['busy_ms'] / (span_s * 1e3):.2f}")
 
-    def serve_subset(sub):
-        with scd_amd.Server(device=0, join_wait_ms=0.0, graph_mode=False, intake_graphs=args.intake_graphs,
+    def serve_subset(sub, arrivals=None, join_wait_ms=0.0):
+        with scd_amd.Server(device=0, join_wait_ms=join_wait_ms, graph_mode=False, intake_graphs=args.intake_graphs,
                             intake_bytes=args.intake_mb << 20) as srv:
-            ids = [srv.submit(g, arrival_ms=0.0) for g in sub]
+            ids = [srv.submit(g, arrival_ms=0.0 if arrivals is None else float(t))
+                   for g, t in zip(sub, arrivals if arrivals is not None else [0.0] * len(sub))]
             t0 = time.perf_counter()
             srv.run()
             wall = time.perf_counter() - t0
@@ -661,6 +662,21 @@ def run_served(args, scd_amd, np):
     small_plain = len(small) / (plain_ms[small].sum() / 1e3)
     log(f"[bench] served small graphs only ({len(small)} planted, all at t=0): {small_cap:.1f} graphs/s vs plain "
         f"{small_plain:.1f} = {small_cap / small_plain:.2f}x; result mismatches {small_mismatch}")
+    # the same small graphs as a Poisson stream at 80% of that capacity
+    lam_s = 0.8 * small_cap
+    arr_s = np.cumsum(np.random.default_rng(11).exponential(1e3 / lam_s, len(small)))
+    res_p, st_p, _ = serve_subset([graphs[i] for i in small], arr_s, args.join_wait_ms)
+    done_p = np.array([r["done_ms"] for r in res_p])
+    lat_p = done_p - arr_s
+    span_p = (done_p.max() - arr_s[0]) / 1e3
+    for r, i in zip(res_p, small):
+        iters, best, w0, lab = plain_res[i]
+        if (r["status"] != 0 or r["iterations"] != iters or r["best_wcc"] != best or r["initial_wcc"] != w0
+                or not np.array_equal(r["labels"], lab)):
+            small_mismatch += 1
+    log(f"[bench] served small graphs at lambda={lam_s:.1f}/s: {len(small) / span_p:.1f} graphs/s, p50 "
+        f"{np.percentile(lat_p, 50):.1f} ms p99 {np.percentile(lat_p, 99):.1f} ms, busy "
+        f"{st_p['busy_ms'] / (span_p * 1e3):.2f}")
     pct = lambda x, q: round(float(np.percentile(x, q)), 3)
     return {"workload": f"{N} graphs, Poisson arrivals at 80% of measured capacity (seed 7): 90% planted partition "
                         "(n log-uniform [1000, 20000], 4-64 communities Dirichlet(1), p_in 0.15, p_out 0.002), "
@@ -682,6 +698,11 @@ def run_served(args, scd_amd, np):
             "small_graphs_only": {"graphs": len(small), "capacity_graphs_per_s": round(small_cap, 1),
                                   "plain_graphs_per_s": round(small_plain, 1),
                                   "speedup": round(small_cap / small_plain, 2), "joins": st_s["joins"],
+                                  "poisson": {"lambda_per_s": round(lam_s, 1),
+                                              "graphs_per_s": round(len(small) / span_p, 1),
+                                              "p50_ms": pct(lat_p, 50), "p99_ms": pct(lat_p, 99),
+                                              "joins": st_p["joins"],
+                                              "busy_fraction": round(st_p["busy_ms"] / (span_p * 1e3), 3)},
                                   "results_identical": small_mismatch == 0}}
 
 

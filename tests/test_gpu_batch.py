@@ -18,8 +18,18 @@ def small_batch():
     return scd_amd.planted_batch(48, seed_base=7, n_min=64, n_max=3000, k_min=4, k_max=64, p_in=0.15, p_out=0.002)
 
 
-def test_batch_equals_single_graph_runs(engine, small_batch):
-    a = small_batch
+@pytest.fixture(scope="module")
+def wide_batch():
+    # the batched workload's whole size range: n log-uniform in [64, 20000]
+    return scd_amd.planted_batch(32, seed_base=4001, n_min=64, n_max=20000, k_min=4, k_max=64, p_in=0.15,
+                                 p_out=0.002)
+
+
+@pytest.mark.parametrize("which", ["small", "wide"])
+def test_batch_equals_single_graph_runs(engine, small_batch, wide_batch, which):
+    a = small_batch if which == "small" else wide_batch
+    if which == "wide":
+        assert a.count == 32 and int(np.diff(a.vtx_off).max()) > 5000
     print(f"[gpu] batch of {a.count}: n={a.graph.n} m={a.graph.m}", flush=True)
     with scd_amd.BatchEngine(device=0) as be:
         be.upload(a)

@@ -10,7 +10,9 @@ from golden_graphs import graph
 
 pytestmark = pytest.mark.gpu
 
-MIX_NAMES = ["pp4x16_s4", "c1", "rmat10", "rmat12", "pprand_s6", "er24_s1", "k9"]
+# one R-MAT scale 18 graph (n 104,290, m 3.4 M) among small planted and
+# R-MAT graphs, as in the served workload's 90 / 10 mix
+MIX_NAMES = ["pp4x16_s4", "c1", "rmat10", "rmat12", "pprand_s6", "er24_s1", "k9", "rmat18"]
 
 
 @pytest.fixture(scope="module")
