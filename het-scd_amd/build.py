@@ -4,6 +4,7 @@ Outputs (git-ignored, shipped with the working tree):
   het-scd_amd/lib/libscd_host.so   binary format, generators, convert (g++)
   het-scd_amd/lib/libscd_hip.so    the gfx950 HIP engine behind include/scd_hip.h
   het-scd_amd/lib/libscd_dropin.so  the reference's cuda_* names over the C ABI
+  het-scd_amd/lib/libscd_refapi.so  the reference's scd.hpp / wcc.hpp / Graph::read names
   het-scd_amd/lib/libscd_serve.so  the serving loop behind include/scd_serve.h
   het-scd_amd/bin/main-gpu         `main-gpu <graph> <device> [group size]`
   het-scd_amd/bin/convert-gpu      `convert-gpu <input file> <output file> [device]`
@@ -72,6 +73,8 @@ def build_dropin_and_cli():
               "-Wl,-rpath,/opt/rocm/lib"]
     run([CXX, *HOST_FLAGS, "-shared", "-o", dropin, src("dropin", "dropin.cpp"),
          "-L", LIB, "-lscd_hip", "-lscd_host", "-Wl,-rpath,$ORIGIN", *hip_rt])
+    run([CXX, *HOST_FLAGS, "-shared", "-o", os.path.join(LIB, "libscd_refapi.so"), src("dropin", "refapi.cpp"),
+         "-L", LIB, "-lscd_hip", "-lscd_host", "-Wl,-rpath,$ORIGIN"])
     run([CXX, *HOST_FLAGS, "-shared", "-o", os.path.join(LIB, "libscd_serve.so"), src("serve", "server.cpp"),
          "-L", LIB, "-lscd_hip", "-Wl,-rpath,$ORIGIN", *hip_rt])
     run([CXX, *HOST_FLAGS, "-o", os.path.join(BIN, "main-gpu"), src("cli", "main_gpu.cpp"),

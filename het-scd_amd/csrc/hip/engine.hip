@@ -1079,6 +1079,8 @@ extern "C" void scd_destroy(scd_ctx *c) {
 
 static int alloc_graph(scd_ctx *c, uint32_t n, uint32_t m, double clustering_coef);
 static int finish_upload(scd_ctx *c);
+template <typename T>
+static int upload_tmp(scd_ctx *c, const T *h, uint32_t count, T **d, std::vector<DevBuf> &tmp);
 
 // Input checks on the device before any kernel indexes by adjacency entries
 // or labels: check_begin clears the check word, the check kernels follow on
@@ -1217,6 +1219,105 @@ extern "C" int scd_labels_to_device(scd_ctx *c, const uint32_t *labels) {
     if ((rc = clear_counters(c)) || (rc = phase_stats(c)) || (rc = finish(c))) return rc;
     c->have_labels = true;
     c->wcc_valid = false;
+    return SCD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// A caller's partition (the CPU call surface, src/scd.hpp:25-31): labels
+// with INVALID allowed, the caller's statistics, one move step alone, the
+// WCC of given labels and sizes.
+// ---------------------------------------------------------------------------
+__global__ void k_stats_pack(const int32_t *__restrict__ sizes, const int32_t *__restrict__ ext,
+                             const int32_t *__restrict__ in, uint32_t nc, int4 *__restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nc) out[i] = make_int4(sizes[i], ext[i], in[i], 0);
+}
+
+__global__ void k_stats_sizes(const int32_t *__restrict__ sizes, uint32_t nc, int4 *__restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nc) out[i].x = sizes[i];
+}
+
+// labels -> current and best labels after the checks of a caller's partition.
+static int put_labels(scd_ctx *c, const char *fn, const uint32_t *labels, uint32_t num_comms) {
+    if (!labels) return fail(SCD_EINVAL, "%s: null labels", fn);
+    if (num_comms > c->n) return fail(SCD_EINVAL, "%s: %u communities for n = %u vertices", fn, num_comms, c->n);
+    for (uint32_t v = 0; v < c->n; v++)
+        if (labels[v] != SCD_INVALID_LABEL && labels[v] >= num_comms)
+            return fail(SCD_EINVAL, "%s: vertex %u has label %u, not below %u communities", fn, v, labels[v],
+                        num_comms);
+    if (c->num_comms != num_comms) drop_graphs(c);
+    c->num_comms = num_comms;
+    HIP_TRY(hipMemcpyAsync(c->cur, labels, sizeof(uint32_t) * c->n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->best, labels, sizeof(uint32_t) * c->n, hipMemcpyHostToDevice, c->stream));
+    c->have_labels = true;
+    c->wcc_valid = false;
+    return SCD_OK;
+}
+
+// Uploads count entries of each of the given arrays to device temporaries.
+static int put_stats(scd_ctx *c, uint32_t nc, const int32_t *sizes, const int32_t *ext, const int32_t *in,
+                     std::vector<DevBuf> &tmp) {
+    int32_t *ds = nullptr, *de = nullptr, *di = nullptr;
+    int rc;
+    if ((rc = upload_tmp(c, sizes, std::max<uint32_t>(nc, 1), &ds, tmp))) return rc;
+    if (ext && in) {
+        if ((rc = upload_tmp(c, ext, std::max<uint32_t>(nc, 1), &de, tmp)) ||
+            (rc = upload_tmp(c, in, std::max<uint32_t>(nc, 1), &di, tmp)))
+            return rc;
+        LAUNCH(c, "stats_pack", cdiv(nc, 256), 256, k_stats_pack, ds, de, di, nc, c->stats);
+    } else {
+        LAUNCH(c, "stats_sizes", cdiv(nc, 256), 256, k_stats_sizes, ds, nc, c->stats);
+    }
+    return check_launch("statistics upload");
+}
+
+extern "C" int scd_set_partition(scd_ctx *c, const uint32_t *labels, uint32_t num_comms, const int32_t *sizes,
+                                 const int32_t *ext_edges, const int32_t *int_edges) {
+    int rc = need_graph(c, "scd_set_partition");
+    if (rc) return rc;
+    const int given = (sizes != nullptr) + (ext_edges != nullptr) + (int_edges != nullptr);
+    if (given != 0 && given != 3)
+        return fail(SCD_EINVAL, "scd_set_partition: give all three statistics arrays or none");
+    if ((rc = put_labels(c, "scd_set_partition", labels, num_comms))) return rc;
+    std::vector<DevBuf> tmp;
+    if (given) rc = put_stats(c, num_comms, sizes, ext_edges, int_edges, tmp);
+    else if (!(rc = clear_counters(c))) rc = phase_stats(c);
+    if (!rc) rc = finish(c);
+    release_tmp(c, tmp);
+    return rc;
+}
+
+extern "C" int scd_move_step(scd_ctx *c, uint32_t *labels_out) {
+    int rc = need_labels(c, "scd_move_step");
+    if (rc) return rc;
+    if ((c->counting && (rc = clear_counters(c))) || (rc = phase_move(c, c->next, nullptr, nullptr))) return rc;
+    std::swap(c->cur, c->next);
+    c->wcc_valid = false;
+    if (labels_out)
+        HIP_TRY(hipMemcpyAsync(labels_out, c->cur, sizeof(uint32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+    return finish(c);
+}
+
+extern "C" int scd_compute_wcc_of(scd_ctx *c, const uint32_t *labels, uint32_t num_comms, const int32_t *sizes,
+                                  double *wcc) {
+    int rc = need_graph(c, "scd_compute_wcc_of");
+    if (rc) return rc;
+    if ((rc = put_labels(c, "scd_compute_wcc_of", labels, num_comms))) return rc;
+    std::vector<DevBuf> tmp;
+    // the internal adjacency and statistics of the labels, then the sizes
+    // the caller gives in place of the computed ones
+    if (!(rc = clear_counters(c)) && !(rc = phase_stats(c)) && sizes)
+        rc = put_stats(c, num_comms, sizes, nullptr, nullptr, tmp);
+    if (!rc) rc = phase_wcc(c, nullptr, nullptr);
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(c->wcc_host, c->wcc_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (e != hipSuccess) rc = fail(SCD_EDEVICE, "scd_compute_wcc_of: %s", hipGetErrorString(e));
+    }
+    if (!rc) rc = finish(c);
+    release_tmp(c, tmp);
+    if (rc) return rc;
+    if (wcc) *wcc = c->wcc_host[0];
     return SCD_OK;
 }
 

@@ -2,8 +2,9 @@
  * scd_graph_layout.hpp — a declaration of the reference's Graph class with
  * the same public data members in the same order (src/graph.hpp:23-33), so
  * that code built without the reference's graph.hpp (the drop-in library and
- * this project's main-gpu) shares its object layout.  Only the members and
- * the inline CSR accessors are declared; construction is left to the caller.
+ * this project's main-gpu) shares its object layout.  Only the members, the
+ * inline CSR accessors and Graph::read are declared; construction is left to
+ * the caller.
  */
 #ifndef SCD_GRAPH_LAYOUT_HPP
 #define SCD_GRAPH_LAYOUT_HPP
@@ -24,6 +25,9 @@ class Graph {
           adj(adjacency) {}
 
     unsigned int degree(unsigned int i) const { return indices[i + 1] - indices[i]; }
+
+    /* Graph::read (src/graph.hpp:41); defined by libscd_refapi.so. */
+    static const Graph *read(const char *filename);
 };
 
 #endif

@@ -183,6 +183,35 @@ int scd_labels_to_device(scd_ctx *ctx, const uint32_t *labels);
  */
 int scd_improve(scd_ctx *ctx, double *wcc, uint32_t *labels_out);
 
+/*
+ * A caller's partition, for the reference's CPU call surface (src/scd.hpp:
+ * 25-40, src/wcc.hpp:24-30), where labels may be INVALID and the caller owns
+ * the statistics:
+ *
+ * scd_set_partition: n labels (INVALID or below num_comms <= n) become the
+ * current and best labels; with sizes, ext_edges and int_edges (num_comms
+ * entries each, all three or none) those are the community statistics the
+ * next move step reads, as improve_partition reads the Partition's arrays
+ * (src/scd.cpp:280-287); without them they are collected (collect_stats,
+ * src/scd.cpp:57-111; read them with scd_get_stats).
+ *
+ * scd_move_step: one synchronous best-move step from the current labels and
+ * statistics (improve_partition, src/scd.cpp:221-341); the new labels become
+ * current and go to labels_out (n entries, may be NULL).  The statistics
+ * are not updated (the reference's collect_stats follows as its own call).
+ *
+ * scd_compute_wcc_of: compute_wcc(g, alpha = 1.0, labels, sizes)
+ * (src/wcc.cpp:31-141) for n labels with num_comms communities; sizes
+ * (num_comms entries, may be NULL: the sizes of the labels) is read for each
+ * vertex's community size as the reference reads it.  The labels stay
+ * current.
+ */
+int scd_set_partition(scd_ctx *ctx, const uint32_t *labels, uint32_t num_comms, const int32_t *sizes,
+                      const int32_t *ext_edges, const int32_t *int_edges);
+int scd_move_step(scd_ctx *ctx, uint32_t *labels_out);
+int scd_compute_wcc_of(scd_ctx *ctx, const uint32_t *labels, uint32_t num_comms, const int32_t *sizes,
+                       double *wcc);
+
 /* Iteration execution mode: on (default) captures each iteration's launch
  * sequence once into a hipGraph (one per label-buffer parity) and replays
  * it; off launches every kernel from the host.  Results are identical.

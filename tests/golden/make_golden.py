@@ -16,6 +16,8 @@ Outputs:
   full_<name>.npz    the complete label and statistics arrays of small graphs
   improve_table.npz  calculate_improvement arguments and result bits,
                      including rows whose int32 products wrap
+  compress_table.npz compress_labels on label arrays: the relabelled
+                     arrays and counts (_ref/ref_compress_table)
 
 Run in the build container (it needs the reference build):
   make -C oracle ref && python tests/golden/make_golden.py [names...]
@@ -153,6 +155,34 @@ def improve_table():
     return len(rows)
 
 
+def compress_table():
+    """The reference's compress_labels (src/scd.cpp:32-55) on label arrays
+    with repeats, INVALID labels and sparse label values."""
+    rng = np.random.default_rng(20261204)
+    arrays = [np.array([7, 7, 0xFFFFFFFF, 3, 7], np.uint32), np.array([5], np.uint32),
+              np.full(4, 0xFFFFFFFF, np.uint32), np.arange(6, dtype=np.uint32)[::-1].copy()]
+    for _ in range(60):
+        n = int(rng.integers(1, 400))
+        a = rng.integers(0, int(rng.integers(1, 2 * n + 2)), n).astype(np.uint32)
+        a[rng.random(n) < rng.random() * 0.3] = 0xFFFFFFFF
+        arrays.append(a)
+    inp = "".join(f"{len(a)}\n" + " ".join(str(int(x)) for x in a) + "\n" for a in arrays)
+    out = subprocess.run([os.path.join(REF, "ref_compress_table")], input=inp, check=True, capture_output=True,
+                         text=True).stdout
+    rows = [l for l in out.splitlines() if l and not l.startswith("compress labels:")]
+    assert len(rows) == len(arrays)
+    counts, outs = [], []
+    for a, row in zip(arrays, rows):
+        v = [int(x) for x in row.split()]
+        assert len(v) == len(a) + 1
+        counts.append(v[0])
+        outs.append(np.array(v[1:], np.uint32))
+    off = np.concatenate([[0], np.cumsum([len(a) for a in arrays])]).astype(np.int64)
+    np.savez_compressed(os.path.join(HERE, "compress_table.npz"), offsets=off, labels_in=np.concatenate(arrays),
+                        labels_out=np.concatenate(outs), counts=np.array(counts, np.uint32))
+    return len(arrays)
+
+
 def main(names=None):
     man_path = os.path.join(HERE, "manifest.json")
     manifest = json.load(open(man_path)) if os.path.exists(man_path) else {"graphs": {}}
@@ -197,6 +227,8 @@ def main(names=None):
             manifest["graphs"][name] = ent
         if not names or "improve_table" in names:
             manifest["improve_table_rows"] = improve_table()
+        if not names or "compress_table" in names:
+            manifest["compress_table_arrays"] = compress_table()
     with open(man_path, "w") as f:
         json.dump(manifest, f, indent=1, sort_keys=True)
 

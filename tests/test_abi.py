@@ -38,6 +38,16 @@ def test_dropin_exports_reference_names():
         assert hasattr(L, sym), sym
 
 
+def test_refapi_exports_reference_names():
+    L = ctypes.CDLL(os.path.join(_lib.LIB_DIR, "libscd_refapi.so"))
+    # Graph::read (src/graph.hpp:41) and the functions of src/scd.hpp:33-36
+    # and src/wcc.hpp:24, C++ linkage
+    for sym in ("_ZN5Graph4readEPKc", "_Z24create_initial_partitionPK5Graph", "_Z15compress_labelsPK5GraphPjS2_",
+                "_Z13collect_statsPK5GraphP9PartitionPKb", "_Z17improve_partitionPK5GraphP9PartitiondPKb",
+                "_Z11compute_wccPK5GraphdPjPiPKb"):
+        assert hasattr(L, sym), sym
+
+
 def test_init_without_device_fails_cleanly():
     if scd_amd.device_count() > 0:
         pytest.skip("a device is visible")

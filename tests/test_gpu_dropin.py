@@ -1,4 +1,4 @@
-"""The reference's own driver, unchanged, on this engine (MI355X).
+"""The reference's own drivers, unchanged, on this engine (MI355X).
 
 oracle/_ref/main-gpu-amd is src/main-gpu.cpp compiled as it stands with the
 reference's src/{graph,scd,wcc,common}.cpp and linked against
@@ -8,7 +8,15 @@ included).  Its results must be the reference CPU run's (oracle/_ref/main,
 the same sources' ./main): nodes, edges, clustering coefficient, initial
 WCC, every iteration's WCC, the iteration count, the final WCC and the
 printed community count.  Both programs are built in the build container
-from the reference tree; without them the test is skipped."""
+from the reference tree; without them the test is skipped.
+
+oracle/_ref/main-amd is the reference's CPU driver src/main.cpp, unchanged,
+with only src/common.cpp and libscd_refapi.so: Graph::read and every
+scd.hpp / wcc.hpp function it calls (create_initial_partition,
+improve_partition, collect_stats, compute_wcc, compress_labels) run on the
+device.  oracle/_ref/main-gpu-amd-dev is src/main-gpu.cpp with the same
+library and libscd_dropin.so: its initial partition and initial WCC run on
+the device as well."""
 import os
 import re
 import subprocess
@@ -21,6 +29,8 @@ pytestmark = pytest.mark.gpu
 
 REF = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref")
 DROPIN_MAIN = os.path.join(REF, "main-gpu-amd")
+REFAPI_MAIN = os.path.join(REF, "main-amd")
+REFAPI_GPU_MAIN = os.path.join(REF, "main-gpu-amd-dev")
 CPU_MAIN = os.path.join(REF, "main")
 NUM = r"([-0-9.e+]+|nan|inf)"
 
@@ -67,3 +77,31 @@ def test_unchanged_reference_main_gpu_matches_reference_cpu(name, tmp_path):
     assert "selected device:" in gpu.stdout and "memory:" in gpu.stdout
     print(f"[gpu] {name}: {a['ms_per_iteration']:.3f} ms per iteration (reference CPU, one thread: "
           f"{b['ms_per_iteration']:.3f})", flush=True)
+
+
+@pytest.mark.skipif(not (os.path.exists(REFAPI_MAIN) and os.path.exists(REFAPI_GPU_MAIN) and os.path.exists(CPU_MAIN)),
+                    reason="oracle/_ref not built (needs the reference tree at build time)")
+@pytest.mark.parametrize("name", ["pp8x40_s5", "c1", "rmat12", "c2"])
+def test_unchanged_reference_drivers_on_device_call_surface(name, tmp_path):
+    g = graph(name)
+    path = str(tmp_path / f"{name}.bin")
+    g.write(path)
+    cpu = subprocess.run([CPU_MAIN, path], capture_output=True, text=True, timeout=300,
+                         env=dict(os.environ, OMP_NUM_THREADS="1"))
+    assert cpu.returncode == 0, cpu.stderr
+    b = parse(cpu.stdout)
+    for exe, args in ((REFAPI_MAIN, [path]), (REFAPI_GPU_MAIN, [path, "0"])):
+        print(f"[gpu] {os.path.basename(exe)} (scd.hpp / wcc.hpp on the device): {name} n={g.n} m={g.m}", flush=True)
+        r = subprocess.run([exe, *args], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+        a = parse(r.stdout)
+        for k in ("nodes", "edges", "iterations", "communities"):
+            assert a[k] == b[k], (exe, k, a[k], b[k])
+        for k in ("avg_cc", "initial_wcc", "final_wcc"):
+            assert a[k] == pytest.approx(b[k], rel=2e-6), (exe, k)
+        assert a["iteration_wcc"] == pytest.approx(b["iteration_wcc"], rel=2e-6)
+        # the reference functions' timer lines are printed as the reference's are
+        for line in ("create initial partition:", "compute WCC:", "compress labels:"):
+            assert line in r.stdout, (exe, line)
+        if exe == REFAPI_MAIN:
+            assert "improving partition:" in r.stdout and "collect community statistics:" in r.stdout

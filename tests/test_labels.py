@@ -2,11 +2,16 @@
 oracle's compress_labels restatement, itself pinned by the reference
 fixtures), the corrected compaction (checked by its defining properties) and
 the labels file."""
+import ctypes
+import os
+
 import numpy as np
 import pytest
 
 import oracle_bind as orc
 import scd_amd
+from conftest import GOLDEN
+from scd_amd import _lib
 
 INVALID = 0xFFFFFFFF
 
@@ -57,3 +62,31 @@ def test_labels_file_round_trip(tmp_path):
         (tmp_path / "bad.bin").write_bytes(bad)
         with pytest.raises(scd_amd.ScdError):
             scd_amd.read_labels(tmp_path / "bad.bin")
+
+
+class GraphHead(ctypes.Structure):
+    """The leading members of the reference's Graph (src/graph.hpp:25-27); the
+    call surface's compress_labels reads only n."""
+    _fields_ = [("n", ctypes.c_uint32), ("m", ctypes.c_uint32), ("clustering_coef", ctypes.c_double),
+                ("p", ctypes.c_void_p * 4)]
+
+
+def test_refapi_compress_labels_equals_reference_table():
+    """libscd_refapi.so's compress_labels (the reference name, C++ linkage)
+    relabels every array of tests/golden/compress_table.npz -- written by the
+    reference's own compress_labels, src/scd.cpp:32-55 -- to the same entries
+    and count (n minus the number of distinct labels, INVALID counted apart)."""
+    L = ctypes.CDLL(os.path.join(_lib.LIB_DIR, "libscd_refapi.so"))
+    f = L._Z15compress_labelsPK5GraphPjS2_
+    f.restype = None
+    f.argtypes = [ctypes.POINTER(GraphHead), ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]
+    t = np.load(os.path.join(GOLDEN, "compress_table.npz"))
+    off = t["offsets"]
+    for k in range(len(t["counts"])):
+        a = np.ascontiguousarray(t["labels_in"][off[k]:off[k + 1]], np.uint32)
+        head = GraphHead(len(a), 0, 0.0)
+        count = ctypes.c_uint32()
+        f(ctypes.byref(head), a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), ctypes.byref(count))
+        assert count.value == t["counts"][k], k
+        assert np.array_equal(a, t["labels_out"][off[k]:off[k + 1]]), k
+        assert count.value == orc.compress_count(t["labels_in"][off[k]:off[k + 1]])
