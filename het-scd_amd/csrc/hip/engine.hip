@@ -182,9 +182,16 @@ struct scd_ctx {
     // chunks launched by each merge pass (a prefix of the chunk list: hubs
     // are ordered by degree, largest first)
     std::vector<uint32_t> merge_pass_chunks;
+    // dense-community WCC (tune.dense_wcc): per-label row offsets and
+    // local-id counters, per-vertex local ids, the list of dense-community
+    // vertices, two counters and the row buffer (dn_units 256-byte units)
+    uint32_t *dn_off = nullptr, *dn_vbase = nullptr, *dn_cursor = nullptr, *dn_lid = nullptr, *dn_list = nullptr,
+             *dn_ctr = nullptr;
+    unsigned long long *dn_bits = nullptr;
+    uint32_t dn_units = 0;
     std::vector<DevBuf> hub_bufs;
 
-    scd_tuning tune = {4096, 2048, 1024, 0, 0, 1, {0, 0}};
+    scd_tuning tune = {4096, 2048, 1024, 0, 0, 1, 0, 0};
     uint32_t last_ip_rounds = 0;
 
     // per-iteration hipGraphs, one per label-buffer parity (cur == lab_a or
@@ -534,6 +541,22 @@ static int build_bins(scd_ctx *c) {
         }
     }
 
+    // dense-community WCC: rows take at most 2 bytes per adjacency entry
+    // (density >= 1/16), so E / 128 units hold the rows of every dense
+    // community of a partition (at least 1 MiB, at most 4 GiB)
+    c->dn_off = c->dn_vbase = c->dn_cursor = c->dn_lid = c->dn_list = c->dn_ctr = nullptr;
+    c->dn_bits = nullptr;
+    c->dn_units = 0;
+    if (c->tune.dense_wcc == 2 || (c->tune.dense_wcc == 1 && c->E >= (1ull << 20))) {
+        const uint64_t units = std::min<uint64_t>(std::max<uint64_t>(c->E / 128, 4096), 1ull << 24);
+        if ((rc = dalloc(c, &c->dn_off, n, &c->hub_bufs)) || (rc = dalloc(c, &c->dn_vbase, n, &c->hub_bufs)) ||
+            (rc = dalloc(c, &c->dn_cursor, n, &c->hub_bufs)) || (rc = dalloc(c, &c->dn_lid, n, &c->hub_bufs)) ||
+            (rc = dalloc(c, &c->dn_list, n, &c->hub_bufs)) || (rc = dalloc(c, &c->dn_ctr, DN_CTR, &c->hub_bufs)) ||
+            (rc = dalloc(c, &c->dn_bits, units * DN_UNIT_WORDS, &c->hub_bufs)))
+            return rc;
+        c->dn_units = (uint32_t)units;
+    }
+
     // hubs
     const uint32_t hb0 = c->bin_begin[K_HUB], nh = c->bin_begin[K_HUB + 1] - hb0;
     c->nh = nh;
@@ -851,10 +874,42 @@ static int scan_u32_dev(scd_ctx *c, const uint32_t *in, uint32_t *out, uint64_t 
 // subgraph of the current labels, or with int_deg == nullptr the whole
 // graph): T[u] = triangles at u, td[u] = edges at u in a triangle; the
 // kernels per degree bin and the work queue of long out-lists.
-static int count_internal_triangles(scd_ctx *c, const WccArgs &w) {
+static int count_internal_triangles(scd_ctx *c, const WccArgs &w0) {
+    WccArgs w = w0;
+    // dense communities of the WCC's internal subgraph: classified, local
+    // ids and rows, counted on their own lane beside the list kernels
+    // (graphs of at least 2^20 adjacency entries: below that the four extra
+    // launches cost more than the list kernels they relieve)
+    const bool dense = c->dn_bits && w.int_deg && c->num_comms;
+    DenseArgs d;
+    if (dense) {
+        d.labels = w.labels;
+        d.stats = w.stats;
+        d.dirty = w.dirty;
+        d.off = c->dn_off;
+        d.vbase = c->dn_vbase;
+        d.cursor = c->dn_cursor;
+        d.lid = c->dn_lid;
+        d.list = c->dn_list;
+        d.ctr = c->dn_ctr;
+        d.bits = c->dn_bits;
+        d.units = c->dn_units;
+        LAUNCH(c, "dn_clear", 1, 64, k_fill_u32, c->dn_ctr, DN_CTR);
+        LAUNCH(c, "dn_classify", cdiv(c->num_comms, 256), 256, k_dn_classify, d, c->num_comms);
+        LAUNCH(c, "dn_lid", cdiv(c->n, 256), 256, k_dn_lid, d, c->n);
+        w.dn_off = c->dn_off;
+    }
     LAUNCH(c, "wcc_dplus", cdiv(c->n, c->block_size), c->block_size, k_wtc_dplus, w, c->n);
-    // lanes: 3 small-degree groups | 2 out-lists up to 64 | 0 work queue
+    // lanes: 3 small-degree groups | 2 out-lists up to 64 | 1 dense
+    // communities | 0 work queue
     fork_lanes(c);
+    if (dense) {
+        on_lane(c, 1);
+        // persistent grids: up to 32 waves per CU, no more than a wave per vertex
+        const uint32_t grid = std::min<uint32_t>(8 * c->num_cus, cdiv(c->n, 4));
+        LAUNCH(c, "dn_rows", grid, 256, k_dn_rows, w, d);
+        LAUNCH(c, "dn_count", grid, 256, k_dn_count, w, d);
+    }
     on_lane(c, 3);
     LAUNCH(c, "wcc_tri_sub8", cdiv(bin_n(c, K_SUB8), 32), 256, k_wtc_group<8>, w, bin_v(c, K_SUB8),
            bin_n(c, K_SUB8));
@@ -1657,7 +1712,9 @@ extern "C" int scd_set_tuning(scd_ctx *c, const scd_tuning *t) {
     if (t->lds_hash > 1 || (t->lds_hash && T > 4096))
         return fail(SCD_EINVAL, "scd_set_tuning: lds_hash %u needs lds_tile <= 4096", t->lds_hash);
     if (t->lanes > 1) return fail(SCD_EINVAL, "scd_set_tuning: lanes %u is not 0 or 1", t->lanes);
-    if (c->tune.hub_table != t->hub_table || c->tune.lds_hash != t->lds_hash || c->tune.lanes != t->lanes)
+    if (t->dense_wcc > 2) return fail(SCD_EINVAL, "scd_set_tuning: dense_wcc %u is not 0, 1 or 2", t->dense_wcc);
+    if (c->tune.hub_table != t->hub_table || c->tune.lds_hash != t->lds_hash || c->tune.lanes != t->lanes ||
+        c->tune.dense_wcc != t->dense_wcc)
         drop_graphs(c);
     c->tune = *t;
     c->tune.wave_max = W;
@@ -1688,6 +1745,18 @@ extern "C" int scd_last_counters(scd_ctx *c, uint64_t out[4]) {
     out[1] = c->counters_host[K_NB];
     out[2] = c->counters_host[K_NB + 1];
     out[3] = c->nh;
+    return SCD_OK;
+}
+
+extern "C" int scd_dense_wcc_info(scd_ctx *c, uint32_t out[2]) {
+    if (!c || !out) return fail(SCD_EINVAL, "scd_dense_wcc_info: null argument");
+    out[0] = out[1] = 0;
+    if (!c->dn_ctr) return SCD_OK;
+    (void)hipSetDevice(c->device);
+    HIP_TRY(hipMemcpyAsync(c->wcc_host + 4, c->dn_ctr, 8, hipMemcpyDeviceToHost, c->stream));
+    int rc = finish(c);
+    if (rc) return rc;
+    memcpy(out, c->wcc_host + 4, 8);
     return SCD_OK;
 }
 
@@ -2820,13 +2889,20 @@ uint32_t ilog2u(uint64_t x) {
 bool read_tune_file(const char *path, const std::string &key, TuneEntry &out) {
     FILE *f = fopen(path, "r");
     if (!f) return false;
-    char k[256];
-    unsigned T, C, W, H, Lh, N;
-    double ms;
+    char line[512], k[256];
     bool found = false;
-    while (fscanf(f, "%255s %u %u %u %u %u %u %lf", k, &T, &C, &W, &H, &Lh, &N, &ms) == 8) {
+    while (fgets(line, sizeof(line), f)) {
+        // key lds_tile merge_chunk wave_max hub_table lds_hash lanes dense_wcc ms
+        // (lines without dense_wcc read it as 0, the default)
+        double v[8];
+        const int got = sscanf(line, "%255s %lf %lf %lf %lf %lf %lf %lf %lf", k, &v[0], &v[1], &v[2], &v[3], &v[4],
+                               &v[5], &v[6], &v[7]);
+        if (got != 8 && got != 9) continue;
+        const unsigned Dn = got == 9 ? (unsigned)v[6] : 0u;
+        const double ms = got == 9 ? v[7] : v[6];
         if (key == k) {  // the last line of a class wins
-            out.t = scd_tuning{T, C, W, H, Lh, N, {0, 0}};
+            out.t = scd_tuning{(unsigned)v[0], (unsigned)v[1], (unsigned)v[2], (unsigned)v[3], (unsigned)v[4],
+                               (unsigned)v[5], Dn, 0};
             out.ms = ms;
             found = true;
         }
@@ -2891,7 +2967,7 @@ extern "C" int scd_autotune(scd_ctx *c, const char *cache_path, int reps, scd_tu
             for (uint32_t C : Cs)
                 for (uint32_t H : Hs)
                     for (uint32_t Lh : Ls)
-                        for (uint32_t N : Ns) cand.push_back(scd_tuning{T, C, W, H, Lh, N, {0, 0}});
+                        for (uint32_t N : Ns) cand.push_back(scd_tuning{T, C, W, H, Lh, N, 0, 0});
         }
 
     // the labels, statistics and best labels to start every timed iteration from
@@ -2926,7 +3002,11 @@ extern "C" int scd_autotune(scd_ctx *c, const char *cache_path, int reps, scd_tu
         release_tmp(c, tmp);
         return rc;
     }
-    for (const scd_tuning &t : cand) {
+    // the shapes, then the fastest one with the dense-community WCC
+    for (size_t i = 0; i <= cand.size(); i++) {
+        scd_tuning t = best_t;
+        if (i < cand.size()) t = cand[i];
+        else t.dense_wcc = 1;
         if ((rc = scd_set_tuning(c, &t)) || (rc = copy_state(false)) || (rc = scd_improve(c, nullptr, nullptr)))
             break;  // the untimed run captures the iteration's graphs
         std::vector<double> ms;
@@ -2957,8 +3037,8 @@ extern "C" int scd_autotune(scd_ctx *c, const char *cache_path, int reps, scd_tu
     if (cache_path) {
         FILE *f = fopen(cache_path, "a");
         if (!f) return fail(SCD_EIO, "scd_autotune: cannot append to %s", cache_path);
-        fprintf(f, "%s %u %u %u %u %u %u %.6f\n", key, best_t.lds_tile, best_t.merge_chunk, best_t.wave_max,
-                best_t.hub_table, best_t.lds_hash, best_t.lanes, best_ms);
+        fprintf(f, "%s %u %u %u %u %u %u %u %.6f\n", key, best_t.lds_tile, best_t.merge_chunk, best_t.wave_max,
+                best_t.hub_table, best_t.lds_hash, best_t.lanes, best_t.dense_wcc, best_ms);
         fclose(f);
     }
     if (chosen) *chosen = best_t;

@@ -57,6 +57,10 @@ struct WccArgs {
     // walked as N+(u) below v against the set of N+(v) (k_wtc_queue<.., true>)
     // instead of N+(v) against N+(u).  0: every edge from u's side.
     uint32_t vc_min;
+    // optional, per label: the row offset of a community counted by the
+    // dense (bitmap) kernels below, DN_NONE for the others.  Members of a
+    // dense community skip the list kernels (their out-lists count as empty).
+    const uint32_t *__restrict__ dn_off = nullptr;
 };
 
 constexpr uint32_t WTC_HOT_MAX = 8192;
@@ -134,6 +138,13 @@ __global__ void k_wtc_dplus(WccArgs a, uint32_t n) {
         const uint32_t lab = a.labels[u];
         if (lab != SCD_INVALID && !a.dirty[lab]) {
             a.dplus[u] = 0;  // community unchanged: no counting, T and td kept
+            return;
+        }
+    }
+    if (a.dn_off) {
+        const uint32_t lab = a.labels[u];
+        if (lab != SCD_INVALID && a.dn_off[lab] != 0xFFFFFFFFu) {
+            a.dplus[u] = 0;  // T and td come from the dense kernels
             return;
         }
     }
@@ -641,6 +652,226 @@ __global__ __launch_bounds__(256) void k_wtc_final_wave(WccArgs a, const uint32_
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off, SCD_WAVE);
             if (lane == 0 && mine) atomicAdd(&a.td[uu], mine);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dense communities: internal triangles by bitmap rows.
+//
+// A community of s members whose internal degrees sum to at least
+// s (s - 1) / 16 (and 32 <= s <= DN_MAX_S) gives each member p a local id
+// and a row of s bits, bit lid(q) set for every internal neighbour q.  Then
+// for every internal neighbour q of p, c(p, q) = |Nc(p) ∩ Nc(q)| is the
+// popcount of row(p) AND row(q): T[p] = (sum over q of c(p, q)) / 2 and
+// td[p] = #{q : c(p, q) > 0} -- the same integers the list kernels count,
+// from coalesced row reads instead of per-entry probes (the rows take at
+// most half the bytes of the community's internal lists).  Local ids come
+// from per-community counters, so their order varies between runs; the
+// counts do not depend on it.  Rows take whole 256-byte units of a buffer
+// of `units` units, handed out by one counter; a community that does not
+// fit stays with the list kernels.
+//
+// The listed vertices lie community after community, and the waves of the
+// row and count kernels stride over the list, so every wave gets members
+// of large and small communities alike.  (Handing out runs of the list from
+// per-XCD slices by atomic counters kept each community's rows in one
+// XCD's L2 but cost more than it saved once arenas listed millions of
+// vertices: 4,096 planted graphs, 193 against 145 ms per iteration.)
+// ---------------------------------------------------------------------------
+constexpr uint32_t DN_NONE = 0xFFFFFFFFu;
+constexpr uint32_t DN_MIN_S = 32;
+constexpr uint32_t DN_MAX_S = 16384;  // rows of at most 256 words
+constexpr uint32_t DN_UNIT_WORDS = 32;  // 256-byte allocation units
+constexpr uint32_t DN_CTR = 2;  // [0] units handed out, [1] vertices listed
+
+struct DenseArgs {
+    const uint32_t *__restrict__ labels;
+    const int4 *__restrict__ stats;
+    const uint8_t *__restrict__ dirty;  // nullptr: every community
+    uint32_t *__restrict__ off;      // per label: first unit of its rows, or DN_NONE
+    uint32_t *__restrict__ vbase;    // per dense label: first list slot of its members
+    uint32_t *__restrict__ cursor;   // per label: local ids handed out
+    uint32_t *__restrict__ lid;      // per vertex of a dense community
+    uint32_t *__restrict__ list;     // the vertices of dense communities, by community
+    uint32_t *__restrict__ ctr;      // DN_CTR counters, cleared before each count
+    unsigned long long *__restrict__ bits;
+    uint32_t units;                  // capacity of bits in units
+};
+
+__device__ __forceinline__ uint32_t dn_words(uint32_t s) { return (s + 63u) >> 6; }
+
+// off[L], vbase[L] and cursor[L] for every label.
+__global__ void k_dn_classify(DenseArgs d, uint32_t num_comms) {
+    const uint32_t L = blockIdx.x * blockDim.x + threadIdx.x;
+    if (L >= num_comms) return;
+    d.cursor[L] = 0;
+    const int4 st = d.stats[L];
+    const uint64_t s = (uint32_t)st.x, z = (uint32_t)st.z;
+    uint32_t off = DN_NONE;
+    if (s >= DN_MIN_S && s <= DN_MAX_S && 16ull * z >= s * (s - 1) && (!d.dirty || d.dirty[L])) {
+        const uint32_t need = (uint32_t)((s * dn_words((uint32_t)s) + DN_UNIT_WORDS - 1) / DN_UNIT_WORDS);
+        const uint32_t at = atomicAdd(&d.ctr[0], need);
+        if ((uint64_t)at + need <= d.units) {
+            off = at;
+            d.vbase[L] = atomicAdd(&d.ctr[1], (uint32_t)s);
+        }
+    }
+    d.off[L] = off;
+}
+
+// Local ids and list slots of the dense-community vertices: per wave, one
+// counter update per distinct label.
+__global__ __launch_bounds__(256) void k_dn_lid(DenseArgs d, uint32_t n) {
+    const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    uint32_t L = SCD_INVALID;
+    if (p < n) L = d.labels[p];
+    const bool dense = L != SCD_INVALID && d.off[L] != DN_NONE;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint64_t todo = __ballot(dense);
+    if (!todo) return;  // uniform per wave
+    uint32_t base = 0, my_leader = 0, my_rank = 0;
+    while (todo) {  // uniform
+        const int l = __builtin_ctzll(todo);
+        const uint32_t Ll = __shfl(L, l, SCD_WAVE);
+        const uint64_t m = __ballot(dense && L == Ll);
+        if (lane == l) base = atomicAdd(&d.cursor[Ll], (uint32_t)__popcll(m));
+        if ((m >> lane) & 1ull) {
+            my_leader = (uint32_t)l;
+            my_rank = (uint32_t)__popcll(m & below);
+        }
+        todo &= ~m;
+    }
+    const uint32_t b = __shfl(base, (int)my_leader, SCD_WAVE);
+    if (dense) {
+        const uint32_t id = b + my_rank;
+        d.lid[p] = id;
+        d.list[d.vbase[L] + id] = p;
+    }
+}
+
+// Row of every listed vertex: a wave per vertex builds it in LDS (64-bit
+// atomic ORs at the neighbours' local ids) and stores it whole.
+__global__ __launch_bounds__(256) void k_dn_rows(WccArgs a, DenseArgs d) {
+    __shared__ unsigned long long s_row[4][DN_MAX_S / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    unsigned long long *row = s_row[wv];
+    const uint32_t nd = d.ctr[1];
+    for (uint32_t slot = blockIdx.x * 4u + wv; slot < nd; slot += gridDim.x * 4u) {  // uniform per wave
+        const uint32_t p = d.list[slot];
+        const uint32_t L = d.labels[p];
+        const uint32_t W = dn_words((uint32_t)d.stats[L].x);
+        for (uint32_t k = lane; k < W; k += 64) row[k] = 0ull;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t beg = a.indices[p], dg = a.int_deg[p];
+        for (uint32_t k = lane; k < dg; k += 64) {
+            const uint32_t l = d.lid[a.int_adj[beg + k]];
+            atomicOr(&row[l >> 6], 1ull << (l & 63u));
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        unsigned long long *out = d.bits + (size_t)d.off[L] * DN_UNIT_WORDS + (size_t)d.lid[p] * W;
+        for (uint32_t k = lane; k < W; k += 64) out[k] = row[k];
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();  // the row is reused by the wave's next vertex
+    }
+}
+
+// T and td of every listed vertex, a wave per vertex, DN_U neighbour rows
+// in flight (2: 4,096 planted graphs, 164 ms for the second iteration
+// against 176 with 4, whose registers cost occupancy, and 165 with 1).  Rows of W <= 64 words: the wave splits into 64 / Wp groups
+// (Wp = W rounded up to a power of two), each ANDing one neighbour's row
+// with its own copy of row(p) per load; longer rows: every lane holds up to
+// four words of row(p), and the neighbour is the same for the whole wave.
+constexpr int DN_U = 2;
+
+__global__ __launch_bounds__(256) void k_dn_count(WccArgs a, DenseArgs d) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t nd = d.ctr[1];
+    for (uint32_t slot = blockIdx.x * 4u + wv; slot < nd; slot += gridDim.x * 4u) {  // uniform per wave
+        const uint32_t p = d.list[slot];
+        const uint32_t L = d.labels[p];
+        const uint32_t W = dn_words((uint32_t)d.stats[L].x);
+        const unsigned long long *rows = d.bits + (size_t)d.off[L] * DN_UNIT_WORDS;
+        const unsigned long long *rp = rows + (size_t)d.lid[p] * W;
+        const uint32_t beg = a.indices[p], dg = a.int_deg[p];
+        uint32_t pc = 0, td = 0;
+        if (W <= 64) {
+            uint32_t Wp = 1;
+            while (Wp < W) Wp <<= 1;
+            const uint32_t G = 64u / Wp, g = (uint32_t)lane / Wp, k = (uint32_t)lane & (Wp - 1u);
+            // bit g * Wp of `firsts`: the first lane of group g
+            unsigned long long firsts = 0;
+            for (uint32_t gg = 0; gg < G; gg++) firsts |= 1ull << (gg * Wp);
+            const unsigned long long mine = k < W ? rp[k] : 0ull;
+            for (uint32_t c0 = 0; c0 < dg; c0 += 64) {
+                const uint32_t nq = min(64u, dg - c0);
+                const uint32_t ql = (uint32_t)lane < nq ? d.lid[a.int_adj[beg + c0 + lane]] : 0u;
+                // batch b: neighbour b * G + g for group g
+                for (uint32_t b0 = 0; b0 * G < nq; b0 += DN_U) {
+                    unsigned long long xv[DN_U];
+#pragma unroll
+                    for (int u = 0; u < DN_U; u++) {
+                        const uint32_t qi = (b0 + u) * G + g;
+                        const uint32_t l = __shfl(ql, (int)(qi & 63u), SCD_WAVE);
+                        xv[u] = (qi < nq && k < W) ? rows[(size_t)l * W + k] : 0ull;
+                    }
+#pragma unroll
+                    for (int u = 0; u < DN_U; u++) {
+                        const unsigned long long v = xv[u] & mine;
+                        pc += (uint32_t)__popcll(v);
+                        unsigned long long f = __ballot(v != 0ull);
+                        for (uint32_t sh = 1; sh < Wp; sh <<= 1) f |= f >> sh;  // a group's OR at its first bit
+                        td += (uint32_t)__popcll(f & firsts);
+                    }
+                }
+            }
+        } else {
+            constexpr int NJ = DN_MAX_S / 64 / 64;
+            unsigned long long mine[NJ];
+#pragma unroll
+            for (int j = 0; j < NJ; j++) {
+                const uint32_t k = (uint32_t)lane + 64u * j;
+                mine[j] = k < W ? rp[k] : 0ull;
+            }
+            const uint32_t nj = (W + 63u) >> 6;  // words per lane in use (2..4)
+            for (uint32_t c0 = 0; c0 < dg; c0 += 64) {
+                const uint32_t nq = min(64u, dg - c0);
+                const uint32_t ql = (uint32_t)lane < nq ? d.lid[a.int_adj[beg + c0 + lane]] : 0u;
+                for (uint32_t i0 = 0; i0 < nq; i0 += DN_U) {
+                    unsigned long long xv[DN_U][NJ];
+#pragma unroll
+                    for (int u = 0; u < DN_U; u++) {
+                        const uint32_t i = min(i0 + u, nq - 1u);  // uniform
+                        const uint32_t l = (uint32_t)__builtin_amdgcn_readlane((int)ql, (int)i);
+                        const unsigned long long *rq = rows + (size_t)l * W;
+#pragma unroll
+                        for (int j = 0; j < NJ; j++) {
+                            const uint32_t k = (uint32_t)lane + 64u * j;
+                            xv[u][j] = ((uint32_t)j < nj && i0 + u < nq && k < W) ? rq[k] : 0ull;
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < DN_U; u++) {
+                        unsigned long long any = 0;
+#pragma unroll
+                        for (int j = 0; j < NJ; j++) {
+                            const unsigned long long v = xv[u][j] & mine[j];
+                            pc += (uint32_t)__popcll(v);
+                            any |= v;
+                        }
+                        td += (i0 + u < nq && __ballot(any != 0ull)) ? 1u : 0u;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) pc += __shfl_xor(pc, o, SCD_WAVE);
+        if (lane == 0) {
+            a.T[p] = pc / 2u;
+            a.td[p] = td;
         }
     }
 }

@@ -169,12 +169,15 @@ extern "C" int scd_server_create(const scd_server_opts *opts, scd_server **out) 
     // 8% slower iterations on a 185-graph planted arena), and the hubs of
     // several R-MAT graphs count their labels in hash tables instead of
     // merge passes over all hub chunks (15% faster arena iterations on a
-    // 200-graph mix of planted graphs and R-MAT s18-22)
+    // 200-graph mix of planted graphs and R-MAT s18-22), and the planted
+    // graphs' dense communities count their internal triangles by bitmap
+    // rows
     for (scd_ctx *c : {s->main, s->prep}) {
         scd_tuning t;
         if (scd_get_tuning(c, &t) == SCD_OK) {
             t.lanes = 0;
             t.hub_table = 1;
+            t.dense_wcc = 1;
             (void)scd_set_tuning(c, &t);
         }
     }

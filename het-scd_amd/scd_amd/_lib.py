@@ -73,7 +73,7 @@ class ScdEdgesC(ctypes.Structure):
 class ScdTuningC(ctypes.Structure):
     _fields_ = [("lds_tile", ctypes.c_uint32), ("merge_chunk", ctypes.c_uint32), ("wave_max", ctypes.c_uint32),
                 ("hub_table", ctypes.c_uint32), ("lds_hash", ctypes.c_uint32), ("lanes", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 2)]
+                ("dense_wcc", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 def host():
@@ -145,6 +145,7 @@ def hip():
         "scd_convert_last_info": (ctypes.c_int, [C, ctypes.POINTER(ScdConvertInfoC)]),
         "scd_initial_partition": (ctypes.c_int, [C, u32p, u32p]),
         "scd_labels_to_device": (ctypes.c_int, [C, u32p]),
+        "scd_set_partition": (ctypes.c_int, [C, u32p, ctypes.c_uint32, i32p, i32p, i32p]),
         "scd_improve": (ctypes.c_int, [C, f64p, u32p]),
         "scd_store_best": (ctypes.c_int, [C]),
         "scd_labels_from_device": (ctypes.c_int, [C, u32p]),
@@ -156,6 +157,7 @@ def hip():
         "scd_get_tuning": (ctypes.c_int, [C, ctypes.POINTER(ScdTuningC)]),
         "scd_set_tuning": (ctypes.c_int, [C, ctypes.POINTER(ScdTuningC)]),
         "scd_last_counters": (ctypes.c_int, [C, u64p]),
+        "scd_dense_wcc_info": (ctypes.c_int, [C, u32p]),
         "scd_prof_count": (ctypes.c_int, [C]),
         "scd_prof_get": (ctypes.c_int, [C, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), u64p, f64p]),
         "scd_prof_reset": (ctypes.c_int, [C]),

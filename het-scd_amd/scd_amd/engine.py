@@ -27,10 +27,11 @@ def _get_tuning(ctx):
     t = _lib.ScdTuningC()
     check_hip(_lib.hip().scd_get_tuning(ctx, ctypes.byref(t)))
     return {"lds_tile": t.lds_tile, "merge_chunk": t.merge_chunk, "wave_max": t.wave_max, "hub_table": t.hub_table,
-            "lds_hash": t.lds_hash, "lanes": t.lanes}
+            "lds_hash": t.lds_hash, "lanes": t.lanes, "dense_wcc": t.dense_wcc}
 
 
-def _set_tuning(ctx, lds_tile=None, merge_chunk=None, wave_max=None, hub_table=None, lds_hash=None, lanes=None):
+def _set_tuning(ctx, lds_tile=None, merge_chunk=None, wave_max=None, hub_table=None, lds_hash=None, lanes=None,
+                dense_wcc=None):
     t = _lib.ScdTuningC()
     check_hip(_lib.hip().scd_get_tuning(ctx, ctypes.byref(t)))
     if lds_tile is not None:
@@ -45,6 +46,8 @@ def _set_tuning(ctx, lds_tile=None, merge_chunk=None, wave_max=None, hub_table=N
         t.lds_hash = lds_hash
     if lanes is not None:
         t.lanes = lanes
+    if dense_wcc is not None:
+        t.dense_wcc = dense_wcc
     check_hip(_lib.hip().scd_set_tuning(ctx, ctypes.byref(t)))
 
 
@@ -54,7 +57,8 @@ def _autotune(ctx, cache_path, reps):
     path = None if cache_path is None else str(cache_path).encode()
     check_hip(_lib.hip().scd_autotune(ctx, path, reps, ctypes.byref(t), ctypes.byref(ms)))
     return {"lds_tile": t.lds_tile, "merge_chunk": t.merge_chunk, "wave_max": t.wave_max,
-            "hub_table": t.hub_table, "lds_hash": t.lds_hash, "lanes": t.lanes}, ms.value
+            "hub_table": t.hub_table, "lds_hash": t.lds_hash, "lanes": t.lanes,
+            "dense_wcc": t.dense_wcc}, ms.value
 
 
 class Engine:
@@ -96,6 +100,18 @@ class Engine:
         if labels.shape != (self.n,):
             raise ValueError(f"labels must hold n = {self.n} entries")
         check_hip(_lib.hip().scd_labels_to_device(self._ctx, ptr(labels, u32p)))
+
+    def set_partition(self, labels, num_comms=None):
+        """scd_set_partition without caller statistics: any labels below
+        num_comms (default: the largest valid label + 1) or INVALID become
+        current, with their statistics computed on the device."""
+        labels = np.ascontiguousarray(labels, dtype=np.uint32)
+        if labels.shape != (self.n,):
+            raise ValueError(f"labels must hold n = {self.n} entries")
+        if num_comms is None:
+            valid = labels[labels != 0xFFFFFFFF]
+            num_comms = int(valid.max()) + 1 if valid.size else 0
+        check_hip(_lib.hip().scd_set_partition(self._ctx, ptr(labels, u32p), num_comms, None, None, None))
 
     def improve_labels(self, return_labels=False):
         wcc = ctypes.c_double()
@@ -234,8 +250,8 @@ class Engine:
         return _get_tuning(self._ctx)
 
     def set_tuning(self, lds_tile=None, merge_chunk=None, wave_max=None, hub_table=None, lds_hash=None,
-                   lanes=None):
-        _set_tuning(self._ctx, lds_tile, merge_chunk, wave_max, hub_table, lds_hash, lanes)
+                   lanes=None, dense_wcc=None):
+        _set_tuning(self._ctx, lds_tile, merge_chunk, wave_max, hub_table, lds_hash, lanes, dense_wcc)
 
     def autotune(self, cache_path=None, reps=3):
         """scd_autotune: fastest tuning for this graph's size class (cached);
@@ -247,6 +263,13 @@ class Engine:
         check_hip(_lib.hip().scd_last_counters(self._ctx, ptr(out, u64p)))
         return {"pairs": int(out[0]), "internal_entries": int(out[1]), "internal_deg_sq": int(out[2]),
                 "hubs": int(out[3])}
+
+    def dense_wcc_info(self):
+        """scd_dense_wcc_info: (row units requested, vertices counted by bitmap
+        rows) of the last internal triangle count."""
+        out = np.zeros(2, np.uint32)
+        check_hip(_lib.hip().scd_dense_wcc_info(self._ctx, ptr(out, u32p)))
+        return int(out[0]), int(out[1])
 
     def profile(self):
         """{kernel name: (launches, total ms)} from the HIP events."""
@@ -399,8 +422,8 @@ class BatchEngine:
         return _get_tuning(self._ctx)
 
     def set_tuning(self, lds_tile=None, merge_chunk=None, wave_max=None, hub_table=None, lds_hash=None,
-                   lanes=None):
-        _set_tuning(self._ctx, lds_tile, merge_chunk, wave_max, hub_table, lds_hash, lanes)
+                   lanes=None, dense_wcc=None):
+        _set_tuning(self._ctx, lds_tile, merge_chunk, wave_max, hub_table, lds_hash, lanes, dense_wcc)
 
     def current_labels(self):
         lab = np.empty(self.arena.graph.n, np.uint32)

@@ -319,6 +319,19 @@ int scd_neighbor_label_pairs(scd_ctx *ctx, uint32_t *sorted_out, uint32_t *run_o
  *                streams (graphs of at least 2^20 adjacency entries); 0: one
  *                stream.  The environment variable SCD_LANES=0 (read by
  *                scd_init) keeps a context on one stream whatever the tuning.
+ *   dense_wcc:   1: on graphs of at least 2^20 adjacency entries
+ *                the WCC counts the internal triangles of dense communities
+ *                (32..16384 members, internal degrees summing to at least
+ *                s (s - 1) / 16) by AND + popcount of per-member bitmap rows,
+ *                the others by the sorted-list kernels (below 2^20 entries
+ *                the extra launches cost more than they save); 2: the same
+ *                on every graph; 0 (default): every community by the list
+ *                kernels.  It pays on arenas of planted-partition graphs
+ *                (communities of thousands of members at densities near
+ *                0.15); the autotuner tries it on its fastest shape.
+ *                Rows take a buffer of 2 bytes per adjacency entry (1 MiB ..
+ *                4 GiB); communities beyond it use the list kernels.
+ *   reserved:    0.
  */
 typedef struct scd_tuning {
     uint32_t lds_tile;
@@ -327,7 +340,8 @@ typedef struct scd_tuning {
     uint32_t hub_table;
     uint32_t lds_hash;
     uint32_t lanes;
-    uint32_t reserved[2];
+    uint32_t dense_wcc;
+    uint32_t reserved;
 } scd_tuning;
 
 int scd_get_tuning(const scd_ctx *ctx, scd_tuning *out);
@@ -360,6 +374,16 @@ int scd_set_counting(scd_ctx *ctx, int on);
  * [3] vertices with degree above lds_tile (hub path).
  */
 int scd_last_counters(scd_ctx *ctx, uint64_t out[4]);
+
+/*
+ * The dense-community WCC of the last internal triangle count (tuning
+ * dense_wcc; no counterpart in the reference): out[0] = 256-byte row units
+ * requested by the communities classified dense (those past the buffer's
+ * capacity were counted by the list kernels), out[1] = vertices whose
+ * counts came from bitmap rows.  Both 0 when dense_wcc is off.  Waits for
+ * the context's stream.
+ */
+int scd_dense_wcc_info(scd_ctx *ctx, uint32_t out[2]);
 
 /*
  * Per degree bin of the last move step: bin names, vertices, adjacency
