@@ -42,10 +42,18 @@ __host__ __device__ inline uint32_t bin_of(uint32_t d, uint32_t lds_tile, uint32
 
 constexpr uint32_t BIN_CHUNK = 4096;  // vertices per workgroup (256 threads x 16)
 
+// A vertex is binned unless gid is given and its graph has stopped
+// (active_g[gid[v]] == 0): a batched arena's bins can leave out the graphs
+// that no longer refine.
+__device__ __forceinline__ bool bin_keep(const uint32_t *gid, const uint32_t *active_g, uint32_t v) {
+    return !gid || active_g[gid[v]];
+}
+
 // cnt[blk * K_NB + b] = vertices of bin b in workgroup blk's chunk;
 // esum[b] += their degrees; *dmax = the largest degree.
 __global__ __launch_bounds__(256) void k_bin_count(const uint32_t *__restrict__ indices, uint32_t n, uint32_t lds_tile,
-                                                   uint32_t wave_max, uint32_t *__restrict__ cnt,
+                                                   uint32_t wave_max, const uint32_t *__restrict__ gid,
+                                                   const uint32_t *__restrict__ active_g, uint32_t *__restrict__ cnt,
                                                    unsigned long long *__restrict__ esum, uint32_t *__restrict__ dmax) {
     __shared__ uint32_t s_cnt[K_NB];
     __shared__ unsigned long long s_e[K_NB];
@@ -61,6 +69,7 @@ __global__ __launch_bounds__(256) void k_bin_count(const uint32_t *__restrict__ 
     for (uint32_t k = threadIdx.x; k < BIN_CHUNK; k += 256) {
         const uint32_t v = base + k;
         if (v >= n) break;
+        if (!bin_keep(gid, active_g, v)) continue;
         const uint32_t d = indices[v + 1] - indices[v];
         const uint32_t b = bin_of(d, lds_tile, wave_max);
         atomicAdd(&s_cnt[b], 1u);
@@ -81,6 +90,8 @@ __global__ __launch_bounds__(256) void k_bin_count(const uint32_t *__restrict__ 
 // (ascending vertex order within every bin).
 __global__ __launch_bounds__(256) void k_bin_scatter(const uint32_t *__restrict__ indices, uint32_t n,
                                                      uint32_t lds_tile, uint32_t wave_max,
+                                                     const uint32_t *__restrict__ gid,
+                                                     const uint32_t *__restrict__ active_g,
                                                      const uint32_t *__restrict__ off,
                                                      uint32_t *__restrict__ bin_verts) {
     __shared__ uint32_t s_off[K_NB];
@@ -91,7 +102,7 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const uint32_t *__restrict_
     const uint32_t base = blockIdx.x * BIN_CHUNK;
     for (uint32_t k = 0; k < BIN_CHUNK; k += 256) {
         const uint32_t v = base + k + threadIdx.x;
-        const bool act = v < n;
+        const bool act = v < n && bin_keep(gid, active_g, v);
         const uint32_t b = act ? bin_of(indices[v + 1] - indices[v], lds_tile, wave_max) : K_NB;
         uint32_t rank = 0;
         for (uint32_t j = 0; j < K_NB; j++) {

@@ -149,9 +149,10 @@ struct scd_ctx {
     uint32_t snap_num_comms = 0;
     bool have_snap = false;
 
-    // bins
+    // bins (bins_partial: only the vertices of the graphs still refining)
     uint32_t *bin_verts = nullptr;
     uint32_t bin_begin[K_NB + 1] = {0};
+    bool bins_partial = false;
 
     // work queue of the internal triangle count (vertices of the W1..HUB
     // bins, WTC_VC out-list entries per item): chunk counts, their scan
@@ -470,11 +471,15 @@ static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b -
 // Degree bins of the resident graph (bins_kernels.hpp) from its CSR offsets
 // on the device; the host receives per-workgroup counts, the degree sums and
 // the largest degree, and sends back each workgroup's offsets.
-static int build_bins(scd_ctx *c) {
+// active_only (batched arenas): the vertices of the graphs still refining
+// (active_g on the device), else all.
+static int build_bins(scd_ctx *c, bool active_only = false) {
     HIP_TRY(hipStreamSynchronize(c->stream));  // the old lists may still be read
     drop_graphs(c);
     release_tmp(c, c->hub_bufs);
     const uint32_t n = c->n, T = c->tune.lds_tile, C = c->tune.merge_chunk, Wm = c->tune.wave_max;
+    const uint32_t *kgid = active_only && c->G ? c->gid : nullptr;
+    c->bins_partial = kgid != nullptr;
     const uint32_t nblk = cdiv(n, BIN_CHUNK);
     std::vector<DevBuf> tmp;
     struct TmpGuard {
@@ -490,7 +495,7 @@ static int build_bins(scd_ctx *c) {
         return rc;
     LAUNCH(c, "bins", 1, 64, k_fill_u64v, esum, (uint64_t)K_NB, 0ull);
     LAUNCH(c, "bins", 1, 64, k_fill_u32, dmax, 1u);
-    LAUNCH(c, "bins", nblk, 256, k_bin_count, c->indices, n, T, Wm, cnt, esum, dmax);
+    LAUNCH(c, "bins", nblk, 256, k_bin_count, c->indices, n, T, Wm, kgid, c->active_g, cnt, esum, dmax);
     if ((rc = check_launch("degree bins"))) return rc;
     std::vector<uint32_t> h_cnt((size_t)nblk * K_NB), h_off((size_t)nblk * K_NB);
     unsigned long long h_esum[K_NB];
@@ -515,7 +520,7 @@ static int build_bins(scd_ctx *c) {
     c->dmax = h_dmax;
     if ((rc = dalloc(c, &c->bin_verts, n, &c->hub_bufs))) return rc;
     HIP_TRY(hipMemcpyAsync(offs, h_off.data(), 4ull * nblk * K_NB, hipMemcpyHostToDevice, c->stream));
-    LAUNCH(c, "bins", nblk, 256, k_bin_scatter, c->indices, n, T, Wm, offs, c->bin_verts);
+    LAUNCH(c, "bins", nblk, 256, k_bin_scatter, c->indices, n, T, Wm, kgid, c->active_g, offs, c->bin_verts);
     if ((rc = check_launch("degree bins"))) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
 
@@ -2644,11 +2649,27 @@ extern "C" int scd_batch_refine(scd_ctx *c, const double *initial_wcc, uint32_t 
     std::vector<double> best(initial_wcc, initial_wcc + G), done(G, 0.0);
     std::vector<uint32_t> active(G, 1u), store(G, 0u), iters(G, 0u);
     uint32_t remaining = G, it = 0;
+    // vertices of the graphs still refining, and in the degree bins
+    uint64_t active_v = c->n, binned_v = c->n;
+    // on an error: every vertex binned again (best effort), the error kept
+    auto unbin = [c](int err) {
+        if (c->bins_partial) (void)build_bins(c, false);
+        return err;
+    };
     const double t0 = wall_ms();
     while (remaining > 0 && it < max_iters) {
         it++;
         HIP_TRY(hipMemcpyAsync(c->active_g, active.data(), 4ull * G, hipMemcpyHostToDevice, c->stream));
-        if ((rc = scd_improve(c, nullptr, nullptr))) return rc;
+        if (2 * active_v <= binned_v) {
+            // half of the binned vertices belong to stopped graphs: bin only
+            // the others (every kernel over the bins then skips the stopped
+            // graphs at no cost), with the stopped graphs' labels in both
+            // label buffers
+            HIP_TRY(hipMemcpyAsync(c->next, c->cur, 4ull * c->n, hipMemcpyDeviceToDevice, c->stream));
+            if ((rc = build_bins(c, true))) return unbin(rc);
+            binned_v = active_v;
+        }
+        if ((rc = scd_improve(c, nullptr, nullptr))) return unbin(rc);
         const double now = wall_ms() - t0;
         bool any_store = false;
         for (uint32_t g = 0; g < G; g++) {
@@ -2664,6 +2685,7 @@ extern "C" int scd_batch_refine(scd_ctx *c, const double *initial_wcc, uint32_t 
             if (w < best[g] * (1 + threshold)) {
                 active[g] = 0;
                 remaining--;
+                active_v -= c->h_vtx_off[g + 1] - c->h_vtx_off[g];
                 done[g] = now;
             } else {
                 best[g] = w;
@@ -2672,7 +2694,7 @@ extern "C" int scd_batch_refine(scd_ctx *c, const double *initial_wcc, uint32_t 
         if (any_store) {
             HIP_TRY(hipMemcpyAsync(c->store_g, store.data(), 4ull * G, hipMemcpyHostToDevice, c->stream));
             LAUNCH(c, "store_best", cdiv(c->n, 256), 256, k_store_best_g, c->cur, c->gid, c->store_g, c->n, c->best);
-            if ((rc = check_launch("store best")) || (rc = finish(c))) return rc;
+            if ((rc = check_launch("store best")) || (rc = finish(c))) return unbin(rc);
         }
     }
     const double end = wall_ms() - t0;
@@ -2681,9 +2703,13 @@ extern "C" int scd_batch_refine(scd_ctx *c, const double *initial_wcc, uint32_t 
     if (best_out) memcpy(best_out, best.data(), 8ull * G);
     if (iters_out) memcpy(iters_out, iters.data(), 4ull * G);
     if (done_ms) memcpy(done_ms, done.data(), 8ull * G);
-    // leave every graph active for later calls
+    // leave every graph active and binned for later calls
     std::vector<uint32_t> ones(G, 1u);
     HIP_TRY(hipMemcpyAsync(c->active_g, ones.data(), 4ull * G, hipMemcpyHostToDevice, c->stream));
+    if (c->bins_partial) {
+        HIP_TRY(hipMemcpyAsync(c->next, c->cur, 4ull * c->n, hipMemcpyDeviceToDevice, c->stream));
+        if ((rc = build_bins(c, false))) return rc;
+    }
     return finish(c);
 }
 

@@ -100,3 +100,23 @@ def test_graphs_labels_one_download(small_batch):
                     assert np.array_equal(lab, full[best][off[g]:off[g + 1]]), (sel, g, best)
         with pytest.raises(scd_amd.ScdError):
             be.graph_labels([a.count])
+
+
+def test_refine_again_after_bins_shrank(small_batch):
+    """scd_batch_refine bins only the graphs still refining once half of the
+    binned vertices belong to stopped graphs, and bins every vertex again at
+    its end: a second pass from a new initial partition repeats the first,
+    and the current labels of every graph equal its labels at its last
+    iteration in both."""
+    a = small_batch
+    with scd_amd.BatchEngine(device=0) as be:
+        be.upload(a)
+        runs = []
+        for _ in range(2):
+            be.initial_partition(want_labels=False)
+            w0 = be.compute_wcc()
+            best, iters, _, hist = be.refine(w0, max_iters=100, history=True)
+            runs.append((w0, best, iters, hist, be.labels(), be.current_labels()))
+    for x, y in zip(runs[0], runs[1]):
+        assert np.array_equal(x, y)
+    assert len(set(runs[0][2].tolist())) > 1  # graphs stopped at different iterations
