@@ -901,7 +901,7 @@ static int count_internal_triangles(scd_ctx *c, const WccArgs &w0) {
         d.units = c->dn_units;
         LAUNCH(c, "dn_clear", 1, 64, k_fill_u32, c->dn_ctr, DN_CTR);
         LAUNCH(c, "dn_classify", cdiv(c->num_comms, 256), 256, k_dn_classify, d, c->num_comms);
-        LAUNCH(c, "dn_lid", cdiv(c->n, 256), 256, k_dn_lid, d, c->n);
+        LAUNCH(c, "dn_lid", cdiv(c->n, 256), 256, k_dn_lid, w, d, c->n);
         w.dn_off = c->dn_off;
     }
     LAUNCH(c, "wcc_dplus", cdiv(c->n, c->block_size), c->block_size, k_wtc_dplus, w, c->n);
@@ -914,6 +914,7 @@ static int count_internal_triangles(scd_ctx *c, const WccArgs &w0) {
         const uint32_t grid = std::min<uint32_t>(8 * c->num_cus, cdiv(c->n, 4));
         LAUNCH(c, "dn_rows", grid, 256, k_dn_rows, w, d);
         LAUNCH(c, "dn_count", grid, 256, k_dn_count, w, d);
+        LAUNCH(c, "dn_halve", std::min<uint32_t>(4 * c->num_cus, cdiv(c->n, 256)), 256, k_dn_halve, w, d);
     }
     on_lane(c, 3);
     LAUNCH(c, "wcc_tri_sub8", cdiv(bin_n(c, K_SUB8), 32), 256, k_wtc_group<8>, w, bin_v(c, K_SUB8),

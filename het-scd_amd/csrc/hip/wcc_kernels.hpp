@@ -720,9 +720,9 @@ __global__ void k_dn_classify(DenseArgs d, uint32_t num_comms) {
     d.off[L] = off;
 }
 
-// Local ids and list slots of the dense-community vertices: per wave, one
-// counter update per distinct label.
-__global__ __launch_bounds__(256) void k_dn_lid(DenseArgs d, uint32_t n) {
+// Local ids and list slots of the dense-community vertices (T and td
+// cleared): per wave, one counter update per distinct label.
+__global__ __launch_bounds__(256) void k_dn_lid(WccArgs a, DenseArgs d, uint32_t n) {
     const uint32_t p = blockIdx.x * 256u + threadIdx.x;
     const int lane = threadIdx.x & 63;
     uint32_t L = SCD_INVALID;
@@ -748,6 +748,8 @@ __global__ __launch_bounds__(256) void k_dn_lid(DenseArgs d, uint32_t n) {
         const uint32_t id = b + my_rank;
         d.lid[p] = id;
         d.list[d.vbase[L] + id] = p;
+        a.T[p] = 0;
+        a.td[p] = 0;
     }
 }
 
@@ -779,16 +781,35 @@ __global__ __launch_bounds__(256) void k_dn_rows(WccArgs a, DenseArgs d) {
     }
 }
 
-// T and td of every listed vertex, a wave per vertex, DN_U neighbour rows
-// in flight (2: 4,096 planted graphs, 164 ms for the second iteration
-// against 176 with 4, whose registers cost occupancy, and 165 with 1).  Rows of W <= 64 words: the wave splits into 64 / Wp groups
-// (Wp = W rounded up to a power of two), each ANDing one neighbour's row
-// with its own copy of row(p) per load; longer rows: every lane holds up to
-// four words of row(p), and the neighbour is the same for the whole wave.
+// Sum over the wave, in every lane.
+__device__ __forceinline__ uint32_t dn_wave_sum(uint32_t x) {
+    x += lane_xor<1>(x);
+    x += lane_xor<2>(x);
+    x += lane_xor<4>(x);
+    x += lane_xor<8>(x);
+    x += lane_xor<16>(x);
+    x += lane_xor<32>(x);
+    return x;
+}
+
+// T (the sum of c(p, q), halved by k_dn_halve) and td of every listed
+// vertex, a wave per vertex, DN_U neighbour rows in flight (2: 4,096
+// planted graphs, 164 ms for the second iteration against 176 with 4, whose
+// registers cost occupancy, and 165 with 1).  Rows of W <= 64 words: the
+// wave splits into 64 / Wp groups (Wp = W rounded up to a power of two),
+// each ANDing one neighbour's row with its own copy of row(p) per load.
+// Longer rows (communities of more than 4,096 members, which take most of
+// the bytes): every lane holds up to four words of row(p), the neighbour is
+// the same for the whole wave, and each internal edge is counted once, from
+// its end with the lower local id, which adds c(p, q) to q's counts by
+// atomics: half the row reads (4,096 planted graphs 574 -> 547 ms; the same
+// for the short rows, with a group sum per neighbour, took it to 598 ms).
 constexpr int DN_U = 2;
 
 __global__ __launch_bounds__(256) void k_dn_count(WccArgs a, DenseArgs d) {
+    __shared__ uint32_t s_qv[4][64], s_ql[4][64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
     const uint32_t nd = d.ctr[1];
     for (uint32_t slot = blockIdx.x * 4u + wv; slot < nd; slot += gridDim.x * 4u) {  // uniform per wave
         const uint32_t p = d.list[slot];
@@ -837,32 +858,58 @@ __global__ __launch_bounds__(256) void k_dn_count(WccArgs a, DenseArgs d) {
                 mine[j] = k < W ? rp[k] : 0ull;
             }
             const uint32_t nj = (W + 63u) >> 6;  // words per lane in use (2..4)
+            const uint32_t lp = d.lid[p];
+            uint32_t *sqv = s_qv[wv], *sql = s_ql[wv];
             for (uint32_t c0 = 0; c0 < dg; c0 += 64) {
-                const uint32_t nq = min(64u, dg - c0);
-                const uint32_t ql = (uint32_t)lane < nq ? d.lid[a.int_adj[beg + c0 + lane]] : 0u;
-                for (uint32_t i0 = 0; i0 < nq; i0 += DN_U) {
+                // the neighbours above p in local id, compacted to the first nk lanes
+                uint32_t qv = 0, ql = 0;
+                bool keep = false;
+                if (c0 + (uint32_t)lane < dg) {
+                    qv = a.int_adj[beg + c0 + lane];
+                    ql = d.lid[qv];
+                    keep = ql > lp;
+                }
+                const uint64_t km = __ballot(keep);
+                const uint32_t nk = (uint32_t)__popcll(km);
+                if (keep) {
+                    sqv[__popcll(km & below)] = qv;
+                    sql[__popcll(km & below)] = ql;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                const uint32_t kv = (uint32_t)lane < nk ? sqv[lane] : 0u;
+                const uint32_t kl = (uint32_t)lane < nk ? sql[lane] : 0u;
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                __builtin_amdgcn_wave_barrier();  // the next chunk refills sqv / sql
+                for (uint32_t i0 = 0; i0 < nk; i0 += DN_U) {
                     unsigned long long xv[DN_U][NJ];
 #pragma unroll
                     for (int u = 0; u < DN_U; u++) {
-                        const uint32_t i = min(i0 + u, nq - 1u);  // uniform
-                        const uint32_t l = (uint32_t)__builtin_amdgcn_readlane((int)ql, (int)i);
+                        const uint32_t i = min(i0 + u, nk - 1u);  // uniform
+                        const uint32_t l = (uint32_t)__builtin_amdgcn_readlane((int)kl, (int)i);
                         const unsigned long long *rq = rows + (size_t)l * W;
 #pragma unroll
                         for (int j = 0; j < NJ; j++) {
                             const uint32_t k = (uint32_t)lane + 64u * j;
-                            xv[u][j] = ((uint32_t)j < nj && i0 + u < nq && k < W) ? rq[k] : 0ull;
+                            xv[u][j] = ((uint32_t)j < nj && i0 + u < nk && k < W) ? rq[k] : 0ull;
                         }
                     }
 #pragma unroll
                     for (int u = 0; u < DN_U; u++) {
-                        unsigned long long any = 0;
+                        if (i0 + u >= nk) break;  // uniform
+                        uint32_t local = 0;
 #pragma unroll
-                        for (int j = 0; j < NJ; j++) {
-                            const unsigned long long v = xv[u][j] & mine[j];
-                            pc += (uint32_t)__popcll(v);
-                            any |= v;
+                        for (int j = 0; j < NJ; j++) local += (uint32_t)__popcll(xv[u][j] & mine[j]);
+                        const uint32_t c = dn_wave_sum(local);
+                        pc += local;
+                        if (c) {
+                            td++;
+                            if (lane == 0) {
+                                const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)kv, (int)(i0 + u));
+                                atomicAdd(&a.T[q], c);
+                                atomicAdd(&a.td[q], 1u);
+                            }
                         }
-                        td += (i0 + u < nq && __ballot(any != 0ull)) ? 1u : 0u;
                     }
                 }
             }
@@ -870,9 +917,18 @@ __global__ __launch_bounds__(256) void k_dn_count(WccArgs a, DenseArgs d) {
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) pc += __shfl_xor(pc, o, SCD_WAVE);
         if (lane == 0) {
-            a.T[p] = pc / 2u;
-            a.td[p] = td;
+            if (pc) atomicAdd(&a.T[p], pc);
+            if (td) atomicAdd(&a.td[p], td);
         }
+    }
+}
+
+// T[p] = (sum over q of c(p, q)) / 2 for the listed vertices.
+__global__ void k_dn_halve(WccArgs a, DenseArgs d) {
+    const uint32_t nd = d.ctr[1];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nd; i += gridDim.x * blockDim.x) {
+        const uint32_t p = d.list[i];
+        a.T[p] >>= 1;
     }
 }
 
