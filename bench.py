@@ -236,13 +236,18 @@ def make_graph(scale, scd_amd, np, on_device):
     return edges, g
 
 
-def dropin_run(path, dev=0):
+def dropin_run(path, dev=0, tune_cache=None):
     """The reference's unchanged src/main-gpu.cpp on libscd_dropin.so
-    (oracle/_ref/main-gpu-amd): its time per iteration and phase lines."""
+    (oracle/_ref/main-gpu-amd): its time per iteration and phase lines.
+    tune_cache: the autotuner's file with this graph's size class
+    (SCD_TUNE_CACHE), so the drop-in runs the timed steps' kernel shapes."""
     exe = os.path.join(ROOT, "oracle", "_ref", "main-gpu-amd")
     if not os.path.exists(exe):
         return None
-    r = subprocess.run([exe, path, str(dev)], capture_output=True, text=True, timeout=600)
+    env = dict(os.environ)
+    if tune_cache:
+        env["SCD_TUNE_CACHE"] = tune_cache
+    r = subprocess.run([exe, path, str(dev)], capture_output=True, text=True, timeout=600, env=env)
     if r.returncode != 0:
         return {"error": r.stderr[-500:]}
     tpi = re.search(r"time per iteration: ([0-9.eE+-]+) ms", r.stdout)
@@ -255,7 +260,10 @@ def dropin_run(path, dev=0):
     return {"ms_per_iteration": float(tpi.group(1)) if tpi else None, "iterations": int(it.group(1)) if it else None,
             "phase_ms_summed": phases,
             "note": "time per iteration = src/main-gpu.cpp's loop time / iterations: phase-stamped iteration, "
-                    "best-label copy, and the driver's own host work"}
+                    "best-label copy, and the driver's own host work; kernel shapes of the timed steps "
+                    "(SCD_TUNE_CACHE)" if tune_cache else
+                    "time per iteration = src/main-gpu.cpp's loop time / iterations: phase-stamped iteration, "
+                    "best-label copy, and the driver's own host work; default kernel shapes"}
 
 
 def refine_leg(args, scale, scd_amd, np, on_device, steps, warmup, cpu_reps, want_tc=True, want_dropin=True):
@@ -276,12 +284,15 @@ def refine_leg(args, scale, scd_amd, np, on_device, steps, warmup, cpu_reps, wan
     t_ip = time.perf_counter() - t
     eng.set_profiling(False)
     t = time.perf_counter()
+    tune_file = None
     if args.tuning:
         eng.set_tuning(**json.loads(args.tuning))
         tune = eng.get_tuning()
         log(f"[bench] tuning set by --tuning: {tune}")
     else:
-        tune, tune_ms = eng.autotune(reps=3)
+        tune_fd, tune_file = tempfile.mkstemp(prefix="scd_tune_", suffix=".txt")
+        os.close(tune_fd)
+        tune, tune_ms = eng.autotune(cache_path=tune_file, reps=3)
         log(f"[bench] autotune {time.perf_counter() - t:.1f} s -> {tune} ({tune_ms:.3f} ms per iteration)")
     t_tune = time.perf_counter() - t
     eng.set_profiling(True)
@@ -393,9 +404,11 @@ def refine_leg(args, scale, scd_amd, np, on_device, steps, warmup, cpu_reps, wan
         if (not args.no_cpu) or want_dropin:
             g.write(path)
         if want_dropin:
-            dropin = dropin_run(path)
+            dropin = dropin_run(path, tune_cache=tune_file)
             if dropin:
                 log(f"[bench] reference main-gpu.cpp on libscd_dropin.so: {dropin}")
+        if tune_file:
+            os.unlink(tune_file)
         if not args.no_cpu:
             cpu = cpu_baseline(path, g.m, cpu_reps, f"R-MAT s{scale}")
             if cpu:

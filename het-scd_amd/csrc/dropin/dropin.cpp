@@ -73,6 +73,11 @@ bool cuda_transfer_graph(const Graph *g, double alpha) {
 
 void cuda_labels_to_device(unsigned int *labels) {
     must(scd_labels_to_device(g_ctx, labels), "cuda_labels_to_device");
+    // with SCD_TUNE_CACHE naming the autotuner's text file, the kernel
+    // shapes of this graph's size class from it (measured and appended
+    // when the class is new; the labels stay as they are)
+    if (const char *tc = getenv("SCD_TUNE_CACHE"))
+        if (*tc) must(scd_autotune(g_ctx, tc, 3, nullptr, nullptr), "cuda_labels_to_device");
     // the driver's loop starts next: record its iterations now
     must(scd_capture_iterations(g_ctx), "cuda_labels_to_device");
 }

@@ -50,13 +50,20 @@ def parse(out):
 
 @pytest.mark.skipif(not (os.path.exists(DROPIN_MAIN) and os.path.exists(CPU_MAIN)),
                     reason="oracle/_ref not built (needs the reference tree at build time)")
-@pytest.mark.parametrize("name", ["c1", "rmat12", "c2"])
-def test_unchanged_reference_main_gpu_matches_reference_cpu(name, tmp_path):
+@pytest.mark.parametrize("name,tuned", [("c1", False), ("rmat12", False), ("c2", False), ("c2", True)])
+def test_unchanged_reference_main_gpu_matches_reference_cpu(name, tuned, tmp_path):
+    """tuned: SCD_TUNE_CACHE names an (empty) autotuner file, so the drop-in
+    tunes the graph's size class, appends it and runs with it."""
     g = graph(name)
     path = str(tmp_path / f"{name}.bin")
     g.write(path)
-    print(f"[gpu] reference main-gpu.cpp on libscd_dropin.so: {name} n={g.n} m={g.m}", flush=True)
-    gpu = subprocess.run([DROPIN_MAIN, path, "0"], capture_output=True, text=True, timeout=300)
+    env = dict(os.environ)
+    env.pop("SCD_TUNE_CACHE", None)
+    cache = tmp_path / "tune.txt"
+    if tuned:
+        env["SCD_TUNE_CACHE"] = str(cache)
+    print(f"[gpu] reference main-gpu.cpp on libscd_dropin.so: {name} n={g.n} m={g.m} tuned={tuned}", flush=True)
+    gpu = subprocess.run([DROPIN_MAIN, path, "0"], capture_output=True, text=True, timeout=300, env=env)
     assert gpu.returncode == 0, gpu.stderr
     cpu = subprocess.run([CPU_MAIN, path], capture_output=True, text=True, timeout=300,
                          env=dict(os.environ, OMP_NUM_THREADS="1"))
@@ -75,6 +82,8 @@ def test_unchanged_reference_main_gpu_matches_reference_cpu(name, tmp_path):
     assert len(phases) == 4 * b["iterations"]
     assert all(float(v) >= 0 for _, v in phases)
     assert "selected device:" in gpu.stdout and "memory:" in gpu.stdout
+    if tuned:
+        assert len(cache.read_text().splitlines()) == 1  # the size class, measured and appended
     print(f"[gpu] {name}: {a['ms_per_iteration']:.3f} ms per iteration (reference CPU, one thread: "
           f"{b['ms_per_iteration']:.3f})", flush=True)
 
