@@ -764,7 +764,7 @@ static int phase_move(scd_ctx *c, uint32_t *labels_out, uint32_t *ex_sorted, uin
         ht.chunk_rem = c->tchunk_rem;
         ht.chunk_has_rem = c->tchunk_has_rem;
         LAUNCH(c, "hub_eval", c->n_tchunks, 256, k_hub_table_eval, at(K_HUB), ht, t, c->tune.merge_chunk);
-        LAUNCH(c, "hub_finalize", cdiv(c->nh, 64), 64, k_hub_finalize, a, ht, c->nh);
+        LAUNCH(c, "hub_finalize", cdiv(c->nh, 4), 256, k_hub_finalize, a, ht, c->nh);
     } else if (c->nh) {
         // merge passes over the hubs still longer than the pass width; each
         // hub's sorted keys end in hub_a or hub_b by the parity of its own
@@ -778,7 +778,7 @@ static int phase_move(scd_ctx *c, uint32_t *labels_out, uint32_t *ex_sorted, uin
             w *= 2;
         }
         LAUNCH(c, "hub_eval", c->n_chunks, 256, k_hub_eval, at(K_HUB), h, c->hub_a, c->hub_b, c->tune.lds_tile, C);
-        LAUNCH(c, "hub_finalize", cdiv(c->nh, 64), 64, k_hub_finalize, a, h, c->nh);
+        LAUNCH(c, "hub_finalize", cdiv(c->nh, 4), 256, k_hub_finalize, a, h, c->nh);
     }
     on_lane(c, 3);
     LAUNCH(c, "move_sub8", cdiv(bin_n(c, K_SUB8), 32), 256, k_move_sub<8>, at(K_SUB8), bin_v(c, K_SUB8), bin_n(c, K_SUB8));

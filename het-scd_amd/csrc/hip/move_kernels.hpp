@@ -699,21 +699,38 @@ __global__ __launch_bounds__(256) void k_hub_table_eval(MoveArgs a, HubArgs h, H
 }
 
 // One thread per hub: combine the hub's chunks and apply the move rule.
-__global__ __launch_bounds__(64) void k_hub_finalize(MoveArgs a, HubArgs h, uint32_t nh) {
-    const uint32_t hb = blockIdx.x * 64u + threadIdx.x;
-    if (hb >= nh) return;
-    const uint32_t p = h.hub_v[hb];
-    const uint32_t my = a.labels[p];
+// One wave per hub (the largest hubs of R-MAT s24 have hundreds of chunks):
+// the chunks' winners reduce in the total (improvement, label) order, and
+// the own-label improvement comes from the last chunk that has one (there
+// is one: the chunk of the own label's run head).
+__global__ __launch_bounds__(256) void k_hub_finalize(MoveArgs a, HubArgs h, uint32_t nh) {
+    const uint32_t hb = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (hb >= nh) return;  // uniform per wave
+    const int lane = threadIdx.x & 63;
     double bi = 0.0, rem = 0.0;
     uint32_t bl = SCD_INVALID;
-    for (uint32_t c = h.hub_chunk_begin[hb]; c < h.hub_chunk_begin[hb + 1]; c++) {
+    int rem_c = -1;  // the lane's last chunk with an own-label improvement
+    for (uint32_t c = h.hub_chunk_begin[hb] + lane; c < h.hub_chunk_begin[hb + 1]; c += 64) {
         if (better(h.chunk_bi[c], h.chunk_bl[c], bi, bl)) {
             bi = h.chunk_bi[c];
             bl = h.chunk_bl[c];
         }
-        if (h.chunk_has_rem[c]) rem = h.chunk_rem[c];
+        if (h.chunk_has_rem[c]) {
+            rem = h.chunk_rem[c];
+            rem_c = (int)c;
+        }
     }
-    if (a.labels_out) a.labels_out[p] = vtx_result(a, p, move_rule(bi, bl, rem, my), my);
+    group_argmax<64>(bi, bl);
+    int last = rem_c;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) last = max(last, __shfl_xor(last, off, SCD_WAVE));
+    const uint64_t owner = __ballot(rem_c == last && last >= 0);
+    const double r = owner ? __shfl(rem, __builtin_ctzll(owner), SCD_WAVE) : 0.0;
+    if (lane == 0 && a.labels_out) {
+        const uint32_t p = h.hub_v[hb];
+        const uint32_t my = a.labels[p];
+        a.labels_out[p] = vtx_result(a, p, move_rule(bi, bl, r, my), my);
+    }
 }
 
 }  // namespace scd_dev
