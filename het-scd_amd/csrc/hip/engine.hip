@@ -334,6 +334,7 @@ static void free_graph(scd_ctx *c, bool keep_cache = true) {
     c->have_graph = c->have_labels = false;
     c->wcc_valid = false;
     c->indices = c->adj = nullptr;
+    c->have_snap = false;  // a snapshot belongs to its graph
     c->nh = c->n_tiles = c->n_chunks = 0;
 }
 

@@ -15,6 +15,7 @@ from golden_graphs import graph
 pytestmark = pytest.mark.gpu
 
 EINVAL = -1
+ESTATE = -6
 
 
 def damaged(g, where, value):
@@ -139,3 +140,22 @@ def test_server_submit_rejects_out_of_range_entry():
         rid = srv.submit(g)
         srv.run()
         assert srv.result(rid)["status"] == 0
+
+
+def test_snapshot_does_not_outlive_its_graph(engine):
+    """A snapshot belongs to the graph it was taken on: after another graph
+    is uploaded, scd_restore reports that no snapshot exists instead of
+    copying the old graph's labels (which may exceed the new n) into it."""
+    big, small = graph("c1"), graph("pp8x40_s5")
+    assert small.n < big.n
+    engine.transfer_graph(big)
+    lab, _ = orc.initial_partition(big)
+    engine.labels_to_device(lab)
+    engine.snapshot()
+    engine.restore()  # same graph: fine
+    engine.transfer_graph(small)
+    engine.labels_to_device(orc.initial_partition(small)[0])
+    with pytest.raises(scd_amd.ScdError) as ei:
+        engine.restore()
+    assert ei.value.code == ESTATE
+    still_works(engine, small)
