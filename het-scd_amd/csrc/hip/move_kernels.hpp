@@ -524,9 +524,20 @@ __global__ __launch_bounds__(256) void k_hub_tile_sort(MoveArgs a, HubArgs h,
 
 // One merge pass: sorted runs of width w -> 2w inside every hub segment.
 // C (outputs per workgroup) is a power of two with C <= 2w, so a chunk never
-// straddles two run pairs.
+// straddles two run pairs.  The chunk goes HM_SUB outputs at a time: the
+// co-ranks of every sub-chunk boundary first (one merge-path search in
+// global memory each, in parallel), then per sub-chunk its two input runs
+// are read into LDS, merged there four outputs per thread, and written out
+// (coalesced reads and writes instead of per-thread sequential merges in
+// global memory).
+constexpr uint32_t HM_SUB = 1024;
+constexpr uint32_t HM_SPLITS = 32768 / HM_SUB + 1;  // merge_chunk <= 2 x 16384
+
 __global__ __launch_bounds__(256) void k_hub_merge(HubArgs h, const uint32_t *__restrict__ src,
                                                    uint32_t *__restrict__ dst, uint32_t w, uint32_t C) {
+    __shared__ uint32_t s_in[HM_SUB], s_out[HM_SUB];
+    __shared__ uint32_t s_split[HM_SPLITS];
+    const int tid = threadIdx.x;
     const uint32_t hb = h.chunk_hub[blockIdx.x];
     const uint32_t o = h.chunk_start[blockIdx.x];
     const uint32_t d = h.hub_d[hb];
@@ -536,18 +547,36 @@ __global__ __launch_bounds__(256) void k_hub_merge(HubArgs h, const uint32_t *__
     const uint32_t lb = (d - ps > w) ? min(w, d - ps - w) : 0u;
     const uint32_t *A = src + base + ps;
     const uint32_t *B = A + la;
+    uint32_t *D = dst + base + ps;
     const uint32_t out_lo = o - ps;
     const uint32_t out_hi = min(o + C, ps + la + lb) - ps;
-    const uint32_t per = C / 256;
-    const uint32_t t0 = out_lo + threadIdx.x * per;
-    if (t0 >= out_hi) return;
-    const uint32_t t1 = min(t0 + per, out_hi);
-    uint32_t i = merge_path(A, la, B, lb, t0);
-    uint32_t j = t0 - i;
-    uint32_t *D = dst + base + ps;
-    for (uint32_t k = t0; k < t1; k++) {
-        const bool takeA = (j >= lb) || (i < la && A[i] <= B[j]);
-        D[k] = takeA ? A[i++] : B[j++];
+    if (out_lo >= out_hi) return;  // uniform
+    const uint32_t ns = (out_hi - out_lo + HM_SUB - 1) / HM_SUB;
+    for (uint32_t s = tid; s <= ns; s += 256) s_split[s] = merge_path(A, la, B, lb, min(out_lo + s * HM_SUB, out_hi));
+    __syncthreads();
+    for (uint32_t s = 0; s < ns; s++) {
+        const uint32_t d0 = out_lo + s * HM_SUB, d1 = min(d0 + HM_SUB, out_hi);
+        const uint32_t i0 = s_split[s], i1 = s_split[s + 1];
+        const uint32_t na = i1 - i0, nb = (d1 - i1) - (d0 - i0), n = d1 - d0;
+        for (uint32_t k = tid; k < na; k += 256) s_in[k] = A[i0 + k];
+        for (uint32_t k = tid; k < nb; k += 256) s_in[na + k] = B[d0 - i0 + k];
+        __syncthreads();
+        const uint32_t t0 = tid * (HM_SUB / 256);
+        if (t0 < n) {
+            const uint32_t *a = s_in, *b = s_in + na;
+            uint32_t i = merge_path(a, na, b, nb, t0);
+            uint32_t j = t0 - i;
+#pragma unroll
+            for (uint32_t k = 0; k < HM_SUB / 256; k++) {
+                if (t0 + k < n) {
+                    const bool takeA = (j >= nb) || (i < na && a[i] <= b[j]);
+                    s_out[t0 + k] = takeA ? a[i++] : b[j++];
+                }
+            }
+        }
+        __syncthreads();
+        for (uint32_t k = tid; k < n; k += 256) D[d0 + k] = s_out[k];
+        __syncthreads();  // s_in and s_out are reused
     }
 }
 
