@@ -164,6 +164,9 @@ struct scd_ctx {
     uint32_t tc_heavy = 128;  // WccArgs::vc_min of graph triangle counting (0: off)
     uint32_t wcc_heavy = 0;   // the same for the WCC's internal triangles
     uint32_t *wq_sum[4] = {nullptr}, *wq_scan[4] = {nullptr};
+    // the WCC's lists of those vertices with 1..WTC_PACK_LO and more
+    // internal out-neighbours (k_wtc_pack), and their two counters
+    uint32_t *wq_lo = nullptr, *wq_hi = nullptr, *wq_pcnt = nullptr;
 
     // hubs
     uint32_t nh = 0, n_tiles = 0, n_chunks = 0, max_hub_deg = 0;
@@ -533,7 +536,8 @@ static int build_bins(scd_ctx *c, bool active_only = false) {
             (rc = dalloc(c, &c->wq_next, 1, &c->hub_bufs)) ||
             (rc = dalloc(c, &c->vq_nch, c->wq_M, &c->hub_bufs)) ||
             (rc = dalloc(c, &c->vq_off, c->wq_M, &c->hub_bufs)) || (rc = dalloc(c, &c->vq_tot, 1, &c->hub_bufs)) ||
-            (rc = dalloc(c, &c->vq_next, 1, &c->hub_bufs)))
+            (rc = dalloc(c, &c->vq_next, 1, &c->hub_bufs)) || (rc = dalloc(c, &c->wq_lo, c->wq_M, &c->hub_bufs)) ||
+            (rc = dalloc(c, &c->wq_hi, c->wq_M, &c->hub_bufs)) || (rc = dalloc(c, &c->wq_pcnt, 2, &c->hub_bufs)))
             return rc;
         uint64_t len = c->wq_M;
         for (int l = 0; l < 4; l++) {
@@ -905,6 +909,13 @@ static int count_internal_triangles(scd_ctx *c, const WccArgs &w0) {
         LAUNCH(c, "dn_lid", cdiv(c->n, 256), 256, k_dn_lid, w, d, c->n);
         w.dn_off = c->dn_off;
     }
+    // the WCC's internal subgraph: the vertices of degree above 32 mostly
+    // have no or a few internal out-neighbours, so their counting kernels
+    // walk lists of those that have some (k_wtc_pack) instead of a wave per
+    // vertex
+    const bool pack = w.int_deg && c->wq_M;
+    if (pack) w.pack_cnt = c->wq_pcnt;
+    const uint32_t gcap = 8 * c->num_cus;  // grids striding over a packed list
     LAUNCH(c, "wcc_dplus", cdiv(c->n, c->block_size), c->block_size, k_wtc_dplus, w, c->n);
     // lanes: 3 small-degree groups | 2 out-lists up to 64 | 1 dense
     // communities | 0 work queue
@@ -919,15 +930,23 @@ static int count_internal_triangles(scd_ctx *c, const WccArgs &w0) {
     }
     on_lane(c, 3);
     LAUNCH(c, "wcc_tri_sub8", cdiv(bin_n(c, K_SUB8), 32), 256, k_wtc_group<8>, w, bin_v(c, K_SUB8),
-           bin_n(c, K_SUB8));
+           bin_n(c, K_SUB8), nullptr);
     LAUNCH(c, "wcc_tri_sub16", cdiv(bin_n(c, K_SUB16), 16), 256, k_wtc_group<16>, w, bin_v(c, K_SUB16),
-           bin_n(c, K_SUB16));
+           bin_n(c, K_SUB16), nullptr);
     LAUNCH(c, "wcc_tri_sub32", cdiv(bin_n(c, K_SUB32), 8), 256, k_wtc_group<32>, w, bin_v(c, K_SUB32),
-           bin_n(c, K_SUB32));
+           bin_n(c, K_SUB32), nullptr);
     if (c->wq_M) {
         const uint32_t M = c->wq_M;
         on_lane(c, 2);
-        LAUNCH(c, "wcc_tri_wave", cdiv(M, 4), 256, k_wtc_group<64>, w, bin_v(c, K_W1), M);
+        if (pack) {
+            LAUNCH(c, "wcc_tri_pack", cdiv(M, WTC_PACK_NT), WTC_PACK_NT, k_wtc_pack, w, bin_v(c, K_W1), M, c->wq_lo,
+                   c->wq_hi);
+            LAUNCH(c, "wcc_tri_wave", std::min(cdiv(M, 32), gcap), 256, k_wtc_group<8>, w, c->wq_lo, M, c->wq_pcnt);
+            LAUNCH(c, "wcc_tri_wave", std::min(cdiv(M, 4), gcap), 256, k_wtc_group<64>, w, c->wq_hi, M,
+                   c->wq_pcnt + 1);
+        } else {
+            LAUNCH(c, "wcc_tri_wave", cdiv(M, 4), 256, k_wtc_group<64>, w, bin_v(c, K_W1), M, nullptr);
+        }
         on_lane(c, 0);
         LAUNCH(c, "wcc_tri_chunks", cdiv(M, 256), 256, k_wtc_count_chunks<false>, w, bin_v(c, K_W1), M, c->wq_nch,
                c->wq_next);
@@ -951,19 +970,23 @@ static int count_internal_triangles(scd_ctx *c, const WccArgs &w0) {
     const int hub_lds = w.int_deg == nullptr ? 1 : 0;
     const uint32_t fin_max = hub_lds ? WTC_FIN_BLOCKS_PER_CU * c->num_cus : 0xFFFFFFFFu;
     LAUNCH(c, "wcc_tri_final", std::min(cdiv(bin_n(c, K_SUB8), 32), fin_max), 256, k_wtc_final<8>, w,
-           bin_v(c, K_SUB8), bin_n(c, K_SUB8), hub_lds);
+           bin_v(c, K_SUB8), bin_n(c, K_SUB8), hub_lds, nullptr);
     on_lane(c, 2);
     LAUNCH(c, "wcc_tri_final", std::min(cdiv(bin_n(c, K_SUB16), 16), fin_max), 256, k_wtc_final<16>, w,
-           bin_v(c, K_SUB16), bin_n(c, K_SUB16), hub_lds);
+           bin_v(c, K_SUB16), bin_n(c, K_SUB16), hub_lds, nullptr);
     on_lane(c, 1);
     LAUNCH(c, "wcc_tri_final", std::min(cdiv(bin_n(c, K_SUB32), 8), fin_max), 256, k_wtc_final<32>, w,
-           bin_v(c, K_SUB32), bin_n(c, K_SUB32), hub_lds);
+           bin_v(c, K_SUB32), bin_n(c, K_SUB32), hub_lds, nullptr);
     on_lane(c, 0);
-    if (!hub_lds && c->wq_M >= (1u << 19))  // a large arena's WCC: 64 list entries per wave
-        LAUNCH(c, "wcc_tri_final", cdiv(c->wq_M, 256), 256, k_wtc_final_wave, w, bin_v(c, K_W1), c->wq_M);
-    else
+    if (pack) {
+        const uint32_t M = c->wq_M;
+        LAUNCH(c, "wcc_tri_final", std::min(cdiv(M, 32), gcap), 256, k_wtc_final<8>, w, c->wq_lo, M, 0, c->wq_pcnt);
+        LAUNCH(c, "wcc_tri_final", std::min(cdiv(M, 4), gcap), 256, k_wtc_final<64>, w, c->wq_hi, M, 0,
+               c->wq_pcnt + 1);
+    } else {
         LAUNCH(c, "wcc_tri_final", std::min(cdiv(c->wq_M, 4), fin_max), 256, k_wtc_final<64>, w, bin_v(c, K_W1),
-               c->wq_M, hub_lds);
+               c->wq_M, hub_lds, nullptr);
+    }
     join_lanes(c);
     LAUNCH(c, "wcc_tri_hot", cdiv(w.hot_n, 256), 256, k_wtc_hot_sum, w);
     return SCD_OK;

@@ -61,6 +61,9 @@ struct WccArgs {
     // dense (bitmap) kernels below, DN_NONE for the others.  Members of a
     // dense community skip the list kernels (their out-lists count as empty).
     const uint32_t *__restrict__ dn_off = nullptr;
+    // optional: the two counters of k_wtc_pack's lists, zeroed by
+    // k_wtc_dplus
+    uint32_t *__restrict__ pack_cnt = nullptr;
 };
 
 constexpr uint32_t WTC_HOT_MAX = 8192;
@@ -133,6 +136,7 @@ __global__ void k_mark_dirty(const uint32_t *__restrict__ old_labels, const uint
 
 __global__ void k_wtc_dplus(WccArgs a, uint32_t n) {
     const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u == 0 && a.pack_cnt) a.pack_cnt[0] = a.pack_cnt[1] = 0;
     if (u >= n) return;
     if (a.dirty) {
         const uint32_t lab = a.labels[u];
@@ -171,80 +175,87 @@ __global__ void k_wtc_dplus(WccArgs a, uint32_t n) {
 constexpr int WTC_UF = 4;                // loads in flight per lane, flattened walks
 constexpr uint32_t WTC_GROUP_MAX = 64;   // longer out-lists go to the work queue
 
+// nv_dev (optional): the list's length on the device (a k_wtc_pack list);
+// the grid then strides over it.
 template <int S>
-__global__ __launch_bounds__(256) void k_wtc_group(WccArgs a, const uint32_t *__restrict__ verts, uint32_t nv) {
+__global__ __launch_bounds__(256) void k_wtc_group(WccArgs a, const uint32_t *__restrict__ verts, uint32_t nv,
+                                                   const uint32_t *__restrict__ nv_dev) {
     __shared__ uint32_t s_n[256], s_vb[256], s_cnt[256], s_inc[256];
     const int tid = threadIdx.x, lane = tid & 63;
     const int lig = lane & (S - 1), gb = tid & ~(S - 1);  // group base in the block's arrays
-    const uint32_t slot = (blockIdx.x * 256u + tid) / S;
-    uint32_t u = 0, beg = 0, dp = 0;
-    if (slot < nv) {
-        u = verts[slot];
-        dp = a.dplus[u];
-        if (dp > (uint32_t)S) dp = 0;  // work queue
-        if (dp) beg = a.indices[u];
-    }
-    if (!__syncthreads_or(dp != 0)) return;  // nothing to count in this workgroup (uniform)
-    uint32_t x = SCD_INVALID, len = 0, vb = 0;
-    if ((uint32_t)lig < dp) {
-        x = a.int_adj[beg + lig];
-        len = a.dplus[x];
-        if (wtc_from_v(a, len, dp)) len = 0;  // counted from x's side
-        vb = a.indices[x];
-    }
-    uint32_t inc = len;  // inclusive prefix over the group
+    const uint32_t count = nv_dev ? *nv_dev : nv;
+    for (uint32_t b0 = blockIdx.x * (256u / S); b0 < count; b0 += gridDim.x * (256u / S)) {  // uniform
+        const uint32_t slot = b0 + (uint32_t)tid / S;
+        uint32_t u = 0, beg = 0, dp = 0;
+        if (slot < count) {
+            u = verts[slot];
+            dp = a.dplus[u];
+            if (dp > (uint32_t)S) dp = 0;  // work queue
+            if (dp) beg = a.indices[u];
+        }
+        if (!__syncthreads_or(dp != 0)) continue;  // nothing to count in this pass (uniform)
+        uint32_t x = SCD_INVALID, len = 0, vb = 0;
+        if ((uint32_t)lig < dp) {
+            x = a.int_adj[beg + lig];
+            len = a.dplus[x];
+            if (wtc_from_v(a, len, dp)) len = 0;  // counted from x's side
+            vb = a.indices[x];
+        }
+        uint32_t inc = len;  // inclusive prefix over the group
 #pragma unroll
-    for (int o = 1; o < S; o <<= 1) {
-        const uint32_t t = __shfl_up(inc, o, SCD_WAVE);
-        if (lig >= o) inc += t;
-    }
-    s_n[tid] = x;
-    s_vb[tid] = vb;
-    s_cnt[tid] = 0;
-    s_inc[tid] = inc;
-    __syncthreads();
-    const uint32_t W = s_inc[gb + S - 1];
-    uint32_t tot = 0;
-    uint32_t cur = 0;  // owner of the lane's entries: the lane's f only grows
-    for (uint32_t f0 = lig; f0 < W; f0 += S * WTC_UF) {
-        uint32_t w[WTC_UF], at[WTC_UF], own[WTC_UF];
+        for (int o = 1; o < S; o <<= 1) {
+            const uint32_t t = __shfl_up(inc, o, SCD_WAVE);
+            if (lig >= o) inc += t;
+        }
+        s_n[tid] = x;
+        s_vb[tid] = vb;
+        s_cnt[tid] = 0;
+        s_inc[tid] = inc;
+        __syncthreads();
+        const uint32_t W = s_inc[gb + S - 1];
+        uint32_t tot = 0;
+        uint32_t cur = 0;  // owner of the lane's entries: the lane's f only grows
+        for (uint32_t f0 = lig; f0 < W; f0 += S * WTC_UF) {
+            uint32_t w[WTC_UF], at[WTC_UF], own[WTC_UF];
 #pragma unroll
-        for (int q = 0; q < WTC_UF; q++) {
-            const uint32_t f = f0 + S * q;
-            w[q] = SCD_INVALID;
-            if (f < W) {
-                while (s_inc[gb + cur] <= f) cur++;  // smallest k with inc[k] > f
-                own[q] = cur;
-                const uint32_t excl = cur ? s_inc[gb + cur - 1] : 0u;
-                at[q] = s_vb[gb + cur] + (f - excl);
-                w[q] = a.int_adj[at[q]];
+            for (int q = 0; q < WTC_UF; q++) {
+                const uint32_t f = f0 + S * q;
+                w[q] = SCD_INVALID;
+                if (f < W) {
+                    while (s_inc[gb + cur] <= f) cur++;  // smallest k with inc[k] > f
+                    own[q] = cur;
+                    const uint32_t excl = cur ? s_inc[gb + cur - 1] : 0u;
+                    at[q] = s_vb[gb + cur] + (f - excl);
+                    w[q] = a.int_adj[at[q]];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < WTC_UF; q++) {
+                if (w[q] == SCD_INVALID) continue;
+                uint32_t lo = 0, hi = dp;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (s_n[gb + mid] < w[q]) lo = mid + 1;
+                    else hi = mid;
+                }
+                if (lo < dp && s_n[gb + lo] == w[q]) {
+                    atomicAdd(&s_cnt[gb + own[q]], 1u);
+                    atomicAdd(&s_cnt[gb + lo], 1u);
+                    a.flag[at[q]] = 1;
+                    tot++;
+                }
             }
         }
-#pragma unroll
-        for (int q = 0; q < WTC_UF; q++) {
-            if (w[q] == SCD_INVALID) continue;
-            uint32_t lo = 0, hi = dp;
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (s_n[gb + mid] < w[q]) lo = mid + 1;
-                else hi = mid;
-            }
-            if (lo < dp && s_n[gb + lo] == w[q]) {
-                atomicAdd(&s_cnt[gb + own[q]], 1u);
-                atomicAdd(&s_cnt[gb + lo], 1u);
-                a.flag[at[q]] = 1;
-                tot++;
-            }
+        __syncthreads();
+        if ((uint32_t)lig < dp && s_cnt[tid]) {
+            wtc_add_T(a, x, s_cnt[tid]);
+            a.flag[beg + lig] = 1;
         }
-    }
-    __syncthreads();
-    if ((uint32_t)lig < dp && s_cnt[tid]) {
-        wtc_add_T(a, x, s_cnt[tid]);
-        a.flag[beg + lig] = 1;
-    }
 #pragma unroll
-    for (int o = S / 2; o > 0; o >>= 1) tot += __shfl_xor(tot, o, SCD_WAVE);
-    if (lig == 0 && tot) atomicAdd(&a.T[u], tot);
+        for (int o = S / 2; o > 0; o >>= 1) tot += __shfl_xor(tot, o, SCD_WAVE);
+        if (lig == 0 && tot) atomicAdd(&a.T[u], tot);
+        __syncthreads();  // the LDS arrays are refilled by the next pass
+    }
 }
 
 // LDS hash set of N+_int(u): slot -> (vertex, position in the out-list).
@@ -277,6 +288,54 @@ constexpr uint32_t WTC_CLAIM = 8;  // items claimed at once by a workgroup
 // side (VC false): chunks of the out-list when it is longer than 64 entries;
 // from v's side (VC true): chunks of the upper list N-(v) (entries past the
 // out-list) of every vertex whose out-list takes edges from its side.
+// The vertices of a list whose internal out-list is not empty, by its
+// length: up to WTC_PACK_LO entries to lo, more to hi, with the counts in
+// a.pack_cnt[0] and [1].  A workgroup of 1024 threads ranks its vertices by
+// wave ballots and reserves its places with one atomic per list (device-
+// scope atomics on one address cost several ns each on this 8-XCD part).
+// Most vertices of degree above 32 have no or a few internal
+// out-neighbours, so the triangle and final-count kernels walk these lists
+// (8 lanes per vertex for lo) instead of a wave per listed vertex.  Order
+// within a list varies between runs; the counts are integer sums.
+constexpr uint32_t WTC_PACK_LO = 8;
+constexpr int WTC_PACK_NT = 1024;
+
+__global__ __launch_bounds__(WTC_PACK_NT) void k_wtc_pack(WccArgs a, const uint32_t *__restrict__ verts,
+                                                          uint32_t M, uint32_t *__restrict__ lo,
+                                                          uint32_t *__restrict__ hi) {
+    constexpr int NW = WTC_PACK_NT / 64;
+    __shared__ uint32_t s_off[2][NW];  // per wave: rank of its first entry, then its list position
+    __shared__ uint32_t s_base[2];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t i = blockIdx.x * (uint32_t)WTC_PACK_NT + threadIdx.x;
+    uint32_t v = 0, dp = 0;
+    if (i < M) {
+        v = verts[i];
+        dp = a.dplus[v];
+    }
+    const bool to_lo = dp != 0 && dp <= WTC_PACK_LO, to_hi = dp > WTC_PACK_LO;
+    const uint64_t m_lo = __ballot(to_lo), m_hi = __ballot(to_hi);
+    if (lane == 0) {
+        s_off[0][w] = (uint32_t)__builtin_popcountll(m_lo);
+        s_off[1][w] = (uint32_t)__builtin_popcountll(m_hi);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const int k = threadIdx.x;
+        uint32_t run = 0;
+        for (int x = 0; x < NW; x++) {
+            const uint32_t c = s_off[k][x];
+            s_off[k][x] = run;
+            run += c;
+        }
+        s_base[k] = run ? atomicAdd(&a.pack_cnt[k], run) : 0u;
+    }
+    __syncthreads();
+    const uint64_t below = (1ull << lane) - 1ull;
+    if (to_lo) lo[s_base[0] + s_off[0][w] + __builtin_popcountll(m_lo & below)] = v;
+    if (to_hi) hi[s_base[1] + s_off[1][w] + __builtin_popcountll(m_hi & below)] = v;
+}
+
 template <bool VC>
 __global__ void k_wtc_count_chunks(WccArgs a, const uint32_t *__restrict__ verts, uint32_t M,
                                    uint32_t *__restrict__ nch, uint32_t *__restrict__ next) {
@@ -586,9 +645,10 @@ constexpr uint32_t WTC_FIN_HOT = 2048;
 constexpr uint32_t WTC_FIN_BLOCKS_PER_CU = 4;
 
 template <int S>
-__global__ __launch_bounds__(256) void k_wtc_final(WccArgs a, const uint32_t *__restrict__ verts, uint32_t nv,
-                                                   int hub_lds) {
+__global__ __launch_bounds__(256) void k_wtc_final(WccArgs a, const uint32_t *__restrict__ verts, uint32_t nv_host,
+                                                   int hub_lds, const uint32_t *__restrict__ nv_dev) {
     __shared__ uint32_t s_td[WTC_FIN_HOT];
+    const uint32_t nv = nv_dev ? *nv_dev : nv_host;
     const int lane = threadIdx.x & 63;
     const int lig = lane & (S - 1);
     constexpr uint32_t G = 256 / S;  // vertices per workgroup pass
@@ -619,41 +679,6 @@ __global__ __launch_bounds__(256) void k_wtc_final(WccArgs a, const uint32_t *__
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < hot; i += 256)
         if (s_td[i]) wtc_add_td(a, i, s_td[i]);
-}
-
-// k_wtc_final for the out-lists of more than 32 entries of a large arena's
-// WCC (no LDS hub counters): a wave takes 64 entries of the list, finds
-// those with a non-empty out-list together (late iterations leave most
-// communities unchanged, their out-lists empty) and walks only those, one
-// after another with all of its lanes.
-__global__ __launch_bounds__(256) void k_wtc_final_wave(WccArgs a, const uint32_t *__restrict__ verts, uint32_t nv) {
-    const int lane = threadIdx.x & 63;
-    const uint32_t step = gridDim.x * 256u;
-    for (uint32_t base = blockIdx.x * 256u + (threadIdx.x & ~63u); base < nv; base += step) {  // wave-uniform
-        const uint32_t slot = base + lane;
-        uint32_t u = 0, dp = 0;
-        if (slot < nv) {
-            u = verts[slot];
-            dp = a.dplus[u];
-        }
-        uint64_t todo = __ballot(dp != 0);
-        while (todo) {  // wave-uniform
-            const int l = __builtin_ctzll(todo);
-            todo &= todo - 1;
-            const uint32_t uu = __shfl(u, l, SCD_WAVE), d = __shfl(dp, l, SCD_WAVE);
-            const uint32_t beg = a.indices[uu];
-            uint32_t mine = 0;
-            for (uint32_t k = lane; k < d; k += 64) {
-                if (a.flag[beg + k]) {
-                    wtc_add_td(a, a.int_adj[beg + k], 1u);
-                    mine++;
-                }
-            }
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off, SCD_WAVE);
-            if (lane == 0 && mine) atomicAdd(&a.td[uu], mine);
-        }
-    }
 }
 
 // ---------------------------------------------------------------------------
