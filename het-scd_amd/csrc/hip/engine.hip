@@ -528,6 +528,11 @@ static int build_bins(scd_ctx *c, bool active_only = false) {
     if ((rc = check_launch("degree bins"))) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
 
+    // the WCC's packed lists (every binned vertex may land in one)
+    if ((rc = dalloc(c, &c->wq_lo, c->bin_begin[K_NB], &c->hub_bufs)) ||
+        (rc = dalloc(c, &c->wq_hi, c->bin_begin[K_NB], &c->hub_bufs)) ||
+        (rc = dalloc(c, &c->wq_pcnt, 2, &c->hub_bufs)))
+        return rc;
     // internal triangle work queue
     c->wq_M = c->bin_begin[K_NB] - c->bin_begin[K_W1];
     if (c->wq_M) {
@@ -536,8 +541,7 @@ static int build_bins(scd_ctx *c, bool active_only = false) {
             (rc = dalloc(c, &c->wq_next, 1, &c->hub_bufs)) ||
             (rc = dalloc(c, &c->vq_nch, c->wq_M, &c->hub_bufs)) ||
             (rc = dalloc(c, &c->vq_off, c->wq_M, &c->hub_bufs)) || (rc = dalloc(c, &c->vq_tot, 1, &c->hub_bufs)) ||
-            (rc = dalloc(c, &c->vq_next, 1, &c->hub_bufs)) || (rc = dalloc(c, &c->wq_lo, c->wq_M, &c->hub_bufs)) ||
-            (rc = dalloc(c, &c->wq_hi, c->wq_M, &c->hub_bufs)) || (rc = dalloc(c, &c->wq_pcnt, 2, &c->hub_bufs)))
+            (rc = dalloc(c, &c->vq_next, 1, &c->hub_bufs)))
             return rc;
         uint64_t len = c->wq_M;
         for (int l = 0; l < 4; l++) {
@@ -909,16 +913,17 @@ static int count_internal_triangles(scd_ctx *c, const WccArgs &w0) {
         LAUNCH(c, "dn_lid", cdiv(c->n, 256), 256, k_dn_lid, w, d, c->n);
         w.dn_off = c->dn_off;
     }
-    // the WCC's internal subgraph: the vertices of degree above 32 mostly
-    // have no or a few internal out-neighbours, so their counting kernels
-    // walk lists of those that have some (k_wtc_pack) instead of a wave per
-    // vertex
-    const bool pack = w.int_deg && c->wq_M;
+    // the WCC's internal subgraph: its vertices mostly have no or a few
+    // internal out-neighbours whatever their degree bin, so the counting
+    // kernels walk lists of those that have some (k_wtc_pack) instead of
+    // S lanes or a wave per binned vertex
+    const bool pack = w.int_deg != nullptr;
+    const uint32_t nb = c->bin_begin[K_NB];  // binned vertices
     if (pack) w.pack_cnt = c->wq_pcnt;
     const uint32_t gcap = 8 * c->num_cus;  // grids striding over a packed list
     LAUNCH(c, "wcc_dplus", cdiv(c->n, c->block_size), c->block_size, k_wtc_dplus, w, c->n);
-    // lanes: 3 small-degree groups | 2 out-lists up to 64 | 1 dense
-    // communities | 0 work queue
+    // lanes: 3 packed lists (the whole graph: small-degree groups) | 2
+    // out-lists up to 64 (whole graph) | 1 dense communities | 0 work queue
     fork_lanes(c);
     if (dense) {
         on_lane(c, 1);
@@ -929,22 +934,23 @@ static int count_internal_triangles(scd_ctx *c, const WccArgs &w0) {
         LAUNCH(c, "dn_halve", std::min<uint32_t>(4 * c->num_cus, cdiv(c->n, 256)), 256, k_dn_halve, w, d);
     }
     on_lane(c, 3);
-    LAUNCH(c, "wcc_tri_sub8", cdiv(bin_n(c, K_SUB8), 32), 256, k_wtc_group<8>, w, bin_v(c, K_SUB8),
-           bin_n(c, K_SUB8), nullptr);
-    LAUNCH(c, "wcc_tri_sub16", cdiv(bin_n(c, K_SUB16), 16), 256, k_wtc_group<16>, w, bin_v(c, K_SUB16),
-           bin_n(c, K_SUB16), nullptr);
-    LAUNCH(c, "wcc_tri_sub32", cdiv(bin_n(c, K_SUB32), 8), 256, k_wtc_group<32>, w, bin_v(c, K_SUB32),
-           bin_n(c, K_SUB32), nullptr);
+    if (pack) {
+        LAUNCH(c, "wcc_tri_pack", cdiv(nb, WTC_PACK_NT), WTC_PACK_NT, k_wtc_pack, w, c->bin_verts, nb, c->wq_lo,
+               c->wq_hi);
+        LAUNCH(c, "wcc_tri_lo", std::min(cdiv(nb, 32), gcap), 256, k_wtc_group<8>, w, c->wq_lo, nb, c->wq_pcnt);
+        LAUNCH(c, "wcc_tri_hi", std::min(cdiv(nb, 4), gcap), 256, k_wtc_group<64>, w, c->wq_hi, nb, c->wq_pcnt + 1);
+    } else {
+        LAUNCH(c, "wcc_tri_sub8", cdiv(bin_n(c, K_SUB8), 32), 256, k_wtc_group<8>, w, bin_v(c, K_SUB8),
+               bin_n(c, K_SUB8), nullptr);
+        LAUNCH(c, "wcc_tri_sub16", cdiv(bin_n(c, K_SUB16), 16), 256, k_wtc_group<16>, w, bin_v(c, K_SUB16),
+               bin_n(c, K_SUB16), nullptr);
+        LAUNCH(c, "wcc_tri_sub32", cdiv(bin_n(c, K_SUB32), 8), 256, k_wtc_group<32>, w, bin_v(c, K_SUB32),
+               bin_n(c, K_SUB32), nullptr);
+    }
     if (c->wq_M) {
         const uint32_t M = c->wq_M;
-        on_lane(c, 2);
-        if (pack) {
-            LAUNCH(c, "wcc_tri_pack", cdiv(M, WTC_PACK_NT), WTC_PACK_NT, k_wtc_pack, w, bin_v(c, K_W1), M, c->wq_lo,
-                   c->wq_hi);
-            LAUNCH(c, "wcc_tri_wave", std::min(cdiv(M, 32), gcap), 256, k_wtc_group<8>, w, c->wq_lo, M, c->wq_pcnt);
-            LAUNCH(c, "wcc_tri_wave", std::min(cdiv(M, 4), gcap), 256, k_wtc_group<64>, w, c->wq_hi, M,
-                   c->wq_pcnt + 1);
-        } else {
+        if (!pack) {
+            on_lane(c, 2);
             LAUNCH(c, "wcc_tri_wave", cdiv(M, 4), 256, k_wtc_group<64>, w, bin_v(c, K_W1), M, nullptr);
         }
         on_lane(c, 0);
@@ -965,27 +971,27 @@ static int count_internal_triangles(scd_ctx *c, const WccArgs &w0) {
     join_lanes(c);
     // edge marks are complete: count them, again on lanes
     fork_lanes(c);
-    on_lane(c, 3);
-    // the whole graph: bounded grids, hub counts summed in LDS per workgroup
-    const int hub_lds = w.int_deg == nullptr ? 1 : 0;
-    const uint32_t fin_max = hub_lds ? WTC_FIN_BLOCKS_PER_CU * c->num_cus : 0xFFFFFFFFu;
-    LAUNCH(c, "wcc_tri_final", std::min(cdiv(bin_n(c, K_SUB8), 32), fin_max), 256, k_wtc_final<8>, w,
-           bin_v(c, K_SUB8), bin_n(c, K_SUB8), hub_lds, nullptr);
-    on_lane(c, 2);
-    LAUNCH(c, "wcc_tri_final", std::min(cdiv(bin_n(c, K_SUB16), 16), fin_max), 256, k_wtc_final<16>, w,
-           bin_v(c, K_SUB16), bin_n(c, K_SUB16), hub_lds, nullptr);
-    on_lane(c, 1);
-    LAUNCH(c, "wcc_tri_final", std::min(cdiv(bin_n(c, K_SUB32), 8), fin_max), 256, k_wtc_final<32>, w,
-           bin_v(c, K_SUB32), bin_n(c, K_SUB32), hub_lds, nullptr);
-    on_lane(c, 0);
     if (pack) {
-        const uint32_t M = c->wq_M;
-        LAUNCH(c, "wcc_tri_final", std::min(cdiv(M, 32), gcap), 256, k_wtc_final<8>, w, c->wq_lo, M, 0, c->wq_pcnt);
-        LAUNCH(c, "wcc_tri_final", std::min(cdiv(M, 4), gcap), 256, k_wtc_final<64>, w, c->wq_hi, M, 0,
+        on_lane(c, 3);
+        LAUNCH(c, "wcc_tri_final", std::min(cdiv(nb, 32), gcap), 256, k_wtc_final<8>, w, c->wq_lo, nb, 0, c->wq_pcnt);
+        on_lane(c, 2);
+        LAUNCH(c, "wcc_tri_final", std::min(cdiv(nb, 4), gcap), 256, k_wtc_final<64>, w, c->wq_hi, nb, 0,
                c->wq_pcnt + 1);
     } else {
+        // the whole graph: bounded grids, hub counts summed in LDS per workgroup
+        const uint32_t fin_max = WTC_FIN_BLOCKS_PER_CU * c->num_cus;
+        on_lane(c, 3);
+        LAUNCH(c, "wcc_tri_final", std::min(cdiv(bin_n(c, K_SUB8), 32), fin_max), 256, k_wtc_final<8>, w,
+               bin_v(c, K_SUB8), bin_n(c, K_SUB8), 1, nullptr);
+        on_lane(c, 2);
+        LAUNCH(c, "wcc_tri_final", std::min(cdiv(bin_n(c, K_SUB16), 16), fin_max), 256, k_wtc_final<16>, w,
+               bin_v(c, K_SUB16), bin_n(c, K_SUB16), 1, nullptr);
+        on_lane(c, 1);
+        LAUNCH(c, "wcc_tri_final", std::min(cdiv(bin_n(c, K_SUB32), 8), fin_max), 256, k_wtc_final<32>, w,
+               bin_v(c, K_SUB32), bin_n(c, K_SUB32), 1, nullptr);
+        on_lane(c, 0);
         LAUNCH(c, "wcc_tri_final", std::min(cdiv(c->wq_M, 4), fin_max), 256, k_wtc_final<64>, w, bin_v(c, K_W1),
-               c->wq_M, hub_lds, nullptr);
+               c->wq_M, 1, nullptr);
     }
     join_lanes(c);
     LAUNCH(c, "wcc_tri_hot", cdiv(w.hot_n, 256), 256, k_wtc_hot_sum, w);
