@@ -246,7 +246,7 @@ struct scd_ctx {
 };
 
 constexpr size_t kTmpCacheBytes = 4ull << 30;
-constexpr double kTuneMargin = 0.03;  // autotuner: required gain over the best so far
+constexpr double kTuneMargin = 0.03;  // autotuner: required gain of the dense-community WCC
 
 template <typename T>
 static int dalloc(scd_ctx *c, T **p, size_t count, std::vector<DevBuf> *list = nullptr) {
@@ -3002,8 +3002,7 @@ extern "C" int scd_autotune(scd_ctx *c, const char *cache_path, int reps, scd_tu
     }
 
     // candidates: only knobs that change a bin or tile for these degrees,
-    // the defaults first; a later candidate must be kTuneMargin faster to
-    // win, so timing noise does not flip the choice between near-equals
+    // the defaults first
     std::vector<uint32_t> Ws = {1024};
     if (dmax > 256) Ws = {1024, 512, 256};
     std::vector<uint32_t> Ts = {2048};
@@ -3059,24 +3058,51 @@ extern "C" int scd_autotune(scd_ctx *c, const char *cache_path, int reps, scd_tu
         release_tmp(c, tmp);
         return rc;
     }
-    // the shapes, then the fastest one with the dense-community WCC
-    for (size_t i = 0; i <= cand.size(); i++) {
-        scd_tuning t = best_t;
-        if (i < cand.size()) t = cand[i];
-        else t.dense_wcc = 1;
-        if ((rc = scd_set_tuning(c, &t)) || (rc = copy_state(false)) || (rc = scd_improve(c, nullptr, nullptr)))
-            break;  // the untimed run captures the iteration's graphs
+    // median of `runs` timed iterations under tuning t (an untimed run
+    // first captures the iteration's graphs)
+    auto time_tuning = [&](const scd_tuning &t, int runs, double *med) -> int {
+        int r = SCD_OK;
+        if ((r = scd_set_tuning(c, &t)) || (r = copy_state(false)) || (r = scd_improve(c, nullptr, nullptr)))
+            return r;
         std::vector<double> ms;
-        for (int r = 0; r < reps && rc == SCD_OK; r++) {
-            if ((rc = copy_state(false))) break;
+        for (int k = 0; k < runs; k++) {
+            if ((r = copy_state(false))) return r;
             const double t0 = wall_ms();
-            rc = scd_improve(c, nullptr, nullptr);
+            if ((r = scd_improve(c, nullptr, nullptr))) return r;
             ms.push_back(wall_ms() - t0);
         }
-        if (rc) break;
         std::sort(ms.begin(), ms.end());
-        const double med = ms[ms.size() / 2];
-        if (med < best_ms * (1.0 - kTuneMargin)) {
+        *med = ms[ms.size() / 2];
+        return SCD_OK;
+    };
+    // every shape once, then the three fastest again with twice the runs
+    // (the fastest median of those wins: shapes a few percent apart are
+    // told apart by the longer medians, not by the order they were tried
+    // in), then the winner with the dense-community WCC, which must be
+    // kTuneMargin faster to replace it
+    std::vector<std::pair<double, scd_tuning>> timed;
+    for (size_t i = 0; i < cand.size() && rc == SCD_OK; i++) {
+        double med = 0.0;
+        if ((rc = time_tuning(cand[i], reps, &med)) == SCD_OK) timed.emplace_back(med, c->tune);
+    }
+    if (rc == SCD_OK) {
+        std::stable_sort(timed.begin(), timed.end(),
+                         [](const std::pair<double, scd_tuning> &x, const std::pair<double, scd_tuning> &y) {
+                             return x.first < y.first;
+                         });
+        for (size_t k = 0; k < std::min<size_t>(3, timed.size()) && rc == SCD_OK; k++) {
+            double med = 0.0;
+            if ((rc = time_tuning(timed[k].second, 2 * reps + 1, &med)) == SCD_OK && med < best_ms) {
+                best_ms = med;
+                best_t = c->tune;
+            }
+        }
+    }
+    if (rc == SCD_OK) {
+        scd_tuning t = best_t;
+        t.dense_wcc = 1;
+        double med = 0.0;
+        if ((rc = time_tuning(t, reps, &med)) == SCD_OK && med < best_ms * (1.0 - kTuneMargin)) {
             best_ms = med;
             best_t = c->tune;
         }
