@@ -852,10 +852,10 @@ static int phase_stats(scd_ctx *c, bool stamps = false) {
     if (c->nh) {
         on_lane(c, 1);
         const uint32_t C = c->tune.merge_chunk;
-        LAUNCH(c, "hub_filter_count", c->n_chunks, 256, k_hub_filter_count, s, c->hub_v, c->hub_d,
-               c->chunk_hub, c->chunk_start, C, c->chunk_cnt);
-        LAUNCH(c, "hub_filter_write", c->n_chunks, 256, k_hub_filter_write, s, c->hub_v, c->hub_d,
-               c->hub_chunk_begin, c->chunk_hub, c->chunk_start, C, c->chunk_cnt);
+        LAUNCH(c, "hub_filter_compact", c->n_chunks, 256, k_hub_filter_compact, s, c->hub_v, c->hub_d, c->hub_off,
+               c->chunk_hub, c->chunk_start, C, c->chunk_cnt, c->hub_a);
+        LAUNCH(c, "hub_filter_place", c->n_chunks, 256, k_hub_filter_place, s, c->hub_v, c->hub_off,
+               c->hub_chunk_begin, c->chunk_hub, c->chunk_start, c->chunk_cnt, c->hub_a);
     }
     join_lanes(c);
     if (stamps) stamp(c, 2);

@@ -277,7 +277,7 @@ __global__ __launch_bounds__(256) void k_filter_block(StatArgs a, const uint32_t
 
 // Order-preserving compaction of one slice of up to 256 entries by a block
 // of 256 threads: returns the slice's count; `same` entries are written at
-// out[run + rank].
+// out[run + rank] (and oflag[run + rank] cleared, when given).
 __device__ __forceinline__ uint32_t block_compact256(bool same, uint32_t q, uint32_t *out, uint8_t *oflag,
                                                      uint32_t run, uint32_t *s_cnt) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -293,18 +293,24 @@ __device__ __forceinline__ uint32_t block_compact256(bool same, uint32_t q, uint
     if (same) {
         const uint32_t at = run + before + __builtin_popcountll(m & ((1ull << lane) - 1ull));
         out[at] = q;
-        oflag[at] = 0;
+        if (oflag) oflag[at] = 0;
     }
     __syncthreads();
     return total;
 }
 
-// Hubs, pass 1: same-label count of each chunk of C entries.
-__global__ __launch_bounds__(256) void k_hub_filter_count(StatArgs a, const uint32_t *__restrict__ hub_v,
-                                                          const uint32_t *__restrict__ hub_d,
-                                                          const uint32_t *__restrict__ chunk_hub,
-                                                          const uint32_t *__restrict__ chunk_start,
-                                                          uint32_t C, uint32_t *__restrict__ chunk_cnt) {
+// Hubs, pass 1: each chunk of C entries gathers its neighbours' labels
+// once and compacts its same-label entries, in adjacency order, into the
+// hub's scratch segment at the chunk's own offset (tmp[hub_off + o ..], the
+// move step's hub buffer, free during the statistics); the count goes to
+// chunk_cnt.
+__global__ __launch_bounds__(256) void k_hub_filter_compact(StatArgs a, const uint32_t *__restrict__ hub_v,
+                                                            const uint32_t *__restrict__ hub_d,
+                                                            const uint32_t *__restrict__ hub_off,
+                                                            const uint32_t *__restrict__ chunk_hub,
+                                                            const uint32_t *__restrict__ chunk_start, uint32_t C,
+                                                            uint32_t *__restrict__ chunk_cnt,
+                                                            uint32_t *__restrict__ tmp) {
     __shared__ uint32_t s_cnt[4];
     const uint32_t hb = chunk_hub[blockIdx.x];
     const uint32_t o = chunk_start[blockIdx.x];
@@ -313,35 +319,36 @@ __global__ __launch_bounds__(256) void k_hub_filter_count(StatArgs a, const uint
     const uint32_t beg = a.indices[p];
     const uint32_t L = a.labels[p];
     const uint32_t end = min(o + C, d);
-    uint32_t mine = 0;
-    for (uint32_t i = o + threadIdx.x; i < end; i += 256)
-        mine += (L != SCD_INVALID && a.labels[a.adj[beg + i]] == L) ? 1u : 0u;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off, SCD_WAVE);
-    if ((threadIdx.x & 63) == 0) s_cnt[threadIdx.x >> 6] = mine;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t cnt = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
-        chunk_cnt[blockIdx.x] = cnt;
+    uint32_t *out = tmp + hub_off[hb] + o;
+    uint32_t run = 0;
+    for (uint32_t b = o; b < end; b += 256) {
+        const uint32_t i = b + threadIdx.x;
+        bool same = false;
+        uint32_t q = 0;
+        if (i < end && L != SCD_INVALID) {
+            q = a.adj[beg + i];
+            same = a.labels[q] == L;
+        }
+        run += block_compact256(same, q, out, nullptr, run, s_cnt);
     }
+    if (threadIdx.x == 0) chunk_cnt[blockIdx.x] = run;
 }
 
-// Hubs, pass 2: compaction of each chunk at the offset given by the counts
-// of the hub's earlier chunks; the hub's first chunk records int_deg.
-__global__ __launch_bounds__(256) void k_hub_filter_write(StatArgs a, const uint32_t *__restrict__ hub_v,
-                                                          const uint32_t *__restrict__ hub_d,
+// Hubs, pass 2: each chunk's entries go after those of the hub's earlier
+// chunks in int_adj (their triangle flags cleared); the hub's first chunk
+// records int_deg.
+__global__ __launch_bounds__(256) void k_hub_filter_place(StatArgs a, const uint32_t *__restrict__ hub_v,
+                                                          const uint32_t *__restrict__ hub_off,
                                                           const uint32_t *__restrict__ hub_chunk_begin,
                                                           const uint32_t *__restrict__ chunk_hub,
                                                           const uint32_t *__restrict__ chunk_start,
-                                                          uint32_t C, const uint32_t *__restrict__ chunk_cnt) {
-    __shared__ uint32_t s_cnt[4];
+                                                          const uint32_t *__restrict__ chunk_cnt,
+                                                          const uint32_t *__restrict__ tmp) {
     const uint32_t c = blockIdx.x;
     const uint32_t hb = chunk_hub[c];
     const uint32_t o = chunk_start[c];
     const uint32_t p = hub_v[hb];
-    const uint32_t d = hub_d[hb];
     const uint32_t beg = a.indices[p];
-    const uint32_t L = a.labels[p];
     uint32_t run = 0;
     for (uint32_t k = hub_chunk_begin[hb]; k < c; k++) run += chunk_cnt[k];
     if (o == 0 && threadIdx.x == 0) {
@@ -353,16 +360,11 @@ __global__ __launch_bounds__(256) void k_hub_filter_write(StatArgs a, const uint
             atomicAdd(a.counters + 1, (unsigned long long)tot * tot);
         }
     }
-    const uint32_t end = min(o + C, d);
-    for (uint32_t b = o; b < end; b += 256) {
-        const uint32_t i = b + threadIdx.x;
-        bool same = false;
-        uint32_t q = 0;
-        if (i < end) {
-            q = a.adj[beg + i];
-            same = (L != SCD_INVALID) && a.labels[q] == L;
-        }
-        run += block_compact256(same, q, a.int_adj + beg, a.iflag + beg, run, s_cnt);
+    const uint32_t cnt = chunk_cnt[c];
+    const uint32_t *src = tmp + hub_off[hb] + o;
+    for (uint32_t k = threadIdx.x; k < cnt; k += 256) {
+        a.int_adj[beg + run + k] = src[k];
+        a.iflag[beg + run + k] = 0;
     }
 }
 
