@@ -215,6 +215,12 @@ struct scd_ctx {
     // three more (default priority), forked and joined with events, so
     // small and tail-bound launches overlap (captured as parallel branches)
     hipStream_t lane[3] = {nullptr, nullptr, nullptr};
+    // an iteration's community statistics run on this stream beside the
+    // WCC's triangle counting, which does not read them; stats_ev (recorded
+    // after them) is waited for before the first kernel that does
+    hipStream_t side = nullptr;
+    hipEvent_t stats_ev = nullptr;
+    bool stats_pending = false;
     // fork and join events: every fork or join point of a launch sequence
     // records its own events (pool reset when the stream is idle, and at the
     // start of each capture; captures draw from a pool of their own)
@@ -447,6 +453,14 @@ static void join_lanes(scd_ctx *c) {
         (void)hipEventRecord(e, c->lane[i]);
         (void)hipStreamWaitEvent(c->stream, e, 0);
     }
+}
+
+// The context's stream waits for the statistics launched on the side
+// stream (no-op when none are pending).
+static void wait_stats(scd_ctx *c) {
+    if (!c->stats_pending) return;
+    (void)hipStreamWaitEvent(c->stream, c->stats_ev, 0);
+    c->stats_pending = false;
 }
 
 // Phase boundary i of an iteration (0: start, 1: after the move, 2: after
@@ -803,10 +817,12 @@ static int phase_move(scd_ctx *c, uint32_t *labels_out, uint32_t *ex_sorted, uin
     return check_launch("move");
 }
 
-// Statistics + internal adjacency of the current labels.  stamps: write the
-// phase boundary after the filter kernels (iterations only).
-static int phase_stats(scd_ctx *c, bool stamps = false) {
-    LAUNCH(c, "clear_stats", cdiv(c->num_comms, 256), 256, k_fill_int4, c->stats, c->num_comms);
+// Statistics + internal adjacency of the current labels.  iteration: within
+// an iteration (the phase boundary after the filter kernels is stamped, and
+// without phase stamps the statistics run on the side stream, beside the
+// WCC's triangle counting; phase_wcc waits for them before the first kernel
+// that reads them).
+static int phase_stats(scd_ctx *c, bool iteration = false) {
     StatArgs s;
     s.indices = c->indices;
     s.adj = c->adj;
@@ -858,9 +874,23 @@ static int phase_stats(scd_ctx *c, bool stamps = false) {
                c->hub_chunk_begin, c->chunk_hub, c->chunk_start, c->chunk_cnt, c->hub_a);
     }
     join_lanes(c);
-    if (stamps) stamp(c, 2);
+    if (iteration) stamp(c, 2);
+    hipEvent_t e0 = (iteration && !c->phase_timing && c->side && lanes_on(c)) ? lane_event(c) : nullptr;
+    hipEvent_t e1 = e0 ? lane_event(c) : nullptr;
+    if (e1) {
+        (void)hipEventRecord(e0, c->stream);
+        (void)hipStreamWaitEvent(c->side, e0, 0);
+        c->ls = c->side;
+    }
+    LAUNCH(c, "clear_stats", cdiv(c->num_comms, 256), 256, k_fill_int4, c->stats, c->num_comms);
     LAUNCH(c, "stats_agg", std::min<uint32_t>(cdiv(c->n, 1024), 2u * c->num_cus), 1024, k_stats_agg, c->indices,
            c->cur, c->int_deg, c->n, c->G ? c->gid : nullptr, c->active_g, c->stats);
+    if (e1) {
+        (void)hipEventRecord(e1, c->side);
+        c->stats_ev = e1;
+        c->stats_pending = true;
+        c->ls = c->stream;
+    }
     return check_launch("stats");
 }
 
@@ -909,6 +939,7 @@ static int count_internal_triangles(scd_ctx *c, const WccArgs &w0) {
         d.bits = c->dn_bits;
         d.units = c->dn_units;
         LAUNCH(c, "dn_clear", 1, 64, k_fill_u32, c->dn_ctr, DN_CTR);
+        wait_stats(c);  // the classification reads the community sizes
         LAUNCH(c, "dn_classify", cdiv(c->num_comms, 256), 256, k_dn_classify, d, c->num_comms);
         LAUNCH(c, "dn_lid", cdiv(c->n, 256), 256, k_dn_lid, w, d, c->n);
         w.dn_off = c->dn_off;
@@ -1021,6 +1052,7 @@ static int phase_wcc(scd_ctx *c, int32_t *int_tri_out, int32_t *int_tri_deg_out,
     w.int_tri_out = int_tri_out;
     w.int_tri_deg_out = int_tri_deg_out;
     int rc = count_internal_triangles(c, w);
+    wait_stats(c);  // the terms read the community sizes
     if (rc) return rc;
     LAUNCH(c, "wcc_terms", cdiv(c->n, c->block_size), c->block_size, k_wcc_terms, w, c->n);
     if (c->G) {
@@ -1122,6 +1154,10 @@ extern "C" int scd_init(int device_id, int work_group_size, int profile, scd_ctx
     {
         bool ok = true;
         for (int i = 0; i < 3 && ok; i++) ok = hipStreamCreateWithFlags(&c->lane[i], hipStreamNonBlocking) == hipSuccess;
+        if (ok && hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) {
+            (void)hipGetLastError();
+            c->side = nullptr;  // statistics stay on the context's stream
+        }
         if (!ok) {  // run everything on the one stream
             (void)hipGetLastError();
             c->use_lanes = false;
@@ -1160,6 +1196,7 @@ extern "C" void scd_destroy(scd_ctx *c) {
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     for (int i = 0; i < 3; i++)
         if (c->lane[i]) (void)hipStreamDestroy(c->lane[i]);
+    if (c->side) (void)hipStreamDestroy(c->side);
     for (auto &pool : c->lane_ev)
         for (auto e : pool) (void)hipEventDestroy(e);
     if (c->stamps) (void)hipFree(c->stamps);
@@ -1426,7 +1463,10 @@ static int enqueue_iteration(scd_ctx *c, bool incremental) {
         LAUNCH(c, "wcc_dirty_mark", cdiv(c->n, B), B, k_mark_dirty, c->cur, c->next, c->n, c->wdirty);
     }
     std::swap(c->cur, c->next);
-    if ((rc = phase_stats(c, true))) return rc;
+    if ((rc = phase_stats(c, true))) {
+        wait_stats(c);
+        return rc;
+    }
     stamp(c, 3);
     if ((rc = phase_wcc(c, nullptr, nullptr, incremental))) return rc;
     stamp(c, 4);
