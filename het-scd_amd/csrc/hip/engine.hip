@@ -918,7 +918,9 @@ static int scan_u32_dev(scd_ctx *c, const uint32_t *in, uint32_t *out, uint64_t 
 // subgraph of the current labels, or with int_deg == nullptr the whole
 // graph): T[u] = triangles at u, td[u] = edges at u in a triangle; the
 // kernels per degree bin and the work queue of long out-lists.
-static int count_internal_triangles(scd_ctx *c, const WccArgs &w0) {
+// fold_hot: fold the hot ids' private copies into T / td at the end (the
+// WCC's terms kernel folds them instead).
+static int count_internal_triangles(scd_ctx *c, const WccArgs &w0, bool fold_hot = true) {
     WccArgs w = w0;
     // dense communities of the WCC's internal subgraph: classified, local
     // ids and rows, counted on their own lane beside the list kernels
@@ -1025,7 +1027,7 @@ static int count_internal_triangles(scd_ctx *c, const WccArgs &w0) {
                c->wq_M, 1, nullptr);
     }
     join_lanes(c);
-    LAUNCH(c, "wcc_tri_hot", cdiv(w.hot_n, 256), 256, k_wtc_hot_sum, w);
+    if (fold_hot) LAUNCH(c, "wcc_tri_hot", cdiv(w.hot_n, 256), 256, k_wtc_hot_sum, w);
     return SCD_OK;
 }
 
@@ -1051,7 +1053,7 @@ static int phase_wcc(scd_ctx *c, int32_t *int_tri_out, int32_t *int_tri_deg_out,
     w.vc_min = c->wcc_heavy;
     w.int_tri_out = int_tri_out;
     w.int_tri_deg_out = int_tri_deg_out;
-    int rc = count_internal_triangles(c, w);
+    int rc = count_internal_triangles(c, w, false);
     wait_stats(c);  // the terms read the community sizes
     if (rc) return rc;
     LAUNCH(c, "wcc_terms", cdiv(c->n, c->block_size), c->block_size, k_wcc_terms, w, c->n);

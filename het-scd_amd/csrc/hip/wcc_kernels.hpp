@@ -97,9 +97,9 @@ __device__ __forceinline__ void wtc_add_td(const WccArgs &a, uint32_t x, uint32_
 }
 
 // T[x] += sum of the copies, td likewise, copies cleared for the next pass.
-__global__ void k_wtc_hot_sum(WccArgs a) {
-    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= a.hot_n) return;
+// Folds the private copies of vertex x (< hot_n) into T[x] / td[x] and
+// clears them.
+__device__ __forceinline__ void wtc_hot_fold(const WccArgs &a, uint32_t x) {
     uint32_t t = 0, d = 0;
 #pragma unroll
     for (uint32_t k = 0; k < WTC_HOT_COPIES; k++) {
@@ -119,6 +119,11 @@ __global__ void k_wtc_hot_sum(WccArgs a) {
     }
     if (t) a.T[x] += t;
     if (d) a.td[x] += d;
+}
+
+__global__ void k_wtc_hot_sum(WccArgs a) {
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < a.hot_n) wtc_hot_fold(a, x);
 }
 
 // dirty[old label] = dirty[new label] = 1 for every vertex that moved
@@ -964,10 +969,13 @@ __global__ void k_wtc_double(const uint32_t *__restrict__ T, uint32_t n, int32_t
 }
 
 // Per-vertex terms (src/wcc.cpp:89-166): vertices without a valid label, in
-// a community of size <= 2 or without triangles contribute 0.
+// a community of size <= 2 or without triangles contribute 0.  The hot ids'
+// private triangle copies are folded into T / td here first (the triangle
+// pass leaves them to this kernel).
 __global__ void k_wcc_terms(WccArgs a, uint32_t n) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
+    if (p < a.hot_n) wtc_hot_fold(a, p);
     const uint32_t L = a.labels[p];
     bool valid = false;
     int size = 0;
