@@ -372,7 +372,10 @@ def refine_leg(args, scale, scd_amd, np, on_device, steps, warmup, cpu_reps, wan
         break
     if roof:
         # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE /
-        # WRITE_SIZE summary of the same kernel on the same graph and tuning
+        # WRITE_SIZE summary of the same kernel on the same graph and bin: an
+        # entry that records its bin's algorithmic bytes matches on those (the
+        # same vertices under the same kernel shape, whichever lds_tile put
+        # them there), an older entry on the tuning knobs that fix the bin
         import glob
         knobs = ("lds_tile", "wave_max", "lds_hash")
         cur = eng.get_tuning()
@@ -381,9 +384,17 @@ def refine_leg(args, scale, scd_amd, np, on_device, steps, warmup, cpu_reps, wan
             if t.get("kernel") != roof["kernel"] or t.get("workload_graph_sha256") != graph_sha:
                 continue
             for ent in t.get("entries", []):
-                if all(ent["tuning"].get(k) == cur[k] for k in knobs):
+                if "algo_bytes_per_launch" in ent:
+                    hit = (ent["algo_bytes_per_launch"] == roof["algo_bytes_per_launch"]
+                           and ent["tuning"].get("lds_hash") == cur["lds_hash"])
+                else:
+                    hit = all(ent["tuning"].get(k) == cur[k] for k in knobs)
+                if hit:
                     roof["traffic"] = ent["traffic_bytes"]
                     roof["traffic_source"] = os.path.relpath(fn, ROOT)
+                    break
+            if roof["traffic"] is not None:
+                break
     per_kernel = {name: {"launches": la, "avg_us": round(ms / la * 1e3, 2), "total_ms": round(ms, 3)}
                   for ms, name, la in kern}
 
