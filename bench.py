@@ -392,6 +392,11 @@ def refine_leg(args, scale, scd_amd, np, on_device, steps, warmup, cpu_reps, wan
                 if hit:
                     roof["traffic"] = ent["traffic_bytes"]
                     roof["traffic_source"] = os.path.relpath(fn, ROOT)
+                    # the counters' bytes over this run's launch duration:
+                    # L2-to-fabric traffic, served by HBM or the Infinity Cache
+                    tr = ent["traffic_bytes"] / (roof["avg_launch_us"] * 1e-6) / 1e9
+                    roof["traffic_GBps"] = round(tr, 1)
+                    roof["traffic_frac_of_peak"] = round(tr / HBM_PEAK_GBPS, 4)
                     break
             if roof["traffic"] is not None:
                 break
@@ -686,21 +691,26 @@ def run_served(args, scd_amd, np):
     small_plain = len(small) / (plain_ms[small].sum() / 1e3)
     log(f"[bench] served small graphs only ({len(small)} planted, all at t=0): {small_cap:.1f} graphs/s vs plain "
         f"{small_plain:.1f} = {small_cap / small_plain:.2f}x; result mismatches {small_mismatch}")
-    # the same small graphs as a Poisson stream at 80% of that capacity
+    # the same small graphs as a Poisson stream at 80% of that capacity,
+    # every graph requested served_reps times (a stream several times longer
+    # than one request's latency, so the rate is the sustained one and not
+    # that of a burst and its tail)
     lam_s = 0.8 * small_cap
-    arr_s = np.cumsum(np.random.default_rng(11).exponential(1e3 / lam_s, len(small)))
-    res_p, st_p, _ = serve_subset([graphs[i] for i in small], arr_s, args.join_wait_ms)
+    stream = small * args.served_reps
+    arr_s = np.cumsum(np.random.default_rng(11).exponential(1e3 / lam_s, len(stream)))
+    res_p, st_p, _ = serve_subset([graphs[i] for i in stream], arr_s, args.join_wait_ms)
     done_p = np.array([r["done_ms"] for r in res_p])
     lat_p = done_p - arr_s
     span_p = (done_p.max() - arr_s[0]) / 1e3
-    for r, i in zip(res_p, small):
+    for r, i in zip(res_p, stream):
         iters, best, w0, lab = plain_res[i]
         if (r["status"] != 0 or r["iterations"] != iters or r["best_wcc"] != best or r["initial_wcc"] != w0
                 or not np.array_equal(r["labels"], lab)):
             small_mismatch += 1
-    log(f"[bench] served small graphs at lambda={lam_s:.1f}/s: {len(small) / span_p:.1f} graphs/s, p50 "
-        f"{np.percentile(lat_p, 50):.1f} ms p99 {np.percentile(lat_p, 99):.1f} ms, busy "
-        f"{st_p['busy_ms'] / (span_p * 1e3):.2f}")
+    log(f"[bench] served small graphs at lambda={lam_s:.1f}/s ({len(stream)} requests over "
+        f"{(arr_s[-1] - arr_s[0]) / 1e3:.3f} s): {len(stream) / span_p:.1f} graphs/s = "
+        f"{len(stream) / span_p / small_plain:.2f}x plain, p50 {np.percentile(lat_p, 50):.1f} ms p99 "
+        f"{np.percentile(lat_p, 99):.1f} ms, busy {st_p['busy_ms'] / (span_p * 1e3):.2f}")
     pct = lambda x, q: round(float(np.percentile(x, q)), 3)
     return {"workload": f"{N} graphs, Poisson arrivals at 80% of measured capacity (seed 7): 90% planted partition "
                         "(n log-uniform [1000, 20000], 4-64 communities Dirichlet(1), p_in 0.15, p_out 0.002), "
@@ -722,8 +732,10 @@ def run_served(args, scd_amd, np):
             "small_graphs_only": {"graphs": len(small), "capacity_graphs_per_s": round(small_cap, 1),
                                   "plain_graphs_per_s": round(small_plain, 1),
                                   "speedup": round(small_cap / small_plain, 2), "joins": st_s["joins"],
-                                  "poisson": {"lambda_per_s": round(lam_s, 1),
-                                              "graphs_per_s": round(len(small) / span_p, 1),
+                                  "poisson": {"lambda_per_s": round(lam_s, 1), "requests": len(stream),
+                                              "arrival_span_s": round((arr_s[-1] - arr_s[0]) / 1e3, 3),
+                                              "graphs_per_s": round(len(stream) / span_p, 1),
+                                              "speedup_vs_plain": round(len(stream) / span_p / small_plain, 2),
                                               "p50_ms": pct(lat_p, 50), "p99_ms": pct(lat_p, 99),
                                               "joins": st_p["joins"],
                                               "busy_fraction": round(st_p["busy_ms"] / (span_p * 1e3), 3)},
@@ -826,6 +838,8 @@ def main():
                     help="skip the served workload (Poisson stream, measured by default)")
     ap.add_argument("--served", action="store_true", default=True, help=argparse.SUPPRESS)
     ap.add_argument("--served-graphs", type=int, default=200)
+    ap.add_argument("--served-reps", type=int, default=4,
+                    help="times each small graph is requested in the served Poisson stream")
     ap.add_argument("--join-wait-ms", type=float, default=5.0)
     ap.add_argument("--intake-graphs", type=int, default=0, help="server intake batch, graphs (0: 64)")
     ap.add_argument("--intake-mb", type=int, default=0, help="server intake batch, MiB of footprint (0: 2 GiB)")
