@@ -183,6 +183,16 @@ struct scd_ctx {
     uint32_t *tchunk_bl = nullptr, *tchunk_has_rem = nullptr;
     double *tchunk_bi = nullptr, *tchunk_rem = nullptr;
     uint32_t n_tchunks = 0;
+    // label-range buckets of the hubs (tune.hub_table 2): per hub the sample
+    // stride, samples per bucket start and first slot; per slot its hub,
+    // bucket start, entry count and result; partition chunks; per hub the
+    // buckets in use and the overflow count; HB_CAP entries per slot
+    uint32_t *hb_sigma = nullptr, *hb_kstep = nullptr, *hb_slot0 = nullptr, *hb_slot_hub = nullptr,
+             *hb_pch_hub = nullptr, *hb_pch_start = nullptr, *hb_lo = nullptr, *hb_nb = nullptr, *hb_cnt = nullptr,
+             *hb_ovf_cnt = nullptr, *hb_region = nullptr, *hb_bl = nullptr, *hb_has_rem = nullptr;
+    double *hb_bi = nullptr, *hb_rem = nullptr;
+    uint32_t hb_slots = 0, hb_pchunks = 0;
+    uint32_t hb_cap = HB_CAP;  // SCD_HUB_BUCKET_CAP (tests) lowers it to reach the overflow path
     // chunks launched by each merge pass (a prefix of the chunk list: hubs
     // are ordered by degree, largest first)
     std::vector<uint32_t> merge_pass_chunks;
@@ -491,6 +501,56 @@ static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b -
 // the largest degree, and sends back each workgroup's offsets.
 // active_only (batched arenas): the vertices of the graphs still refining
 // (active_g on the device), else all.
+// Label-range buckets of the hubs (tune.hub_table 2), hubs as ordered in
+// build_bins (degrees hd): per hub a sample stride that keeps its samples
+// within HB_SAMPLES, kstep samples per bucket start (about HB_TARGET entries
+// a bucket) and room for 1 + 2 (samples - 1) / kstep buckets (each
+// candidate start may add one more bucket after a frequent label), at most
+// HB_MAXB; partition chunks of HB_PCH entries.
+static int build_hub_buckets(scd_ctx *c, const std::vector<uint32_t> &hd) {
+    const uint32_t nh = c->nh;
+    std::vector<uint32_t> sig(nh), ks(nh), slot0(nh + 1), slot_hub, ph, ps;
+    uint64_t slots = 0;
+    for (uint32_t h = 0; h < nh; h++) {
+        const uint32_t d = hd[h];
+        uint32_t sg = HB_TARGET / 16;  // 16 samples per bucket start: sizes within about 1/4 of the aim
+        while ((d + sg - 1) / sg > HB_SAMPLES) sg *= 2;
+        const uint32_t m = (d + sg - 1) / sg;
+        uint32_t k = std::max<uint32_t>(1, HB_TARGET / sg);
+        while (1 + 2 * ((m - 1) / k) > HB_MAXB) k *= 2;
+        sig[h] = sg;
+        ks[h] = k;
+        slot0[h] = (uint32_t)slots;
+        const uint32_t nsl = 1 + 2 * ((m - 1) / k);
+        slots += nsl;
+        for (uint32_t i = 0; i < nsl; i++) slot_hub.push_back(h);
+        for (uint32_t st = 0; st < d; st += HB_PCH) {
+            ph.push_back(h);
+            ps.push_back(st);
+        }
+    }
+    if (slots >= (1ull << 32) / HB_CAP) return fail(SCD_ELIMIT, "hub buckets exceed 2^32 entries");
+    slot0[nh] = (uint32_t)slots;
+    c->hb_slots = (uint32_t)slots;
+    c->hb_pchunks = (uint32_t)ph.size();
+    auto up = [&](uint32_t **dst, const std::vector<uint32_t> &src) -> int {
+        int r = dalloc(c, dst, src.size(), &c->hub_bufs);
+        if (r) return r;
+        HIP_TRY(hipMemcpy(*dst, src.data(), sizeof(uint32_t) * src.size(), hipMemcpyHostToDevice));
+        return SCD_OK;
+    };
+    int rc;
+    if ((rc = up(&c->hb_sigma, sig)) || (rc = up(&c->hb_kstep, ks)) || (rc = up(&c->hb_slot0, slot0)) ||
+        (rc = up(&c->hb_slot_hub, slot_hub)) || (rc = up(&c->hb_pch_hub, ph)) || (rc = up(&c->hb_pch_start, ps)) ||
+        (rc = dalloc(c, &c->hb_lo, slots, &c->hub_bufs)) || (rc = dalloc(c, &c->hb_nb, nh, &c->hub_bufs)) ||
+        (rc = dalloc(c, &c->hb_cnt, slots, &c->hub_bufs)) || (rc = dalloc(c, &c->hb_ovf_cnt, nh, &c->hub_bufs)) ||
+        (rc = dalloc(c, &c->hb_region, slots * HB_CAP, &c->hub_bufs)) ||
+        (rc = dalloc(c, &c->hb_bl, slots, &c->hub_bufs)) || (rc = dalloc(c, &c->hb_has_rem, slots, &c->hub_bufs)) ||
+        (rc = dalloc(c, &c->hb_bi, slots, &c->hub_bufs)) || (rc = dalloc(c, &c->hb_rem, slots, &c->hub_bufs)))
+        return rc;
+    return SCD_OK;
+}
+
 static int build_bins(scd_ctx *c, bool active_only = false) {
     HIP_TRY(hipStreamSynchronize(c->stream));  // the old lists may still be read
     drop_graphs(c);
@@ -589,6 +649,7 @@ static int build_bins(scd_ctx *c, bool active_only = false) {
     const uint32_t hb0 = c->bin_begin[K_HUB], nh = c->bin_begin[K_HUB + 1] - hb0;
     c->nh = nh;
     c->n_tiles = c->n_chunks = c->max_hub_deg = 0;
+    c->hb_slots = c->hb_pchunks = 0;
     if (nh == 0) return SCD_OK;
     std::vector<uint32_t> hv(nh), hd(nh), hoff(nh), hcb(nh + 1), th, ts, ch, cs;
     {
@@ -686,7 +747,8 @@ static int build_bins(scd_ctx *c, bool active_only = false) {
         return rc;
     HIP_TRY(hipMemcpy(c->tab_off, toff.data(), 8ull * nh, hipMemcpyHostToDevice));
     LAUNCH(c, "hub_table_clear", cdiv(slots, 256), 256, k_fill_u64v, c->hub_slots, slots, HUB_EMPTY);
-    return check_launch("hub tables");
+    if ((rc = check_launch("hub tables"))) return rc;
+    return c->tune.hub_table == 2 ? build_hub_buckets(c, hd) : SCD_OK;
 }
 
 static HubArgs hub_args(scd_ctx *c) {
@@ -729,9 +791,9 @@ static void launch_lds_and_tiles(scd_ctx *c, const MoveArgs &a, const HubArgs &h
         LAUNCH(c, "move_lds", bin_n(c, K_LDS), 256, (k_move_lds<TH>), a, bin_v(c, K_LDS), bin_n(c, K_LDS));
     }
     on_lane(c, 1);  // the hub chain: tiles, then merges / table, evaluation, finalize
-    if (a.ex_sorted || !c->tune.hub_table)
+    if (a.ex_sorted || c->tune.hub_table == 0)
         LAUNCH(c, "hub_tile_sort", c->n_tiles, 256, (k_hub_tile_sort<T>), a, h, c->hub_a);
-    else
+    else if (c->tune.hub_table == 1)
         LAUNCH(c, "hub_tile_count", c->n_tiles, 256, (k_hub_tile_count<T>), a, h, t);
 }
 
@@ -779,7 +841,32 @@ static int phase_move(scd_ctx *c, uint32_t *labels_out, uint32_t *ex_sorted, uin
         default: launch_lds_and_tiles<16384>(c, a, h, t); break;
     }
     on_lane(c, 1);
-    if (c->nh && !ex_sorted && c->tune.hub_table) {
+    if (c->nh && !ex_sorted && c->tune.hub_table == 2) {
+        HubArgs hb = h;  // results per bucket slot
+        hb.hub_chunk_begin = c->hb_slot0;
+        hb.chunk_bi = c->hb_bi;
+        hb.chunk_bl = c->hb_bl;
+        hb.chunk_rem = c->hb_rem;
+        hb.chunk_has_rem = c->hb_has_rem;
+        HubBuckets k;
+        k.sigma = c->hb_sigma;
+        k.kstep = c->hb_kstep;
+        k.slot0 = c->hb_slot0;
+        k.slot_hub = c->hb_slot_hub;
+        k.pch_hub = c->hb_pch_hub;
+        k.pch_start = c->hb_pch_start;
+        k.lo = c->hb_lo;
+        k.nb = c->hb_nb;
+        k.cnt = c->hb_cnt;
+        k.ovf_cnt = c->hb_ovf_cnt;
+        k.region = c->hb_region;
+        k.ovf = c->hub_b;
+        k.cap = c->hb_cap;
+        LAUNCH(c, "hub_split", c->nh, 256, k_hb_split, a, hb, k);
+        LAUNCH(c, "hub_part", c->hb_pchunks, 256, k_hb_part, a, hb, k);
+        LAUNCH(c, "hub_eval", c->hb_slots, 256, k_hb_eval, at(K_HUB), hb, k);
+        LAUNCH(c, "hub_finalize", cdiv(c->nh, 4), 256, k_hub_finalize, a, hb, c->nh);
+    } else if (c->nh && !ex_sorted && c->tune.hub_table) {
         HubArgs ht = h;  // results per table chunk
         ht.hub_chunk_begin = c->hub_tchunk_begin;
         ht.chunk_bi = c->tchunk_bi;
@@ -1168,6 +1255,12 @@ extern "C" int scd_init(int device_id, int work_group_size, int profile, scd_ctx
         // kernels may not follow work forked across captured streams)
         if (const char *env = getenv("SCD_LANES"))
             if (atoi(env) == 0) c->use_lanes = false;
+    }
+    // SCD_HUB_BUCKET_CAP: entries per hub bucket region below HB_CAP, so that
+    // tests reach the buckets' overflow path
+    if (const char *env = getenv("SCD_HUB_BUCKET_CAP")) {
+        const long v = atol(env);
+        if (v >= 1 && v < (long)HB_CAP) c->hb_cap = (uint32_t)v;
     }
     e = hipHostMalloc((void **)&c->wcc_host, 512, hipHostMallocDefault);
     if (e != hipSuccess) {
@@ -1786,7 +1879,7 @@ extern "C" int scd_set_tuning(scd_ctx *c, const scd_tuning *t) {
     const uint32_t W = t->wave_max ? t->wave_max : 1024u;
     if (W != 256 && W != 512 && W != 1024)
         return fail(SCD_EINVAL, "scd_set_tuning: wave_max %u is not 256, 512 or 1024", W);
-    if (t->hub_table > 1) return fail(SCD_EINVAL, "scd_set_tuning: hub_table %u is not 0 or 1", t->hub_table);
+    if (t->hub_table > 2) return fail(SCD_EINVAL, "scd_set_tuning: hub_table %u is not 0, 1 or 2", t->hub_table);
     if (t->lds_hash > 1 || (t->lds_hash && T > 4096))
         return fail(SCD_EINVAL, "scd_set_tuning: lds_hash %u needs lds_tile <= 4096", t->lds_hash);
     if (t->lanes > 1) return fail(SCD_EINVAL, "scd_set_tuning: lanes %u is not 0 or 1", t->lanes);
@@ -3056,16 +3149,18 @@ extern "C" int scd_autotune(scd_ctx *c, const char *cache_path, int reps, scd_tu
             std::vector<uint32_t> Hs = {0};
             if (dmax > T) {
                 Cs = {1024, 2048, 4096};
-                Hs = {0, 1};
+                Hs = {0, 1, 2};
             }
             std::vector<uint32_t> Ls = {0};
             if (T <= 4096 && dmax > W) Ls = {0, 1};
             std::vector<uint32_t> Ns = {1};
             if (c->E >= (1ull << 20)) Ns = {0, 1};
             for (uint32_t C : Cs)
-                for (uint32_t H : Hs)
+                for (uint32_t H : Hs) {
+                    if (H == 2 && C != Cs[0]) continue;  // the buckets use no merge chunks
                     for (uint32_t Lh : Ls)
                         for (uint32_t N : Ns) cand.push_back(scd_tuning{T, C, W, H, Lh, N, 0, 0});
+                }
         }
 
     // the labels, statistics and best labels to start every timed iteration from

@@ -762,6 +762,300 @@ __global__ __launch_bounds__(256) void k_hub_finalize(MoveArgs a, HubArgs h, uin
     }
 }
 
+// ---------------------------------------------------------------------------
+// Hub path by label-range buckets (move without export).  A hub's neighbour
+// labels are split at sampled label values into buckets of about HB_TARGET
+// entries; each bucket is sorted and evaluated in LDS by one workgroup, and
+// k_hub_finalize combines the buckets' best candidates per hub.  One label
+// gather, one scatter and one LDS sort per entry: no tiles merged in global
+// memory.
+//   k_hb_split: per hub, at most HB_SAMPLES labels sampled (every sigma-th
+//     neighbour) and sorted in LDS; every kstep-th sample value starts a
+//     bucket, and a value met at kstep or more samples gets a bucket of its
+//     own (a single label of any multiplicity is one candidate: counted,
+//     never sorted).  Clears the hub's bucket counters.
+//   k_hb_part: per chunk of HB_PCH entries, the labels gathered and counted
+//     per bucket in LDS; one atomic per bucket reserves the chunk's range in
+//     the bucket's region of HB_CAP entries, and entries past HB_CAP go to
+//     the hub's overflow area.  Single-label buckets are only counted.
+//   k_hb_eval: per bucket slot: one label -> one evaluation; at most HB_CAP
+//     entries -> sort, reduce-by-key and evaluation in LDS; more (a sampling
+//     outlier: a bucket aims at HB_TARGET entries and holds at most about
+//     twice that in expectation) -> its distinct labels one at a time by block
+//     reductions over the region and the overflow area, slower but exact.
+// ---------------------------------------------------------------------------
+constexpr uint32_t HB_SAMPLES = 4096;  // sampled labels per hub, at most
+constexpr uint32_t HB_TARGET = 1024;   // entries per bucket aimed at
+constexpr uint32_t HB_CAP = 4096;      // entries per bucket region
+constexpr uint32_t HB_PCH = 4096;      // entries per partition workgroup
+constexpr uint32_t HB_MAXB = 2048;     // buckets per hub, at most
+
+struct HubBuckets {
+    const uint32_t *__restrict__ sigma;      // per hub: sample stride
+    const uint32_t *__restrict__ kstep;      // per hub: samples per bucket start
+    const uint32_t *__restrict__ slot0;      // per hub (nh + 1): first bucket slot
+    const uint32_t *__restrict__ slot_hub;   // per slot: its hub
+    const uint32_t *__restrict__ pch_hub;    // per partition chunk: hub, first entry
+    const uint32_t *__restrict__ pch_start;
+    uint32_t *__restrict__ lo;       // per slot: bucket j holds labels [lo[j], lo[j + 1])
+    uint32_t *__restrict__ nb;       // per hub: buckets in use
+    uint32_t *__restrict__ cnt;      // per slot: entries
+    uint32_t *__restrict__ ovf_cnt;  // per hub: entries in the overflow area
+    uint32_t *__restrict__ region;   // HB_CAP entries per slot
+    uint32_t *__restrict__ ovf;      // overflow area of hub h at hub_off[h]
+    uint32_t cap;                    // entries a region takes (HB_CAP; less only to test the overflow path)
+};
+
+// The bucket of label L: the largest j < K with lo[j] <= L (lo[0] = 0).
+__device__ __forceinline__ uint32_t hb_bucket(const uint32_t *lo, uint32_t K, uint32_t L) {
+    uint32_t a = 0, b = K;
+    while (b - a > 1) {
+        const uint32_t mid = (a + b) >> 1;
+        if (lo[mid] <= L) a = mid;
+        else b = mid;
+    }
+    return a;
+}
+
+__device__ __forceinline__ uint32_t hb_hi(const uint32_t *lo, uint32_t K, uint32_t j) {
+    return j + 1 < K ? lo[j + 1] : SCD_INVALID;
+}
+
+__global__ __launch_bounds__(256) void k_hb_split(MoveArgs a, HubArgs h, HubBuckets b) {
+    __shared__ uint32_t s[HB_SAMPLES];
+    __shared__ uint32_t s_off[257];
+    const int tid = threadIdx.x;
+    const uint32_t hb = blockIdx.x;
+    const uint32_t p = h.hub_v[hb], d = h.hub_d[hb];
+    const uint32_t s0 = b.slot0[hb], ns = b.slot0[hb + 1] - s0;
+    for (uint32_t i = tid; i < ns; i += 256) b.cnt[s0 + i] = 0;
+    if (tid == 0) b.ovf_cnt[hb] = 0;
+    if (vtx_stopped(a, p)) {  // uniform per block
+        if (tid == 0) b.nb[hb] = 0;
+        return;
+    }
+    const uint32_t sg = b.sigma[hb], k = b.kstep[hb];
+    const uint32_t beg = a.indices[p];
+    const uint32_t m = (d + sg - 1) / sg;  // <= HB_SAMPLES (host)
+    const uint32_t n2 = max(next_pow2(m), 1024u);
+    {
+        constexpr int U = HB_SAMPLES / 256;
+        uint32_t q[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const uint32_t j = u * 256 + tid;
+            q[u] = j < m ? a.adj[beg + j * sg] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const uint32_t j = u * 256 + tid;
+            if (j < n2) s[j] = j < m ? a.labels[q[u]] : SCD_INVALID;
+        }
+    }
+    __syncthreads();
+    block_sort_lds<HB_SAMPLES>(s, n2, m);
+    // valid samples: those before the first INVALID
+    uint32_t mv;
+    {
+        uint32_t lo_ = 0, hi_ = m;
+        while (lo_ < hi_) {
+            const uint32_t mid = (lo_ + hi_) >> 1;
+            if (s[mid] != SCD_INVALID) lo_ = mid + 1;
+            else hi_ = mid;
+        }
+        mv = lo_;
+    }
+    // bucket starts: candidate j (1..J) is the sample value v_j = s[j k]; it
+    // starts a bucket unless it equals v_{j-1} (or is 0, the first start);
+    // a value met at k or more samples also ends its own bucket at v + 1
+    // (after its last candidate, unless v + 1 starts the next one anyway)
+    const uint32_t J = mv ? (mv - 1) / k : 0u;
+    const uint32_t per = (J + 255) / 256;
+    const uint32_t j0 = 1 + tid * per, j1 = min(J, (tid + 1) * per);  // [j0, j1]
+    auto heavy = [&](uint32_t v) {
+        uint32_t l0 = 0, h0 = mv;  // first index with s >= v
+        while (l0 < h0) {
+            const uint32_t mid = (l0 + h0) >> 1;
+            if (s[mid] < v) l0 = mid + 1;
+            else h0 = mid;
+        }
+        return run_end_u32(s, mv, l0, v) - l0 >= k;
+    };
+    auto emits = [&](uint32_t j, uint32_t *out) {
+        const uint32_t v = s[j * k];
+        uint32_t e = 0;
+        const bool fresh = j == 1 ? v != 0u : v != s[(j - 1) * k];
+        if (fresh) out[e++] = v;
+        const bool last = j == J || s[(j + 1) * k] != v;
+        if (last && v + 1u != SCD_INVALID && (j == J || s[(j + 1) * k] != v + 1u) && heavy(v)) out[e++] = v + 1u;
+        return e;
+    };
+    uint32_t mine = 0;
+    for (uint32_t j = j0; j <= j1; j++) {
+        uint32_t tmp[2];
+        mine += emits(j, tmp);
+    }
+    s_off[tid + 1] = mine;
+    __syncthreads();
+    if (tid == 0) {
+        s_off[0] = 1;  // slot 0 holds the start 0
+        for (int t = 1; t <= 256; t++) s_off[t] += s_off[t - 1];
+        b.lo[s0] = 0;
+        b.nb[hb] = s_off[256];
+    }
+    __syncthreads();
+    uint32_t at = s0 + s_off[tid];
+    for (uint32_t j = j0; j <= j1; j++) {
+        uint32_t tmp[2];
+        const uint32_t e = emits(j, tmp);
+        for (uint32_t i = 0; i < e; i++) b.lo[at++] = tmp[i];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_hb_part(MoveArgs a, HubArgs h, HubBuckets b) {
+    __shared__ uint32_t keys[HB_PCH];
+    __shared__ uint32_t s_lo[HB_MAXB], s_cnt[HB_MAXB], s_base[HB_MAXB];
+    const int tid = threadIdx.x;
+    const uint32_t hb = b.pch_hub[blockIdx.x], st = b.pch_start[blockIdx.x];
+    const uint32_t p = h.hub_v[hb];
+    if (vtx_stopped(a, p)) return;  // uniform per block
+    const uint32_t d = h.hub_d[hb];
+    const uint32_t s0 = b.slot0[hb], K = b.nb[hb];
+    for (uint32_t i = tid; i < K; i += 256) {
+        s_lo[i] = b.lo[s0 + i];
+        s_cnt[i] = 0;
+    }
+    const uint32_t len = min(HB_PCH, d - st);
+    gather_labels<HB_PCH>(a, a.indices[p] + st, len, len, keys);  // ends with a barrier
+    for (uint32_t i = tid; i < len; i += 256) {
+        const uint32_t L = keys[i];
+        if (L != SCD_INVALID) atomicAdd(&s_cnt[hb_bucket(s_lo, K, L)], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < K; i += 256) {
+        const uint32_t c = s_cnt[i];
+        s_base[i] = c ? atomicAdd(&b.cnt[s0 + i], c) : 0u;
+        s_cnt[i] = 0;
+    }
+    __syncthreads();
+    uint32_t *ov = b.ovf + h.hub_off[hb];
+    for (uint32_t i = tid; i < len; i += 256) {
+        const uint32_t L = keys[i];
+        if (L == SCD_INVALID) continue;
+        const uint32_t j = hb_bucket(s_lo, K, L);
+        if (hb_hi(s_lo, K, j) - s_lo[j] == 1u) continue;  // one label: counted only
+        const uint32_t pos = s_base[j] + atomicAdd(&s_cnt[j], 1u);
+        if (pos < b.cap) b.region[(size_t)(s0 + j) * HB_CAP + pos] = L;
+        else ov[atomicAdd(&b.ovf_cnt[hb], 1u)] = L;
+    }
+}
+
+// Minimum / sum over the 256 threads of a block, in every thread.
+__device__ __forceinline__ uint32_t hb_block_min(uint32_t x, uint32_t *s_red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x = min(x, (uint32_t)__shfl_xor(x, o, SCD_WAVE));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = x;
+    __syncthreads();
+    return min(min(s_red[0], s_red[1]), min(s_red[2], s_red[3]));
+}
+
+__device__ __forceinline__ uint32_t hb_block_sum(uint32_t x, uint32_t *s_red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += (uint32_t)__shfl_xor(x, o, SCD_WAVE);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = x;
+    __syncthreads();
+    return s_red[0] + s_red[1] + s_red[2] + s_red[3];
+}
+
+__global__ __launch_bounds__(256) void k_hb_eval(MoveArgs a, HubArgs h, HubBuckets b) {
+    __shared__ uint32_t keys[HB_CAP];
+    __shared__ double s_bi[4];
+    __shared__ uint32_t s_bl[4], s_red[4];
+    __shared__ double s_rem;
+    __shared__ int s_has_rem;
+    const int tid = threadIdx.x;
+    const uint32_t slot = blockIdx.x;
+    const uint32_t hb = b.slot_hub[slot];
+    const uint32_t j = slot - b.slot0[hb], K = b.nb[hb];
+    const uint32_t R = j < K ? b.cnt[slot] : 0u;
+    if (R == 0) {  // uniform: a slot not in use, or an empty bucket
+        if (tid == 0) {
+            h.chunk_bi[slot] = 0.0;
+            h.chunk_bl[slot] = SCD_INVALID;
+            h.chunk_has_rem[slot] = 0;
+            h.chunk_rem[slot] = 0.0;
+        }
+        return;
+    }
+    const uint32_t p = h.hub_v[hb], d = h.hub_d[hb], my = a.labels[p];
+    const double pext = vtx_pext(a, p);
+    const uint32_t lo = b.lo[slot], hi = j + 1 < K ? b.lo[slot + 1] : SCD_INVALID;
+    if (tid == 0) s_has_rem = 0;
+    __syncthreads();
+    double bi = 0.0;
+    uint32_t bl = SCD_INVALID;
+    uint32_t heads = 0;
+    auto one = [&](uint32_t L, uint32_t c) {  // a candidate, thread 0
+        const double imp = candidate_improvement(L, c, d, my, a.stats, pext);
+        if (L == my) {
+            s_rem = -imp;
+            s_has_rem = 1;
+        }
+        if (imp > 0.0 && better(imp, L, bi, bl)) {
+            bi = imp;
+            bl = L;
+        }
+        heads++;
+    };
+    if (hi - lo == 1u) {
+        if (tid == 0) one(lo, R);
+    } else if (R <= b.cap) {
+        const uint32_t n2 = max(next_pow2(R), 1024u);
+        const uint32_t *src = b.region + (size_t)slot * HB_CAP;
+        for (uint32_t i = tid; i < n2; i += 256) keys[i] = i < R ? src[i] : SCD_INVALID;
+        __syncthreads();
+        block_sort_lds<HB_CAP>(keys, n2, R);
+        eval_sorted_batched(a, keys, n2, 0, R, d, my, 0, pext, bi, bl, heads, &s_rem, &s_has_rem);
+    } else {
+        // the region holds cap of the bucket's entries, the overflow area
+        // the rest (among other buckets' overflow)
+        const uint32_t *src = b.region + (size_t)slot * HB_CAP;
+        const uint32_t *ov = b.ovf + h.hub_off[hb];
+        const uint32_t no = b.ovf_cnt[hb];
+        uint32_t cur = lo;
+        while (true) {  // uniform
+            uint32_t mn = SCD_INVALID;
+            for (uint32_t i = tid; i < b.cap; i += 256) {
+                const uint32_t v = src[i];
+                if (v >= cur && v < mn) mn = v;
+            }
+            for (uint32_t i = tid; i < no; i += 256) {
+                const uint32_t v = ov[i];
+                if (v >= cur && v >= lo && v < hi && v < mn) mn = v;
+            }
+            mn = hb_block_min(mn, s_red);
+            if (mn == SCD_INVALID) break;
+            uint32_t c = 0;
+            for (uint32_t i = tid; i < b.cap; i += 256) c += src[i] == mn;
+            for (uint32_t i = tid; i < no; i += 256) c += ov[i] == mn;
+            c = hb_block_sum(c, s_red);
+            if (tid == 0) one(mn, c);
+            cur = mn + 1u;
+        }
+    }
+    count_pairs(a.counters, heads);
+    block_argmax256(bi, bl, s_bi, s_bl);
+    if (tid == 0) {
+        h.chunk_bi[slot] = bi;
+        h.chunk_bl[slot] = bl;
+        h.chunk_has_rem[slot] = (uint32_t)s_has_rem;
+        h.chunk_rem[slot] = s_has_rem ? s_rem : 0.0;
+    }
+}
+
 }  // namespace scd_dev
+
 
 #endif

@@ -311,7 +311,10 @@ int scd_neighbor_label_pairs(scd_ctx *ctx, uint32_t *sorted_out, uint32_t *run_o
  *                take one workgroup each.
  *   hub_table:   0: hub candidates by merge passes over sorted tiles; 1: by
  *                per-hub hash tables of (label, count) filled from sorted
- *                tiles (no merge passes).
+ *                tiles (no merge passes); 2: by label-range buckets of about
+ *                1,024 entries cut at sampled label values, each sorted and
+ *                evaluated in LDS (no merge passes; a region of 16 KiB per
+ *                bucket slot, about 32 bytes per hub adjacency entry).
  *   lds_hash:    0: the lds bin sorts each vertex's neighbour labels; 1: it
  *                counts them in an LDS hash table (lds_tile 2048 or 4096).
  *   Export (scd_neighbor_label_pairs) always sorts.

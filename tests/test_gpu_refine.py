@@ -150,7 +150,8 @@ def test_tuning_invariance(engine, manifest):
     for tile, chunk, wmax, table, lh, ln in ((2048, 256, 1024, 0, 0, 1), (2048, 4096, 256, 1, 1, 0),
                                              (4096, 1024, 512, 1, 0, 1), (16384, 2048, 1024, 0, 0, 0),
                                              (8192, 2048, 256, 0, 0, 1), (2048, 1024, 512, 1, 1, 1),
-                                             (4096, 2048, 256, 0, 1, 0)):
+                                             (4096, 2048, 256, 0, 1, 0), (2048, 1024, 1024, 2, 0, 1),
+                                             (4096, 2048, 512, 2, 1, 0), (8192, 1024, 256, 2, 0, 1)):
         engine.transfer_graph(g)
         engine.set_tuning(lds_tile=tile, merge_chunk=chunk, wave_max=wmax, hub_table=table, lds_hash=lh, lanes=ln)
         engine.labels_to_device(initial_labels(name, ent, g))
@@ -183,7 +184,8 @@ def test_error_paths(engine):
         engine.set_tuning(lds_tile=3000)
 
 
-@pytest.mark.parametrize("name,hub_table", [("pp4x16_s4", 0), ("c1", 0), ("rmat16", 0), ("rmat16", 1)])
+@pytest.mark.parametrize("name,hub_table", [("pp4x16_s4", 0), ("c1", 0), ("rmat16", 0), ("rmat16", 1),
+                                            ("rmat16", 2)])
 def test_graph_replays_match_eager(engine, manifest, name, hub_table):
     """Captured-iteration replays (several, with eager copies and host
     synchronisation between them) give the eager launches' labels and WCC
@@ -191,7 +193,7 @@ def test_graph_replays_match_eager(engine, manifest, name, hub_table):
     ent = manifest["graphs"][name]
     g = graph(name)
     engine.transfer_graph(g)
-    engine.set_tuning(lds_tile=2048 if hub_table else 4096, hub_table=hub_table, lds_hash=hub_table)
+    engine.set_tuning(lds_tile=2048 if hub_table else 4096, hub_table=hub_table, lds_hash=int(hub_table == 1))
     engine.labels_to_device(initial_labels(name, ent, g))
     engine.snapshot()
     engine.set_graph_mode(False)
@@ -264,3 +266,31 @@ def test_next_step_captures(engine):
     engine.set_graph_mode(False)
     assert _lib.hip().scd_next_step_captures(engine._ctx) == 0
     engine.set_graph_mode(True)
+
+
+@pytest.mark.parametrize("cap", [1, 64])
+def test_hub_buckets_overflow_path(manifest, monkeypatch, cap):
+    """Hub buckets whose entries exceed the region (forced here by a region
+    of `cap` entries; with the full region a sampling outlier) are evaluated
+    from the region and the overflow area: labels as the reference's, and
+    as with the merge passes, for every iteration."""
+    name = "rmat16"
+    ent = manifest["graphs"][name]
+    g = graph(name)
+    monkeypatch.setenv("SCD_HUB_BUCKET_CAP", str(cap))
+    eng = scd_amd.Engine(device=0)
+    try:
+        eng.transfer_graph(g)
+        eng.set_tuning(lds_tile=2048, hub_table=2, lds_hash=0)
+        assert eng.get_tuning()["hub_table"] == 2
+        eng.labels_to_device(initial_labels(name, ent, g))
+        best = eng.compute_wcc()
+        for it in range(1, ent["iterations"] + 1):
+            wcc, lab = eng.improve_labels(return_labels=True)
+            assert sha(lab) == ent["labels_sha256"][it], f"labels differ at iteration {it}"
+            assert wcc == pytest.approx(unhex(ent["wcc_hex"][it - 1]), rel=WCC_RTOL, abs=0)
+            if wcc < best * 1.01:
+                break
+            best = wcc
+    finally:
+        eng.close()
