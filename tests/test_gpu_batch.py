@@ -7,6 +7,7 @@ import pytest
 
 import oracle_bind as orc
 import scd_amd
+from golden_graphs import graph
 
 pytestmark = pytest.mark.gpu
 
@@ -120,3 +121,33 @@ def test_refine_again_after_bins_shrank(small_batch):
     for x, y in zip(runs[0], runs[1]):
         assert np.array_equal(x, y)
     assert len(set(runs[0][2].tolist())) > 1  # graphs stopped at different iterations
+
+
+def test_batch_hub_buckets_equal_single_graph_runs(engine):
+    """An arena whose largest graph has hubs, refined with the hub buckets
+    (lds_tile 2048, hub_table 2) while smaller graphs stop earlier: every
+    graph gets what the single-graph path gives it with the default tuning."""
+    names = ["rmat16", "pp8x40_s5", "rmat14", "c1"]
+    gs = [graph(n) for n in names]
+    assert gs[0].degree().max() > 2048
+    a = scd_amd.Arena.concat(gs)
+    print(f"[gpu] hub-bucket batch {names}: n={a.graph.n} m={a.graph.m}", flush=True)
+    with scd_amd.BatchEngine(device=0) as be:
+        be.upload(a)
+        be.set_tuning(lds_tile=2048, hub_table=2, lds_hash=0)
+        lab0, k = be.initial_partition()
+        w0 = be.compute_wcc()
+        best, iters, done, hist = be.refine(w0, max_iters=100, history=True)
+        lab_best = be.labels()
+    assert len(set(iters.tolist())) > 1  # graphs stop at different iterations
+    for i, g in enumerate(gs):
+        lo, hi = int(a.vtx_off[i]), int(a.vtx_off[i + 1])
+        engine.transfer_graph(g)
+        l0, ki = engine.initial_partition()
+        assert ki == k[i] and np.array_equal(l0, lab0[lo:hi]), f"{names[i]}: initial partition"
+        wi = engine.compute_wcc()
+        assert wi == w0[i]
+        bi, it, h = engine.refine(wi)
+        assert it == iters[i] and np.array_equal(h, hist[i, :it]), f"{names[i]}: iterations / WCC"
+        assert bi == best[i]
+        assert np.array_equal(engine.labels_from_device(), lab_best[lo:hi]), f"{names[i]}: best labels"
