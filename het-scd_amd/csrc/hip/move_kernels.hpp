@@ -787,7 +787,7 @@ __global__ __launch_bounds__(256) void k_hub_finalize(MoveArgs a, HubArgs h, uin
 constexpr uint32_t HB_SAMPLES = 4096;  // sampled labels per hub, at most
 constexpr uint32_t HB_TARGET = 1024;   // entries per bucket aimed at
 constexpr uint32_t HB_CAP = 4096;      // entries per bucket region
-constexpr uint32_t HB_PCH = 4096;      // entries per partition workgroup
+constexpr uint32_t HB_PCH = 2048;      // entries per partition workgroup
 constexpr uint32_t HB_MAXB = 2048;     // buckets per hub, at most
 
 struct HubBuckets {
