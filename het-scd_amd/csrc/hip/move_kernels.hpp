@@ -776,8 +776,9 @@ __global__ __launch_bounds__(256) void k_hub_finalize(MoveArgs a, HubArgs h, uin
 //     never sorted).  Clears the hub's bucket counters.
 //   k_hb_part: per chunk of HB_PCH entries, the labels gathered and counted
 //     per bucket in LDS; one atomic per bucket reserves the chunk's range in
-//     the bucket's region of HB_CAP entries, and entries past HB_CAP go to
-//     the hub's overflow area.  Single-label buckets are only counted.
+//     the bucket's region of HB_CAP entries (fewer only when a test asks),
+//     and entries past it go to the hub's overflow area.  Single-label
+//     buckets are only counted.
 //   k_hb_eval: per bucket slot: one label -> one evaluation; at most HB_CAP
 //     entries -> sort, reduce-by-key and evaluation in LDS; more (a sampling
 //     outlier: a bucket aims at HB_TARGET entries and holds at most about
