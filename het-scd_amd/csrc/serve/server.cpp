@@ -143,7 +143,10 @@ extern "C" const char *scd_server_last_error(void) { return g_err; }
 extern "C" uint64_t scd_server_footprint(uint32_t n, uint32_t m) {
     // resident: CSR (n + 1 + 2m) and tri, tri_deg, two label arrays (4n);
     // arena share: the engine's per-vertex (about 100 B) and per-entry
-    // (adj, internal adjacency, flags: 9 B) buffers
+    // (adj, internal adjacency, flags: 9 B) buffers.  The hub buffers come on
+    // top (about 40 B per adjacency entry of a vertex with more than
+    // lds_tile neighbours: two scratch copies and the bucket regions), as
+    // the degrees are not known here.
     const uint64_t N = n, E = 2ull * m;
     return 4 * (N + 1 + E) + 16 * N + 104 * N + 9 * E;
 }
@@ -166,17 +169,17 @@ extern "C" int scd_server_create(const scd_server_opts *opts, scd_server **out) 
     scd_set_graph_mode(s->prep, 0);
     // arenas of many graphs fill the device bin by bin: the per-bin kernels
     // run on one stream (concurrent bin lanes only added synchronization:
-    // 8% slower iterations on a 185-graph planted arena), and the hubs of
-    // several R-MAT graphs count their labels in hash tables instead of
-    // merge passes over all hub chunks (15% faster arena iterations on a
-    // 200-graph mix of planted graphs and R-MAT s18-22), and the planted
-    // graphs' dense communities count their internal triangles by bitmap
-    // rows
+    // 8% slower iterations on a 185-graph planted arena), the hubs of
+    // several R-MAT graphs take the label-range buckets (on a 200-graph mix
+    // of planted graphs and R-MAT s18-22 the hash tables made arena
+    // iterations 15% faster than merge passes over all hub chunks, and the
+    // buckets 8% faster again), and the planted graphs' dense communities
+    // count their internal triangles by bitmap rows
     for (scd_ctx *c : {s->main, s->prep}) {
         scd_tuning t;
         if (scd_get_tuning(c, &t) == SCD_OK) {
             t.lanes = 0;
-            t.hub_table = 1;
+            t.hub_table = 2;
             t.dense_wcc = 1;
             (void)scd_set_tuning(c, &t);
         }
