@@ -288,8 +288,11 @@ template <int T>
 __device__ __forceinline__ void block_sort_lds(uint32_t *s, uint32_t n2, uint32_t n) {
     // positions n .. n2-1 hold padding (SCD_INVALID, the largest key): a
     // chunk of padding only is sorted already, and merging a run with a
-    // run of padding only leaves both in place.  Chunks of n2 / 4 keys (at
-    // most 1024), so all four waves sort in registers
+    // run of padding only leaves both in place.  Positions below n hold the
+    // real keys before and after every pass, so a thread whose outputs all
+    // lie at n or above would write padding over padding: it skips.
+    // Chunks of n2 / 4 keys (at most 1024), so all four waves sort in
+    // registers
     constexpr int PER_MAX = T / 256;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t chunk = min(n2 / 4, 1024u);
@@ -301,7 +304,7 @@ __device__ __forceinline__ void block_sort_lds(uint32_t *s, uint32_t n2, uint32_
     const uint32_t o0 = threadIdx.x * per;
     for (uint32_t w2 = chunk; w2 < n2; w2 *= 2) {
         const uint32_t ps = (o0 / (2 * w2)) * (2 * w2);
-        const bool skip = ps + w2 >= n;  // B is padding only
+        const bool skip = ps + w2 >= n || o0 >= n;  // B is padding only, or so are the outputs
         const uint32_t *A = s + ps;
         const uint32_t *B = A + w2;
         uint32_t out[PER_MAX];
