@@ -3221,7 +3221,7 @@ extern "C" int scd_autotune(scd_ctx *c, const char *cache_path, int reps, scd_tu
     for (size_t i = 0; i < cand.size() && rc == SCD_OK; i++) {
         double med = 0.0;
         if ((rc = time_tuning(cand[i], reps, &med)) == SCD_OK) timed.emplace_back(med, c->tune);
-        else if (rc == SCD_ENOMEM) rc = SCD_OK;  // a shape whose buffers do not fit: not a candidate
+        else if (rc == SCD_ENOMEM || rc == SCD_ELIMIT) rc = SCD_OK;  // a shape whose buffers do not fit: not a candidate
     }
     if (rc == SCD_OK) {
         std::stable_sort(timed.begin(), timed.end(),
