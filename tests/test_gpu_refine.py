@@ -294,3 +294,29 @@ def test_hub_buckets_overflow_path(manifest, monkeypatch, cap):
             best = wcc
     finally:
         eng.close()
+
+
+def test_hub_buckets_from_random_labels(engine):
+    """Hub buckets (lds_tile 2048, hub_table 2) from a random partition: two
+    steps, the second with many INVALID labels; the move and the export
+    (which sorts hubs by tiles and merge passes whatever the tuning) against
+    the oracle."""
+    g = graph("rmat16")
+    rng = np.random.default_rng(20000)
+    lab = rng.integers(0, 20000, g.n).astype(np.uint32)
+    engine.transfer_graph(g)
+    engine.set_tuning(lds_tile=2048, hub_table=2, lds_hash=0)
+    try:
+        engine.labels_to_device(lab)
+        for step in range(2):
+            srt, off, rl, rc = engine.neighbor_label_pairs()
+            osrt, ooff, orl, orc_ = orc.neighbor_label_pairs(g, lab)
+            assert np.array_equal(srt, osrt) and np.array_equal(rl, orl) and np.array_equal(rc, orc_)
+            new, st, w, terms, it, itd = oracle_step(g, lab, 20000, g.clustering_coef)
+            wcc, got = engine.improve_labels(return_labels=True)
+            assert np.array_equal(got, new), f"labels differ at step {step}"
+            assert wcc == pytest.approx(w, rel=WCC_RTOL, abs=0)
+            lab = new
+        assert (lab == INVALID).sum() > 0
+    finally:
+        engine.set_tuning(lds_tile=4096, merge_chunk=2048, wave_max=1024, hub_table=0, lds_hash=0, lanes=1)
