@@ -982,21 +982,21 @@ static int phase_stats(scd_ctx *c, bool iteration = false) {
 }
 
 // Exclusive scan of n uint32 values (in -> out, no aliasing) on the stream
-// without host involvement: tile sums, scan of the tile sums (recursively,
-// into the level buffers preallocated in wq_sum / wq_scan), tile scans; the
-// total goes to total_dev when given.  Capturable: no host copies, no
+// without host involvement: tile sums, scan of the tile sums (one workgroup
+// up to 2048 tiles, else recursively into the level buffers preallocated in
+// wq_sum / wq_scan), tile scans; the total goes to total_dev when given.  Capturable: no host copies, no
 // memset nodes.
 static int scan_u32_dev(scd_ctx *c, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *total_dev, int level) {
     const uint32_t nt = cdiv(n, 2048);
     uint32_t *sums = c->wq_sum[level], *scan = c->wq_scan[level];
     LAUNCH(c, "scan_tile_sums", nt, 256, k_scan_tile_sums, in, n, sums);
-    if (nt > 1) {
+    if (nt <= 2048) {  // the tile sums' scan and the total in one workgroup
+        LAUNCH(c, "scan_top", 1, 256, k_scan_top, sums, nt, scan, total_dev);
+    } else {
         int rc = scan_u32_dev(c, sums, scan, nt, nullptr, level + 1);
         if (rc) return rc;
-    } else {
-        LAUNCH(c, "scan_zero", 1, 64, k_fill_u32, scan, 1u);
+        if (total_dev) LAUNCH(c, "scan_total", 1, 64, k_scan_total, scan, sums, nt, total_dev);
     }
-    if (total_dev) LAUNCH(c, "scan_total", 1, 64, k_scan_total, scan, sums, nt, total_dev);
     LAUNCH(c, "scan_tiles", nt, 256, k_scan_tiles, in, n, scan, out);
     return SCD_OK;
 }

@@ -516,6 +516,29 @@ __global__ __launch_bounds__(256) void k_scan_tiles(const uint32_t *__restrict__
 }
 
 // total[0] = exclusive-scan offset of the last tile + its sum.
+// Exclusive scan of at most 2048 tile sums by one workgroup, and their total:
+// the top level of scan_u32_dev in one launch.
+__global__ __launch_bounds__(256) void k_scan_top(const uint32_t *__restrict__ sums, uint32_t nt,
+                                                  uint32_t *__restrict__ scan, uint32_t *__restrict__ total) {
+    __shared__ uint32_t s_w[4];
+    uint32_t loc[8], acc = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const uint32_t i = threadIdx.x * 8u + k;
+        loc[k] = i < nt ? sums[i] : 0u;
+        acc += loc[k];
+    }
+    uint32_t tot;
+    uint32_t run = block_excl_scan256(acc, s_w, tot);
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const uint32_t i = threadIdx.x * 8u + k;
+        if (i < nt) scan[i] = run;
+        run += loc[k];
+    }
+    if (threadIdx.x == 0 && total) total[0] = tot;
+}
+
 __global__ void k_scan_total(const uint32_t *__restrict__ tile_off, const uint32_t *__restrict__ tile_sums,
                              uint32_t ntiles, uint32_t *__restrict__ total) {
     if (threadIdx.x == 0 && blockIdx.x == 0) total[0] = tile_off[ntiles - 1] + tile_sums[ntiles - 1];
