@@ -790,6 +790,7 @@ constexpr uint32_t HB_TARGET = 1024;   // entries per bucket aimed at
 constexpr uint32_t HB_CAP = 4096;      // entries per bucket region
 constexpr uint32_t HB_PCH = 2048;      // entries per partition workgroup
 constexpr uint32_t HB_MAXB = 2048;     // buckets per hub, at most
+static_assert(HB_MAXB <= 65536, "k_hb_part keeps bucket ids in 16 bits");
 
 struct HubBuckets {
     const uint32_t *__restrict__ sigma;      // per hub: sample stride
@@ -915,6 +916,7 @@ __global__ __launch_bounds__(256) void k_hb_split(MoveArgs a, HubArgs h, HubBuck
 
 __global__ __launch_bounds__(256) void k_hb_part(MoveArgs a, HubArgs h, HubBuckets b) {
     __shared__ uint32_t keys[HB_PCH];
+    __shared__ uint16_t s_bk[HB_PCH];  // each entry's bucket, found once
     __shared__ uint32_t s_lo[HB_MAXB], s_cnt[HB_MAXB], s_base[HB_MAXB];
     const int tid = threadIdx.x;
     const uint32_t hb = b.pch_hub[blockIdx.x], st = b.pch_start[blockIdx.x];
@@ -930,7 +932,11 @@ __global__ __launch_bounds__(256) void k_hb_part(MoveArgs a, HubArgs h, HubBucke
     gather_labels<HB_PCH>(a, a.indices[p] + st, len, len, keys);  // ends with a barrier
     for (uint32_t i = tid; i < len; i += 256) {
         const uint32_t L = keys[i];
-        if (L != SCD_INVALID) atomicAdd(&s_cnt[hb_bucket(s_lo, K, L)], 1u);
+        if (L != SCD_INVALID) {
+            const uint32_t j = hb_bucket(s_lo, K, L);
+            s_bk[i] = (uint16_t)j;
+            atomicAdd(&s_cnt[j], 1u);
+        }
     }
     __syncthreads();
     for (uint32_t i = tid; i < K; i += 256) {
@@ -943,7 +949,7 @@ __global__ __launch_bounds__(256) void k_hb_part(MoveArgs a, HubArgs h, HubBucke
     for (uint32_t i = tid; i < len; i += 256) {
         const uint32_t L = keys[i];
         if (L == SCD_INVALID) continue;
-        const uint32_t j = hb_bucket(s_lo, K, L);
+        const uint32_t j = s_bk[i];
         if (hb_hi(s_lo, K, j) - s_lo[j] == 1u) continue;  // one label: counted only
         const uint32_t pos = s_base[j] + atomicAdd(&s_cnt[j], 1u);
         if (pos < b.cap) b.region[(size_t)(s0 + j) * HB_CAP + pos] = L;
