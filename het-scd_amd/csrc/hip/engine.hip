@@ -3157,7 +3157,9 @@ extern "C" int scd_autotune(scd_ctx *c, const char *cache_path, int reps, scd_tu
             if (c->E >= (1ull << 20)) Ns = {0, 1};
             for (uint32_t C : Cs)
                 for (uint32_t H : Hs) {
-                    if (H == 2 && C != Cs[0]) continue;  // the buckets use no merge chunks
+                    // with the buckets the chunk size only shapes the hub
+                    // statistics filter: its two largest
+                    if (H == 2 && Cs.size() > 1 && C == Cs[0]) continue;
                     for (uint32_t Lh : Ls)
                         for (uint32_t N : Ns) cand.push_back(scd_tuning{T, C, W, H, Lh, N, 0, 0});
                 }
